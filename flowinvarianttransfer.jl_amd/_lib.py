@@ -1,0 +1,76 @@
+"""ctypes binding of libfit_hip.so (declared in include/fit_hip.h).
+
+The library is the product path: nothing here falls back to a CPU implementation.  If the
+shared library has not been built, or a call fails on the device, an exception is raised.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libfit_hip.so")
+
+F32, F64 = 0, 1
+DEALIAS_NONE, DEALIAS_TWOTHIRDS, DEALIAS_PADDED = 0, 1, 2
+KINETIC_ENERGY, HELICITY, ENSTROPHY, PASSIVE_SCALAR = 0, 1, 2, 3
+PTR_HOST, PTR_DEVICE = 0, 1
+K_CLASSES = ("fft_column", "fft_row_inverse", "fft_row_forward", "density_shell", "reduce", "other")
+
+# every entry point of include/fit_hip.h with its ctypes signature
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.POINTER(ctypes.c_double)
+SIGNATURES = {
+    "fit_last_error": (ctypes.c_char_p, []),
+    "fit_version": (ctypes.c_char_p, []),
+    "fit_plan_create": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(ctypes.c_int64), _I, _I, _I, _I,
+                             ctypes.POINTER(ctypes.POINTER(ctypes.c_double)),
+                             ctypes.POINTER(ctypes.c_int64), _I, _I]),
+    "fit_plan_destroy": (_I, [_P]),
+    "fit_plan_device_bytes": (ctypes.c_int64, [_P]),
+    "fit_nonlinear_term": (_I, [_P, _P, _P, _P, _I, _I]),
+    "fit_transfer_density": (_I, [_P, _I, _P, _P, _P, _I, _I]),
+    "fit_spectral_flux": (_I, [_P, _I, _P, _P, _D, _D, _I, _I]),
+    "fit_spectral_flux_with_nhat": (_I, [_P, _I, _P, _P, _D, _D, _I, _I]),
+    "fit_shell_to_shell": (_I, [_P, _I, _P, _P, _D, _D, _D, _I, _I, _I]),
+    "fit_mode_to_mode": (_I, [_P, _I, _P, _P, _P, _P, _I, _I]),
+    "fit_helical_decompose": (_I, [_P, _P, _P, _P, _I, _I]),
+    "fit_profile_enable": (_I, [_P, _I]),
+    "fit_profile_query": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int64), _D, _D]),
+    "fit_plan_stream": (_P, [_P]),
+}
+
+
+class DimensionMismatch(ValueError):
+    """Shape errors (the reference throws Julia's DimensionMismatch)."""
+
+
+_ERRORS = {1: ValueError, 2: DimensionMismatch, 3: NotImplementedError, 4: MemoryError,
+           5: RuntimeError}
+
+_lib = None
+
+
+def load_library(path=None):
+    """Load libfit_hip.so once; raises if it is missing (no fallback path exists)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"{p} is missing: build the HIP backend first (python -c 'import __graft_entry__ as g; "
+            "g.build()' or python flowinvarianttransfer.jl_amd/build_lib.py)")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load_library().fit_last_error().decode(errors="replace")
+        raise _ERRORS.get(rc, RuntimeError)(msg or f"libfit_hip error code {rc}")

@@ -1,0 +1,460 @@
+"""Host-side mirror of the reference's public API for the pseudospectral transfer path, with every
+computation routed to libfit_hip.so (MI355X).
+
+Same names, argument meaning, defaults and error behaviour as the reference (src/SpectralFlux.jl,
+src/ShellToShell/ShellToShellTransfer.jl, src/ScaleToScale/ScaleToScaleTransfer.jl,
+src/NonlinearTerm.jl, src/Invariants.jl), with the backend singletons replaced by
+``HIPSpectral()`` / ``HIPExecution()``.  Arrays use the reference's layout: shape (ns..., C) with
+the component axis last, column-major in memory.  Host inputs are numpy arrays; device inputs are
+torch tensors on a HIP device (results then stay on the device).
+"""
+import ctypes
+from collections import OrderedDict
+
+import numpy as np
+
+from . import _lib
+from .grid import as_component_field, shell_setup
+from .types import (AbstractDealiasing, AbstractInvariant, HelicalDecomposition, HIPExecution,
+                    HIPSpectral, KineticEnergy, ModeToModeTriadResult, NoDecomposition,
+                    OrszagTwoThirds, PassiveScalar, ShellMagnitude, ShellToShellResult,
+                    SpectralFluxResult)
+
+try:  # torch is only plumbing for device arrays
+    import torch
+except Exception:  # pragma: no cover
+    torch = None
+
+
+# ---------------------------------------------------------------------------------------------
+# array marshalling
+# ---------------------------------------------------------------------------------------------
+
+def _is_device(a):
+    return torch is not None and isinstance(a, torch.Tensor) and a.device.type == "cuda"
+
+
+def _real_dtype(a):
+    if _is_device(a):
+        return np.float64 if a.dtype in (torch.complex128, torch.float64) else np.float32
+    dt = np.asarray(a).dtype
+    return np.float64 if dt in (np.complex128, np.float64) else np.float32
+
+
+def _marshal(a, io):
+    """-> (pointer, keep-alive object, ptr_kind) for a complex array in io precision (F32/F64),
+    column-major."""
+    if _is_device(a):
+        want = torch.complex64 if io == _lib.F32 else torch.complex128
+        t = a if a.dtype == want else a.to(want)
+        rev = tuple(reversed(range(t.dim())))
+        if not t.permute(rev).is_contiguous():
+            t = t.permute(rev).contiguous().permute(rev)
+        return t.data_ptr(), t, _lib.PTR_DEVICE
+    want = np.complex64 if io == _lib.F32 else np.complex128
+    arr = np.asfortranarray(np.asarray(a, dtype=want))
+    return arr.ctypes.data, arr, _lib.PTR_HOST
+
+
+def _alloc(like_device, shape, io, complex_=True):
+    if like_device is not None:
+        if complex_:
+            dt = torch.complex64 if io == _lib.F32 else torch.complex128
+        else:
+            dt = torch.float32 if io == _lib.F32 else torch.float64
+        rev = tuple(reversed(range(len(shape))))
+        t = torch.empty(tuple(reversed(shape)), dtype=dt, device=like_device.device).permute(rev)
+        return t, t.data_ptr()
+    if complex_:
+        dt = np.complex64 if io == _lib.F32 else np.complex128
+    else:
+        dt = np.float32 if io == _lib.F32 else np.float64
+    arr = np.empty(shape, dtype=dt, order="F")
+    return arr, arr.ctypes.data
+
+
+def _dptr(arr):
+    return arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check_invariant(invariant, nd, M):
+    """Invariants.jl:63 and :99-101 -- raised before any device work."""
+    from .types import Enstrophy, Helicity
+    if isinstance(invariant, Helicity) and nd != 3:
+        raise ValueError(f"Helicity transfer is defined in 3D only (got nd={nd}).")
+    if isinstance(invariant, Enstrophy) and nd not in (2, 3):
+        raise ValueError("Enstrophy transfer is defined in 2D (scalar vorticity, conserved) or 3D "
+                         f"(vector vorticity, non-conservative via stretching); got nd={nd}.")
+    if isinstance(invariant, (Helicity, Enstrophy)) and M < nd:
+        raise _lib.DimensionMismatch("vorticity-weighted invariants need nd velocity components")
+
+
+def _check_spectral(spectral):
+    if not isinstance(spectral, HIPSpectral):
+        raise ValueError(f"spectral={spectral!r}: this package provides the HIP backend only "
+                         "(pass spectral=HIPSpectral()).")
+
+
+# ---------------------------------------------------------------------------------------------
+# plans (device workspaces)
+# ---------------------------------------------------------------------------------------------
+
+class Plan:
+    """Device-resident workspace for one grid (the reference's NonlinearTermWorkspace /
+    SpectralFluxWorkspace / ShellToShellWorkspace, Workspaces.jl:46-170, plus FFT plans)."""
+
+    def __init__(self, ns, M, D, ks, dealiasing=OrszagTwoThirds(), precision=np.float32,
+                 shell_idx=None, n_shells=0, device=0):
+        self.lib = _lib.load_library()
+        self.ns = tuple(int(n) for n in ns)
+        self.nd = len(self.ns)
+        self.M, self.D = int(M), int(D)
+        self.io = _lib.F64 if np.dtype(precision) == np.float64 else _lib.F32
+        self.n_shells = int(n_shells)
+        nsarr = (ctypes.c_int64 * self.nd)(*self.ns)
+        self._ks = [np.ascontiguousarray(np.asarray(k, dtype=np.float64)) for k in ks]
+        if len(self._ks) != self.nd or any(len(k) != n for k, n in zip(self._ks, self.ns)):
+            raise _lib.DimensionMismatch("ks must hold nd vectors of lengths ns")
+        ksp = (ctypes.POINTER(ctypes.c_double) * self.nd)(*[_dptr(k) for k in self._ks])
+        sp = None
+        if self.n_shells > 0:
+            self._shell = np.asfortranarray(np.asarray(shell_idx, dtype=np.int64))
+            if self._shell.shape != self.ns:
+                raise _lib.DimensionMismatch("shell_idx must have shape ns")
+            sp = self._shell.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        self.handle = ctypes.c_void_p()
+        rc = self.lib.fit_plan_create(ctypes.byref(self.handle), self.nd, nsarr, self.M, self.D,
+                                      self.io, dealiasing.code, ksp, sp, self.n_shells, device)
+        _lib.check(rc)
+
+    def close(self):
+        if getattr(self, "handle", None) and self.handle.value:
+            self.lib.fit_plan_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def device_bytes(self):
+        return int(self.lib.fit_plan_device_bytes(self.handle))
+
+    @property
+    def stream(self):
+        return self.lib.fit_plan_stream(self.handle)
+
+    # -- profiling hooks -----------------------------------------------------------------------
+    def profile(self, on=True):
+        _lib.check(self.lib.fit_profile_enable(self.handle, int(on)))
+
+    def profile_stats(self):
+        out = {}
+        for i, name in enumerate(_lib.K_CLASSES):
+            n = ctypes.c_int64()
+            ms = ctypes.c_double()
+            by = ctypes.c_double()
+            _lib.check(self.lib.fit_profile_query(self.handle, i, ctypes.byref(n), ctypes.byref(ms),
+                                                  ctypes.byref(by)))
+            out[name] = {"launches": n.value, "ms": ms.value, "alg_bytes": by.value}
+        return out
+
+    # -- entry points --------------------------------------------------------------------------
+    def _fa(self, f, adv):
+        fp, fk, kind = _marshal(f, self.io)
+        if adv is None:
+            return fp, None, (fk,), kind
+        ap, ak, kind2 = _marshal(adv, self.io)
+        if kind2 != kind:
+            raise ValueError("advected and advecting fields must both be host or both device arrays")
+        return fp, ap, (fk, ak), kind
+
+    def nonlinear_term(self, f, adv=None):
+        fp, ap, keep, kind = self._fa(f, adv)
+        out, op = _alloc(keep[0] if kind == _lib.PTR_DEVICE else None, self.ns + (self.M,), self.io)
+        _lib.check(self.lib.fit_nonlinear_term(self.handle, fp, ap, op, self.io, kind))
+        return out
+
+    def transfer_density(self, invariant, c_hat, N_hat):
+        cp, ck, kind = _marshal(c_hat, self.io)
+        npp, nk, _ = _marshal(N_hat, self.io)
+        out, op = _alloc(ck if kind == _lib.PTR_DEVICE else None, self.ns, self.io, complex_=False)
+        _lib.check(self.lib.fit_transfer_density(self.handle, invariant.code, cp, npp, op, self.io,
+                                                 kind))
+        return out
+
+    def spectral_flux(self, invariant, f, adv=None):
+        fp, ap, keep, kind = self._fa(f, adv)
+        T = np.zeros(self.n_shells)
+        Pi = np.zeros(self.n_shells)
+        _lib.check(self.lib.fit_spectral_flux(self.handle, invariant.code, fp, ap, _dptr(T),
+                                              _dptr(Pi), self.io, kind))
+        return T, Pi
+
+    def spectral_flux_with_nhat(self, invariant, c_hat, N_hat):
+        cp, ck, kind = _marshal(c_hat, self.io)
+        npp, nk, _ = _marshal(N_hat, self.io)
+        T = np.zeros(self.n_shells)
+        Pi = np.zeros(self.n_shells)
+        _lib.check(self.lib.fit_spectral_flux_with_nhat(self.handle, invariant.code, cp, npp,
+                                                        _dptr(T), _dptr(Pi), self.io, kind))
+        return T, Pi
+
+    def shell_to_shell(self, invariant, f, adv=None, verify=True):
+        fp, ap, keep, kind = self._fa(f, adv)
+        n = self.n_shells
+        T = np.zeros((n, n), order="F")
+        net = np.zeros(n)
+        asym = np.zeros(1)
+        _lib.check(self.lib.fit_shell_to_shell(self.handle, invariant.code, fp, ap, _dptr(T),
+                                               _dptr(net), _dptr(asym), int(bool(verify)), self.io,
+                                               kind))
+        return T, net, float(asym[0])
+
+    def mode_to_mode(self, invariant, f, adv=None):
+        fp, ap, keep, kind = self._fa(f, adv)
+        like = keep[0] if kind == _lib.PTR_DEVICE else None
+        S, sp = _alloc(like, self.ns + self.ns, self.io, complex_=False)
+        net, npp = _alloc(like, self.ns, self.io, complex_=False)
+        _lib.check(self.lib.fit_mode_to_mode(self.handle, invariant.code, fp, ap, sp, npp, self.io,
+                                             kind))
+        return net, S
+
+    def helical_decompose(self, u):
+        up, uk, kind = _marshal(u, self.io)
+        like = uk if kind == _lib.PTR_DEVICE else None
+        a, ap = _alloc(like, self.ns + (self.M,), self.io)
+        b, bp = _alloc(like, self.ns + (self.M,), self.io)
+        _lib.check(self.lib.fit_helical_decompose(self.handle, up, ap, bp, self.io, kind))
+        return a, b
+
+
+_PLANS = OrderedDict()
+_MAX_PLANS = 4
+
+
+def _plan_for(ns, M, D, ks, dealiasing, precision, device, binning=None, geometry=None,
+              shells=False):
+    key = (tuple(ns), M, D, tuple(np.asarray(k, dtype=np.float64).tobytes() for k in ks),
+           type(dealiasing).__name__, np.dtype(precision).name, device,
+           repr(binning) if shells else None, repr(geometry) if shells else None)
+    if key in _PLANS:
+        _PLANS.move_to_end(key)
+        return _PLANS[key]
+    centers = edges = None
+    sidx = None
+    nsh = 0
+    if shells:
+        centers, edges, sidx = shell_setup(ks, binning, geometry)
+        nsh = len(centers)
+    plan = Plan(ns, M, D, ks, dealiasing, precision, sidx, nsh, device)
+    plan.centers, plan.edges = centers, edges
+    _PLANS[key] = plan
+    while len(_PLANS) > _MAX_PLANS:
+        _PLANS.popitem(last=False)[1].close()
+    return plan
+
+
+def clear_plan_cache():
+    while _PLANS:
+        _PLANS.popitem()[1].close()
+
+
+def _shape_info(f_hat, ks, advecting_hat):
+    nd = len(ks)
+    ns = tuple(int(s) for s in f_hat.shape[:nd])
+    if len(f_hat.shape) != nd + 1:
+        raise _lib.DimensionMismatch(f"expected an (ns..., M) array with nd={nd}")
+    M = int(f_hat.shape[nd])
+    D = int(advecting_hat.shape[nd]) if advecting_hat is not None else M
+    if advecting_hat is not None and tuple(advecting_hat.shape[:nd]) != ns:
+        raise _lib.DimensionMismatch("advecting_hat grid differs from the advected field's")
+    return nd, ns, M, D
+
+
+def _ft_cast(x, ft):
+    return np.asarray(x).astype(ft)
+
+
+# ---------------------------------------------------------------------------------------------
+# public API (same names / keywords as the reference)
+# ---------------------------------------------------------------------------------------------
+
+def compute_nonlinear_term(velocity_hat, ks, *, dealiasing=OrszagTwoThirds(),
+                           spectral=HIPSpectral(), advecting_hat=None):
+    """NonlinearTerm.jl:65-76 -- N̂ = FFT[(u_adv·∇) f] with the reference's normalisation."""
+    _check_spectral(spectral)
+    nd, ns, M, D = _shape_info(velocity_hat, ks, advecting_hat)
+    ft = _real_dtype(velocity_hat)
+    plan = _plan_for(ns, M, D, ks, dealiasing, spectral.compute_dtype(ft), spectral.device)
+    plan_io = plan.io
+    out = plan.nonlinear_term(velocity_hat, advecting_hat)
+    if not _is_device(out):
+        out = out.astype(np.complex64 if ft == np.float32 else np.complex128)
+    return out
+
+
+def transfer_density(invariant, velocity_hat, N_hat, ks, *, spectral=HIPSpectral()):
+    """Invariants.jl:112-118 (allocating form of transfer_density!)."""
+    _check_spectral(spectral)
+    nd = len(ks)
+    ns = tuple(int(s) for s in velocity_hat.shape[:nd])
+    M = int(velocity_hat.shape[nd])
+    _check_invariant(invariant, nd, M)
+    ft = _real_dtype(velocity_hat)
+    plan = _plan_for(ns, M, max(M, nd), ks, OrszagTwoThirds(), spectral.compute_dtype(ft),
+                     spectral.device)
+    t = plan.transfer_density(invariant, velocity_hat, N_hat)
+    return t if _is_device(t) else t.astype(ft)
+
+
+def calculate_spectral_flux(velocity_hat, ks, *, binning=None, dealiasing=OrszagTwoThirds(),
+                            invariant=KineticEnergy(), decomposition=NoDecomposition(),
+                            spectral=HIPSpectral(), advecting_hat=None, geometry=None):
+    """SpectralFlux.jl:49-94: T(k_n) = Σ_{shell n} t(k), Π = +cumsum(T)."""
+    _check_spectral(spectral)
+    nd, ns, M, D = _shape_info(velocity_hat, ks, advecting_hat)
+    _check_invariant(invariant, nd, M)
+    ft = _real_dtype(velocity_hat)
+    geometry = geometry if geometry is not None else ShellMagnitude(None)
+    plan = _plan_for(ns, M, D, ks, dealiasing, spectral.compute_dtype(ft), spectral.device,
+                     binning, geometry, shells=True)
+    centers = _ft_cast(plan.centers, ft)
+    if isinstance(decomposition, HelicalDecomposition):
+        N = plan.nonlinear_term(velocity_hat, advecting_hat)
+        up, um = plan.helical_decompose(velocity_hat)
+        res = {}
+        for name, comp in (("positive", up), ("negative", um)):
+            T, Pi = plan.spectral_flux_with_nhat(invariant, comp, N)
+            res[name] = SpectralFluxResult(centers, _ft_cast(T, ft), _ft_cast(Pi, ft))
+        return res
+    if not isinstance(decomposition, NoDecomposition):
+        raise ValueError(f"decomposition {decomposition!r} is not on the HIP path")
+    T, Pi = plan.spectral_flux(invariant, velocity_hat, advecting_hat)
+    return SpectralFluxResult(centers, _ft_cast(T, ft), _ft_cast(Pi, ft))
+
+
+def calculate_scalar_flux(velocity_hat, scalar_hat, ks, *, binning=None,
+                          dealiasing=OrszagTwoThirds(), spectral=HIPSpectral(), geometry=None):
+    """SpectralFlux.jl:203-215."""
+    th = as_component_field(scalar_hat, len(ks))
+    return calculate_spectral_flux(th, ks, binning=binning, dealiasing=dealiasing,
+                                   invariant=PassiveScalar(), advecting_hat=velocity_hat,
+                                   spectral=spectral, geometry=geometry)
+
+
+def calculate_shell_to_shell_transfer(velocity_hat, ks, *, binning=None,
+                                      dealiasing=OrszagTwoThirds(), verify_antisymmetry=True,
+                                      invariant=KineticEnergy(), spectral=HIPSpectral(),
+                                      execution=HIPExecution(), advecting_hat=None, geometry=None):
+    """ShellToShellTransfer.jl:78-104 with the giver loop on the device."""
+    _check_spectral(spectral)
+    if not isinstance(execution, HIPExecution):
+        raise ValueError(f"execution={execution!r}: pass execution=HIPExecution()")
+    nd, ns, M, D = _shape_info(velocity_hat, ks, advecting_hat)
+    _check_invariant(invariant, nd, M)
+    ft = _real_dtype(velocity_hat)
+    geometry = geometry if geometry is not None else ShellMagnitude(None)
+    plan = _plan_for(ns, M, D, ks, dealiasing, spectral.compute_dtype(ft), spectral.device,
+                     binning, geometry, shells=True)
+    T, net, asym = plan.shell_to_shell(invariant, velocity_hat, advecting_hat,
+                                       verify_antisymmetry)
+    return ShellToShellResult(_ft_cast(plan.centers, ft), _ft_cast(plan.edges, ft),
+                              np.asfortranarray(T.astype(ft)), _ft_cast(net, ft), ft(asym))
+
+
+def calculate_scalar_shell_to_shell_transfer(velocity_hat, scalar_hat, ks, **kw):
+    """ShellToShellTransfer.jl:143-147."""
+    th = as_component_field(scalar_hat, len(ks))
+    return calculate_shell_to_shell_transfer(th, ks, invariant=PassiveScalar(),
+                                             advecting_hat=velocity_hat, **kw)
+
+
+def calculate_mode_to_mode_transfer(velocity_hat, ks, *, invariant=KineticEnergy(),
+                                    dealiasing=OrszagTwoThirds(), spectral=HIPSpectral(),
+                                    max_modes=1024, force=False, advecting_hat=None):
+    """ScaleToScaleTransfer.jl:49-91 (guard at :64-67)."""
+    _check_spectral(spectral)
+    nd, ns, M, D = _shape_info(velocity_hat, ks, advecting_hat)
+    _check_invariant(invariant, nd, M)
+    Nm = int(np.prod(ns))
+    if not (force or Nm <= max_modes):
+        raise ValueError(
+            f"calculate_mode_to_mode_transfer resolves S(k|p) over all {Nm}×{Nm} mode pairs "
+            f"(O(N^{{2D}}) time/memory); N_modes={Nm} exceeds max_modes={max_modes}. Use "
+            "calculate_shell_to_shell_transfer / calculate_spectral_flux for the aggregates, or "
+            "pass force=True.")
+    ft = _real_dtype(velocity_hat)
+    plan = _plan_for(ns, M, D, ks, dealiasing, spectral.compute_dtype(ft), spectral.device)
+    net, S = plan.mode_to_mode(invariant, velocity_hat, advecting_hat)
+    if not _is_device(S):
+        net, S = net.astype(ft), S.astype(ft)
+    return ModeToModeTriadResult(invariant, ks, net, S)
+
+
+def calculate_scalar_mode_to_mode_transfer(velocity_hat, scalar_hat, ks, **kw):
+    """ScaleToScaleTransfer.jl:101-105."""
+    th = as_component_field(scalar_hat, len(ks))
+    return calculate_mode_to_mode_transfer(th, ks, invariant=PassiveScalar(),
+                                           advecting_hat=velocity_hat, **kw)
+
+
+def decompose_field(decomposition, velocity_hat, ks, *, spectral=HIPSpectral()):
+    """Decomposition.jl:42-98 (NoDecomposition -> identity; helical on the device)."""
+    if isinstance(decomposition, NoDecomposition):
+        return velocity_hat
+    if not isinstance(decomposition, HelicalDecomposition):
+        raise ValueError(f"decomposition {decomposition!r} is not on the HIP path")
+    nd = len(ks)
+    if nd != 3:
+        raise ValueError(f"HelicalDecomposition is defined in 3D only (got nd={nd}).")
+    ns = tuple(int(s) for s in velocity_hat.shape[:nd])
+    M = int(velocity_hat.shape[nd])
+    ft = _real_dtype(velocity_hat)
+    plan = _plan_for(ns, M, max(M, nd), ks, OrszagTwoThirds(), spectral.compute_dtype(ft),
+                     spectral.device)
+    up, um = plan.helical_decompose(velocity_hat)
+    return {"positive": up, "negative": um}
+
+
+def calculate_partial_fluxes(velocity_hat, ks, *, decomposition=HelicalDecomposition(),
+                             binning=None, dealiasing=OrszagTwoThirds(), spectral=HIPSpectral(),
+                             geometry=None):
+    """SpectralFlux.jl:240-285: channels[(s_k, s_p, s_q)] from (u_{s_p}·∇)u_{s_q} carried by
+    u_{s_k}; returns {"channels": ..., "total": ..., "k_shells": ...}."""
+    _check_spectral(spectral)
+    comps = decompose_field(decomposition, velocity_hat, ks, spectral=spectral)
+    if not isinstance(comps, dict):
+        raise ValueError("calculate_partial_fluxes needs a decomposition that splits u into ≥2 "
+                         f"named components; got {decomposition!r}.")
+    nd, ns, M, D = _shape_info(velocity_hat, ks, None)
+    ft = _real_dtype(velocity_hat)
+    geometry = geometry if geometry is not None else ShellMagnitude(None)
+    plan = _plan_for(ns, M, M, ks, dealiasing, spectral.compute_dtype(ft), spectral.device,
+                     binning, geometry, shells=True)
+    centers = _ft_cast(plan.centers, ft)
+    channels = {}
+    for sp in comps:
+        for sq in comps:
+            N = plan.nonlinear_term(comps[sq], comps[sp])
+            for sk in comps:
+                T, Pi = plan.spectral_flux_with_nhat(KineticEnergy(), comps[sk], N)
+                channels[(sk, sp, sq)] = SpectralFluxResult(centers, _ft_cast(T, ft),
+                                                            _ft_cast(Pi, ft))
+    totT = sum(c.transfer_spectrum.astype(np.float64) for c in channels.values())
+    totP = sum(c.flux.astype(np.float64) for c in channels.values())
+    total = SpectralFluxResult(centers, totT.astype(ft), totP.astype(ft))
+    return {"channels": channels, "total": total, "k_shells": centers}
+
+
+def calculate_helical_partial_fluxes(velocity_hat, ks, **kw):
+    """SpectralFlux.jl:295-296."""
+    return calculate_partial_fluxes(velocity_hat, ks, decomposition=HelicalDecomposition(), **kw)

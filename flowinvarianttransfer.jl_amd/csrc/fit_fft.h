@@ -1,0 +1,176 @@
+// fit_fft.h -- device-side building blocks: complex helpers, radix-2/3/4/5/8 butterflies and a
+// mixed-radix Stockham FFT that runs B interleaved lines of length L out of LDS.
+//
+// LDS layout: point i of line l lives at buf[i*B + l] (lines interleaved), so a wavefront that
+// walks l fastest touches consecutive LDS words and, in column passes, consecutive HBM words.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fit {
+
+template <typename R> struct CT;
+template <> struct CT<float> { using T = float2; };
+template <> struct CT<double> { using T = double2; };
+template <typename R> using cpx = typename CT<R>::T;
+
+template <typename R> __device__ __forceinline__ cpx<R> mk(R x, R y) { cpx<R> v; v.x = x; v.y = y; return v; }
+template <typename V> __device__ __forceinline__ V cadd(V a, V b) { V r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename V> __device__ __forceinline__ V csub(V a, V b) { V r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
+  V r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r;
+}
+template <typename V> __device__ __forceinline__ V cconj(V a) { V r; r.x = a.x; r.y = -a.y; return r; }
+template <typename V, typename R> __device__ __forceinline__ V cscale(V a, R s) { V r; r.x = a.x * s; r.y = a.y * s; return r; }
+// multiply by (SIGN * i):  SIGN=+1 -> i*a = (-a.y, a.x);  SIGN=-1 -> -i*a = (a.y, -a.x)
+template <int SIGN, typename V> __device__ __forceinline__ V cmuli(V a) {
+  V r;
+  if (SIGN > 0) { r.x = -a.y; r.y = a.x; } else { r.x = a.y; r.y = -a.x; }
+  return r;
+}
+
+constexpr int FIT_MAX_STAGES = 16;
+
+// Radix plan of one transform length (host fills it; passed by value to kernels).
+struct FftDesc {
+  int L;
+  int nst;
+  int rad[FIT_MAX_STAGES];
+};
+
+// ---------------------------------------------------------------------------------------------
+// Butterflies: y_q = Σ_k x_k exp(SIGN * 2πi q k / r)
+// ---------------------------------------------------------------------------------------------
+
+template <int SIGN, typename R> __device__ __forceinline__ void dft2(cpx<R>* v) {
+  cpx<R> a = v[0], b = v[1];
+  v[0] = cadd(a, b); v[1] = csub(a, b);
+}
+
+template <int SIGN, typename R> __device__ __forceinline__ void dft3(cpx<R>* v) {
+  const R h = (R)0.86602540378443864676;  // sqrt(3)/2
+  cpx<R> t = cadd(v[1], v[2]);
+  cpx<R> d = csub(v[1], v[2]);
+  cpx<R> m = mk<R>(v[0].x - (R)0.5 * t.x, v[0].y - (R)0.5 * t.y);
+  cpx<R> s = cscale(cmuli<SIGN>(d), h);
+  v[0] = cadd(v[0], t);
+  v[1] = cadd(m, s);
+  v[2] = csub(m, s);
+}
+
+template <int SIGN, typename R> __device__ __forceinline__ void dft4(cpx<R>* v) {
+  cpx<R> a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+  cpx<R> c = cadd(v[1], v[3]), d = cmuli<SIGN>(csub(v[1], v[3]));
+  v[0] = cadd(a, c); v[2] = csub(a, c);
+  v[1] = cadd(b, d); v[3] = csub(b, d);
+}
+
+template <int SIGN, typename R> __device__ __forceinline__ void dft5(cpx<R>* v) {
+  const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
+  const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;
+  cpx<R> a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+  cpx<R> a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+  cpx<R> x0 = v[0];
+  cpx<R> p1 = mk<R>(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
+  cpx<R> p2 = mk<R>(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
+  cpx<R> q1 = cmuli<SIGN>(mk<R>(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+  cpx<R> q2 = cmuli<SIGN>(mk<R>(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+  v[0] = mk<R>(x0.x + a1.x + a2.x, x0.y + a1.y + a2.y);
+  v[1] = cadd(p1, q1); v[4] = csub(p1, q1);
+  v[2] = cadd(p2, q2); v[3] = csub(p2, q2);
+}
+
+template <int SIGN, typename R> __device__ __forceinline__ void dft8(cpx<R>* v) {
+  const R r2 = (R)0.70710678118654752440;
+  cpx<R> e[4] = {v[0], v[2], v[4], v[6]};
+  cpx<R> o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<SIGN, R>(e);
+  dft4<SIGN, R>(o);
+  // o[q] *= w8^q, w8 = exp(SIGN i pi/4)
+  o[1] = mk<R>(r2 * (o[1].x - (R)SIGN * o[1].y), r2 * (o[1].y + (R)SIGN * o[1].x));
+  o[2] = cmuli<SIGN>(o[2]);
+  o[3] = mk<R>(r2 * (-o[3].x - (R)SIGN * o[3].y), r2 * (-o[3].y + (R)SIGN * o[3].x));
+#pragma unroll
+  for (int q = 0; q < 4; ++q) { v[q] = cadd(e[q], o[q]); v[q + 4] = csub(e[q], o[q]); }
+}
+
+template <int RADIX, int SIGN, typename R> __device__ __forceinline__ void dft(cpx<R>* v) {
+  if constexpr (RADIX == 2) dft2<SIGN, R>(v);
+  else if constexpr (RADIX == 3) dft3<SIGN, R>(v);
+  else if constexpr (RADIX == 4) dft4<SIGN, R>(v);
+  else if constexpr (RADIX == 5) dft5<SIGN, R>(v);
+  else dft8<SIGN, R>(v);
+}
+
+// twiddle exp(SIGN 2πi idx / L) from the table tw[idx] = exp(-2πi idx / L)
+template <int SIGN, typename R> __device__ __forceinline__ cpx<R> twiddle(const cpx<R>* __restrict__ tw, int idx) {
+  cpx<R> w = tw[idx];
+  if (SIGN > 0) w.y = -w.y;
+  return w;
+}
+
+// One Stockham stage of radix RADIX over B interleaved lines: src -> dst.
+template <int RADIX, int SIGN, typename R>
+__device__ __forceinline__ void stockham_stage(const cpx<R>* src, cpx<R>* dst, int L, int B, int Ns,
+                                               const cpx<R>* __restrict__ tw, int tid, int nth) {
+  const int Lr = L / RADIX;
+  const int tstep = L / (Ns * RADIX);
+  const int nbf = Lr * B;
+  for (int q = tid; q < nbf; q += nth) {
+    const int l = q % B;
+    const int j = q / B;
+    const int m = j % Ns;
+    cpx<R> v[RADIX];
+#pragma unroll
+    for (int k = 0; k < RADIX; ++k) v[k] = src[(j + k * Lr) * B + l];
+    if (m > 0) {
+#pragma unroll
+      for (int k = 1; k < RADIX; ++k) v[k] = cmul(v[k], twiddle<SIGN, R>(tw, k * m * tstep));
+    }
+    dft<RADIX, SIGN, R>(v);
+    const int o = (j - m) * RADIX + m;
+#pragma unroll
+    for (int k = 0; k < RADIX; ++k) dst[(o + k * Ns) * B + l] = v[k];
+  }
+}
+
+// Full mixed-radix transform of B interleaved lines held in LDS buffer a (b = scratch of the
+// same size).  Returns the buffer holding the natural-order result.  Caller must have synced
+// after filling a; the routine ends with a barrier.
+template <int SIGN, typename R>
+__device__ cpx<R>* lds_fft(cpx<R>* a, cpx<R>* b, const FftDesc& d, int B, const cpx<R>* __restrict__ tw,
+                           int tid, int nth) {
+  int Ns = 1;
+  const int L = d.L;
+  for (int s = 0; s < d.nst; ++s) {
+    const int r = d.rad[s];
+    switch (r) {
+      case 2: stockham_stage<2, SIGN, R>(a, b, L, B, Ns, tw, tid, nth); break;
+      case 3: stockham_stage<3, SIGN, R>(a, b, L, B, Ns, tw, tid, nth); break;
+      case 4: stockham_stage<4, SIGN, R>(a, b, L, B, Ns, tw, tid, nth); break;
+      case 5: stockham_stage<5, SIGN, R>(a, b, L, B, Ns, tw, tid, nth); break;
+      default: stockham_stage<8, SIGN, R>(a, b, L, B, Ns, tw, tid, nth); break;
+    }
+    __syncthreads();
+    cpx<R>* t = a; a = b; b = t;
+    Ns *= r;
+  }
+  return a;
+}
+
+// 2/3 rule on one axis (NonlinearTerm.jl:134-141): folded |k| >= n ÷ 3
+__device__ __forceinline__ bool dealiased1(int i, int n) {
+  const int k = i <= n / 2 ? i : n - i;
+  return k >= n / 3;
+}
+
+// padded-grid index -> original-grid index (inverse of the fftshift embedding of
+// FFTWExt.jl:134-149); false when the padded bin is outside the embedded block.
+__device__ __forceinline__ bool pad_pre(int i, int n, int G, int& o) {
+  if (G == n) { o = i; return true; }
+  if (i < (n + 1) / 2) { o = i; return true; }
+  if (i >= G - n / 2) { o = n - (G - i); return true; }
+  return false;
+}
+
+}  // namespace fit

@@ -1,0 +1,855 @@
+// fit_hip.hip -- plan management, pipeline orchestration and the C ABI of libfit_hip.so.
+//
+// One plan = one grid on one GPU: device-resident twiddles, wavenumbers, shell map and
+// workspaces, one HIP stream.  Every call runs the whole hot path on the device and returns
+// FP64 shell sums (or device/host arrays) to the caller.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fit_hip.h"
+#include "fit_kernels.h"
+
+using namespace fit;
+
+namespace {
+
+thread_local std::string g_last_error;
+std::mutex g_create_mutex;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define FIT_HIP(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) {                                                                    \
+      return fail(e_ == hipErrorOutOfMemory ? FIT_E_OUT_OF_MEMORY : FIT_E_DEVICE,              \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                          \
+    }                                                                                          \
+  } while (0)
+
+// radix plan: as many 8s as possible, then 4 / 2, then 3s and 5s (any order is valid Stockham)
+bool make_desc(int L, FftDesc& d) {
+  d.L = L;
+  d.nst = 0;
+  int r = L;
+  while (r % 8 == 0 && d.nst < FIT_MAX_STAGES) { d.rad[d.nst++] = 8; r /= 8; }
+  while (r % 4 == 0 && d.nst < FIT_MAX_STAGES) { d.rad[d.nst++] = 4; r /= 4; }
+  while (r % 2 == 0 && d.nst < FIT_MAX_STAGES) { d.rad[d.nst++] = 2; r /= 2; }
+  while (r % 3 == 0 && d.nst < FIT_MAX_STAGES) { d.rad[d.nst++] = 3; r /= 3; }
+  while (r % 5 == 0 && d.nst < FIT_MAX_STAGES) { d.rad[d.nst++] = 5; r /= 5; }
+  return r == 1;
+}
+
+int padded_len(int n) {  // FFTWExt.jl:128-131
+  int m = (3 * n + 1) / 2;
+  return ((m - n) % 2 == 0) ? m : m + 1;
+}
+
+struct ProfRec {
+  hipEvent_t a = nullptr, b = nullptr;
+  int cls = 0;
+  double bytes = 0;
+};
+
+}  // namespace
+
+struct fit_plan {
+  int nd = 0, M = 0, D = 0, dtype = 0, dealias = 0, device = 0;
+  int n[3] = {1, 1, 1}, G[3] = {1, 1, 1};
+  int h1 = 1;
+  int64_t Np = 1, Gp = 1, Hp = 1;
+  int nsh = 0;
+  bool shell_symmetric = true;
+  size_t rsz = 4;  // sizeof(R)
+  hipStream_t stream = nullptr;
+  FftDesc fd[3];
+  void* tw[3] = {nullptr, nullptr, nullptr};
+  void* ks[3] = {nullptr, nullptr, nullptr};
+  int* shell = nullptr;
+  void* work = nullptr;    // one half spectrum on the transform grid
+  void* uphys = nullptr;   // nd physical advecting components
+  void* nphys = nullptr;   // one physical product
+  void* nhat = nullptr;    // M full original-grid N̂
+  void* in_f = nullptr;    // staged primary field (M comps)
+  void* in_a = nullptr;    // staged advecting field (D comps)
+  void* in_c = nullptr;    // staged carrier / extra input (M comps)
+  void* out_tmp = nullptr; // staged output
+  size_t out_tmp_bytes = 0;
+  double* partials = nullptr;
+  int nblk_density = 0;
+  double* dcol = nullptr;  // nsh doubles
+  int64_t bytes = 0;
+  // profiling
+  bool prof = false;
+  std::vector<ProfRec> recs;
+  size_t nrec = 0;
+  int64_t launches[FIT_K_COUNT] = {0};
+  double ms[FIT_K_COUNT] = {0};
+  double abytes[FIT_K_COUNT] = {0};
+};
+
+namespace {
+
+int dalloc(fit_plan* p, void** ptr, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return fail(e == hipErrorOutOfMemory ? FIT_E_OUT_OF_MEMORY : FIT_E_DEVICE,
+                "hipMalloc(" + std::to_string(bytes) + " B): " + hipGetErrorString(e));
+  }
+  p->bytes += (int64_t)bytes;
+  return FIT_OK;
+}
+
+void dfree(void*& ptr) {
+  if (ptr) hipFree(ptr);
+  ptr = nullptr;
+}
+
+// profiling brackets
+void prof_begin(fit_plan* p, int cls, double bytes) {
+  if (!p->prof) return;
+  if (p->nrec >= p->recs.size()) {
+    ProfRec r;
+    hipEventCreate(&r.a);
+    hipEventCreate(&r.b);
+    p->recs.push_back(r);
+  }
+  ProfRec& r = p->recs[p->nrec];
+  r.cls = cls;
+  r.bytes = bytes;
+  hipEventRecord(r.a, p->stream);
+}
+
+void prof_end(fit_plan* p) {
+  if (!p->prof) return;
+  hipEventRecord(p->recs[p->nrec].b, p->stream);
+  p->nrec++;
+}
+
+void prof_collect(fit_plan* p) {
+  if (!p->prof) return;
+  for (size_t i = 0; i < p->nrec; ++i) {
+    float t = 0.f;
+    hipEventElapsedTime(&t, p->recs[i].a, p->recs[i].b);
+    p->launches[p->recs[i].cls] += 1;
+    p->ms[p->recs[i].cls] += t;
+    p->abytes[p->recs[i].cls] += p->recs[i].bytes;
+  }
+  p->nrec = 0;
+}
+
+template <typename R> Grid grid_of(const fit_plan* p) {
+  Grid g;
+  g.nd = p->nd;
+  g.n1 = p->n[0]; g.n2 = p->n[1]; g.n3 = p->n[2];
+  g.G1 = p->G[0]; g.G2 = p->G[1]; g.G3 = p->G[2];
+  g.h1 = p->h1;
+  g.Np = p->Np;
+  return g;
+}
+
+// largest power-of-two tile with 2*L*T*sizeof(cpx) <= 64 KiB (ping-pong LDS), capped at cap
+int tile_for(int L, size_t csz, int cap) {
+  int t = cap;
+  while (t > 1 && (size_t)2 * L * t * csz > 65536) t /= 2;
+  return t;
+}
+
+constexpr int kThreads = 256;
+
+// ------------------------------ pass launchers ------------------------------------------------
+
+template <typename R, int SIGN, class LD, class ST>
+int launch_col(fit_plan* p, LD ld, ST st, int axis, int nfields) {
+  const int ax = axis;  // 1 = y, 2 = z
+  const int L = p->G[ax];
+  const size_t csz = sizeof(cpx<R>);
+  const int TX = tile_for(L, csz, 16);
+  const int nx = p->h1;
+  const int no = axis == 1 ? p->G[2] : p->G[1];
+  dim3 grid((nx + TX - 1) / TX, no, nfields);
+  const size_t lds = (size_t)2 * L * TX * csz;
+  prof_begin(p, FIT_K_FFT_COLUMN, 2.0 * (double)nx * no * L * csz * nfields);
+  hipLaunchKernelGGL((k_col<R, SIGN, LD, ST>), grid, dim3(kThreads), lds, p->stream, ld, st, p->fd[ax],
+                     (const cpx<R>*)p->tw[ax], nx, TX, axis);
+  prof_end(p);
+  FIT_HIP(hipGetLastError());
+  return FIT_OK;
+}
+
+template <typename R, class LD, class ST>
+int launch_row_inv(fit_plan* p, LD ld, ST st, int nfields, double extra_bytes_per_row) {
+  const int L = p->G[0];
+  const size_t csz = sizeof(cpx<R>);
+  const int P = tile_for(L, csz, 8);
+  const int nrows = p->G[1] * p->G[2];
+  const int npairs = (nrows + 1) / 2;
+  dim3 grid((npairs + P - 1) / P, 1, nfields);
+  const size_t lds = (size_t)2 * L * P * csz;
+  prof_begin(p, FIT_K_FFT_ROW_INVERSE,
+             ((double)p->h1 * csz + (double)L * sizeof(R) + extra_bytes_per_row) * nrows * nfields);
+  hipLaunchKernelGGL((k_row_inv<R, LD, ST>), grid, dim3(kThreads), lds, p->stream, ld, st, p->fd[0],
+                     (const cpx<R>*)p->tw[0], p->G[1], nrows, P);
+  prof_end(p);
+  FIT_HIP(hipGetLastError());
+  return FIT_OK;
+}
+
+template <typename R, class LD, class ST>
+int launch_row_fwd(fit_plan* p, LD ld, ST st, int nfields) {
+  const int L = p->G[0];
+  const size_t csz = sizeof(cpx<R>);
+  const int P = tile_for(L, csz, 8);
+  const int nrows = p->G[1] * p->G[2];
+  const int npairs = (nrows + 1) / 2;
+  dim3 grid((npairs + P - 1) / P, 1, nfields);
+  const size_t lds = (size_t)2 * L * P * csz;
+  prof_begin(p, FIT_K_FFT_ROW_FORWARD, ((double)p->h1 * csz + (double)L * sizeof(R)) * nrows * nfields);
+  hipLaunchKernelGGL((k_row_fwd<R, LD, ST>), grid, dim3(kThreads), lds, p->stream, ld, st, p->fd[0],
+                     (const cpx<R>*)p->tw[0], p->G[1], nrows, P, p->h1);
+  prof_end(p);
+  FIT_HIP(hipGetLastError());
+  return FIT_OK;
+}
+
+template <typename R> LdHalf<R> ld_work(fit_plan* p) {
+  return LdHalf<R>{(const cpx<R>*)p->work, (int64_t)p->h1, (int64_t)p->h1 * p->G[1], p->Hp};
+}
+template <typename R> StHalf<R> st_work(fit_plan* p) {
+  return StHalf<R>{(cpx<R>*)p->work, (int64_t)p->h1, (int64_t)p->h1 * p->G[1], p->Hp};
+}
+
+// Inverse transform of one spectral-source field into a physical row storer RS.
+template <typename R, class RS>
+int inverse_field(fit_plan* p, const LdSpectral<R>& ls, RS rs, double extra_row_bytes) {
+  int rc;
+  if (p->nd == 3) {
+    if ((rc = launch_col<R, 1>(p, ls, st_work<R>(p), 2, 1))) return rc;
+    if ((rc = launch_col<R, 1>(p, ld_work<R>(p), st_work<R>(p), 1, 1))) return rc;
+    return launch_row_inv<R>(p, ld_work<R>(p), rs, 1, extra_row_bytes);
+  }
+  if (p->nd == 2) {
+    if ((rc = launch_col<R, 1>(p, ls, st_work<R>(p), 1, 1))) return rc;
+    return launch_row_inv<R>(p, ld_work<R>(p), rs, 1, extra_row_bytes);
+  }
+  return launch_row_inv<R>(p, ls, rs, 1, extra_row_bytes);
+}
+
+// Forward transform of the physical product (nphys) into full original-grid N̂ component c.
+template <typename R>
+int forward_field(fit_plan* p, int c, double scale) {
+  Grid g = grid_of<R>(p);
+  LdReal<R> lr{(const R*)p->nphys, (int64_t)p->G[0], (int64_t)p->G[0] * p->G[1], 0};
+  StNhat<R> sn{g, (cpx<R>*)p->nhat + (int64_t)c * p->Np, 0, p->dealias == FIT_DEALIAS_TWOTHIRDS,
+               (R)scale};
+  int rc;
+  if (p->nd == 1) return launch_row_fwd<R>(p, lr, sn, 1);
+  if ((rc = launch_row_fwd<R>(p, lr, st_work<R>(p), 1))) return rc;
+  if (p->nd == 2) return launch_col<R, -1>(p, ld_work<R>(p), sn, 1, 1);
+  if ((rc = launch_col<R, -1>(p, ld_work<R>(p), st_work<R>(p), 1, 1))) return rc;
+  return launch_col<R, -1>(p, ld_work<R>(p), sn, 2, 1);
+}
+
+template <typename R> LdSpectral<R> spectral_loader(fit_plan* p, const void* src, int gradj, int giver) {
+  LdSpectral<R> ls;
+  ls.g = grid_of<R>(p);
+  ls.src = (const cpx<R>*)src;
+  ls.sf = p->Np;
+  ls.trunc = p->dealias == FIT_DEALIAS_TWOTHIRDS;
+  ls.gradj = gradj;
+  ls.gradj_per_field = 0;
+  ls.src_per_field_div = 1;
+  ls.ks0 = (const R*)p->ks[0];
+  ls.ks1 = (const R*)p->ks[1];
+  ls.ks2 = (const R*)p->ks[2];
+  ls.shell = p->shell;
+  ls.giver = giver;
+  return ls;
+}
+
+// Advecting velocity u_j on the transform grid (FFTWExt.jl:80-86 / :171): u_j = Re ifft(keep â_j)
+template <typename R>
+int compute_u(fit_plan* p, const void* adv) {
+  const double inv_mtot = 1.0 / (double)p->Gp;
+  for (int j = 0; j < p->nd; ++j) {
+    LdSpectral<R> ls = spectral_loader<R>(p, (const cpx<R>*)adv + (int64_t)j * p->Np, -1, 0);
+    StReal<R> sr{(R*)p->uphys + (int64_t)j * p->Gp, (int64_t)p->G[0], (int64_t)p->G[0] * p->G[1], 0,
+                 (R)inv_mtot};
+    int rc = inverse_field<R>(p, ls, sr, 0.0);
+    if (rc) return rc;
+  }
+  return FIT_OK;
+}
+
+// N̂ for the primary field f (optionally restricted to giver shell m) with u already computed.
+template <typename R>
+int nonlinear_from_u(fit_plan* p, const void* f, int giver) {
+  const double inv_mtot = 1.0 / (double)p->Gp;
+  const double fscale = (double)p->Gp / ((double)p->Np * (double)p->Np);
+  const int64_t sy = p->G[0], sz = (int64_t)p->G[0] * p->G[1];
+  for (int c = 0; c < p->M; ++c) {
+    for (int j = 0; j < p->nd; ++j) {
+      LdSpectral<R> ls = spectral_loader<R>(p, (const cpx<R>*)f + (int64_t)c * p->Np, j, giver);
+      StProduct<R> sp{(R*)p->nphys, (const R*)p->uphys + (int64_t)j * p->Gp, sy, sz, (R)inv_mtot, j == 0};
+      int rc = inverse_field<R>(p, ls, sp, (double)p->G[0] * sizeof(R) * (j == 0 ? 2.0 : 3.0));
+      if (rc) return rc;
+    }
+    int rc = forward_field<R>(p, c, fscale);
+    if (rc) return rc;
+  }
+  return FIT_OK;
+}
+
+// density (+ shell partial sums, reduced into p->dcol) from carrier and N̂ (both full grid)
+template <typename R>
+int density_launch(fit_plan* p, int inv, const void* car, const void* nh, void* t_out, bool shells) {
+  Grid g = grid_of<R>(p);
+  const int nsh = shells ? p->nsh : 0;
+  const int nblk = p->nblk_density;
+  const size_t lds = (size_t)std::max(nsh, 1) * sizeof(double);
+  const size_t csz = sizeof(cpx<R>);
+  prof_begin(p, FIT_K_DENSITY_SHELL,
+             (double)p->Np * (2.0 * p->M * csz + (shells ? 4.0 : 0.0) + (t_out ? sizeof(R) : 0.0)));
+  hipLaunchKernelGGL(k_density<R>, dim3(nblk), dim3(kThreads), lds, p->stream, g, inv, p->M,
+                     (const cpx<R>*)car, p->Np, (const cpx<R>*)nh, p->Np, (const R*)p->ks[0],
+                     (const R*)p->ks[1], (const R*)p->ks[2], shells ? p->shell : nullptr, nsh,
+                     p->partials, (R*)t_out);
+  prof_end(p);
+  FIT_HIP(hipGetLastError());
+  if (shells) {
+    prof_begin(p, FIT_K_REDUCE, (double)nblk * nsh * 8.0 + nsh * 8.0);
+    hipLaunchKernelGGL(k_reduce_partials, dim3((nsh + 127) / 128), dim3(128), 0, p->stream, p->partials,
+                       nblk, nsh, p->dcol);
+    prof_end(p);
+    FIT_HIP(hipGetLastError());
+  }
+  return FIT_OK;
+}
+
+// ------------------------------ input / output staging ----------------------------------------
+
+size_t csize(int dtype) { return dtype == FIT_F64 ? 16 : 8; }
+
+// Make a device array of `ncomp` full-grid complex components in compute precision available.
+// Returns the device pointer to use (user pointer if usable as is, else the staging buffer).
+int stage_in(fit_plan* p, const void* src, int ncomp, int io_dtype, int ptr_kind, void** staging,
+             const void** out) {
+  const int64_t n = (int64_t)ncomp * p->Np;
+  const size_t cb = csize(p->dtype);
+  if (ptr_kind == FIT_PTR_DEVICE && io_dtype == p->dtype) {
+    *out = src;
+    return FIT_OK;
+  }
+  if (!*staging) {
+    int rc = dalloc(p, staging, (size_t)std::max(p->M, p->D) * p->Np * cb);
+    if (rc) return rc;
+  }
+  if (io_dtype == p->dtype) {
+    FIT_HIP(hipMemcpyAsync(*staging, src, n * cb, hipMemcpyHostToDevice, p->stream));
+  } else {
+    const void* dsrc = src;
+    void* tmp = nullptr;
+    if (ptr_kind == FIT_PTR_HOST) {
+      FIT_HIP(hipMalloc(&tmp, n * csize(io_dtype)));
+      FIT_HIP(hipMemcpyAsync(tmp, src, n * csize(io_dtype), hipMemcpyHostToDevice, p->stream));
+      dsrc = tmp;
+    }
+    const int64_t nr = 2 * n;
+    if (io_dtype == FIT_F64)
+      hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(256), 0, p->stream, (const double*)dsrc,
+                         (float*)*staging, nr);
+    else
+      hipLaunchKernelGGL((k_convert<float, double>), dim3(1024), dim3(256), 0, p->stream, (const float*)dsrc,
+                         (double*)*staging, nr);
+    FIT_HIP(hipGetLastError());
+    if (tmp) {
+      FIT_HIP(hipStreamSynchronize(p->stream));
+      hipFree(tmp);
+    }
+  }
+  *out = *staging;
+  return FIT_OK;
+}
+
+// copy nvals reals (compute precision) from device src to dst (io_dtype, host or device)
+int stage_out_real(fit_plan* p, const void* dsrc, void* dst, int64_t nvals, int io_dtype, int ptr_kind) {
+  const size_t rs = p->rsz;
+  const size_t os = io_dtype == FIT_F64 ? 8 : 4;
+  if (io_dtype == p->dtype) {
+    FIT_HIP(hipMemcpyAsync(dst, dsrc, nvals * rs,
+                           ptr_kind == FIT_PTR_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
+                           p->stream));
+    return FIT_OK;
+  }
+  void* conv = dst;
+  if (ptr_kind == FIT_PTR_HOST) {
+    if (p->out_tmp_bytes < nvals * os) {
+      dfree(p->out_tmp);
+      int rc = dalloc(p, &p->out_tmp, nvals * os);
+      if (rc) return rc;
+      p->out_tmp_bytes = nvals * os;
+    }
+    conv = p->out_tmp;
+  }
+  if (io_dtype == FIT_F64)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(1024), dim3(256), 0, p->stream, (const float*)dsrc,
+                       (double*)conv, nvals);
+  else
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(256), 0, p->stream, (const double*)dsrc,
+                       (float*)conv, nvals);
+  FIT_HIP(hipGetLastError());
+  if (ptr_kind == FIT_PTR_HOST)
+    FIT_HIP(hipMemcpyAsync(dst, conv, nvals * os, hipMemcpyDeviceToHost, p->stream));
+  return FIT_OK;
+}
+
+int check_plan(fit_plan* p) {
+  if (!p) return fail(FIT_E_ARGUMENT, "null plan");
+  FIT_HIP(hipSetDevice(p->device));
+  return FIT_OK;
+}
+
+int check_invariant(fit_plan* p, int inv, int ncomp) {
+  if (inv < 0 || inv > 3) return fail(FIT_E_ARGUMENT, "unknown invariant code");
+  if (inv == FIT_HELICITY && p->nd != 3)
+    return fail(FIT_E_ARGUMENT, "Helicity transfer is defined in 3D only (got nd=" + std::to_string(p->nd) + ").");
+  if (inv == FIT_ENSTROPHY && p->nd != 2 && p->nd != 3)
+    return fail(FIT_E_ARGUMENT, "Enstrophy transfer is defined in 2D or 3D; got nd=" + std::to_string(p->nd) + ".");
+  if ((inv == FIT_HELICITY || inv == FIT_ENSTROPHY) && ncomp < p->nd)
+    return fail(FIT_E_DIMENSION, "vorticity-weighted invariants need M >= nd components");
+  return FIT_OK;
+}
+
+// stage f / adv for a nonlinear-term call
+int stage_fa(fit_plan* p, const void* f_hat, const void* adv_hat, int io_dtype, int ptr_kind, const void** f,
+             const void** a) {
+  if (!f_hat) return fail(FIT_E_ARGUMENT, "f_hat is NULL");
+  if (io_dtype != FIT_F32 && io_dtype != FIT_F64) return fail(FIT_E_ARGUMENT, "io_dtype must be FIT_F32 or FIT_F64");
+  if (ptr_kind != FIT_PTR_HOST && ptr_kind != FIT_PTR_DEVICE) return fail(FIT_E_ARGUMENT, "bad ptr_kind");
+  int rc = stage_in(p, f_hat, p->M, io_dtype, ptr_kind, &p->in_f, f);
+  if (rc) return rc;
+  if (adv_hat) {
+    rc = stage_in(p, adv_hat, p->D, io_dtype, ptr_kind, &p->in_a, a);
+    if (rc) return rc;
+  } else {
+    if (p->M < p->nd)
+      return fail(FIT_E_DIMENSION, "self-advection needs M >= nd velocity components; pass adv_hat");
+    *a = *f;
+  }
+  return FIT_OK;
+}
+
+template <typename R>
+int nonlinear_term_impl(fit_plan* p, const void* f, const void* a) {
+  int rc = compute_u<R>(p, a);
+  if (rc) return rc;
+  return nonlinear_from_u<R>(p, f, 0);
+}
+
+int finish(fit_plan* p) {
+  FIT_HIP(hipStreamSynchronize(p->stream));
+  prof_collect(p);
+  return FIT_OK;
+}
+
+int spectral_flux_tail(fit_plan* p, double* T, double* Pi) {
+  std::vector<double> h(p->nsh);
+  FIT_HIP(hipMemcpyAsync(h.data(), p->dcol, p->nsh * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  int rc = finish(p);
+  if (rc) return rc;
+  double acc = 0.0;
+  for (int s = 0; s < p->nsh; ++s) {
+    if (T) T[s] = h[s];
+    acc += h[s];
+    if (Pi) Pi[s] = acc;  // Π = +cumsum(T) (SpectralFlux.jl:172-177)
+  }
+  return FIT_OK;
+}
+
+template <typename R>
+int s2s_impl(fit_plan* p, int inv, const void* f, const void* a, std::vector<double>& Tm) {
+  int rc = compute_u<R>(p, a);
+  if (rc) return rc;
+  const int nsh = p->nsh;
+  for (int m = 1; m <= nsh; ++m) {
+    if ((rc = nonlinear_from_u<R>(p, f, m))) return rc;
+    if ((rc = density_launch<R>(p, inv, f, p->nhat, nullptr, true))) return rc;
+    FIT_HIP(hipMemcpyAsync(Tm.data() + (size_t)(m - 1) * nsh, p->dcol, nsh * sizeof(double), hipMemcpyDeviceToHost,
+                           p->stream));
+    // dcol is reused by the next giver: order the copy before the next reduction
+    FIT_HIP(hipStreamSynchronize(p->stream));
+  }
+  return FIT_OK;
+}
+
+template <typename R>
+int m2m_impl(fit_plan* p, int inv, const void* f, const void* a, void* S, void* net, int io_dtype,
+                    int ptr_kind) {
+  Grid g = grid_of<R>(p);
+  const size_t cs = 2 * sizeof(R);
+  void *A = nullptr, *E = nullptr, *Sd = nullptr, *netpart = nullptr, *netd = nullptr;
+  int rc = FIT_OK;
+  const int64_t Np = p->Np;
+  const int64_t pchunk = 256;
+  const int nchunk = (int)((Np + pchunk - 1) / pchunk);
+  const size_t os = io_dtype == FIT_F64 ? 8 : 4;
+  auto cleanup = [&]() { if (A) (void)hipFree(A); if (E) (void)hipFree(E); if (Sd && Sd != S) (void)hipFree(Sd); if (netpart) (void)hipFree(netpart); if (netd) (void)hipFree(netd); };
+  if (hipMalloc(&A, (size_t)p->nd * Np * cs) != hipSuccess || hipMalloc(&E, (size_t)p->M * Np * cs) != hipSuccess ||
+      hipMalloc(&netpart, (size_t)nchunk * Np * sizeof(double)) != hipSuccess ||
+      hipMalloc(&netd, (size_t)Np * sizeof(double)) != hipSuccess) {
+    cleanup();
+    return fail(FIT_E_OUT_OF_MEMORY, "mode-to-mode scratch allocation failed");
+  }
+  if (ptr_kind == FIT_PTR_DEVICE) Sd = S;
+  else if (hipMalloc(&Sd, (size_t)Np * Np * os) != hipSuccess) { cleanup(); return fail(FIT_E_OUT_OF_MEMORY, "S staging"); }
+  const int trunc = p->dealias == FIT_DEALIAS_TWOTHIRDS;
+  hipLaunchKernelGGL(k_m2m_prep<R>, dim3(512), dim3(256), 0, p->stream, g, trunc, inv, p->M, (const cpx<R>*)f, Np,
+                     (const cpx<R>*)a, Np, (const R*)p->ks[0], (const R*)p->ks[1], (const R*)p->ks[2], (cpx<R>*)A,
+                     (cpx<R>*)E);
+  dim3 grid((unsigned)((Np + 255) / 256), nchunk);
+  prof_begin(p, FIT_K_OTHER, (double)Np * Np * os);
+  if (io_dtype == FIT_F64)
+    hipLaunchKernelGGL((k_m2m<R, double>), grid, dim3(256), 0, p->stream, g, trunc, p->M, (const cpx<R>*)f, Np,
+                       (const cpx<R>*)A, (const cpx<R>*)E, (const R*)p->ks[0], (const R*)p->ks[1], (const R*)p->ks[2],
+                       pchunk, (double*)Sd, (double*)netpart);
+  else
+    hipLaunchKernelGGL((k_m2m<R, float>), grid, dim3(256), 0, p->stream, g, trunc, p->M, (const cpx<R>*)f, Np,
+                       (const cpx<R>*)A, (const cpx<R>*)E, (const R*)p->ks[0], (const R*)p->ks[1], (const R*)p->ks[2],
+                       pchunk, (float*)Sd, (double*)netpart);
+  prof_end(p);
+  hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((Np + 127) / 128)), dim3(128), 0, p->stream,
+                     (const double*)netpart, nchunk, (int)Np, (double*)netd);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { cleanup(); return fail(FIT_E_DEVICE, hipGetErrorString(e)); }
+  // net in io precision
+  if (io_dtype == FIT_F64) {
+    e = hipMemcpyAsync(net, netd, Np * 8, ptr_kind == FIT_PTR_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
+                       p->stream);
+  } else {
+    void* nf = nullptr;
+    if (hipMalloc(&nf, Np * 4) != hipSuccess) { cleanup(); return fail(FIT_E_OUT_OF_MEMORY, "net staging"); }
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(256), dim3(256), 0, p->stream, (const double*)netd, (float*)nf, Np);
+    e = hipMemcpyAsync(net, nf, Np * 4, ptr_kind == FIT_PTR_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
+                       p->stream);
+    (void)hipStreamSynchronize(p->stream);
+    (void)hipFree(nf);
+  }
+  if (e == hipSuccess && ptr_kind == FIT_PTR_HOST)
+    e = hipMemcpyAsync(S, Sd, (size_t)Np * Np * os, hipMemcpyDeviceToHost, p->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+  cleanup();
+  if (e != hipSuccess) return fail(FIT_E_DEVICE, hipGetErrorString(e));
+  (void)rc;
+  return FIT_OK;
+}
+
+}  // namespace
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+extern "C" {
+
+const char* fit_last_error(void) { return g_last_error.c_str(); }
+
+const char* fit_version(void) { return "fit_hip 0.1 (gfx950)"; }
+
+int fit_plan_create(fit_plan** out, int nd, const int64_t* ns, int M, int D, int compute_dtype, int dealias,
+                    const double* const* ks, const int64_t* shell_idx, int n_shells, int device) {
+  std::lock_guard<std::mutex> lock(g_create_mutex);
+  g_last_error.clear();
+  if (!out) return fail(FIT_E_ARGUMENT, "plan out-pointer is NULL");
+  *out = nullptr;
+  if (nd < 1 || nd > 3) return fail(FIT_E_ARGUMENT, "nd must be 1, 2 or 3");
+  if (!ns || !ks) return fail(FIT_E_ARGUMENT, "ns / ks is NULL");
+  if (M < 1) return fail(FIT_E_DIMENSION, "M must be >= 1");
+  if (D < nd) return fail(FIT_E_DIMENSION, "the advecting velocity needs D >= nd components");
+  if (compute_dtype != FIT_F32 && compute_dtype != FIT_F64) return fail(FIT_E_ARGUMENT, "bad compute dtype");
+  if (dealias < 0 || dealias > 2) return fail(FIT_E_ARGUMENT, "bad dealiasing code");
+  if (n_shells < 0 || (n_shells > 0 && !shell_idx)) return fail(FIT_E_ARGUMENT, "shell_idx missing");
+  if (n_shells > 8000) return fail(FIT_E_UNSUPPORTED, "at most 8000 shells per plan");
+  fit_plan* p = new fit_plan();
+  p->nd = nd; p->M = M; p->D = D; p->dtype = compute_dtype; p->dealias = dealias; p->device = device;
+  p->rsz = compute_dtype == FIT_F64 ? 8 : 4;
+  for (int d = 0; d < nd; ++d) {
+    if (ns[d] < 1 || ns[d] > (1 << 20)) { delete p; return fail(FIT_E_DIMENSION, "bad grid size"); }
+    p->n[d] = (int)ns[d];
+    p->G[d] = dealias == FIT_DEALIAS_PADDED ? padded_len((int)ns[d]) : (int)ns[d];
+  }
+  for (int d = 0; d < 3; ++d) {
+    if (!make_desc(p->G[d], p->fd[d])) {
+      int bad = p->G[d];
+      delete p;
+      return fail(FIT_E_UNSUPPORTED, "transform length " + std::to_string(bad) + " is not of the form 2^a 3^b 5^c");
+    }
+    if ((size_t)2 * p->G[d] * (compute_dtype == FIT_F64 ? 16 : 8) > 65536) {
+      int bad = p->G[d];
+      delete p;
+      return fail(FIT_E_UNSUPPORTED, "transform length " + std::to_string(bad) + " exceeds the LDS line limit");
+    }
+  }
+  p->h1 = p->G[0] / 2 + 1;
+  p->Np = (int64_t)p->n[0] * p->n[1] * p->n[2];
+  p->Gp = (int64_t)p->G[0] * p->G[1] * p->G[2];
+  p->Hp = (int64_t)p->h1 * p->G[1] * p->G[2];
+  p->nsh = n_shells;
+  int rc = FIT_OK;
+  auto bail = [&](int code) { fit_plan_destroy(p); return code; };
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) { delete p; return fail(FIT_E_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e)); }
+  e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete p; return fail(FIT_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+  const size_t rs = p->rsz, cs = 2 * rs;
+  // twiddles exp(-2πi k/L) (FP64 host arithmetic, rounded once) and wavenumbers
+  for (int d = 0; d < 3; ++d) {
+    const int L = p->G[d];
+    std::vector<double> tw(2 * (size_t)L);
+    for (int k = 0; k < L; ++k) {
+      const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)L;
+      tw[2 * k] = (double)cosl(ang);
+      tw[2 * k + 1] = (double)sinl(ang);
+    }
+    if ((rc = dalloc(p, &p->tw[d], (size_t)L * cs))) return bail(rc);
+    const int nk = d < nd ? p->n[d] : 1;
+    std::vector<double> kv(nk, 0.0);
+    if (d < nd) for (int i = 0; i < nk; ++i) kv[i] = ks[d][i];
+    if ((rc = dalloc(p, &p->ks[d], (size_t)nk * rs))) return bail(rc);
+    if (compute_dtype == FIT_F64) {
+      hipMemcpy(p->tw[d], tw.data(), (size_t)L * cs, hipMemcpyHostToDevice);
+      hipMemcpy(p->ks[d], kv.data(), (size_t)nk * rs, hipMemcpyHostToDevice);
+    } else {
+      std::vector<float> twf(2 * (size_t)L), kf(nk);
+      for (size_t i = 0; i < twf.size(); ++i) twf[i] = (float)tw[i];
+      for (int i = 0; i < nk; ++i) kf[i] = (float)kv[i];
+      hipMemcpy(p->tw[d], twf.data(), (size_t)L * cs, hipMemcpyHostToDevice);
+      hipMemcpy(p->ks[d], kf.data(), (size_t)nk * rs, hipMemcpyHostToDevice);
+    }
+  }
+  // shell map (int32 on device); symmetry under k -> -k is recorded
+  if (n_shells > 0) {
+    std::vector<int> sh((size_t)p->Np);
+    for (int64_t i = 0; i < p->Np; ++i) {
+      const int64_t s = shell_idx[i];
+      if (s < 0 || s > n_shells) { fit_plan_destroy(p); return fail(FIT_E_ARGUMENT, "shell index out of range"); }
+      sh[i] = (int)s;
+    }
+    for (int64_t o = 0; o < p->Np && p->shell_symmetric; ++o) {
+      const int ix = (int)(o % p->n[0]), iy = (int)((o / p->n[0]) % p->n[1]), iz = (int)(o / ((int64_t)p->n[0] * p->n[1]));
+      const int mx = ix ? p->n[0] - ix : 0, my = iy ? p->n[1] - iy : 0, mz = iz ? p->n[2] - iz : 0;
+      if (sh[o] != sh[mx + (int64_t)p->n[0] * (my + (int64_t)p->n[1] * mz)]) p->shell_symmetric = false;
+    }
+    if ((rc = dalloc(p, (void**)&p->shell, (size_t)p->Np * sizeof(int)))) return bail(rc);
+    hipMemcpy(p->shell, sh.data(), (size_t)p->Np * sizeof(int), hipMemcpyHostToDevice);
+  }
+  if ((rc = dalloc(p, &p->work, (size_t)p->Hp * cs))) return bail(rc);
+  if ((rc = dalloc(p, &p->uphys, (size_t)nd * p->Gp * rs))) return bail(rc);
+  if ((rc = dalloc(p, &p->nphys, (size_t)p->Gp * rs))) return bail(rc);
+  if ((rc = dalloc(p, &p->nhat, (size_t)std::max(M, 3) * p->Np * cs))) return bail(rc);
+  p->nblk_density = 1024;
+  if ((rc = dalloc(p, (void**)&p->partials, (size_t)p->nblk_density * std::max(n_shells, 1) * sizeof(double))))
+    return bail(rc);
+  if ((rc = dalloc(p, (void**)&p->dcol, (size_t)std::max(n_shells, 1) * sizeof(double)))) return bail(rc);
+  hipError_t le = hipDeviceSynchronize();
+  if (le != hipSuccess) { fit_plan_destroy(p); return fail(FIT_E_DEVICE, hipGetErrorString(le)); }
+  *out = p;
+  return FIT_OK;
+}
+
+int fit_plan_destroy(fit_plan* p) {
+  if (!p) return FIT_OK;
+  hipSetDevice(p->device);
+  if (p->stream) hipStreamSynchronize(p->stream);
+  for (int d = 0; d < 3; ++d) { dfree(p->tw[d]); dfree(p->ks[d]); }
+  void* sh = p->shell; dfree(sh);
+  dfree(p->work); dfree(p->uphys); dfree(p->nphys); dfree(p->nhat);
+  dfree(p->in_f); dfree(p->in_a); dfree(p->in_c); dfree(p->out_tmp);
+  void* pa = p->partials; dfree(pa);
+  void* dc = p->dcol; dfree(dc);
+  for (auto& r : p->recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+  if (p->stream) hipStreamDestroy(p->stream);
+  delete p;
+  return FIT_OK;
+}
+
+int64_t fit_plan_device_bytes(const fit_plan* p) { return p ? p->bytes : 0; }
+
+void* fit_plan_stream(const fit_plan* p) { return p ? (void*)p->stream : nullptr; }
+
+int fit_profile_enable(fit_plan* p, int on) {
+  if (!p) return fail(FIT_E_ARGUMENT, "null plan");
+  p->prof = on != 0;
+  p->nrec = 0;
+  for (int i = 0; i < FIT_K_COUNT; ++i) { p->launches[i] = 0; p->ms[i] = 0; p->abytes[i] = 0; }
+  return FIT_OK;
+}
+
+int fit_profile_query(const fit_plan* p, int cls, int64_t* launches, double* total_ms, double* alg_bytes) {
+  if (!p || cls < 0 || cls >= FIT_K_COUNT) return fail(FIT_E_ARGUMENT, "bad plan / kernel class");
+  if (launches) *launches = p->launches[cls];
+  if (total_ms) *total_ms = p->ms[cls];
+  if (alg_bytes) *alg_bytes = p->abytes[cls];
+  return FIT_OK;
+}
+
+int fit_nonlinear_term(fit_plan* p, const void* f_hat, const void* adv_hat, void* N_hat, int io_dtype,
+                       int ptr_kind) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!N_hat) return fail(FIT_E_ARGUMENT, "N_hat is NULL");
+  const void *f, *a;
+  if ((rc = stage_fa(p, f_hat, adv_hat, io_dtype, ptr_kind, &f, &a))) return rc;
+  rc = p->dtype == FIT_F64 ? nonlinear_term_impl<double>(p, f, a) : nonlinear_term_impl<float>(p, f, a);
+  if (rc) return rc;
+  if ((rc = stage_out_real(p, p->nhat, N_hat, 2 * (int64_t)p->M * p->Np, io_dtype, ptr_kind))) return rc;
+  return finish(p);
+}
+
+int fit_transfer_density(fit_plan* p, int inv, const void* c_hat, const void* N_hat, void* t, int io_dtype,
+                         int ptr_kind) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!c_hat || !N_hat || !t) return fail(FIT_E_ARGUMENT, "NULL array");
+  if ((rc = check_invariant(p, inv, p->M))) return rc;
+  const void *c, *n;
+  if ((rc = stage_in(p, c_hat, p->M, io_dtype, ptr_kind, &p->in_f, &c))) return rc;
+  if ((rc = stage_in(p, N_hat, p->M, io_dtype, ptr_kind, &p->in_c, &n))) return rc;
+  void* tdev = p->nphys;  // Np <= Gp reals
+  rc = p->dtype == FIT_F64 ? density_launch<double>(p, inv, c, n, tdev, false)
+                           : density_launch<float>(p, inv, c, n, tdev, false);
+  if (rc) return rc;
+  if ((rc = stage_out_real(p, tdev, t, p->Np, io_dtype, ptr_kind))) return rc;
+  return finish(p);
+}
+
+int fit_spectral_flux(fit_plan* p, int inv, const void* f_hat, const void* adv_hat, double* T, double* Pi,
+                      int io_dtype, int ptr_kind) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (p->nsh <= 0) return fail(FIT_E_ARGUMENT, "plan has no shells");
+  if ((rc = check_invariant(p, inv, p->M))) return rc;
+  const void *f, *a;
+  if ((rc = stage_fa(p, f_hat, adv_hat, io_dtype, ptr_kind, &f, &a))) return rc;
+  if (p->dtype == FIT_F64) {
+    if ((rc = nonlinear_term_impl<double>(p, f, a))) return rc;
+    if ((rc = density_launch<double>(p, inv, f, p->nhat, nullptr, true))) return rc;
+  } else {
+    if ((rc = nonlinear_term_impl<float>(p, f, a))) return rc;
+    if ((rc = density_launch<float>(p, inv, f, p->nhat, nullptr, true))) return rc;
+  }
+  return spectral_flux_tail(p, T, Pi);
+}
+
+int fit_spectral_flux_with_nhat(fit_plan* p, int inv, const void* c_hat, const void* N_hat, double* T,
+                                double* Pi, int io_dtype, int ptr_kind) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (p->nsh <= 0) return fail(FIT_E_ARGUMENT, "plan has no shells");
+  if ((rc = check_invariant(p, inv, p->M))) return rc;
+  const void *c, *n;
+  if ((rc = stage_in(p, c_hat, p->M, io_dtype, ptr_kind, &p->in_f, &c))) return rc;
+  if ((rc = stage_in(p, N_hat, p->M, io_dtype, ptr_kind, &p->in_c, &n))) return rc;
+  rc = p->dtype == FIT_F64 ? density_launch<double>(p, inv, c, n, nullptr, true)
+                           : density_launch<float>(p, inv, c, n, nullptr, true);
+  if (rc) return rc;
+  return spectral_flux_tail(p, T, Pi);
+}
+
+int fit_shell_to_shell(fit_plan* p, int inv, const void* f_hat, const void* adv_hat, double* Tmat, double* net,
+                       double* max_asym, int verify, int io_dtype, int ptr_kind) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (p->nsh <= 0) return fail(FIT_E_ARGUMENT, "plan has no shells");
+  if ((rc = check_invariant(p, inv, p->M))) return rc;
+  const void *f, *a;
+  if ((rc = stage_fa(p, f_hat, adv_hat, io_dtype, ptr_kind, &f, &a))) return rc;
+  const int nsh = p->nsh;
+  std::vector<double> Tm((size_t)nsh * nsh, 0.0);
+  rc = p->dtype == FIT_F64 ? s2s_impl<double>(p, inv, f, a, Tm) : s2s_impl<float>(p, inv, f, a, Tm);
+  if (rc) return rc;
+  if ((rc = finish(p))) return rc;
+  if (Tmat) std::memcpy(Tmat, Tm.data(), Tm.size() * sizeof(double));
+  if (net) {
+    for (int n = 0; n < nsh; ++n) {
+      double s = 0.0;
+      for (int m = 0; m < nsh; ++m) s += Tm[n + (size_t)nsh * m];
+      net[n] = s;
+    }
+  }
+  if (max_asym) {
+    if (verify) {
+      double v = 0.0;
+      for (int n = 0; n < nsh; ++n)
+        for (int m = 0; m < nsh; ++m) v = std::max(v, std::fabs(Tm[n + (size_t)nsh * m] + Tm[m + (size_t)nsh * n]));
+      *max_asym = v;
+    } else {
+      *max_asym = std::nan("");
+    }
+  }
+  return FIT_OK;
+}
+
+int fit_mode_to_mode(fit_plan* p, int inv, const void* f_hat, const void* adv_hat, void* S, void* net, int io_dtype,
+                     int ptr_kind) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!S || !net) return fail(FIT_E_ARGUMENT, "NULL output");
+  if (p->dealias == FIT_DEALIAS_PADDED)
+    return fail(FIT_E_UNSUPPORTED, "mode-to-mode supports NoDealiasing and OrszagTwoThirds");
+  if ((rc = check_invariant(p, inv, p->M))) return rc;
+  if (p->M > 3 && (inv == FIT_HELICITY || inv == FIT_ENSTROPHY)) return fail(FIT_E_DIMENSION, "M > 3");
+  const void *f, *a;
+  if ((rc = stage_fa(p, f_hat, adv_hat, io_dtype, ptr_kind, &f, &a))) return rc;
+  rc = p->dtype == FIT_F64 ? m2m_impl<double>(p, inv, f, a, S, net, io_dtype, ptr_kind)
+                           : m2m_impl<float>(p, inv, f, a, S, net, io_dtype, ptr_kind);
+  if (rc) return rc;
+  return finish(p);
+}
+
+int fit_helical_decompose(fit_plan* p, const void* u_hat, void* u_plus, void* u_minus, int io_dtype, int ptr_kind) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (p->nd != 3) return fail(FIT_E_ARGUMENT, "HelicalDecomposition is defined in 3D only (got nd=" + std::to_string(p->nd) + ").");
+  if (p->M < 3) return fail(FIT_E_ARGUMENT, "HelicalDecomposition needs >=3 velocity components");
+  if (!u_hat || !u_plus || !u_minus) return fail(FIT_E_ARGUMENT, "NULL array");
+  const void* u;
+  if ((rc = stage_in(p, u_hat, p->M, io_dtype, ptr_kind, &p->in_f, &u))) return rc;
+  const int64_t Np = p->Np;
+  const size_t cs = 2 * p->rsz;
+  // outputs into nhat (3 comps) and a second buffer (in_c)
+  if (!p->in_c) { if ((rc = dalloc(p, &p->in_c, (size_t)std::max(p->M, p->D) * Np * cs))) return rc; }
+  if (p->dtype == FIT_F64) {
+    Grid g = grid_of<double>(p);
+    hipLaunchKernelGGL(k_helical<double>, dim3(1024), dim3(256), 0, p->stream, g, (const double2*)u, Np,
+                       (const double*)p->ks[0], (const double*)p->ks[1], (const double*)p->ks[2], (double2*)p->nhat,
+                       (double2*)p->in_c, Np);
+  } else {
+    Grid g = grid_of<float>(p);
+    hipLaunchKernelGGL(k_helical<float>, dim3(1024), dim3(256), 0, p->stream, g, (const float2*)u, Np,
+                       (const float*)p->ks[0], (const float*)p->ks[1], (const float*)p->ks[2], (float2*)p->nhat,
+                       (float2*)p->in_c, Np);
+  }
+  FIT_HIP(hipGetLastError());
+  // components beyond 3 are zero in both outputs
+  if (p->M > 3) {
+    FIT_HIP(hipMemsetAsync((char*)p->nhat + 3 * Np * cs, 0, (size_t)(p->M - 3) * Np * cs, p->stream));
+    FIT_HIP(hipMemsetAsync((char*)p->in_c + 3 * Np * cs, 0, (size_t)(p->M - 3) * Np * cs, p->stream));
+  }
+  if ((rc = stage_out_real(p, p->nhat, u_plus, 2 * (int64_t)p->M * Np, io_dtype, ptr_kind))) return rc;
+  if ((rc = finish(p))) return rc;
+  if ((rc = stage_out_real(p, p->in_c, u_minus, 2 * (int64_t)p->M * Np, io_dtype, ptr_kind))) return rc;
+  return finish(p);
+}
+
+}  // extern "C"

@@ -1,0 +1,480 @@
+// fit_kernels.h -- HIP kernels of the pseudospectral transfer path (gfx950, wave64).
+//
+// Data in HBM (compute precision R = float or double):
+//   API arrays    : full complex (n1, n2, n3, C), x fastest, components planar (Julia layout)
+//   half spectra  : complex (h1, G2, G3), h1 = G1/2 + 1, on the transform grid G (= n, or the
+//                   3/2-padded grid); inverse passes run z -> y -> x(c2r), forward x(r2c) -> y -> z
+//   physical      : real (G1, G2, G3)
+// Missing dimensions (nd < 3) have extent 1.
+#pragma once
+#include "fit_fft.h"
+
+namespace fit {
+
+// ---------------------------------------------------------------------------------------------
+// Geometry shared by loaders / storers
+// ---------------------------------------------------------------------------------------------
+struct Grid {
+  int nd;
+  int n1, n2, n3;      // original grid
+  int G1, G2, G3;      // transform grid
+  int h1;              // G1/2 + 1
+  int64_t Np;          // n1 n2 n3
+  __device__ __forceinline__ bool dealiased(int ox, int oy, int oz) const {
+    if (dealiased1(ox, n1)) return true;
+    if (nd >= 2 && dealiased1(oy, n2)) return true;
+    if (nd >= 3 && dealiased1(oz, n3)) return true;
+    return false;
+  }
+  __device__ __forceinline__ bool pre(int x, int y, int z, int& ox, int& oy, int& oz) const {
+    return pad_pre(x, n1, G1, ox) && pad_pre(y, n2, G2, oy) && pad_pre(z, n3, G3, oz);
+  }
+  __device__ __forceinline__ int64_t oidx(int ox, int oy, int oz) const {
+    return (int64_t)ox + (int64_t)n1 * ((int64_t)oy + (int64_t)n2 * oz);
+  }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Loaders: value at logical (x, y, z) of field f
+// ---------------------------------------------------------------------------------------------
+
+template <typename R> struct LdHalf {
+  const cpx<R>* p; int64_t sy, sz, sf;
+  __device__ __forceinline__ cpx<R> operator()(int x, int y, int z, int f) const {
+    return p[x + y * sy + z * sz + f * sf];
+  }
+};
+
+template <typename R> struct LdReal {
+  const R* p; int64_t sy, sz, sf;
+  __device__ __forceinline__ R operator()(int x, int y, int z, int f) const {
+    return p[x + y * sy + z * sz + f * sf];
+  }
+};
+
+// Hermitian part, on the transform grid, of  mult(o) * src(o)  where o is the original-grid
+// preimage of the bin (zero outside the embedded block) and
+//   mult = keep(o) [2/3] * (i ks_j[o_j] if gradj >= 0) * 1[shell(o) == giver] (if giver > 0).
+// This is exactly what real.(ifft(...)) of the reference sees (FFTWExt.jl:82-96, 171-177, 256).
+template <typename R> struct LdSpectral {
+  Grid g;
+  const cpx<R>* src; int64_t sf;    // component f at src + f*sf (full original grid)
+  int trunc;                        // Orszag 2/3 input truncation
+  int gradj;                        // -1: no derivative; else derivative direction
+  int gradj_per_field;              // 1: gradj = f % nd  (batched gradient fields)
+  int src_per_field_div;            // component index = f / src_per_field_div
+  const R* ks0; const R* ks1; const R* ks2;
+  const int* shell; int giver;
+  __device__ __forceinline__ cpx<R> G(int x, int y, int z, int comp, int gj) const {
+    int ox, oy, oz;
+    cpx<R> z0 = mk<R>((R)0, (R)0);
+    if (!g.pre(x, y, z, ox, oy, oz)) return z0;
+    if (trunc && g.dealiased(ox, oy, oz)) return z0;
+    const int64_t o = g.oidx(ox, oy, oz);
+    if (giver > 0 && shell[o] != giver) return z0;
+    cpx<R> v = src[o + comp * sf];
+    if (gj >= 0) {
+      const R k = gj == 0 ? ks0[ox] : (gj == 1 ? ks1[oy] : ks2[oz]);
+      v = mk<R>(-k * v.y, k * v.x);
+    }
+    return v;
+  }
+  __device__ __forceinline__ cpx<R> operator()(int x, int y, int z, int f) const {
+    const int comp = f / src_per_field_div;
+    const int gj = gradj_per_field ? (f % g.nd) : gradj;
+    const cpx<R> a = G(x, y, z, comp, gj);
+    const int mx = x == 0 ? 0 : g.G1 - x;
+    const int my = y == 0 ? 0 : g.G2 - y;
+    const int mz = z == 0 ? 0 : g.G3 - z;
+    const cpx<R> b = G(mx, my, mz, comp, gj);
+    return mk<R>((R)0.5 * (a.x + b.x), (R)0.5 * (a.y - b.y));
+  }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Storers
+// ---------------------------------------------------------------------------------------------
+
+template <typename R> struct StHalf {
+  cpx<R>* p; int64_t sy, sz, sf;
+  __device__ __forceinline__ void operator()(int x, int y, int z, int f, cpx<R> v) const {
+    p[x + y * sy + z * sz + f * sf] = v;
+  }
+};
+
+template <typename R> struct StReal {
+  R* p; int64_t sy, sz, sf; R scale;
+  __device__ __forceinline__ void operator()(int x, int y, int z, int f, R v) const {
+    p[x + y * sy + z * sz + f * sf] = v * scale;
+  }
+};
+
+// acc (+)= u_j * scale * v   -- the physical-space product Σ_j u_j ∂_j f_c (FFTWExt.jl:100-109)
+template <typename R> struct StProduct {
+  R* acc; const R* u; int64_t sy, sz; R scale; int first;
+  __device__ __forceinline__ void operator()(int x, int y, int z, int f, R v) const {
+    const int64_t i = x + y * sy + z * sz;
+    const R g = u[i] * (v * scale);
+    acc[i] = first ? g : acc[i] + g;
+  }
+};
+
+// Final forward pass: transform-grid half-spectrum bin -> full original-grid N̂ (truncation of the
+// padded spectrum, FFTWExt.jl:144-149 and :180; 2/3 output mask, :117), times `scale`.
+// Every original index is produced exactly once: either from its own bin or, when its bin lies in
+// the unstored half, as the conjugate of the mirrored stored bin.
+template <typename R> struct StNhat {
+  Grid g; cpx<R>* out; int64_t sf; int trunc; R scale;
+  __device__ __forceinline__ void put(int ox, int oy, int oz, int f, cpx<R> v) const {
+    if (trunc && g.dealiased(ox, oy, oz)) v = mk<R>((R)0, (R)0);
+    out[g.oidx(ox, oy, oz) + f * sf] = cscale(v, scale);
+  }
+  __device__ __forceinline__ void operator()(int x, int y, int z, int f, cpx<R> v) const {
+    int ox, oy, oz;
+    if (g.pre(x, y, z, ox, oy, oz)) put(ox, oy, oz, f, v);
+    const int mx = x == 0 ? 0 : g.G1 - x;
+    if (mx > g.G1 / 2) {
+      const int my = y == 0 ? 0 : g.G2 - y;
+      const int mz = z == 0 ? 0 : g.G3 - z;
+      if (g.pre(mx, my, mz, ox, oy, oz)) put(ox, oy, oz, f, cconj(v));
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------------------------
+// FFT pass kernels
+// ---------------------------------------------------------------------------------------------
+
+// Strided-axis pass: each workgroup transforms TX adjacent columns (consecutive x) of length L
+// along y (axis 1, other index = z) or z (axis 2, other index = y).
+template <typename R, int SIGN, class LD, class ST>
+__global__ void __launch_bounds__(256) k_col(LD ld, ST st, FftDesc d, const cpx<R>* __restrict__ tw,
+                                             int nx, int TX, int axis) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* a = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<R>* b = a + (size_t)d.L * TX;
+  const int L = d.L;
+  const int x0 = blockIdx.x * TX;
+  const int o = blockIdx.y;
+  const int f = blockIdx.z;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  for (int e = tid; e < L * TX; e += nth) {
+    const int xl = e % TX, i = e / TX, x = x0 + xl;
+    cpx<R> v = mk<R>((R)0, (R)0);
+    if (x < nx) v = axis == 1 ? ld(x, i, o, f) : ld(x, o, i, f);
+    a[i * TX + xl] = v;
+  }
+  __syncthreads();
+  cpx<R>* res = lds_fft<SIGN, R>(a, b, d, TX, tw, tid, nth);
+  for (int e = tid; e < L * TX; e += nth) {
+    const int xl = e % TX, i = e / TX, x = x0 + xl;
+    if (x < nx) {
+      if (axis == 1) st(x, i, o, f, res[i * TX + xl]);
+      else st(x, o, i, f, res[i * TX + xl]);
+    }
+  }
+}
+
+// Contiguous-axis inverse pass (c2r): two Hermitian half rows A, B of length h1 are packed into
+// Z = A + iB (Hermitian extension to L points), one complex inverse FFT, a = Re z, b = Im z.
+// The imaginary parts of the self-conjugate bins (k = 0, L/2) are dropped, as c2r / real() do.
+template <typename R, class LD, class ST>
+__global__ void __launch_bounds__(256) k_row_inv(LD ld, ST st, FftDesc d, const cpx<R>* __restrict__ tw,
+                                                 int G2, int nrows, int P) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* a = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int L = d.L;
+  cpx<R>* b = a + (size_t)L * P;
+  const int f = blockIdx.z;
+  const int pair0 = blockIdx.x * P;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  for (int e = tid; e < L * P; e += nth) {
+    const int k = e % L, p = e / L;
+    const int r0 = 2 * (pair0 + p), r1 = r0 + 1;
+    const int kk = k <= L / 2 ? k : L - k;
+    cpx<R> A = mk<R>((R)0, (R)0), B = A;
+    if (r0 < nrows) A = ld(kk, r0 % G2, r0 / G2, f);
+    if (r1 < nrows) B = ld(kk, r1 % G2, r1 / G2, f);
+    if (k == 0 || 2 * k == L) { A.y = (R)0; B.y = (R)0; }
+    if (k > L / 2) { A.y = -A.y; B.y = -B.y; }
+    a[k * P + p] = mk<R>(A.x - B.y, A.y + B.x);
+  }
+  __syncthreads();
+  cpx<R>* res = lds_fft<1, R>(a, b, d, P, tw, tid, nth);
+  for (int e = tid; e < L * P; e += nth) {
+    const int x = e % L, p = e / L;
+    const int r0 = 2 * (pair0 + p), r1 = r0 + 1;
+    const cpx<R> z = res[x * P + p];
+    if (r0 < nrows) st(x, r0 % G2, r0 / G2, f, z.x);
+    if (r1 < nrows) st(x, r1 % G2, r1 / G2, f, z.y);
+  }
+}
+
+// Contiguous-axis forward pass (r2c) on row pairs: z = a + ib, Z = FFT(z),
+// A(k) = (Z(k) + conj Z(L-k))/2,  B(k) = (Z(k) - conj Z(L-k))/(2i),  k = 0..L/2.
+template <typename R, class LD, class ST>
+__global__ void __launch_bounds__(256) k_row_fwd(LD ld, ST st, FftDesc d, const cpx<R>* __restrict__ tw,
+                                                 int G2, int nrows, int P, int h1) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* a = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int L = d.L;
+  cpx<R>* b = a + (size_t)L * P;
+  const int f = blockIdx.z;
+  const int pair0 = blockIdx.x * P;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  for (int e = tid; e < L * P; e += nth) {
+    const int x = e % L, p = e / L;
+    const int r0 = 2 * (pair0 + p), r1 = r0 + 1;
+    R va = (R)0, vb = (R)0;
+    if (r0 < nrows) va = ld(x, r0 % G2, r0 / G2, f);
+    if (r1 < nrows) vb = ld(x, r1 % G2, r1 / G2, f);
+    a[x * P + p] = mk<R>(va, vb);
+  }
+  __syncthreads();
+  cpx<R>* res = lds_fft<-1, R>(a, b, d, P, tw, tid, nth);
+  for (int e = tid; e < h1 * P; e += nth) {
+    const int k = e % h1, p = e / h1;
+    const int r0 = 2 * (pair0 + p), r1 = r0 + 1;
+    const cpx<R> Zk = res[k * P + p];
+    const cpx<R> Zm = res[((L - k) % L) * P + p];
+    const cpx<R> A = mk<R>((R)0.5 * (Zk.x + Zm.x), (R)0.5 * (Zk.y - Zm.y));
+    const cpx<R> B = mk<R>((R)0.5 * (Zk.y + Zm.y), (R)-0.5 * (Zk.x - Zm.x));
+    if (r0 < nrows) st(k, r0 % G2, r0 / G2, f, A);
+    if (r1 < nrows) st(k, r1 % G2, r1 / G2, f, B);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Densities (Invariants.jl:40-104) written as t = Re Σ_c conj(e_c) N̂_c, on the full grid
+// ---------------------------------------------------------------------------------------------
+
+template <typename R>
+__device__ __forceinline__ R density_at(int inv, int nd, int M, const cpx<R>* car, int64_t csf,
+                                        const cpx<R>* nh, int64_t nsf, int64_t o, R kx, R ky, R kz) {
+  if (inv == 0 || inv == 3) {  // kinetic energy / passive scalar: Σ_c Re{conj(c_c) N̂_c}
+    R t = (R)0;
+    for (int c = 0; c < M; ++c) {
+      const cpx<R> u = car[o + c * csf], n = nh[o + c * nsf];
+      t += u.x * n.x + u.y * n.y;
+    }
+    return t;
+  }
+  if (inv == 1) {  // helicity: ω̂ = i k × û
+    const cpx<R> ux = car[o], uy = car[o + csf], uz = car[o + 2 * csf];
+    const cpx<R> wx = csub(cscale(uz, ky), cscale(uy, kz));
+    const cpx<R> wy = csub(cscale(ux, kz), cscale(uz, kx));
+    const cpx<R> wz = csub(cscale(uy, kx), cscale(ux, ky));
+    // ω = i w;  Re{conj(i w) N} = Re{-i conj(w) N} = Im{conj(w) N} = w.x N.y - w.y N.x
+    const cpx<R> n0 = nh[o], n1 = nh[o + nsf], n2 = nh[o + 2 * nsf];
+    return (wx.x * n0.y - wx.y * n0.x) + (wy.x * n1.y - wy.y * n1.x) + (wz.x * n2.y - wz.y * n2.x);
+  }
+  // enstrophy
+  if (nd == 2) {
+    const cpx<R> ux = car[o], uy = car[o + csf];
+    const cpx<R> nx = nh[o], ny = nh[o + nsf];
+    const cpx<R> w = csub(cscale(uy, kx), cscale(ux, ky));    // ω̂ = i w
+    const cpx<R> v = csub(cscale(ny, kx), cscale(nx, ky));    // N̂_ω = i v
+    return w.x * v.x + w.y * v.y;                              // Re{conj(iw) iv} = Re{conj(w) v}
+  }
+  const cpx<R> ux = car[o], uy = car[o + csf], uz = car[o + 2 * csf];
+  const cpx<R> n0 = nh[o], n1 = nh[o + nsf], n2 = nh[o + 2 * nsf];
+  const cpx<R> wx = csub(cscale(uz, ky), cscale(uy, kz));
+  const cpx<R> wy = csub(cscale(ux, kz), cscale(uz, kx));
+  const cpx<R> wz = csub(cscale(uy, kx), cscale(ux, ky));
+  const cpx<R> vx = csub(cscale(n2, ky), cscale(n1, kz));
+  const cpx<R> vy = csub(cscale(n0, kz), cscale(n2, kx));
+  const cpx<R> vz = csub(cscale(n1, kx), cscale(n0, ky));
+  return (wx.x * vx.x + wx.y * vx.y) + (wy.x * vy.x + wy.y * vy.y) + (wz.x * vz.x + wz.y * vz.y);
+}
+
+// density over the full original grid; optional per-mode output t_out; optional shell partial
+// sums (FP64 LDS bins, one row of nsh partials per workgroup, reduced later in fixed order).
+template <typename R>
+__global__ void __launch_bounds__(256) k_density(Grid g, int inv, int M, const cpx<R>* __restrict__ car,
+                                                 int64_t csf, const cpx<R>* __restrict__ nh, int64_t nsf,
+                                                 const R* __restrict__ ks0, const R* __restrict__ ks1,
+                                                 const R* __restrict__ ks2, const int* __restrict__ shell,
+                                                 int nsh, double* __restrict__ partials, R* __restrict__ t_out) {
+  extern __shared__ double dbins[];
+  double* bins = dbins;
+  for (int s = threadIdx.x; s < nsh; s += blockDim.x) bins[s] = 0.0;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < g.Np; o += stride) {
+    const int ix = (int)(o % g.n1);
+    const int iy = (int)((o / g.n1) % g.n2);
+    const int iz = (int)(o / ((int64_t)g.n1 * g.n2));
+    const R kx = ks0[ix];
+    const R ky = g.nd >= 2 ? ks1[iy] : (R)0;
+    const R kz = g.nd >= 3 ? ks2[iz] : (R)0;
+    const R t = density_at<R>(inv, g.nd, M, car, csf, nh, nsf, o, kx, ky, kz);
+    if (t_out) t_out[o] = t;
+    if (shell) {
+      const int s = shell[o];
+      if (s > 0) atomicAdd(&bins[s - 1], (double)t);
+    }
+  }
+  if (shell) {
+    __syncthreads();
+    for (int s = threadIdx.x; s < nsh; s += blockDim.x) partials[(int64_t)blockIdx.x * nsh + s] = bins[s];
+  }
+}
+
+// out[s] = Σ_b partials[b][s], b in increasing order (deterministic FP64 reduction)
+__global__ void k_reduce_partials(const double* __restrict__ partials, int nblk, int nsh, double* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsh) return;
+  double acc = 0.0;
+  for (int b = 0; b < nblk; ++b) acc += partials[(int64_t)b * nsh + s];
+  out[s] = acc;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Precision conversion of API arrays
+// ---------------------------------------------------------------------------------------------
+template <typename A, typename B>
+__global__ void k_convert(const A* __restrict__ in, B* __restrict__ out, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = (B)in[i];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Mode-to-mode triads.  For giver p (single-mode field entering ifft through its Hermitian part):
+//   N̂_p(k) = keep(k) keep(p) / (2 Np²) Σ_j p_j [ i f̂(p) A_j(k-p) - i conj(f̂(p)) A_j(k+p) ],
+//   A_j = Hermitian part of keep·â_j,  S(k|p) = Re Σ_c conj(e_c(k)) N̂_{p,c}(k)
+// which is what ScaleToScaleTransfer.jl:75-88 evaluates with one full FFT nonlinear term per p.
+// ---------------------------------------------------------------------------------------------
+
+// Hermitian part of keep·â_j over the full grid (nd arrays) and effective carrier e_c (M arrays)
+template <typename R>
+__global__ void k_m2m_prep(Grid g, int trunc, int inv, int M, const cpx<R>* __restrict__ f, int64_t fsf,
+                           const cpx<R>* __restrict__ adv, int64_t asf, const R* ks0, const R* ks1,
+                           const R* ks2, cpx<R>* __restrict__ Aout, cpx<R>* __restrict__ eout) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < g.Np; o += stride) {
+    const int ix = (int)(o % g.n1), iy = (int)((o / g.n1) % g.n2), iz = (int)(o / ((int64_t)g.n1 * g.n2));
+    const int mx = ix == 0 ? 0 : g.n1 - ix, my = iy == 0 ? 0 : g.n2 - iy, mz = iz == 0 ? 0 : g.n3 - iz;
+    const int64_t om = g.oidx(mx, my, mz);
+    const bool kp = !(trunc && g.dealiased(ix, iy, iz));
+    const bool km = !(trunc && g.dealiased(mx, my, mz));
+    for (int j = 0; j < g.nd; ++j) {
+      const cpx<R> a = kp ? adv[o + j * asf] : mk<R>((R)0, (R)0);
+      const cpx<R> b = km ? adv[om + j * asf] : mk<R>((R)0, (R)0);
+      Aout[o + j * g.Np] = mk<R>((R)0.5 * (a.x + b.x), (R)0.5 * (a.y - b.y));
+    }
+    const R kx = ks0[ix], ky = g.nd >= 2 ? ks1[iy] : (R)0, kz = g.nd >= 3 ? ks2[iz] : (R)0;
+    if (inv == 0 || inv == 3) {
+      for (int c = 0; c < M; ++c) eout[o + c * g.Np] = f[o + c * fsf];
+    } else if (inv == 1 || g.nd == 3) {
+      const cpx<R> ux = f[o], uy = f[o + fsf], uz = f[o + 2 * fsf];
+      cpx<R> wx = csub(cscale(uz, ky), cscale(uy, kz));
+      cpx<R> wy = csub(cscale(ux, kz), cscale(uz, kx));
+      cpx<R> wz = csub(cscale(uy, kx), cscale(ux, ky));
+      wx = cmuli<1>(wx); wy = cmuli<1>(wy); wz = cmuli<1>(wz);                 // ω = i k × u
+      if (inv == 1) {
+        eout[o] = wx; eout[o + g.Np] = wy; eout[o + 2 * g.Np] = wz;
+      } else {                                                                  // e = i k × ω
+        const cpx<R> vx = csub(cscale(wz, ky), cscale(wy, kz));
+        const cpx<R> vy = csub(cscale(wx, kz), cscale(wz, kx));
+        const cpx<R> vz = csub(cscale(wy, kx), cscale(wx, ky));
+        eout[o] = cmuli<1>(vx); eout[o + g.Np] = cmuli<1>(vy); eout[o + 2 * g.Np] = cmuli<1>(vz);
+      }
+    } else {                                                                    // 2D enstrophy
+      const cpx<R> ux = f[o], uy = f[o + fsf];
+      const cpx<R> w = cmuli<1>(csub(cscale(uy, kx), cscale(ux, ky)));        // ω̂ = i(kx uy - ky ux)
+      eout[o] = cmuli<1>(cscale(w, ky));                                        // e_x =  i ky ω̂
+      eout[o + g.Np] = cmuli<-1>(cscale(w, kx));                                // e_y = -i kx ω̂
+    }
+  }
+}
+
+// One thread per receiver k, looping over a chunk of givers p.  S written column-major
+// S[k + Np p] in OUT precision; FP64 partial net sums per chunk.
+template <typename R, typename OUT>
+__global__ void __launch_bounds__(256) k_m2m(Grid g, int trunc, int M, const cpx<R>* __restrict__ f, int64_t fsf,
+                                             const cpx<R>* __restrict__ A, const cpx<R>* __restrict__ e,
+                                             const R* __restrict__ ks0, const R* __restrict__ ks1,
+                                             const R* __restrict__ ks2, int64_t pchunk, OUT* __restrict__ S,
+                                             double* __restrict__ netpart) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t p0 = (int64_t)blockIdx.y * pchunk;
+  const int64_t p1 = p0 + pchunk < g.Np ? p0 + pchunk : g.Np;
+  if (k >= g.Np) return;
+  const int kx = (int)(k % g.n1), ky = (int)((k / g.n1) % g.n2), kz = (int)(k / ((int64_t)g.n1 * g.n2));
+  const bool keepk = !(trunc && g.dealiased(kx, ky, kz));
+  cpx<R> ek[3];
+  for (int c = 0; c < M && c < 3; ++c) ek[c] = e[k + c * g.Np];
+  const R inv2 = (R)(0.5 / ((double)g.Np * (double)g.Np));
+  double net = 0.0;
+  for (int64_t p = p0; p < p1; ++p) {
+    const int px = (int)(p % g.n1), py = (int)((p / g.n1) % g.n2), pz = (int)(p / ((int64_t)g.n1 * g.n2));
+    R s = (R)0;
+    if (keepk && !(trunc && g.dealiased(px, py, pz))) {
+      const int ax = kx - px < 0 ? kx - px + g.n1 : kx - px;
+      const int ay = ky - py < 0 ? ky - py + g.n2 : ky - py;
+      const int az = kz - pz < 0 ? kz - pz + g.n3 : kz - pz;
+      const int bx = kx + px >= g.n1 ? kx + px - g.n1 : kx + px;
+      const int by = ky + py >= g.n2 ? ky + py - g.n2 : ky + py;
+      const int bz = kz + pz >= g.n3 ? kz + pz - g.n3 : kz + pz;
+      const int64_t om = g.oidx(ax, ay, az), op = g.oidx(bx, by, bz);
+      const R pj[3] = {ks0[px], g.nd >= 2 ? ks1[py] : (R)0, g.nd >= 3 ? ks2[pz] : (R)0};
+      cpx<R> dm = mk<R>((R)0, (R)0), dp = dm;
+      for (int j = 0; j < g.nd; ++j) {
+        dm = cadd(dm, cscale(A[om + j * g.Np], pj[j]));
+        dp = cadd(dp, cscale(A[op + j * g.Np], pj[j]));
+      }
+      for (int c = 0; c < M; ++c) {
+        const cpx<R> fp = f[p + c * fsf];
+        // i dm fp - i dp conj(fp)
+        const cpx<R> n = csub(cmuli<1>(cmul(dm, fp)), cmuli<1>(cmul(dp, cconj(fp))));
+        const cpx<R> ec = M <= 3 ? ek[c] : e[k + c * g.Np];
+        s += ec.x * n.x + ec.y * n.y;
+      }
+      s *= inv2;
+    }
+    S[k + g.Np * p] = (OUT)s;
+    net += (double)s;
+  }
+  netpart[(int64_t)blockIdx.y * g.Np + k] = net;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Helical split (Decomposition.jl:63-98)
+// ---------------------------------------------------------------------------------------------
+template <typename R>
+__global__ void k_helical(Grid g, const cpx<R>* __restrict__ u, int64_t usf, const R* ks0, const R* ks1,
+                          const R* ks2, cpx<R>* __restrict__ up, cpx<R>* __restrict__ um, int64_t osf) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const R is2 = (R)1 / sqrt((R)2);
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < g.Np; o += stride) {
+    const int ix = (int)(o % g.n1), iy = (int)((o / g.n1) % g.n2), iz = (int)(o / ((int64_t)g.n1 * g.n2));
+    const R kx = ks0[ix], ky = ks1[iy], kz = ks2[iz];
+    const R kk = sqrt(kx * kx + ky * ky + kz * kz);
+    cpx<R> zero = mk<R>((R)0, (R)0);
+    if (kk == (R)0) {
+      for (int c = 0; c < 3; ++c) { up[o + c * osf] = zero; um[o + c * osf] = zero; }
+      continue;
+    }
+    const R hx = kx / kk, hy = ky / kk, hz = kz / kk;
+    const bool usez = fabs(hz) < (R)0.9;
+    const R rx = usez ? (R)0 : (R)1, ry = (R)0, rz = usez ? (R)1 : (R)0;
+    R e1x = hy * rz - hz * ry, e1y = hz * rx - hx * rz, e1z = hx * ry - hy * rx;
+    const R n1 = sqrt(e1x * e1x + e1y * e1y + e1z * e1z);
+    e1x /= n1; e1y /= n1; e1z /= n1;
+    const R e2x = hy * e1z - hz * e1y, e2y = hz * e1x - hx * e1z, e2z = hx * e1y - hy * e1x;
+    const cpx<R> u1 = u[o], u2 = u[o + usf], u3 = u[o + 2 * usf];
+    const cpx<R> ue1 = cadd(cadd(cscale(u1, e1x), cscale(u2, e1y)), cscale(u3, e1z));
+    const cpx<R> ue2 = cadd(cadd(cscale(u1, e2x), cscale(u2, e2y)), cscale(u3, e2z));
+    const cpx<R> upc = cscale(csub(ue1, cmuli<1>(ue2)), is2);
+    const cpx<R> umc = cscale(cadd(ue1, cmuli<1>(ue2)), is2);
+    const R e1[3] = {e1x, e1y, e1z}, e2[3] = {e2x, e2y, e2z};
+    for (int c = 0; c < 3; ++c) {
+      const cpx<R> hp = mk<R>(e1[c] * is2, e2[c] * is2);
+      const cpx<R> hm = mk<R>(e1[c] * is2, -e2[c] * is2);
+      up[o + c * osf] = cmul(upc, hp);
+      um[o + c * osf] = cmul(umc, hm);
+    }
+  }
+}
+
+}  // namespace fit

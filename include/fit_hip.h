@@ -1,0 +1,156 @@
+/*
+ * fit_hip.h -- C ABI of libfit_hip.so, the MI355X (gfx950) backend for FlowInvariantTransfer.jl's
+ * pseudospectral transfer path.
+ *
+ * Every entry point replaces one override point of the reference (file:line in the reference
+ * repository).  Arrays follow the reference's Julia layout: column-major (first index fastest),
+ * shape (ns..., C) with the component axis last, complex values interleaved (re, im).
+ *   - complex inputs/outputs: FIT_F32 -> ComplexF32 (8 B), FIT_F64 -> ComplexF64 (16 B)
+ *   - ks: nd arrays of ns[d] doubles in FFTW index order (wavenumber_grid, Utils.jl:51-63)
+ *   - shell_idx: Np Int64 values, 1-based shell number, 0 = outside every shell
+ *     (assign_shells, ShellBinning.jl:126-140)
+ *   - T, Pi, Tmat, net are host arrays of doubles; Tmat is column-major n_sh x n_sh,
+ *     Tmat[n + n_sh*m] = transfer INTO shell n FROM giver shell m.
+ * Return value: 0 on success, otherwise a FIT_E* code; fit_last_error() gives the message
+ * (thread-local).  A plan is not safe for concurrent calls; use one plan per host thread.
+ */
+#ifndef FIT_HIP_H
+#define FIT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fit_plan fit_plan;
+
+/* compute / io precision */
+enum { FIT_F32 = 0, FIT_F64 = 1 };
+/* dealiasing: NoDealiasing, OrszagTwoThirds, PaddedThreeHalves (Types.jl:451-476) */
+enum { FIT_DEALIAS_NONE = 0, FIT_DEALIAS_TWOTHIRDS = 1, FIT_DEALIAS_PADDED = 2 };
+/* invariants (Types.jl:69-132) */
+enum { FIT_KINETIC_ENERGY = 0, FIT_HELICITY = 1, FIT_ENSTROPHY = 2, FIT_PASSIVE_SCALAR = 3 };
+/* where a data pointer lives */
+enum { FIT_PTR_HOST = 0, FIT_PTR_DEVICE = 1 };
+
+/* error codes: ARGUMENT ~ Julia ArgumentError, DIMENSION ~ DimensionMismatch */
+enum {
+  FIT_OK = 0,
+  FIT_E_ARGUMENT = 1,
+  FIT_E_DIMENSION = 2,
+  FIT_E_UNSUPPORTED = 3,
+  FIT_E_OUT_OF_MEMORY = 4,
+  FIT_E_DEVICE = 5
+};
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* fit_last_error(void);
+
+/* Library version string. */
+const char* fit_version(void);
+
+/*
+ * Build a plan: device-resident workspaces, twiddle tables, wavenumbers and shell map for one
+ * grid.  Replaces Workspaces._make_fft_plans (Workspaces.jl:22, FFTWExt.jl:37-49) together with
+ * the NonlinearTermWorkspace / SpectralFluxWorkspace / ShellToShellWorkspace constructors
+ * (Workspaces.jl:63-75, 104-117, 154-170).
+ *   nd in 1..3; ns[d] of the form 2^a 3^b 5^c; M = advected components; D = advecting velocity
+ *   components (>= nd); shell_idx may be NULL when n_shells == 0 (nonlinear term only).
+ */
+int fit_plan_create(fit_plan** plan, int nd, const int64_t* ns, int M, int D, int compute_dtype,
+                    int dealias, const double* const* ks, const int64_t* shell_idx, int n_shells,
+                    int device);
+
+int fit_plan_destroy(fit_plan* plan);
+
+/* Device bytes the plan holds. */
+int64_t fit_plan_device_bytes(const fit_plan* plan);
+
+/*
+ * N̂ = keep·FFT[(u_adv·∇) f]/Np into N_hat (ns..., M).  Replaces
+ * NonlinearTerm._compute_nonlinear_term! for the FFT path (NonlinearTerm.jl:102-116) i.e.
+ * FFTWExt._nonlinear_term_fft! (FFTWExt.jl:61-121) and _nonlinear_term_padded_fft! (:160-183).
+ *   f_hat (ns..., M); adv_hat (ns..., D) or NULL for self-advection (M == D required).
+ */
+int fit_nonlinear_term(fit_plan* plan, const void* f_hat, const void* adv_hat, void* N_hat,
+                       int io_dtype, int ptr_kind);
+
+/*
+ * Per-mode density t(k) (ns...) in io_dtype real.  Replaces Invariants.transfer_density!
+ * (Invariants.jl:55-104).  c_hat: carrier (ns..., M); N_hat: (ns..., M).
+ */
+int fit_transfer_density(fit_plan* plan, int invariant, const void* c_hat, const void* N_hat,
+                         void* t, int io_dtype, int ptr_kind);
+
+/*
+ * Spectral transfer T(k_n) and flux Pi(K_n) = +cumsum(T) over the plan's n_shells shells.
+ * Replaces SpectralFlux.calculate_spectral_flux! (SpectralFlux.jl:131-180): nonlinear term,
+ * density and shell scatter-sum all on the device.  T and Pi: host arrays of n_shells doubles.
+ */
+int fit_spectral_flux(fit_plan* plan, int invariant, const void* f_hat, const void* adv_hat,
+                      double* T, double* Pi, int io_dtype, int ptr_kind);
+
+/*
+ * T and Pi from a given carrier and N̂ (both (ns..., M)): the density + shell sum + cumsum tail
+ * of the flux, SpectralFlux._calculate_spectral_flux_with_N̂! (SpectralFlux.jl:148-180).  Used for
+ * decomposed carriers (helical components, SpectralFlux.jl:96-122) and partial fluxes (:240-296).
+ */
+int fit_spectral_flux_with_nhat(fit_plan* plan, int invariant, const void* c_hat, const void* N_hat,
+                                double* T, double* Pi, int io_dtype, int ptr_kind);
+
+/*
+ * Shell-to-shell matrix T(n,m) over the plan's shells (every shell is a giver).  Replaces
+ * ShellToShellTransfer._calculate_shell_to_shell! (ShellToShellTransfer.jl:152-159), i.e. the
+ * FFT giver loop FFTWExt._shell_to_shell_fft! (FFTWExt.jl:209-295) and the KernelAbstractions
+ * GPUBackend loop (KernelAbstractionsExt.jl:68-138).  Tmat: n_sh*n_sh doubles (column-major),
+ * net: n_sh doubles (Σ_m T(n,m)), *max_asym = max|T+Tᵀ| or NaN when verify == 0.
+ */
+int fit_shell_to_shell(fit_plan* plan, int invariant, const void* f_hat, const void* adv_hat,
+                       double* Tmat, double* net, double* max_asym, int verify, int io_dtype,
+                       int ptr_kind);
+
+/*
+ * Mode-resolved triad transfer S(k|p) (ns..., ns...) in io_dtype real (receiver k first) and
+ * net(k) = Σ_p S(k|p) (ns...) in io_dtype real.  Replaces calculate_mode_to_mode_transfer's giver
+ * loop (ScaleToScaleTransfer.jl:75-88).  S and net must be DEVICE pointers when Np^2 values do
+ * not fit host memory; ptr_kind applies to all four pointers.  Dealias NONE or TWOTHIRDS.
+ */
+int fit_mode_to_mode(fit_plan* plan, int invariant, const void* f_hat, const void* adv_hat,
+                     void* S, void* net, int io_dtype, int ptr_kind);
+
+/*
+ * Helical (Craya-Herring) split of a 3D velocity into u+ and u- (each (ns..., 3)).
+ * Replaces Decomposition._decompose_field_spectral(::HelicalDecomposition) (Decomposition.jl:63-98).
+ */
+int fit_helical_decompose(fit_plan* plan, const void* u_hat, void* u_plus, void* u_minus,
+                          int io_dtype, int ptr_kind);
+
+/* ---- instrumentation (no reference counterpart: measurement hooks for the benchmark) ---- */
+
+/* Enable (1) / disable (0) per-kernel HIP-event timing on the plan's stream; resets counters. */
+int fit_profile_enable(fit_plan* plan, int on);
+
+/* Kernel classes timed by the profiler. */
+enum {
+  FIT_K_FFT_COLUMN = 0,      /* strided-axis FFT pass */
+  FIT_K_FFT_ROW_INVERSE = 1, /* contiguous-axis c2r pass (+ fused product) */
+  FIT_K_FFT_ROW_FORWARD = 2, /* contiguous-axis r2c pass */
+  FIT_K_DENSITY_SHELL = 3,   /* density + shell partial sums */
+  FIT_K_REDUCE = 4,          /* fixed-order FP64 partial reduction */
+  FIT_K_OTHER = 5,
+  FIT_K_COUNT = 6
+};
+
+/* Launch count, summed kernel time (ms) and summed algorithmic HBM bytes for one class. */
+int fit_profile_query(const fit_plan* plan, int kernel_class, int64_t* launches, double* total_ms,
+                      double* alg_bytes);
+
+/* The HIP stream the plan launches on (as void*: hipStream_t). */
+void* fit_plan_stream(const fit_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FIT_HIP_H */

@@ -1,0 +1,261 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle on the same seeded inputs.
+
+Tolerances (normwise, max|x - x_ref| / max|x_ref|):
+  FP64 compute on the device : 1e-11  (roundoff of a different FFT algorithm)
+  FP32 compute on the device : 2e-5 per-mode quantities, 1e-5 shell sums
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-11
+TOL32 = 2e-5
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-300))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rand_c(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def ks_of(oracle, ns):
+    return oracle.wavenumber_grid(ns, (2 * math.pi,) * len(ns))
+
+
+CASES_NL = [((8,), "none"), ((12,), "23"), ((16,), "pad"), ((16, 16), "23"), ((12, 10), "none"),
+            ((8, 12), "pad"), ((8, 8, 8), "23"), ((6, 8, 10), "none"), ((8, 6, 4), "pad"),
+            ((16, 12, 8), "23"), ((6, 6, 6), "pad")]
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("ns,dea", CASES_NL)
+def test_nonlinear_term_matches_oracle(fit, oracle, ns, dea, prec):
+    say("case nonlinear", ns, dea, prec)
+    rng = np.random.default_rng(hash((ns, dea)) % 2 ** 32)
+    nd = len(ns)
+    ks = ks_of(oracle, ns)
+    D = max(nd, 2)
+    u = rand_c(rng, ns + (D,))
+    th = rand_c(rng, ns + (1,))
+    deal = {"none": fit.NoDealiasing(), "23": fit.OrszagTwoThirds(), "pad": fit.PaddedThreeHalves()}[dea]
+    dt = np.complex128 if prec == "f64" else np.complex64
+    tol = TOL64 if prec == "f64" else TOL32
+    ref = oracle.compute_nonlinear_term(u, ks, dea)
+    got = fit.compute_nonlinear_term(u.astype(dt), ks, dealiasing=deal)
+    e = rel(got, ref)
+    say("  self-advection err", e)
+    assert e < tol
+    ref = oracle.compute_nonlinear_term(th, ks, dea, advecting_hat=u)
+    got = fit.compute_nonlinear_term(th.astype(dt), ks, dealiasing=deal, advecting_hat=u.astype(dt))
+    e = rel(got, ref)
+    say("  scalar err", e)
+    assert e < tol
+
+
+@pytest.mark.parametrize("ns,inv", [((8, 6, 10), "ke"), ((8, 6, 10), "helicity"), ((8, 6, 10), "enstrophy"),
+                                    ((12, 10), "enstrophy"), ((12, 10), "ke"), ((16,), "ke")])
+def test_transfer_density_matches_oracle(fit, oracle, ns, inv):
+    say("case density", ns, inv)
+    rng = np.random.default_rng(7)
+    nd = len(ns)
+    ks = ks_of(oracle, ns)
+    u = rand_c(rng, ns + (nd,))
+    N = rand_c(rng, ns + (nd,))
+    I = {"ke": fit.KineticEnergy(), "helicity": fit.Helicity(), "enstrophy": fit.Enstrophy()}[inv]
+    got = fit.transfer_density(I, u, N, ks)
+    assert rel(got, oracle.transfer_density(inv, u, N, ks)) < TOL64
+
+
+FLUX_CASES = [
+    ((16, 16), "23", "ke", "lin", None),
+    ((16, 16), "23", "enstrophy", "lin", None),
+    ((24, 24), "pad", "ke", "lin", None),
+    ((12, 12, 12), "23", "ke", "lin", None),
+    ((12, 12, 12), "23", "helicity", "log", None),
+    ((12, 12, 12), "none", "enstrophy", "lin", None),
+    ((16, 16, 16), "23", "ke", "lin", "perp"),
+    ((16, 16, 16), "pad", "helicity", "lin", "par"),
+    ((8,), "none", "ke", "lin", None),
+]
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("ns,dea,inv,binn,geom", FLUX_CASES)
+def test_spectral_flux_matches_oracle(fit, oracle, ns, dea, inv, binn, geom, prec):
+    say("case flux", ns, dea, inv, binn, geom, prec)
+    nd = len(ns)
+    ks = ks_of(oracle, ns)
+    if nd == 1:
+        x = np.arange(ns[0]) * 2 * math.pi / ns[0]
+        u = (np.fft.fft(np.cos(x) + 0.3 * np.sin(2 * x) + 0.1 * np.cos(3 * x)) / ns[0]).reshape(ns[0], 1)
+    elif nd == 2:
+        u, _ = oracle.streamfunction_velocity_2d(ns[0], 11)
+    else:
+        u = oracle.synthetic_field(ns, 3, seed=5)
+    b_f = {"lin": fit.LinearBinning(1.0), "log": fit.LogarithmicBinning(1.0, 1.5)}[binn]
+    b_o = {"lin": oracle.LinearBinning(1.0), "log": oracle.LogarithmicBinning(1.0, 1.5)}[binn]
+    g_f = {None: None, "perp": fit.PerpendicularShells(), "par": fit.ParallelShells()}[geom]
+    g_o = {None: None, "perp": oracle.PerpendicularShells(), "par": oracle.ParallelShells()}[geom]
+    I = {"ke": fit.KineticEnergy(), "helicity": fit.Helicity(), "enstrophy": fit.Enstrophy()}[inv]
+    deal = {"none": fit.NoDealiasing(), "23": fit.OrszagTwoThirds(), "pad": fit.PaddedThreeHalves()}[dea]
+    dt = np.complex128 if prec == "f64" else np.complex64
+    c, T, Pi = oracle.calculate_spectral_flux(u, ks, b_o, dea, inv, geometry=g_o)
+    r = fit.calculate_spectral_flux(u.astype(dt), ks, binning=b_f, dealiasing=deal, invariant=I,
+                                    geometry=g_f)
+    eT, eP = rel(r.transfer_spectrum, T), rel(r.flux, Pi)
+    say("  T err", eT, "Pi err", eP)
+    tol = TOL64 if prec == "f64" else 1e-5
+    assert np.allclose(r.k_shells, c)
+    assert eT < tol and eP < tol
+    assert r.transfer_spectrum.dtype == (np.float64 if prec == "f64" else np.float32)
+
+
+def test_scalar_flux_matches_oracle(fit, oracle):
+    say("case scalar flux")
+    rng = np.random.default_rng(32)
+    u, ks = oracle.streamfunction_velocity_2d(16, 32)
+    th = np.fft.fft2(rng.standard_normal((16, 16))) / 256
+    _, T, Pi = oracle.calculate_scalar_flux(u, th, ks, oracle.LinearBinning(1.0), "23")
+    r = fit.calculate_scalar_flux(u, th, ks, binning=fit.LinearBinning(1.0))
+    assert rel(r.transfer_spectrum, T) < TOL64 and rel(r.flux, Pi) < TOL64
+    sT = math.sqrt(np.sum(r.transfer_spectrum ** 2))
+    assert abs(r.flux[-1]) < 1e-9 * sT                                   # runtests.jl:347-368
+
+
+S2S_CASES = [((16, 16), "23", "ke"), ((12, 12), "none", "ke"), ((12, 12), "pad", "ke"),
+             ((8, 8, 8), "23", "ke"), ((8, 8, 8), "23", "helicity"), ((16, 16), "23", "enstrophy")]
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("ns,dea,inv", S2S_CASES)
+def test_shell_to_shell_matches_oracle(fit, oracle, ns, dea, inv, prec):
+    say("case s2s", ns, dea, inv, prec)
+    nd = len(ns)
+    ks = ks_of(oracle, ns)
+    if nd == 2:
+        u, _ = oracle.streamfunction_velocity_2d(ns[0], 13)
+    else:
+        u = oracle.synthetic_field(ns, 3, seed=8)
+    I = {"ke": fit.KineticEnergy(), "helicity": fit.Helicity(), "enstrophy": fit.Enstrophy()}[inv]
+    deal = {"none": fit.NoDealiasing(), "23": fit.OrszagTwoThirds(), "pad": fit.PaddedThreeHalves()}[dea]
+    dt = np.complex128 if prec == "f64" else np.complex64
+    c, e, T, net, asym = oracle.calculate_shell_to_shell_transfer(u, ks, oracle.LinearBinning(1.0), dea,
+                                                                   invariant=inv)
+    r = fit.calculate_shell_to_shell_transfer(u.astype(dt), ks, binning=fit.LinearBinning(1.0),
+                                              dealiasing=deal, invariant=I)
+    eT = rel(r.transfer_matrix, T)
+    say("  T(n,m) err", eT, "asym", r.max_antisymmetry_error)
+    tol = TOL64 if prec == "f64" else TOL32
+    assert eT < tol and rel(r.net_transfer, net) < tol
+    assert np.allclose(r.shell_edges, e)
+
+
+def test_scalar_shell_to_shell(fit, oracle):
+    say("case scalar s2s")
+    rng = np.random.default_rng(14)
+    u, ks = oracle.streamfunction_velocity_2d(16, 14)
+    th = np.fft.fft2(rng.standard_normal((16, 16))) / 256
+    _, _, T, net, _ = oracle.calculate_scalar_shell_to_shell_transfer(u, th, ks,
+                                                                       binning=oracle.LinearBinning(1.0))
+    r = fit.calculate_scalar_shell_to_shell_transfer(u, th, ks, binning=fit.LinearBinning(1.0))
+    nrm = math.sqrt(np.sum(T ** 2))
+    assert rel(r.transfer_matrix, T) < TOL64
+    assert r.max_antisymmetry_error < 1e-10 * nrm                         # runtests.jl:441-464
+
+
+M2M_CASES = [((8, 8), "ke", "23"), ((6, 6), "ke", "none"), ((8, 8), "enstrophy", "23"),
+             ((4, 4, 4), "ke", "none"), ((4, 4, 4), "helicity", "none"), ((4, 4, 4), "enstrophy", "none"),
+             ((8, 8), "scalar", "23")]
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("ns,inv,dea", M2M_CASES)
+def test_mode_to_mode_matches_oracle(fit, oracle, ns, inv, dea, prec):
+    say("case m2m", ns, inv, dea, prec)
+    rng = np.random.default_rng(3)
+    nd = len(ns)
+    ks = ks_of(oracle, ns)
+    if nd == 2:
+        u, _ = oracle.streamfunction_velocity_2d(ns[0], 21)
+    else:
+        u = oracle.synthetic_field(ns, 3, seed=2)
+    dt = np.complex128 if prec == "f64" else np.complex64
+    tol = TOL64 if prec == "f64" else TOL32
+    if inv == "scalar":
+        th = rand_c(rng, ns + (1,))
+        net_o, S_o = oracle.calculate_mode_to_mode_transfer(th, ks, "scalar", dea, force=True,
+                                                            advecting_hat=u)
+        r = fit.calculate_scalar_mode_to_mode_transfer(u.astype(dt), th.astype(dt), ks,
+                                                       dealiasing=fit.OrszagTwoThirds())
+    else:
+        prim = u + 0.2 * rand_c(rng, u.shape)        # non-Hermitian primary, Hermitian advecting
+        I = {"ke": fit.KineticEnergy(), "helicity": fit.Helicity(), "enstrophy": fit.Enstrophy()}[inv]
+        deal = {"none": fit.NoDealiasing(), "23": fit.OrszagTwoThirds()}[dea]
+        net_o, S_o = oracle.calculate_mode_to_mode_transfer(prim, ks, inv, dea, force=True,
+                                                            advecting_hat=u)
+        r = fit.calculate_mode_to_mode_transfer(prim.astype(dt), ks, invariant=I, dealiasing=deal,
+                                                advecting_hat=u.astype(dt))
+    eS, eN = rel(r.transfer, S_o), rel(r.net_transfer, net_o)
+    say("  S err", eS, "net err", eN)
+    assert r.transfer.shape == ns + ns
+    assert eS < tol and eN < tol
+
+
+def test_helical_decomposition_and_partial_fluxes(fit, oracle):
+    say("case helical")
+    u, ks = oracle.curl_velocity_3d(16, 91)
+    dec_o = oracle.helical_decompose(u, ks)
+    dec = fit.decompose_field(fit.HelicalDecomposition(), u, ks)
+    assert rel(dec["positive"], dec_o["positive"]) < TOL64
+    assert rel(dec["negative"], dec_o["negative"]) < TOL64
+    b = fit.LinearBinning(1.0)
+    ch_o, tot_o, _ = oracle.calculate_partial_fluxes(u, ks, oracle.LinearBinning(1.0), "23")
+    hp = fit.calculate_helical_partial_fluxes(u, ks, binning=b)
+    assert len(hp["channels"]) == 8
+    scale = max(math.sqrt(np.sum(v[1] ** 2)) for v in ch_o.values())
+    for key, (c, T, P) in ch_o.items():
+        assert np.max(np.abs(hp["channels"][key].transfer_spectrum - T)) < 1e-10 * scale
+    hel = fit.calculate_spectral_flux(u, ks, binning=b, decomposition=fit.HelicalDecomposition())
+    hel_o = oracle.calculate_spectral_flux(u, ks, oracle.LinearBinning(1.0), "23",
+                                           decomposition="HelicalDecomposition")
+    for name in ("positive", "negative"):
+        assert rel(hel[name].transfer_spectrum, hel_o[name][1]) < TOL64
+
+
+def test_zero_field_gives_zero(fit, oracle):
+    # runtests.jl:738-776: !-variants on a zero field return zeros
+    ks = ks_of(oracle, (6, 6))
+    r = fit.calculate_shell_to_shell_transfer(np.zeros((6, 6, 2), complex), ks,
+                                              binning=fit.LinearBinning(1.0),
+                                              dealiasing=fit.NoDealiasing(), verify_antisymmetry=False)
+    assert np.all(np.abs(r.transfer_matrix) < 1e-14) and math.isnan(r.max_antisymmetry_error)
+
+
+def test_device_arrays_roundtrip(fit, oracle):
+    # torch tensors on the HIP device go through FIT_PTR_DEVICE without host copies
+    import torch
+    say("case device arrays")
+    ns = (16, 12, 8)
+    u = oracle.synthetic_field(ns, 3, seed=4)
+    ks = ks_of(oracle, ns)
+    ud = torch.from_numpy(np.ascontiguousarray(u.transpose(3, 2, 1, 0))).to("cuda").permute(3, 2, 1, 0)
+    N = fit.compute_nonlinear_term(ud, ks)
+    assert N.is_cuda
+    ref = oracle.compute_nonlinear_term(u, ks, "23")
+    assert rel(N.cpu().numpy(), ref) < TOL64
+    r = fit.calculate_spectral_flux(ud.to(torch.complex64), ks, binning=fit.LinearBinning(1.0))
+    _, T, _ = oracle.calculate_spectral_flux(u, ks, oracle.LinearBinning(1.0), "23")
+    assert rel(r.transfer_spectrum, T) < 1e-5
