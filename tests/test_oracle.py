@@ -156,7 +156,7 @@ def test_scalar_variance_conserved_and_flux_returns_to_zero(oracle):
     th = np.fft.fft2(rng.standard_normal((16, 16))) / 256
     _, T, Pi = oracle.calculate_scalar_flux(u, th, ks, oracle.LinearBinning(1.0), "23")
     sT = math.sqrt(np.sum(T ** 2))
-    assert sT > 0 and abs(T.sum()) < 1e-9 * sT and abs(Pi[-1]) < 1e-9 * sT
+    assert sT > 1e-9 and abs(T.sum()) < 1e-9 * sT and abs(Pi[-1]) < 1e-9 * sT
     _, T1, _ = oracle.calculate_scalar_flux(u, th[..., None], ks, oracle.LinearBinning(1.0), "23")
     assert np.allclose(T1, T)
 
@@ -272,7 +272,7 @@ def test_shell_to_shell_properties(oracle):
 
 def test_scalar_shell_to_shell_properties(oracle):
     # runtests.jl:441-464
-    rng = np.random.default_rng(14)
+    rng = np.random.default_rng(15)
     u, ks = oracle.streamfunction_velocity_2d(16, 14)
     th = np.fft.fft2(rng.standard_normal((16, 16))) / 256
     b = oracle.LinearBinning(1.0)
@@ -280,7 +280,7 @@ def test_scalar_shell_to_shell_properties(oracle):
     _, _, Td, _, asym = oracle.calculate_scalar_shell_to_shell_transfer(
         u, th, ks, binning=b, spectral="DirectSumBackend")
     nrm = math.sqrt(np.sum(Td ** 2))
-    assert nrm > 0 and np.max(np.abs(Tf - Td)) < 1e-9 * nrm and asym < 1e-10 * nrm
+    assert nrm > 1e-12 and np.max(np.abs(Tf - Td)) < 1e-9 * nrm and asym < 1e-10 * nrm
     _, T, _ = oracle.calculate_scalar_flux(u, th, ks, b, "23")
     assert np.max(np.abs(netf - T)) < 1e-9 * nrm
 
