@@ -144,6 +144,17 @@ class Plan:
     def __exit__(self, *exc):
         self.close()
 
+    def set_option(self, option, value):
+        """FIT_OPT_* knobs: plain=True runs the unfused reference dataflow (comparison baseline);
+        giver_batch caps the givers per batched launch."""
+        _lib.check(self.lib.fit_plan_set_option(self.handle, option, int(value)))
+
+    def set_plain(self, on=True):
+        self.set_option(_lib.OPT_PLAIN_PIPELINE, int(bool(on)))
+
+    def set_giver_batch(self, n):
+        self.set_option(_lib.OPT_GIVER_BATCH, int(n))
+
     @property
     def device_bytes(self):
         return int(self.lib.fit_plan_device_bytes(self.handle))

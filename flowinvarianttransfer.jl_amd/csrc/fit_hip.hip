@@ -89,6 +89,20 @@ struct fit_plan {
   int nblk_density = 0;
   double* dcol = nullptr;  // nsh doubles
   int64_t bytes = 0;
+  // fused / pruned pipeline state
+  int plain = 0;                 // 1: unfused, unpruned, one giver per launch (reference dataflow)
+  int band_in[3] = {0, 0, 0};    // folded support of spectral inputs (keep / embedded block)
+  int band_rx[3] = {0, 0, 0};    // folded support of the receiving shells (∩ output mask)
+  std::vector<int4> shell_band;  // per shell (index s-1): folded support ∩ band_in (-1: empty)
+  int4* d_shell_band = nullptr;
+  int* shh = nullptr;            // shell map on the original half grid (x <= n1/2)
+  void* eh = nullptr;            // Hermitian part of the masked carrier, M comps, half grid
+  void* work2 = nullptr; size_t work2_bytes = 0;
+  void* nxbuf = nullptr; size_t nxbuf_bytes = 0;
+  double* partials2 = nullptr; size_t partials2_bytes = 0;
+  double* dcolG = nullptr; size_t dcolG_bytes = 0;
+  size_t eh_bytes = 0;
+  int giver_batch = 0;           // cap on givers per batch (0 = memory budget only)
   // profiling
   bool prof = false;
   std::vector<ProfRec> recs;
@@ -171,26 +185,38 @@ constexpr int kThreads = 256;
 
 // ------------------------------ pass launchers ------------------------------------------------
 
+Band full_band(int G) { return Band{G, G}; }
+
+// Column pass over fields [0, nfields): x < nx, other-index band ob, input band ib, output band wb.
 template <typename R, int SIGN, class LD, class ST>
-int launch_col(fit_plan* p, LD ld, ST st, int axis, int nfields) {
-  const int ax = axis;  // 1 = y, 2 = z
-  const int L = p->G[ax];
+int launch_col(fit_plan* p, LD ld, ST st, int axis, int nfields, int nx, Band ob, Band ib, Band wb,
+               const int4* fband = nullptr, int fdiv = 1) {
+  const int L = p->G[axis];
   const size_t csz = sizeof(cpx<R>);
   const int TX = tile_for(L, csz, 16);
-  const int nx = p->h1;
-  const int no = axis == 1 ? p->G[2] : p->G[1];
-  dim3 grid((nx + TX - 1) / TX, no, nfields);
+  const int nob = ob.count();
+  if (nx <= 0 || nob <= 0 || nfields <= 0) return FIT_OK;
+  dim3 grid((nx + TX - 1) / TX, nob, nfields);
   const size_t lds = (size_t)2 * L * TX * csz;
-  prof_begin(p, FIT_K_FFT_COLUMN, 2.0 * (double)nx * no * L * csz * nfields);
-  hipLaunchKernelGGL((k_col<R, SIGN, LD, ST>), grid, dim3(kThreads), lds, p->stream, ld, st, p->fd[ax],
-                     (const cpx<R>*)p->tw[ax], nx, TX, axis);
+  const double rows_in = ib.count(), rows_out = wb.count();
+  prof_begin(p, FIT_K_FFT_COLUMN, (double)nx * nob * (rows_in + rows_out) * csz * nfields);
+  hipLaunchKernelGGL((k_col<R, SIGN, LD, ST>), grid, dim3(kThreads), lds, p->stream, ld, st, p->fd[axis],
+                     (const cpx<R>*)p->tw[axis], nx, TX, axis, ob, ib, wb, fband, fdiv,
+                     axis == 2 ? p->G[1] : p->G[2]);
   prof_end(p);
   FIT_HIP(hipGetLastError());
   return FIT_OK;
 }
 
+template <typename R, int SIGN, class LD, class ST>
+int launch_col_full(fit_plan* p, LD ld, ST st, int axis, int nfields) {
+  const int other = axis == 1 ? p->G[2] : p->G[1];
+  return launch_col<R, SIGN>(p, ld, st, axis, nfields, p->h1, full_band(other), full_band(p->G[axis]),
+                             full_band(p->G[axis]));
+}
+
 template <typename R, class LD, class ST>
-int launch_row_inv(fit_plan* p, LD ld, ST st, int nfields, double extra_bytes_per_row) {
+int launch_row_inv(fit_plan* p, LD ld, ST st, int nfields, double extra_bytes_per_row, int kxa) {
   const int L = p->G[0];
   const size_t csz = sizeof(cpx<R>);
   const int P = tile_for(L, csz, 8);
@@ -198,17 +224,18 @@ int launch_row_inv(fit_plan* p, LD ld, ST st, int nfields, double extra_bytes_pe
   const int npairs = (nrows + 1) / 2;
   dim3 grid((npairs + P - 1) / P, 1, nfields);
   const size_t lds = (size_t)2 * L * P * csz;
+  const int nin = std::min(kxa + 1, p->h1);
   prof_begin(p, FIT_K_FFT_ROW_INVERSE,
-             ((double)p->h1 * csz + (double)L * sizeof(R) + extra_bytes_per_row) * nrows * nfields);
+             ((double)nin * csz + (double)L * sizeof(R) + extra_bytes_per_row) * nrows * nfields);
   hipLaunchKernelGGL((k_row_inv<R, LD, ST>), grid, dim3(kThreads), lds, p->stream, ld, st, p->fd[0],
-                     (const cpx<R>*)p->tw[0], p->G[1], nrows, P);
+                     (const cpx<R>*)p->tw[0], p->G[1], nrows, P, kxa);
   prof_end(p);
   FIT_HIP(hipGetLastError());
   return FIT_OK;
 }
 
 template <typename R, class LD, class ST>
-int launch_row_fwd(fit_plan* p, LD ld, ST st, int nfields) {
+int launch_row_fwd(fit_plan* p, LD ld, ST st, int nfields, int nout) {
   const int L = p->G[0];
   const size_t csz = sizeof(cpx<R>);
   const int P = tile_for(L, csz, 8);
@@ -216,9 +243,9 @@ int launch_row_fwd(fit_plan* p, LD ld, ST st, int nfields) {
   const int npairs = (nrows + 1) / 2;
   dim3 grid((npairs + P - 1) / P, 1, nfields);
   const size_t lds = (size_t)2 * L * P * csz;
-  prof_begin(p, FIT_K_FFT_ROW_FORWARD, ((double)p->h1 * csz + (double)L * sizeof(R)) * nrows * nfields);
+  prof_begin(p, FIT_K_FFT_ROW_FORWARD, ((double)nout * csz + (double)L * sizeof(R)) * nrows * nfields);
   hipLaunchKernelGGL((k_row_fwd<R, LD, ST>), grid, dim3(kThreads), lds, p->stream, ld, st, p->fd[0],
-                     (const cpx<R>*)p->tw[0], p->G[1], nrows, P, p->h1);
+                     (const cpx<R>*)p->tw[0], p->G[1], nrows, P, nout);
   prof_end(p);
   FIT_HIP(hipGetLastError());
   return FIT_OK;
@@ -231,20 +258,28 @@ template <typename R> StHalf<R> st_work(fit_plan* p) {
   return StHalf<R>{(cpx<R>*)p->work, (int64_t)p->h1, (int64_t)p->h1 * p->G[1], p->Hp};
 }
 
-// Inverse transform of one spectral-source field into a physical row storer RS.
+// Inverse transform of one spectral-source field into a physical row storer RS.  `bi` is the
+// folded support of the source (pruning); the plain pipeline passes full bands.
 template <typename R, class RS>
-int inverse_field(fit_plan* p, const LdSpectral<R>& ls, RS rs, double extra_row_bytes) {
+int inverse_field(fit_plan* p, const LdSpectral<R>& ls, RS rs, double extra_row_bytes, const int* bi) {
   int rc;
+  const int xa = std::min(bi[0], p->G[0] / 2) + 1;
   if (p->nd == 3) {
-    if ((rc = launch_col<R, 1>(p, ls, st_work<R>(p), 2, 1))) return rc;
-    if ((rc = launch_col<R, 1>(p, ld_work<R>(p), st_work<R>(p), 1, 1))) return rc;
-    return launch_row_inv<R>(p, ld_work<R>(p), rs, 1, extra_row_bytes);
+    if ((rc = launch_col<R, 1>(p, ls, st_work<R>(p), 2, 1, xa, Band{bi[1], p->G[1]}, Band{bi[2], p->G[2]},
+                               full_band(p->G[2]))))
+      return rc;
+    if ((rc = launch_col<R, 1>(p, ld_work<R>(p), st_work<R>(p), 1, 1, xa, full_band(p->G[2]),
+                               Band{bi[1], p->G[1]}, full_band(p->G[1]))))
+      return rc;
+    return launch_row_inv<R>(p, ld_work<R>(p), rs, 1, extra_row_bytes, xa - 1);
   }
   if (p->nd == 2) {
-    if ((rc = launch_col<R, 1>(p, ls, st_work<R>(p), 1, 1))) return rc;
-    return launch_row_inv<R>(p, ld_work<R>(p), rs, 1, extra_row_bytes);
+    if ((rc = launch_col<R, 1>(p, ls, st_work<R>(p), 1, 1, xa, full_band(1), Band{bi[1], p->G[1]},
+                               full_band(p->G[1]))))
+      return rc;
+    return launch_row_inv<R>(p, ld_work<R>(p), rs, 1, extra_row_bytes, xa - 1);
   }
-  return launch_row_inv<R>(p, ls, rs, 1, extra_row_bytes);
+  return launch_row_inv<R>(p, ls, rs, 1, extra_row_bytes, xa - 1);
 }
 
 // Forward transform of the physical product (nphys) into full original-grid N̂ component c.
@@ -255,11 +290,11 @@ int forward_field(fit_plan* p, int c, double scale) {
   StNhat<R> sn{g, (cpx<R>*)p->nhat + (int64_t)c * p->Np, 0, p->dealias == FIT_DEALIAS_TWOTHIRDS,
                (R)scale};
   int rc;
-  if (p->nd == 1) return launch_row_fwd<R>(p, lr, sn, 1);
-  if ((rc = launch_row_fwd<R>(p, lr, st_work<R>(p), 1))) return rc;
-  if (p->nd == 2) return launch_col<R, -1>(p, ld_work<R>(p), sn, 1, 1);
-  if ((rc = launch_col<R, -1>(p, ld_work<R>(p), st_work<R>(p), 1, 1))) return rc;
-  return launch_col<R, -1>(p, ld_work<R>(p), sn, 2, 1);
+  if (p->nd == 1) return launch_row_fwd<R>(p, lr, sn, 1, p->h1);
+  if ((rc = launch_row_fwd<R>(p, lr, st_work<R>(p), 1, p->h1))) return rc;
+  if (p->nd == 2) return launch_col_full<R, -1>(p, ld_work<R>(p), sn, 1, 1);
+  if ((rc = launch_col_full<R, -1>(p, ld_work<R>(p), st_work<R>(p), 1, 1))) return rc;
+  return launch_col_full<R, -1>(p, ld_work<R>(p), sn, 2, 1);
 }
 
 template <typename R> LdSpectral<R> spectral_loader(fit_plan* p, const void* src, int gradj, int giver) {
@@ -279,21 +314,28 @@ template <typename R> LdSpectral<R> spectral_loader(fit_plan* p, const void* src
   return ls;
 }
 
+const int* full_bi(fit_plan* p) {
+  static thread_local int b[3];
+  b[0] = p->G[0]; b[1] = p->G[1]; b[2] = p->G[2];
+  return b;
+}
+
 // Advecting velocity u_j on the transform grid (FFTWExt.jl:80-86 / :171): u_j = Re ifft(keep â_j)
 template <typename R>
 int compute_u(fit_plan* p, const void* adv) {
   const double inv_mtot = 1.0 / (double)p->Gp;
+  const int* bi = p->plain ? full_bi(p) : p->band_in;
   for (int j = 0; j < p->nd; ++j) {
     LdSpectral<R> ls = spectral_loader<R>(p, (const cpx<R>*)adv + (int64_t)j * p->Np, -1, 0);
     StReal<R> sr{(R*)p->uphys + (int64_t)j * p->Gp, (int64_t)p->G[0], (int64_t)p->G[0] * p->G[1], 0,
                  (R)inv_mtot};
-    int rc = inverse_field<R>(p, ls, sr, 0.0);
+    int rc = inverse_field<R>(p, ls, sr, 0.0, bi);
     if (rc) return rc;
   }
   return FIT_OK;
 }
 
-// N̂ for the primary field f (optionally restricted to giver shell m) with u already computed.
+// Plain pipeline: N̂ for the primary field f (optionally restricted to giver shell m), u computed.
 template <typename R>
 int nonlinear_from_u(fit_plan* p, const void* f, int giver) {
   const double inv_mtot = 1.0 / (double)p->Gp;
@@ -303,13 +345,114 @@ int nonlinear_from_u(fit_plan* p, const void* f, int giver) {
     for (int j = 0; j < p->nd; ++j) {
       LdSpectral<R> ls = spectral_loader<R>(p, (const cpx<R>*)f + (int64_t)c * p->Np, j, giver);
       StProduct<R> sp{(R*)p->nphys, (const R*)p->uphys + (int64_t)j * p->Gp, sy, sz, (R)inv_mtot, j == 0};
-      int rc = inverse_field<R>(p, ls, sp, (double)p->G[0] * sizeof(R) * (j == 0 ? 2.0 : 3.0));
+      int rc = inverse_field<R>(p, ls, sp, (double)p->G[0] * sizeof(R) * (j == 0 ? 2.0 : 3.0), full_bi(p));
       if (rc) return rc;
     }
     int rc = forward_field<R>(p, c, fscale);
     if (rc) return rc;
   }
   return FIT_OK;
+}
+
+// ------------------------------ fused / pruned pipeline ---------------------------------------
+
+int grow(fit_plan* p, void** buf, size_t* have, size_t need) {
+  if (*have >= need) return FIT_OK;
+  if (*buf) { (void)hipStreamSynchronize(p->stream); (void)hipFree(*buf); p->bytes -= (int64_t)*have; *buf = nullptr; *have = 0; }
+  int rc = dalloc(p, buf, need);
+  if (rc) return rc;
+  *have = need;
+  return FIT_OK;
+}
+
+// x-extent of compact storage for a folded band on the half grid
+int xext(fit_plan* p, int a) { return std::min(a, p->G[0] / 2) + 1; }
+
+// Inverse passes (all but x) of ng*M*nd gradient fields (giver g, component c, direction j) of
+// f̂ into work2 (compact, pitch xa).  giver0 = 0: no shell filter.  fband: per-giver bounds.
+template <typename R>
+int v2_inverse_grads(fit_plan* p, const void* f, int giver0, int ng, const int* bi, const int4* fband) {
+  const int nf = ng * p->M * p->nd;
+  const int xa = xext(p, bi[0]);
+  const size_t csz = sizeof(cpx<R>);
+  int rc = grow(p, &p->work2, &p->work2_bytes, (size_t)nf * xa * p->G[1] * p->G[2] * csz);
+  if (rc) return rc;
+  LdSpectralB<R> ls{spectral_loader<R>(p, f, 0, 0), p->M, giver0};
+  const int64_t sy = xa, sz = (int64_t)xa * p->G[1], sf = sz * p->G[2];
+  StHalf<R> st{(cpx<R>*)p->work2, sy, sz, sf};
+  LdHalf<R> ld{(const cpx<R>*)p->work2, sy, sz, sf};
+  const int fdiv = p->M * p->nd;
+  if (p->nd == 3) {
+    if ((rc = launch_col<R, 1>(p, ls, st, 2, nf, xa, Band{bi[1], p->G[1]}, Band{bi[2], p->G[2]},
+                               full_band(p->G[2]), fband, fdiv)))
+      return rc;
+    return launch_col<R, 1>(p, ld, st, 1, nf, xa, full_band(p->G[2]), Band{bi[1], p->G[1]},
+                            full_band(p->G[1]), fband, fdiv);
+  }
+  return launch_col<R, 1>(p, ls, st, 1, nf, xa, full_band(1), Band{bi[1], p->G[1]}, full_band(p->G[1]),
+                          fband, fdiv);
+}
+
+// Fused x pass: ng*M field groups -> forward-x half rows kx <= bo[0] in nxbuf (pitch xb+1).
+template <typename R>
+int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* bo) {
+  const int nfg = ng * p->M;
+  const int xa = xext(p, bi[0]);
+  const int xb = xext(p, bo[0]);
+  const size_t csz = sizeof(cpx<R>);
+  int rc = grow(p, &p->nxbuf, &p->nxbuf_bytes, (size_t)nfg * xb * p->G[1] * p->G[2] * csz);
+  if (rc) return rc;
+  const int L = p->G[0];
+  int P = 8;
+  while (P > 1 && (size_t)3 * L * P * csz > 65536) --P;
+  const int nrows = p->G[1] * p->G[2];
+  const int npairs = (nrows + 1) / 2;
+  const int nrg = (npairs + P - 1) / P;
+  const int64_t gsy = xa, gsz = (int64_t)xa * p->G[1], gsf = gsz * p->G[2];
+  const int64_t osy = xb, osz = (int64_t)xb * p->G[1], osf = osz * p->G[2];
+  const int64_t usy = L, usz = (int64_t)L * p->G[1];
+  const double bytes = (double)nfg * nrows * (p->nd * (xa * csz) + xb * csz) + (double)p->nd * nrows * L * sizeof(R);
+  prof_begin(p, FIT_K_FFT_ROW_INVERSE, bytes);
+  hipLaunchKernelGGL(k_xfused<R>, dim3((unsigned)((int64_t)nrg * nfg)), dim3(kThreads), (size_t)3 * L * P * csz,
+                     p->stream, p->fd[0], (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf,
+                     (const R*)p->uphys, usy, usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M, p->G[1],
+                     nrows, P, (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, nfg);
+  prof_end(p);
+  FIT_HIP(hipGetLastError());
+  return FIT_OK;
+}
+
+// Remaining forward passes of nxbuf (M fields, one field group per component) into full N̂.
+template <typename R>
+int v2_forward_nhat(fit_plan* p, const int* bo) {
+  const int xb = xext(p, bo[0]);
+  const int64_t sy = xb, sz = (int64_t)xb * p->G[1], sf = sz * p->G[2];
+  LdHalf<R> ld{(const cpx<R>*)p->nxbuf, sy, sz, sf};
+  StHalf<R> st{(cpx<R>*)p->nxbuf, sy, sz, sf};
+  Grid g = grid_of<R>(p);
+  const bool trunc = p->dealias == FIT_DEALIAS_TWOTHIRDS;
+  const double scale = (double)p->Gp / ((double)p->Np * (double)p->Np);
+  StNhat<R> sn{g, (cpx<R>*)p->nhat, p->Np, trunc, (R)scale};
+  int rc;
+  if (trunc) FIT_HIP(hipMemsetAsync(p->nhat, 0, (size_t)p->M * p->Np * sizeof(cpx<R>), p->stream));
+  if (p->nd == 3) {
+    if ((rc = launch_col<R, -1>(p, ld, st, 1, p->M, xb, full_band(p->G[2]), full_band(p->G[1]),
+                                Band{bo[1], p->G[1]})))
+      return rc;
+    return launch_col<R, -1>(p, ld, sn, 2, p->M, xb, Band{bo[1], p->G[1]}, full_band(p->G[2]),
+                             Band{bo[2], p->G[2]});
+  }
+  return launch_col<R, -1>(p, ld, sn, 1, p->M, xb, full_band(1), full_band(p->G[1]), Band{bo[1], p->G[1]});
+}
+
+// N̂ of f advected by u (already in uphys) through the fused pipeline.
+template <typename R>
+int v2_nonlinear(fit_plan* p, const void* f) {
+  const int* bo = p->band_in;  // every mode the input band feeds (2/3: kept box; padded: block)
+  int rc;
+  if ((rc = v2_inverse_grads<R>(p, f, 0, 1, p->band_in, nullptr))) return rc;
+  if ((rc = v2_xfused<R>(p, 1, p->band_in, nullptr, bo))) return rc;
+  return v2_forward_nhat<R>(p, bo);
 }
 
 // density (+ shell partial sums, reduced into p->dcol) from carrier and N̂ (both full grid)
@@ -455,7 +598,8 @@ template <typename R>
 int nonlinear_term_impl(fit_plan* p, const void* f, const void* a) {
   int rc = compute_u<R>(p, a);
   if (rc) return rc;
-  return nonlinear_from_u<R>(p, f, 0);
+  if (p->plain || p->nd == 1) return nonlinear_from_u<R>(p, f, 0);
+  return v2_nonlinear<R>(p, f);
 }
 
 int finish(fit_plan* p) {
@@ -478,10 +622,10 @@ int spectral_flux_tail(fit_plan* p, double* T, double* Pi) {
   return FIT_OK;
 }
 
+// Plain giver loop: one giver per pipeline pass, full transforms, N̂ through HBM.
 template <typename R>
-int s2s_impl(fit_plan* p, int inv, const void* f, const void* a, std::vector<double>& Tm) {
-  int rc = compute_u<R>(p, a);
-  if (rc) return rc;
+int s2s_plain(fit_plan* p, int inv, const void* f, std::vector<double>& Tm) {
+  int rc;
   const int nsh = p->nsh;
   for (int m = 1; m <= nsh; ++m) {
     if ((rc = nonlinear_from_u<R>(p, f, m))) return rc;
@@ -492,6 +636,104 @@ int s2s_impl(fit_plan* p, int inv, const void* f, const void* a, std::vector<dou
     FIT_HIP(hipStreamSynchronize(p->stream));
   }
   return FIT_OK;
+}
+
+// Batched giver loop: every giver of a batch goes through each pruned pass in one launch; the
+// last forward pass computes the pair density and the shell sums in place (no N̂ in HBM).
+template <typename R>
+int s2s_batched(fit_plan* p, int inv, const void* f, std::vector<double>& Tm) {
+  const int nsh = p->nsh;
+  const int* bo = p->band_rx;
+  const size_t csz = sizeof(cpx<R>);
+  const int n1h = p->n[0] / 2 + 1;
+  const int64_t Hn = (int64_t)n1h * p->n[1] * p->n[2];
+  int rc;
+  // carrier Hermitian part on the receiving band
+  if ((rc = grow(p, &p->eh, &p->eh_bytes, (size_t)p->M * Hn * csz))) return rc;
+  {
+    Grid g = grid_of<R>(p);
+    prof_begin(p, FIT_K_OTHER, (double)Hn * p->M * csz * 3.0);
+    hipLaunchKernelGGL(k_carrier_half<R>, dim3(2048), dim3(256), 0, p->stream, g, inv, p->M,
+                       p->dealias == FIT_DEALIAS_TWOTHIRDS ? 1 : 0, (const cpx<R>*)f, p->Np, (const R*)p->ks[0],
+                       (const R*)p->ks[1], (const R*)p->ks[2], Band{bo[1], p->n[1]}, Band{bo[2], p->n[2]},
+                       bo[0], (cpx<R>*)p->eh, Hn);
+    prof_end(p);
+    FIT_HIP(hipGetLastError());
+  }
+  // batch size from the device memory budget
+  int bmax[3] = {-1, -1, -1};
+  for (int s = 0; s < nsh; ++s) {
+    bmax[0] = std::max(bmax[0], p->shell_band[s].x);
+    bmax[1] = std::max(bmax[1], p->shell_band[s].y);
+    bmax[2] = std::max(bmax[2], p->shell_band[s].z);
+  }
+  const int xa_max = std::max(1, xext(p, bmax[0]));
+  const int xb = std::max(1, xext(p, bo[0]));
+  const double per_giver = ((double)p->M * p->nd * xa_max + (double)p->M * xb) * p->G[1] * p->G[2] * csz;
+  size_t freeb = 0, totalb = 0;
+  FIT_HIP(hipMemGetInfo(&freeb, &totalb));
+  const double budget = 0.6 * ((double)freeb + (double)p->work2_bytes + (double)p->nxbuf_bytes);
+  int ng = (int)std::max(1.0, std::min((double)nsh, budget / per_giver));
+  if (p->giver_batch > 0) ng = std::min(ng, p->giver_batch);
+  // last-pass launch geometry
+  const int lax = p->nd == 3 ? 2 : 1;
+  const int L = p->G[lax];
+  int TX = 16;
+  while (TX > 1 && ((size_t)2 * L * TX * csz + (size_t)nsh * 8 > 65536 || TX * L > 32 * kThreads)) TX /= 2;
+  const Band ob = p->nd == 3 ? Band{bo[1], p->G[1]} : full_band(1);
+  const Band wb = p->nd == 3 ? Band{bo[2], p->G[2]} : Band{bo[1], p->G[1]};
+  const int gx = (xb + TX - 1) / TX, gy = ob.count();
+  const int nblk = gx * gy;
+  if ((rc = grow(p, (void**)&p->partials2, &p->partials2_bytes, (size_t)ng * nblk * nsh * sizeof(double)))) return rc;
+  if ((rc = grow(p, (void**)&p->dcolG, &p->dcolG_bytes, (size_t)ng * nsh * sizeof(double)))) return rc;
+  const double scale = 1.0 / (double)p->Np;
+  for (int m0 = 1; m0 <= nsh; m0 += ng) {
+    const int gb = std::min(ng, nsh - m0 + 1);
+    int bi[3] = {-1, -1, -1};
+    for (int s = m0 - 1; s < m0 - 1 + gb; ++s) {
+      bi[0] = std::max(bi[0], p->shell_band[s].x);
+      bi[1] = std::max(bi[1], p->shell_band[s].y);
+      bi[2] = std::max(bi[2], p->shell_band[s].z);
+    }
+    if (bi[0] < 0) continue;  // empty givers: zero columns
+    const int4* fb = p->d_shell_band + (m0 - 1);
+    if ((rc = v2_inverse_grads<R>(p, f, m0, gb, bi, fb))) return rc;
+    if ((rc = v2_xfused<R>(p, gb, bi, fb, bo))) return rc;
+    const int64_t sy = xb, sz = (int64_t)xb * p->G[1], sf = sz * p->G[2];
+    if (p->nd == 3) {
+      LdHalf<R> ld{(const cpx<R>*)p->nxbuf, sy, sz, sf};
+      StHalf<R> st{(cpx<R>*)p->nxbuf, sy, sz, sf};
+      if ((rc = launch_col<R, -1>(p, ld, st, 1, gb * p->M, xb, full_band(p->G[2]), full_band(p->G[1]),
+                                  Band{bo[1], p->G[1]})))
+        return rc;
+    }
+    prof_begin(p, FIT_K_DENSITY_SHELL,
+               (double)gb * p->M * xb * gy * L * csz + (double)gb * p->M * xb * gy * wb.count() * csz);
+    hipLaunchKernelGGL(k_lastdens<R>, dim3(gx, gy, gb), dim3(kThreads), (size_t)2 * L * TX * csz + (size_t)nsh * 8,
+                       p->stream, p->fd[lax], (const cpx<R>*)p->tw[lax], (const cpx<R>*)p->nxbuf, sy, sz, sf, xb, TX,
+                       lax, ob, wb, p->M, (R)scale, (const cpx<R>*)p->eh, (int64_t)n1h, (int64_t)n1h * p->n[1], Hn,
+                       p->shh, nsh, p->n[0], p->partials2);
+    prof_end(p);
+    FIT_HIP(hipGetLastError());
+    prof_begin(p, FIT_K_REDUCE, (double)gb * nblk * nsh * 8.0);
+    hipLaunchKernelGGL(k_reduce_batched, dim3((nsh + 127) / 128, gb), dim3(128), 0, p->stream, p->partials2, nblk,
+                       nsh, gb, p->dcolG);
+    prof_end(p);
+    FIT_HIP(hipGetLastError());
+    FIT_HIP(hipMemcpyAsync(Tm.data() + (size_t)(m0 - 1) * nsh, p->dcolG, (size_t)gb * nsh * sizeof(double),
+                           hipMemcpyDeviceToHost, p->stream));
+    FIT_HIP(hipStreamSynchronize(p->stream));
+  }
+  return FIT_OK;
+}
+
+template <typename R>
+int s2s_impl(fit_plan* p, int inv, const void* f, const void* a, std::vector<double>& Tm) {
+  int rc = compute_u<R>(p, a);
+  if (rc) return rc;
+  if (p->plain || p->nd == 1 || p->dealias == FIT_DEALIAS_PADDED || !p->shell_symmetric)
+    return s2s_plain<R>(p, inv, f, Tm);
+  return s2s_batched<R>(p, inv, f, Tm);
 }
 
 template <typename R>
@@ -637,6 +879,12 @@ int fit_plan_create(fit_plan** out, int nd, const int64_t* ns, int M, int D, int
       hipMemcpy(p->ks[d], kf.data(), (size_t)nk * rs, hipMemcpyHostToDevice);
     }
   }
+  // folded support of the spectral inputs: kept box (2/3), all modes otherwise (the embedded
+  // block on the padded grid has the same folded extent n/2)
+  for (int d = 0; d < 3; ++d) {
+    if (d >= nd) { p->band_in[d] = 0; continue; }
+    p->band_in[d] = dealias == FIT_DEALIAS_TWOTHIRDS ? p->n[d] / 3 - 1 : p->n[d] / 2;
+  }
   // shell map (int32 on device); symmetry under k -> -k is recorded
   if (n_shells > 0) {
     std::vector<int> sh((size_t)p->Np);
@@ -652,6 +900,40 @@ int fit_plan_create(fit_plan** out, int nd, const int64_t* ns, int M, int D, int
     }
     if ((rc = dalloc(p, (void**)&p->shell, (size_t)p->Np * sizeof(int)))) return bail(rc);
     hipMemcpy(p->shell, sh.data(), (size_t)p->Np * sizeof(int), hipMemcpyHostToDevice);
+    // per-shell folded support (∩ input band) and the receivers' support (∩ output mask)
+    p->shell_band.assign(n_shells, int4{-1, -1, -1, 0});
+    for (int64_t o = 0; o < p->Np; ++o) {
+      const int s = sh[o];
+      if (s <= 0) continue;
+      const int ix = (int)(o % p->n[0]), iy = (int)((o / p->n[0]) % p->n[1]), iz = (int)(o / ((int64_t)p->n[0] * p->n[1]));
+      const int fx = std::min(ix, p->n[0] - ix), fy = std::min(iy, p->n[1] - iy), fz = std::min(iz, p->n[2] - iz);
+      int4& b = p->shell_band[s - 1];
+      b.x = std::max(b.x, fx); b.y = std::max(b.y, fy); b.z = std::max(b.z, fz);
+    }
+    p->band_rx[0] = p->band_rx[1] = p->band_rx[2] = -1;
+    for (auto& b : p->shell_band) {
+      if (b.x < 0) continue;
+      b.x = std::min(b.x, p->band_in[0]); b.y = std::min(b.y, p->band_in[1]); b.z = std::min(b.z, p->band_in[2]);
+      if (b.x < 0 || b.y < 0 || b.z < 0) { b = int4{-1, -1, -1, 0}; continue; }
+      p->band_rx[0] = std::max(p->band_rx[0], b.x);
+      p->band_rx[1] = std::max(p->band_rx[1], b.y);
+      p->band_rx[2] = std::max(p->band_rx[2], b.z);
+    }
+    for (int d = 0; d < 3; ++d) p->band_rx[d] = std::max(p->band_rx[d], 0);
+    if ((rc = dalloc(p, (void**)&p->d_shell_band, (size_t)n_shells * sizeof(int4)))) return bail(rc);
+    hipMemcpy(p->d_shell_band, p->shell_band.data(), (size_t)n_shells * sizeof(int4), hipMemcpyHostToDevice);
+    // shell map on the original half grid (x <= n1/2) for the fused density pass
+    if (dealias != FIT_DEALIAS_PADDED) {
+      const int n1h = p->n[0] / 2 + 1;
+      const int64_t Hn = (int64_t)n1h * p->n[1] * p->n[2];
+      std::vector<int> shh((size_t)Hn);
+      for (int64_t h = 0; h < Hn; ++h) {
+        const int64_t ix = h % n1h, rest = h / n1h;
+        shh[h] = sh[ix + (int64_t)p->n[0] * rest];
+      }
+      if ((rc = dalloc(p, (void**)&p->shh, (size_t)Hn * sizeof(int)))) return bail(rc);
+      hipMemcpy(p->shh, shh.data(), (size_t)Hn * sizeof(int), hipMemcpyHostToDevice);
+    }
   }
   if ((rc = dalloc(p, &p->work, (size_t)p->Hp * cs))) return bail(rc);
   if ((rc = dalloc(p, &p->uphys, (size_t)nd * p->Gp * rs))) return bail(rc);
@@ -676,6 +958,11 @@ int fit_plan_destroy(fit_plan* p) {
   dfree(p->work); dfree(p->uphys); dfree(p->nphys); dfree(p->nhat);
   dfree(p->in_f); dfree(p->in_a); dfree(p->in_c); dfree(p->out_tmp);
   void* pa = p->partials; dfree(pa);
+  { void* q = p->d_shell_band; dfree(q); }
+  { void* q = p->shh; dfree(q); }
+  dfree(p->eh); dfree(p->work2); dfree(p->nxbuf);
+  { void* q = p->partials2; dfree(q); }
+  { void* q = p->dcolG; dfree(q); }
   void* dc = p->dcol; dfree(dc);
   for (auto& r : p->recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
   if (p->stream) hipStreamDestroy(p->stream);
@@ -684,6 +971,18 @@ int fit_plan_destroy(fit_plan* p) {
 }
 
 int64_t fit_plan_device_bytes(const fit_plan* p) { return p ? p->bytes : 0; }
+
+int fit_plan_set_option(fit_plan* p, int option, int value) {
+  if (!p) return fail(FIT_E_ARGUMENT, "null plan");
+  switch (option) {
+    case FIT_OPT_PLAIN_PIPELINE: p->plain = value != 0; return FIT_OK;
+    case FIT_OPT_GIVER_BATCH:
+      if (value < 0) return fail(FIT_E_ARGUMENT, "giver batch must be >= 0");
+      p->giver_batch = value;
+      return FIT_OK;
+    default: return fail(FIT_E_ARGUMENT, "unknown plan option");
+  }
+}
 
 void* fit_plan_stream(const fit_plan* p) { return p ? (void*)p->stream : nullptr; }
 

@@ -34,6 +34,16 @@ struct Grid {
   }
 };
 
+// Folded-index band {i : min(i, G - i) <= a} of one axis (all of it when 2a+1 >= G); used to
+// prune FFT passes to the spectral support of their input and the modes their output feeds.
+struct Band {
+  int a, G;
+  __host__ __device__ __forceinline__ bool full() const { return 2 * a + 1 >= G; }
+  __host__ __device__ __forceinline__ int count() const { return full() ? G : (a < 0 ? 0 : 2 * a + 1); }
+  __host__ __device__ __forceinline__ bool has(int i) const { return full() || i <= a || i >= G - a; }
+  __host__ __device__ __forceinline__ int at(int t) const { return (full() || t <= a) ? t : G - (2 * a + 1 - t); }
+};
+
 // ---------------------------------------------------------------------------------------------
 // Loaders: value at logical (x, y, z) of field f
 // ---------------------------------------------------------------------------------------------
@@ -87,6 +97,27 @@ template <typename R> struct LdSpectral {
     const int my = y == 0 ? 0 : g.G2 - y;
     const int mz = z == 0 ? 0 : g.G3 - z;
     const cpx<R> b = G(mx, my, mz, comp, gj);
+    return mk<R>((R)0.5 * (a.x + b.x), (R)0.5 * (a.y - b.y));
+  }
+};
+
+// Batched gradient sources: field f = (giver g, component c, direction j), f = (g*M + c)*nd + j.
+// Hermitian part of keep·(i ks_j)·1[shell == giver0 + g]·f̂_c on the transform grid.
+template <typename R> struct LdSpectralB {
+  LdSpectral<R> base;   // geometry, masks, ks, shell map
+  int M, giver0;        // giver0 = 0: no shell filter
+  __device__ __forceinline__ cpx<R> operator()(int x, int y, int z, int f) const {
+    const int nd = base.g.nd;
+    const int j = f % nd;
+    const int c = (f / nd) % M;
+    const int g = f / (nd * M);
+    LdSpectral<R> s = base;
+    s.giver = giver0 > 0 ? giver0 + g : 0;
+    const cpx<R> a = s.G(x, y, z, c, j);
+    const int mx = x == 0 ? 0 : s.g.G1 - x;
+    const int my = y == 0 ? 0 : s.g.G2 - y;
+    const int mz = z == 0 ? 0 : s.g.G3 - z;
+    const cpx<R> b = s.G(mx, my, mz, c, j);
     return mk<R>((R)0.5 * (a.x + b.x), (R)0.5 * (a.y - b.y));
   }
 };
@@ -146,29 +177,43 @@ template <typename R> struct StNhat {
 // ---------------------------------------------------------------------------------------------
 
 // Strided-axis pass: each workgroup transforms TX adjacent columns (consecutive x) of length L
-// along y (axis 1, other index = z) or z (axis 2, other index = y).
+// along y (axis 1, other index = z) or z (axis 2, other index = y).  Pruning: only columns in
+// x < nx and other-index band `ob` run; input rows outside `ib` are zero (not loaded); output rows
+// outside `wb` are not stored.  With fband != nullptr, field f uses the per-giver folded bounds
+// fband[f / fdiv] = (ax, ay, az): whole tiles outside them exit before any barrier.
 template <typename R, int SIGN, class LD, class ST>
 __global__ void __launch_bounds__(256) k_col(LD ld, ST st, FftDesc d, const cpx<R>* __restrict__ tw,
-                                             int nx, int TX, int axis) {
+                                             int nx, int TX, int axis, Band ob, Band ib, Band wb,
+                                             const int4* __restrict__ fband, int fdiv, int Gother) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* a = reinterpret_cast<cpx<R>*>(smem_raw);
   cpx<R>* b = a + (size_t)d.L * TX;
   const int L = d.L;
   const int x0 = blockIdx.x * TX;
-  const int o = blockIdx.y;
+  const int o = ob.at(blockIdx.y);
   const int f = blockIdx.z;
+  if (fband) {
+    const int4 fb = fband[f / fdiv];
+    if (x0 > fb.x) return;
+    if (axis == 2) {
+      if (!Band{fb.y, Gother}.has(o)) return;
+      ib = Band{fb.z, L};
+    } else {
+      ib = Band{fb.y, L};
+    }
+  }
   const int tid = threadIdx.x, nth = blockDim.x;
   for (int e = tid; e < L * TX; e += nth) {
     const int xl = e % TX, i = e / TX, x = x0 + xl;
     cpx<R> v = mk<R>((R)0, (R)0);
-    if (x < nx) v = axis == 1 ? ld(x, i, o, f) : ld(x, o, i, f);
+    if (x < nx && ib.has(i)) v = axis == 1 ? ld(x, i, o, f) : ld(x, o, i, f);
     a[i * TX + xl] = v;
   }
   __syncthreads();
   cpx<R>* res = lds_fft<SIGN, R>(a, b, d, TX, tw, tid, nth);
   for (int e = tid; e < L * TX; e += nth) {
     const int xl = e % TX, i = e / TX, x = x0 + xl;
-    if (x < nx) {
+    if (x < nx && wb.has(i)) {
       if (axis == 1) st(x, i, o, f, res[i * TX + xl]);
       else st(x, o, i, f, res[i * TX + xl]);
     }
@@ -180,7 +225,7 @@ __global__ void __launch_bounds__(256) k_col(LD ld, ST st, FftDesc d, const cpx<
 // The imaginary parts of the self-conjugate bins (k = 0, L/2) are dropped, as c2r / real() do.
 template <typename R, class LD, class ST>
 __global__ void __launch_bounds__(256) k_row_inv(LD ld, ST st, FftDesc d, const cpx<R>* __restrict__ tw,
-                                                 int G2, int nrows, int P) {
+                                                 int G2, int nrows, int P, int kxa) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* a = reinterpret_cast<cpx<R>*>(smem_raw);
   const int L = d.L;
@@ -193,8 +238,8 @@ __global__ void __launch_bounds__(256) k_row_inv(LD ld, ST st, FftDesc d, const 
     const int r0 = 2 * (pair0 + p), r1 = r0 + 1;
     const int kk = k <= L / 2 ? k : L - k;
     cpx<R> A = mk<R>((R)0, (R)0), B = A;
-    if (r0 < nrows) A = ld(kk, r0 % G2, r0 / G2, f);
-    if (r1 < nrows) B = ld(kk, r1 % G2, r1 / G2, f);
+    if (r0 < nrows && kk <= kxa) A = ld(kk, r0 % G2, r0 / G2, f);
+    if (r1 < nrows && kk <= kxa) B = ld(kk, r1 % G2, r1 / G2, f);
     if (k == 0 || 2 * k == L) { A.y = (R)0; B.y = (R)0; }
     if (k > L / 2) { A.y = -A.y; B.y = -B.y; }
     a[k * P + p] = mk<R>(A.x - B.y, A.y + B.x);
@@ -215,6 +260,7 @@ __global__ void __launch_bounds__(256) k_row_inv(LD ld, ST st, FftDesc d, const 
 template <typename R, class LD, class ST>
 __global__ void __launch_bounds__(256) k_row_fwd(LD ld, ST st, FftDesc d, const cpx<R>* __restrict__ tw,
                                                  int G2, int nrows, int P, int h1) {
+  // h1 here is the number of output bins stored (kx < h1): the output band of the pass
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* a = reinterpret_cast<cpx<R>*>(smem_raw);
   const int L = d.L;
@@ -242,6 +288,211 @@ __global__ void __launch_bounds__(256) k_row_fwd(LD ld, ST st, FftDesc d, const 
     if (r0 < nrows) st(k, r0 % G2, r0 / G2, f, A);
     if (r1 < nrows) st(k, r1 % G2, r1 / G2, f, B);
   }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fused contiguous-axis pass for one field group (giver g, component c) and a block of row pairs:
+//   for j < nd:  ∂_j f_c rows = c2r of the half rows of gradient field (fg*nd + j)   [kx <= kxa_g]
+//                N += u_j * ∂_j f_c                         (u_j physical, rows read from HBM)
+//   then r2c of N and store the half rows kx <= kxb  (N never touches HBM)
+// The two rows of a pair ride in the real / imaginary parts of one complex FFT both ways.
+// ---------------------------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256) k_xfused(FftDesc d, const cpx<R>* __restrict__ tw,
+                                                const cpx<R>* __restrict__ grad, int64_t gsy, int64_t gsz,
+                                                int64_t gsf, const R* __restrict__ u, int64_t usy, int64_t usz,
+                                                int64_t usf, cpx<R>* __restrict__ out, int64_t osy, int64_t osz,
+                                                int64_t osf, int nd, int M, int G2, int nrows, int P,
+                                                R gscale, int kxa, const int4* __restrict__ fband, int kxb,
+                                                int nfg) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int L = d.L;
+  cpx<R>* a = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<R>* b = a + (size_t)L * P;
+  cpx<R>* acc = b + (size_t)L * P;
+  // field group fastest: consecutive workgroups share the same u rows (L2 reuse)
+  const int fg = blockIdx.x % nfg;
+  const int pair0 = (blockIdx.x / nfg) * P;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int ka = fband ? min(kxa, fband[fg / M].x) : kxa;
+  for (int j = 0; j < nd; ++j) {
+    const cpx<R>* gf = grad + (int64_t)(fg * nd + j) * gsf;
+    for (int e = tid; e < L * P; e += nth) {
+      const int k = e % L, p = e / L;
+      const int r0 = 2 * (pair0 + p), r1 = r0 + 1;
+      const int kk = k <= L / 2 ? k : L - k;
+      cpx<R> A = mk<R>((R)0, (R)0), B = A;
+      if (kk <= ka) {
+        if (r0 < nrows) A = gf[kk + (r0 % G2) * gsy + (r0 / G2) * gsz];
+        if (r1 < nrows) B = gf[kk + (r1 % G2) * gsy + (r1 / G2) * gsz];
+      }
+      if (k == 0 || 2 * k == L) { A.y = (R)0; B.y = (R)0; }
+      if (k > L / 2) { A.y = -A.y; B.y = -B.y; }
+      a[k * P + p] = mk<R>(A.x - B.y, A.y + B.x);
+    }
+    __syncthreads();
+    cpx<R>* res = lds_fft<1, R>(a, b, d, P, tw, tid, nth);
+    const R* uj = u + (int64_t)j * usf;
+    for (int e = tid; e < L * P; e += nth) {
+      const int x = e % L, p = e / L;
+      const int r0 = 2 * (pair0 + p), r1 = r0 + 1;
+      const cpx<R> z = res[x * P + p];
+      const R u0 = r0 < nrows ? uj[x + (r0 % G2) * usy + (r0 / G2) * usz] : (R)0;
+      const R u1 = r1 < nrows ? uj[x + (r1 % G2) * usy + (r1 / G2) * usz] : (R)0;
+      const cpx<R> t = mk<R>(u0 * (z.x * gscale), u1 * (z.y * gscale));
+      acc[x * P + p] = j == 0 ? t : cadd(acc[x * P + p], t);
+    }
+    __syncthreads();
+  }
+  cpx<R>* res = lds_fft<-1, R>(acc, a, d, P, tw, tid, nth);
+  cpx<R>* of = out + (int64_t)fg * osf;
+  const int nk = kxb + 1;
+  for (int e = tid; e < nk * P; e += nth) {
+    const int k = e % nk, p = e / nk;
+    const int r0 = 2 * (pair0 + p), r1 = r0 + 1;
+    const cpx<R> Zk = res[k * P + p];
+    const cpx<R> Zm = res[((L - k) % L) * P + p];
+    const cpx<R> A = mk<R>((R)0.5 * (Zk.x + Zm.x), (R)0.5 * (Zk.y - Zm.y));
+    const cpx<R> B = mk<R>((R)0.5 * (Zk.y + Zm.y), (R)-0.5 * (Zk.x - Zm.x));
+    if (r0 < nrows) of[k + (r0 % G2) * osy + (r0 / G2) * osz] = A;
+    if (r1 < nrows) of[k + (r1 % G2) * osy + (r1 / G2) * osz] = B;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fused last forward pass + transfer density + shell sums (non-padded plans, where N̂ is
+// Hermitian): for the M component lines of giver g at TX adjacent columns, N̂_c = scale·FFT(line)
+// on the output band, t_pair = w(kx)·Re Σ_c conj(eh_c) N̂_c with eh the Hermitian part of the
+// masked effective carrier (the pair k, -k at once: w = 2 off the kx = 0, n1/2 planes), and
+// FP64 LDS shell bins -> one partial row per workgroup.  N̂ never touches HBM.
+// ---------------------------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256) k_lastdens(FftDesc d, const cpx<R>* __restrict__ tw,
+                                                  const cpx<R>* __restrict__ src, int64_t sy, int64_t sz,
+                                                  int64_t sf, int nx, int TX, int axis, Band ob, Band wb, int M,
+                                                  R scale, const cpx<R>* __restrict__ eh, int64_t ehsy,
+                                                  int64_t ehsz, int64_t ehsf, const int* __restrict__ shh,
+                                                  int nsh, int n1, double* __restrict__ partials) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int L = d.L;
+  cpx<R>* a = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<R>* b = a + (size_t)L * TX;
+  double* bins = reinterpret_cast<double*>(b + (size_t)L * TX);
+  const int x0 = blockIdx.x * TX;
+  const int o = ob.at(blockIdx.y);
+  const int g = blockIdx.z;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  for (int s = tid; s < nsh; s += nth) bins[s] = 0.0;
+  constexpr int QMAX = 32;  // TX*L <= QMAX*256 per launch
+  R tq[QMAX];
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) tq[q] = (R)0;
+  for (int c = 0; c < M; ++c) {
+    const cpx<R>* fsrc = src + (int64_t)(g * M + c) * sf;
+    for (int e = tid; e < L * TX; e += nth) {
+      const int xl = e % TX, i = e / TX, x = x0 + xl;
+      cpx<R> v = mk<R>((R)0, (R)0);
+      if (x < nx) v = axis == 1 ? fsrc[x + i * sy + o * sz] : fsrc[x + o * sy + i * sz];
+      a[i * TX + xl] = v;
+    }
+    __syncthreads();
+    cpx<R>* res = lds_fft<-1, R>(a, b, d, TX, tw, tid, nth);
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) {
+      const int e = tid + q * nth;
+      if (e < L * TX) {
+        const int xl = e % TX, i = e / TX, x = x0 + xl;
+        if (x < nx && wb.has(i)) {
+          const cpx<R> nv = cscale(res[i * TX + xl], scale);
+          const int64_t hi = axis == 1 ? (x + i * ehsy + o * ehsz) : (x + o * ehsy + i * ehsz);
+          const cpx<R> ev = eh[hi + c * ehsf];
+          tq[q] += ev.x * nv.x + ev.y * nv.y;
+        }
+      }
+    }
+    __syncthreads();  // a / b are refilled by the next component
+  }
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) {
+    const int e = tid + q * nth;
+    if (e < L * TX) {
+      const int xl = e % TX, i = e / TX, x = x0 + xl;
+      if (x < nx && wb.has(i)) {
+        const int64_t hi = axis == 1 ? (x + i * ehsy + o * ehsz) : (x + o * ehsy + i * ehsz);
+        const int sh = shh[hi];
+        if (sh > 0) {
+          const double w = (x == 0 || 2 * x == n1) ? 1.0 : 2.0;
+          atomicAdd(&bins[sh - 1], w * (double)tq[q]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int64_t blk = (int64_t)g * gridDim.x * gridDim.y + (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+  for (int s = tid; s < nsh; s += nth) partials[blk * nsh + s] = bins[s];
+}
+
+// Hermitian part of the masked effective carrier on the original half grid (x <= n1/2), only on
+// the output band: eh_c(k) = keep(k)·(e_c(k) + conj e_c(-k))/2, e from Invariants.jl:40-104 as
+// t = Re Σ_c conj(e_c) N̂_c (KE/scalar: ĉ; helicity: i k×û; enstrophy: i k×(i k×û), 2D form).
+template <typename R>
+__global__ void k_carrier_half(Grid g, int inv, int M, int trunc, const cpx<R>* __restrict__ f, int64_t fsf,
+                               const R* __restrict__ ks0, const R* __restrict__ ks1, const R* __restrict__ ks2,
+                               Band by, Band bz, int xb, cpx<R>* __restrict__ eh, int64_t ehsf) {
+  const int n1h = g.n1 / 2 + 1;
+  const int64_t Hn = (int64_t)n1h * g.n2 * g.n3;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t hidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; hidx < Hn; hidx += stride) {
+    const int ix = (int)(hidx % n1h), iy = (int)((hidx / n1h) % g.n2), iz = (int)(hidx / ((int64_t)n1h * g.n2));
+    if (ix > xb || !by.has(iy) || !bz.has(iz)) continue;
+    const int mx = ix == 0 ? 0 : g.n1 - ix, my = iy == 0 ? 0 : g.n2 - iy, mz = iz == 0 ? 0 : g.n3 - iz;
+    const bool keep = !(trunc && g.dealiased(ix, iy, iz));
+    cpx<R> ep[3], em[3];
+    const int64_t op = g.oidx(ix, iy, iz), om = g.oidx(mx, my, mz);
+    for (int side = 0; side < 2; ++side) {
+      const int64_t o = side == 0 ? op : om;
+      const int jx = side == 0 ? ix : mx, jy = side == 0 ? iy : my, jz = side == 0 ? iz : mz;
+      cpx<R>* e = side == 0 ? ep : em;
+      const R kx = ks0[jx], ky = g.nd >= 2 ? ks1[jy] : (R)0, kz = g.nd >= 3 ? ks2[jz] : (R)0;
+      if (inv == 0 || inv == 3) {
+        for (int c = 0; c < M && c < 3; ++c) e[c] = f[o + c * fsf];
+      } else if (inv == 1 || g.nd == 3) {
+        const cpx<R> ux = f[o], uy = f[o + fsf], uz = f[o + 2 * fsf];
+        cpx<R> wx = cmuli<1>(csub(cscale(uz, ky), cscale(uy, kz)));
+        cpx<R> wy = cmuli<1>(csub(cscale(ux, kz), cscale(uz, kx)));
+        cpx<R> wz = cmuli<1>(csub(cscale(uy, kx), cscale(ux, ky)));
+        if (inv == 1) {
+          e[0] = wx; e[1] = wy; e[2] = wz;
+        } else {
+          e[0] = cmuli<1>(csub(cscale(wz, ky), cscale(wy, kz)));
+          e[1] = cmuli<1>(csub(cscale(wx, kz), cscale(wz, kx)));
+          e[2] = cmuli<1>(csub(cscale(wy, kx), cscale(wx, ky)));
+        }
+      } else {
+        const cpx<R> ux = f[o], uy = f[o + fsf];
+        const cpx<R> w = cmuli<1>(csub(cscale(uy, kx), cscale(ux, ky)));
+        e[0] = cmuli<1>(cscale(w, ky));
+        e[1] = cmuli<-1>(cscale(w, kx));
+      }
+    }
+    for (int c = 0; c < M && c < 3; ++c) {
+      cpx<R> v = mk<R>((R)0.5 * (ep[c].x + em[c].x), (R)0.5 * (ep[c].y - em[c].y));
+      if (!keep) v = mk<R>((R)0, (R)0);
+      eh[hidx + c * ehsf] = v;
+    }
+  }
+}
+
+// out[g*nsh + s] = Σ_b partials[(g*nblk + b)*nsh + s] in increasing b (deterministic FP64)
+__global__ void k_reduce_batched(const double* __restrict__ partials, int nblk, int nsh, int ng,
+                                 double* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.y;
+  if (s >= nsh || g >= ng) return;
+  double acc = 0.0;
+  const double* p = partials + (int64_t)g * nblk * nsh;
+  for (int b = 0; b < nblk; ++b) acc += p[(int64_t)b * nsh + s];
+  out[(int64_t)g * nsh + s] = acc;
 }
 
 // ---------------------------------------------------------------------------------------------

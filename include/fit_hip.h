@@ -67,6 +67,13 @@ int fit_plan_destroy(fit_plan* plan);
 /* Device bytes the plan holds. */
 int64_t fit_plan_device_bytes(const fit_plan* plan);
 
+/* Plan options (no reference counterpart: execution knobs of this backend).
+ *   FIT_OPT_PLAIN_PIPELINE = 1: value 1 runs the plain dataflow (unfused, unpruned passes, one
+ *     giver at a time, N̂ through HBM) -- the comparison baseline; 0 (default) the fused one.
+ *   FIT_OPT_GIVER_BATCH = 2: cap on giver shells per batched launch (0 = device memory budget). */
+enum { FIT_OPT_PLAIN_PIPELINE = 1, FIT_OPT_GIVER_BATCH = 2 };
+int fit_plan_set_option(fit_plan* plan, int option, int value);
+
 /*
  * N̂ = keep·FFT[(u_adv·∇) f]/Np into N_hat (ns..., M).  Replaces
  * NonlinearTerm._compute_nonlinear_term! for the FFT path (NonlinearTerm.jl:102-116) i.e.

@@ -264,3 +264,45 @@ def test_device_arrays_roundtrip(fit, oracle):
     r = fit.calculate_spectral_flux(ud.to(torch.complex64), ks, binning=fit.LinearBinning(1.0))
     _, T, _ = oracle.calculate_spectral_flux(u, ks, oracle.LinearBinning(1.0), "23")
     assert rel(r.transfer_spectrum, T) < 1e-5
+
+
+@pytest.mark.parametrize("ns,dea,inv,batch", [((16, 16, 16), "23", "ke", 0), ((16, 16, 16), "23", "helicity", 3),
+                                              ((24, 20), "none", "enstrophy", 2), ((12, 12, 12), "none", "ke", 5)])
+def test_fused_pipeline_equals_plain(fit, oracle, ns, dea, inv, batch):
+    """The fused / pruned / batched giver loop against the plain one (same plan, FP64) and the
+    oracle, with a giver-batch cap that splits the givers into several launches."""
+    say("case fused vs plain", ns, dea, inv, batch)
+    nd = len(ns)
+    ks = ks_of(oracle, ns)
+    u = oracle.synthetic_field(ns, nd, seed=12)
+    deal = {"none": fit.NoDealiasing(), "23": fit.OrszagTwoThirds()}[dea]
+    I = {"ke": fit.KineticEnergy(), "helicity": fit.Helicity(), "enstrophy": fit.Enstrophy()}[inv]
+    from fit_amd.grid import shell_setup
+    c, e, sidx = shell_setup(ks, fit.CustomBinning(range(0, 7)), None)
+    with fit.Plan(ns, nd, nd, ks, deal, np.float64, sidx, len(c)) as plan:
+        plan.set_giver_batch(batch)
+        Tf, netf, _ = plan.shell_to_shell(I, u)
+        plan.set_plain(True)
+        Tp, netp, _ = plan.shell_to_shell(I, u)
+    _, _, To, _, _ = oracle.calculate_shell_to_shell_transfer(u, ks, oracle.CustomBinning(tuple(range(0, 7))), dea,
+                                                               invariant=inv)
+    assert np.max(np.abs(To)) > 1e-12
+    e1, e2 = rel(Tf, To), rel(Tp, To)
+    say("  fused err", e1, "plain err", e2)
+    assert e1 < TOL64 and e2 < TOL64
+
+
+@pytest.mark.parametrize("ns,dea", [((16, 16, 16), "23"), ((12, 10, 8), "pad"), ((20, 24), "23")])
+def test_fused_nonlinear_equals_plain(fit, oracle, ns, dea):
+    say("case fused NL vs plain", ns, dea)
+    nd = len(ns)
+    ks = ks_of(oracle, ns)
+    rng = np.random.default_rng(2)
+    u = rand_c(rng, ns + (nd,))
+    deal = {"none": fit.NoDealiasing(), "23": fit.OrszagTwoThirds(), "pad": fit.PaddedThreeHalves()}[dea]
+    with fit.Plan(ns, nd, nd, ks, deal, np.float64) as plan:
+        Nf = plan.nonlinear_term(u)
+        plan.set_plain(True)
+        Np_ = plan.nonlinear_term(u)
+    ref = oracle.compute_nonlinear_term(u, ks, dea)
+    assert rel(Nf, ref) < TOL64 and rel(Np_, ref) < TOL64
