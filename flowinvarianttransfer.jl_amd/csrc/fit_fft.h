@@ -174,3 +174,87 @@ __device__ __forceinline__ bool pad_pre(int i, int n, int G, int& o) {
 }
 
 }  // namespace fit
+
+namespace fit {
+
+// ---------------------------------------------------------------------------------------------
+// Register-resident Stockham FFT for compile-time lengths L = 2^a (E = 8 points per thread) or
+// 3·2^a (E = 24).  A line is held by T = L/E threads; thread t owns points t + T·e (e < E) before
+// and after every stage, so only the Stockham output scatter goes through LDS (double-buffered:
+// one barrier per stage).  Radix plan: 8s, then 4 / 2, then 3.
+// ---------------------------------------------------------------------------------------------
+constexpr int rp_nst(int L) {
+  int n = 0, r = L;
+  while (r % 8 == 0) { r /= 8; ++n; }
+  while (r % 4 == 0) { r /= 4; ++n; }
+  while (r % 2 == 0) { r /= 2; ++n; }
+  while (r % 3 == 0) { r /= 3; ++n; }
+  return n;
+}
+constexpr int rp_rad(int L, int s) {
+  int n = 0, r = L;
+  while (r % 8 == 0) { if (n == s) return 8; r /= 8; ++n; }
+  while (r % 4 == 0) { if (n == s) return 4; r /= 4; ++n; }
+  while (r % 2 == 0) { if (n == s) return 2; r /= 2; ++n; }
+  while (r % 3 == 0) { if (n == s) return 3; r /= 3; ++n; }
+  return 1;
+}
+constexpr bool rp_pow2(int L) { return L > 0 && (L & (L - 1)) == 0; }
+constexpr int rp_E(int L) { return rp_pow2(L) ? 8 : 24; }
+
+template <int L, int E, int S, int NS, int SIGN, bool DB, typename R, class IDX>
+__device__ __forceinline__ void reg_stage(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* lds1, const IDX& idx, int t,
+                                          const cpx<R>* __restrict__ tw) {
+  if constexpr (S < rp_nst(L)) {
+    constexpr int r = rp_rad(L, S);
+    constexpr int T = L / E;
+    constexpr int NB = E / r;
+    static_assert(E % r == 0, "radix must divide the points per thread");
+    cpx<R>* lds = (DB && (S & 1)) ? lds1 : lds0;
+    if constexpr (!DB) __syncthreads();   // single buffer: previous readers are done
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int j = t + i * T;
+      const int m = j % NS;
+      cpx<R> w[r];
+#pragma unroll
+      for (int k = 0; k < r; ++k) w[k] = v[i + k * NB];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int k = 1; k < r; ++k) w[k] = cmul(w[k], twiddle<SIGN, R>(tw, k * m * (L / (NS * r))));
+      }
+      dft<r, SIGN, R>(w);
+      const int o = (j - m) * r + m;
+#pragma unroll
+      for (int k = 0; k < r; ++k) lds[idx(o + k * NS)] = w[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = lds[idx(t + T * e)];
+    reg_stage<L, E, S + 1, NS * r, SIGN, DB, R, IDX>(v, lds0, lds1, idx, t, tw);
+  }
+}
+
+// v[e] = x[t + T e] in, X[t + T e] out (natural order).  DB: lds0 / lds1 are two exchange
+// buffers (one barrier per stage); otherwise lds0 is used alone (two barriers per stage).  The
+// caller must place a barrier between two calls that share the buffers.
+template <int L, int SIGN, bool DB, typename R, class IDX>
+__device__ __forceinline__ void reg_fft(cpx<R> (&v)[rp_E(L)], cpx<R>* lds0, cpx<R>* lds1, const IDX& idx, int t,
+                                        const cpx<R>* __restrict__ tw) {
+  reg_stage<L, rp_E(L), 0, 1, SIGN, DB, R, IDX>(v, lds0, lds1, idx, t, tw);
+}
+
+// line-major LDS index with one pad slot every 16 points (breaks power-of-two strides)
+struct IdxLine {
+  int base;
+  __device__ __forceinline__ int operator()(int pos) const { return base + pos + (pos >> 4); }
+};
+// column tiles: point-major, TX lines interleaved (x fastest in both HBM and LDS)
+struct IdxCol {
+  int x, TX;
+  __device__ __forceinline__ int operator()(int pos) const { return pos * TX + x; }
+};
+
+constexpr int ldsline(int L) { return L + L / 16; }
+
+}  // namespace fit

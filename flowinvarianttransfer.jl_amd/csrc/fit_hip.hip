@@ -187,22 +187,54 @@ constexpr int kThreads = 256;
 
 Band full_band(int G) { return Band{G, G}; }
 
+// lengths with a compile-time register FFT (others use the generic LDS engine)
+#define FIT_REG_LENGTHS(X) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048) X(3072)
+constexpr int reg_T(int L) { return L / (((L & (L - 1)) == 0) ? 8 : 24); }
+bool reg_supported(int L) {
+  switch (L) {
+#define FIT_CASE(LL) case LL:
+    FIT_REG_LENGTHS(FIT_CASE)
+#undef FIT_CASE
+    return true;
+    default: return false;
+  }
+}
+size_t g_max_lds = 65536;  // per-workgroup LDS the launches may use
+
 // Column pass over fields [0, nfields): x < nx, other-index band ob, input band ib, output band wb.
 template <typename R, int SIGN, class LD, class ST>
 int launch_col(fit_plan* p, LD ld, ST st, int axis, int nfields, int nx, Band ob, Band ib, Band wb,
                const int4* fband = nullptr, int fdiv = 1) {
   const int L = p->G[axis];
   const size_t csz = sizeof(cpx<R>);
-  const int TX = tile_for(L, csz, 16);
   const int nob = ob.count();
   if (nx <= 0 || nob <= 0 || nfields <= 0) return FIT_OK;
-  dim3 grid((nx + TX - 1) / TX, nob, nfields);
-  const size_t lds = (size_t)2 * L * TX * csz;
   const double rows_in = ib.count(), rows_out = wb.count();
+  const int gother = axis == 2 ? p->G[1] : p->G[2];
   prof_begin(p, FIT_K_FFT_COLUMN, (double)nx * nob * (rows_in + rows_out) * csz * nfields);
-  hipLaunchKernelGGL((k_col<R, SIGN, LD, ST>), grid, dim3(kThreads), lds, p->stream, ld, st, p->fd[axis],
-                     (const cpx<R>*)p->tw[axis], nx, TX, axis, ob, ib, wb, fband, fdiv,
-                     axis == 2 ? p->G[1] : p->G[2]);
+  if (!p->plain && reg_supported(L)) {
+    const int T = reg_T(L);
+    int TX = std::min(16, 1024 / T);
+    while (TX > 1 && (size_t)L * TX * csz > g_max_lds) TX /= 2;
+    dim3 grid((nx + TX - 1) / TX, nob, nfields);
+    const size_t lds = (size_t)L * TX * csz;
+    switch (L) {
+#define FIT_CASE(LL)                                                                                       \
+  case LL:                                                                                                 \
+    hipLaunchKernelGGL((k_col_r<R, LL, SIGN, LD, ST>), grid, dim3(reg_T(LL) * TX), lds, p->stream, ld, st, \
+                       (const cpx<R>*)p->tw[axis], nx, TX, axis, ob, ib, wb, fband, fdiv, gother);           \
+    break;
+      FIT_REG_LENGTHS(FIT_CASE)
+#undef FIT_CASE
+      default: break;
+    }
+  } else {
+    const int TX = tile_for(L, csz, 16);
+    dim3 grid((nx + TX - 1) / TX, nob, nfields);
+    const size_t lds = (size_t)2 * L * TX * csz;
+    hipLaunchKernelGGL((k_col<R, SIGN, LD, ST>), grid, dim3(kThreads), lds, p->stream, ld, st, p->fd[axis],
+                       (const cpx<R>*)p->tw[axis], nx, TX, axis, ob, ib, wb, fband, fdiv, gother);
+  }
   prof_end(p);
   FIT_HIP(hipGetLastError());
   return FIT_OK;
@@ -403,20 +435,41 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
   int rc = grow(p, &p->nxbuf, &p->nxbuf_bytes, (size_t)nfg * xb * p->G[1] * p->G[2] * csz);
   if (rc) return rc;
   const int L = p->G[0];
-  int P = 8;
-  while (P > 1 && (size_t)3 * L * P * csz > 65536) --P;
   const int nrows = p->G[1] * p->G[2];
   const int npairs = (nrows + 1) / 2;
-  const int nrg = (npairs + P - 1) / P;
   const int64_t gsy = xa, gsz = (int64_t)xa * p->G[1], gsf = gsz * p->G[2];
   const int64_t osy = xb, osz = (int64_t)xb * p->G[1], osf = osz * p->G[2];
   const int64_t usy = L, usz = (int64_t)L * p->G[1];
   const double bytes = (double)nfg * nrows * (p->nd * (xa * csz) + xb * csz) + (double)p->nd * nrows * L * sizeof(R);
   prof_begin(p, FIT_K_FFT_ROW_INVERSE, bytes);
-  hipLaunchKernelGGL(k_xfused<R>, dim3((unsigned)((int64_t)nrg * nfg)), dim3(kThreads), (size_t)3 * L * P * csz,
-                     p->stream, p->fd[0], (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf,
-                     (const R*)p->uphys, usy, usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M, p->G[1],
-                     nrows, P, (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, nfg);
+  if (reg_supported(L)) {
+    const int T = reg_T(L);
+    const size_t LL = (size_t)L + L / 16;
+    int B = std::max(1, 256 / T);
+    while (B > 1 && 2 * B * LL * csz > g_max_lds) --B;
+    const int nrg = (npairs + B - 1) / B;
+    const size_t lds = 2 * B * LL * csz;
+    switch (L) {
+#define FIT_CASE(LLEN)                                                                                         \
+  case LLEN:                                                                                                   \
+    hipLaunchKernelGGL((k_xfused_r<R, LLEN>), dim3((unsigned)((int64_t)nrg * nfg)), dim3(B * reg_T(LLEN)), lds, \
+                       p->stream, (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf,             \
+                       (const R*)p->uphys, usy, usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M,      \
+                       p->G[1], nrows, B, (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, nfg);               \
+    break;
+      FIT_REG_LENGTHS(FIT_CASE)
+#undef FIT_CASE
+      default: break;
+    }
+  } else {
+    int P = 8;
+    while (P > 1 && (size_t)3 * L * P * csz > 65536) --P;
+    const int nrg = (npairs + P - 1) / P;
+    hipLaunchKernelGGL(k_xfused<R>, dim3((unsigned)((int64_t)nrg * nfg)), dim3(kThreads), (size_t)3 * L * P * csz,
+                       p->stream, p->fd[0], (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf,
+                       (const R*)p->uphys, usy, usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M, p->G[1],
+                       nrows, P, (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, nfg);
+  }
   prof_end(p);
   FIT_HIP(hipGetLastError());
   return FIT_OK;

@@ -360,6 +360,124 @@ __global__ void __launch_bounds__(256) k_xfused(FftDesc d, const cpx<R>* __restr
 }
 
 // ---------------------------------------------------------------------------------------------
+// Register-FFT variants (compile-time L) of the strided pass and of the fused x pass.
+// ---------------------------------------------------------------------------------------------
+
+// Column pass: thread (t, x) with x fastest; TX columns per workgroup; single LDS buffer.
+template <typename R, int L, int SIGN, class LD, class ST>
+__global__ void __launch_bounds__(1024) k_col_r(LD ld, ST st, const cpx<R>* __restrict__ tw, int nx, int TX,
+                                                int axis, Band ob, Band ib, Band wb,
+                                                const int4* __restrict__ fband, int fdiv, int Gother) {
+  constexpr int E = rp_E(L), T = L / E;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int xl = threadIdx.x % TX, t = threadIdx.x / TX;
+  const int x0 = blockIdx.x * TX;
+  const int o = ob.at(blockIdx.y);
+  const int f = blockIdx.z;
+  if (fband) {
+    const int4 fb = fband[f / fdiv];
+    if (x0 > fb.x) return;
+    if (axis == 2) {
+      if (!Band{fb.y, Gother}.has(o)) return;
+      ib = Band{fb.z, L};
+    } else {
+      ib = Band{fb.y, L};
+    }
+  }
+  const int x = x0 + xl;
+  cpx<R> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = t + T * e;
+    v[e] = mk<R>((R)0, (R)0);
+    if (x < nx && ib.has(i)) v[e] = axis == 1 ? ld(x, i, o, f) : ld(x, o, i, f);
+  }
+  reg_fft<L, SIGN, false, R>(v, lds, lds, IdxCol{xl, TX}, t, tw);
+  if (x < nx) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = t + T * e;
+      if (wb.has(i)) {
+        if (axis == 1) st(x, i, o, f, v[e]);
+        else st(x, o, i, f, v[e]);
+      }
+    }
+  }
+}
+
+// Fused x pass (see k_xfused) with the register FFT: B row pairs per workgroup, T threads each.
+template <typename R, int L>
+__global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw, const cpx<R>* __restrict__ grad,
+                                                  int64_t gsy, int64_t gsz, int64_t gsf, const R* __restrict__ u,
+                                                  int64_t usy, int64_t usz, int64_t usf, cpx<R>* __restrict__ out,
+                                                  int64_t osy, int64_t osz, int64_t osf, int nd, int M, int G2,
+                                                  int nrows, int B, R gscale, int kxa,
+                                                  const int4* __restrict__ fband, int kxb, int nfg) {
+  constexpr int E = rp_E(L), T = L / E, LL = ldsline(L);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* lds0 = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<R>* lds1 = lds0 + B * LL;
+  const int line = threadIdx.x / T, t = threadIdx.x % T;
+  const int fg = blockIdx.x % nfg;
+  const int pair = (blockIdx.x / nfg) * B + line;
+  const int r0 = 2 * pair, r1 = r0 + 1;
+  const bool ok0 = r0 < nrows, ok1 = r1 < nrows;
+  const int64_t ro0 = ok0 ? (int64_t)(r0 % G2) * gsy + (int64_t)(r0 / G2) * gsz : 0;
+  const int64_t ro1 = ok1 ? (int64_t)(r1 % G2) * gsy + (int64_t)(r1 / G2) * gsz : 0;
+  const int64_t uo0 = ok0 ? (int64_t)(r0 % G2) * usy + (int64_t)(r0 / G2) * usz : 0;
+  const int64_t uo1 = ok1 ? (int64_t)(r1 % G2) * usy + (int64_t)(r1 / G2) * usz : 0;
+  const IdxLine idx{line * LL};
+  const int ka = fband ? min(kxa, fband[fg / M].x) : kxa;
+  cpx<R> acc[E];
+  for (int j = 0; j < nd; ++j) {
+    const cpx<R>* gf = grad + (int64_t)(fg * nd + j) * gsf;
+    cpx<R> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = t + T * e;
+      const int kk = k <= L / 2 ? k : L - k;
+      cpx<R> A = mk<R>((R)0, (R)0), Bv = A;
+      if (kk <= ka) {
+        if (ok0) A = gf[kk + ro0];
+        if (ok1) Bv = gf[kk + ro1];
+      }
+      if (k == 0 || 2 * k == L) { A.y = (R)0; Bv.y = (R)0; }
+      if (k > L / 2) { A.y = -A.y; Bv.y = -Bv.y; }
+      v[e] = mk<R>(A.x - Bv.y, A.y + Bv.x);
+    }
+    __syncthreads();
+    reg_fft<L, 1, true, R>(v, lds0, lds1, idx, t, tw);
+    const R* uj = u + (int64_t)j * usf;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int xx = t + T * e;
+      const R u0 = ok0 ? uj[xx + uo0] : (R)0;
+      const R u1 = ok1 ? uj[xx + uo1] : (R)0;
+      const cpx<R> tt = mk<R>(u0 * (v[e].x * gscale), u1 * (v[e].y * gscale));
+      acc[e] = j == 0 ? tt : cadd(acc[e], tt);
+    }
+  }
+  __syncthreads();
+  reg_fft<L, -1, true, R>(acc, lds0, lds1, idx, t, tw);
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; ++e) lds0[idx(t + T * e)] = acc[e];
+  __syncthreads();
+  cpx<R>* of = out + (int64_t)fg * osf;
+  const int64_t oo0 = ok0 ? (int64_t)(r0 % G2) * osy + (int64_t)(r0 / G2) * osz : 0;
+  const int64_t oo1 = ok1 ? (int64_t)(r1 % G2) * osy + (int64_t)(r1 / G2) * osz : 0;
+  for (int k = t; k <= kxb; k += T) {
+    const cpx<R> Zk = lds0[idx(k)];
+    const cpx<R> Zm = lds0[idx((L - k) % L)];
+    const cpx<R> A = mk<R>((R)0.5 * (Zk.x + Zm.x), (R)0.5 * (Zk.y - Zm.y));
+    const cpx<R> Bv = mk<R>((R)0.5 * (Zk.y + Zm.y), (R)-0.5 * (Zk.x - Zm.x));
+    if (ok0) of[k + oo0] = A;
+    if (ok1) of[k + oo1] = Bv;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Fused last forward pass + transfer density + shell sums (non-padded plans, where N̂ is
 // Hermitian): for the M component lines of giver g at TX adjacent columns, N̂_c = scale·FFT(line)
 // on the output band, t_pair = w(kx)·Re Σ_c conj(eh_c) N̂_c with eh the Hermitian part of the
