@@ -258,3 +258,62 @@ struct IdxCol {
 constexpr int ldsline(int L) { return L + L / 16; }
 
 }  // namespace fit
+
+namespace fit {
+
+// ---------------------------------------------------------------------------------------------
+// Wave-private register FFT: a line of length L is held by T = L/E <= 64 lanes of ONE wavefront,
+// so the Stockham exchanges need no workgroup barrier -- only that the wave's own LDS writes
+// have completed (s_waitcnt lgkmcnt(0)) before its lanes read them, plus a code-motion fence.
+// ---------------------------------------------------------------------------------------------
+constexpr int wE(int L) {
+  return L == 256 ? 8 : L == 384 ? 24 : L == 512 ? 8 : L == 768 ? 24 : L == 1024 ? 16
+       : L == 1536 ? 24 : L == 2048 ? 32 : L == 3072 ? 48 : 0;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): this wave's LDS accesses are complete
+  __builtin_amdgcn_wave_barrier();      // no code motion across this point
+}
+
+template <int L, int E, int S, int NS, int SIGN, typename R, class IDX>
+__device__ __forceinline__ void wave_stage(cpx<R> (&v)[E], cpx<R>* lds, const IDX& idx, int t,
+                                           const cpx<R>* __restrict__ tw) {
+  if constexpr (S < rp_nst(L)) {
+    constexpr int r = rp_rad(L, S);
+    constexpr int T = L / E;
+    constexpr int NB = E / r;
+    static_assert(E % r == 0, "radix must divide the points per lane");
+    wave_lds_sync();  // reads of the previous stage are complete before overwriting
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int j = t + i * T;
+      const int m = j % NS;
+      cpx<R> w[r];
+#pragma unroll
+      for (int k = 0; k < r; ++k) w[k] = v[i + k * NB];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int k = 1; k < r; ++k) w[k] = cmul(w[k], twiddle<SIGN, R>(tw, k * m * (L / (NS * r))));
+      }
+      dft<r, SIGN, R>(w);
+      const int o = (j - m) * r + m;
+#pragma unroll
+      for (int k = 0; k < r; ++k) lds[idx(o + k * NS)] = w[k];
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = lds[idx(t + T * e)];
+    wave_stage<L, E, S + 1, NS * r, SIGN, R, IDX>(v, lds, idx, t, tw);
+  }
+}
+
+// v[e] = x[t + T e] in, X[t + T e] out; lds: this line's private region (wave-owned).
+template <int L, int SIGN, typename R, class IDX>
+__device__ __forceinline__ void wave_fft(cpx<R> (&v)[wE(L)], cpx<R>* lds, const IDX& idx, int t,
+                                         const cpx<R>* __restrict__ tw) {
+  wave_stage<L, wE(L), 0, 1, SIGN, R, IDX>(v, lds, idx, t, tw);
+  wave_lds_sync();
+}
+
+}  // namespace fit

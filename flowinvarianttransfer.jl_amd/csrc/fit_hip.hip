@@ -103,6 +103,8 @@ struct fit_plan {
   double* dcolG = nullptr; size_t dcolG_bytes = 0;
   size_t eh_bytes = 0;
   int giver_batch = 0;           // cap on givers per batch (0 = memory budget only)
+  int xfused_gc = 0;             // givers per workgroup in the wave-per-line x pass (0 = 4)
+  int xfused_block = 0;          // 1: use the workgroup-barrier x pass instead
   // profiling
   bool prof = false;
   std::vector<ProfRec> recs;
@@ -442,25 +444,33 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
   const int64_t usy = L, usz = (int64_t)L * p->G[1];
   const double bytes = (double)nfg * nrows * (p->nd * (xa * csz) + xb * csz) + (double)p->nd * nrows * L * sizeof(R);
   prof_begin(p, FIT_K_FFT_ROW_INVERSE, bytes);
-  if (reg_supported(L)) {
-    const int T = reg_T(L);
+  const bool wave_ok = reg_supported(L) && !p->xfused_block &&
+                       (size_t)4 * (64 / (L / wE(L))) * ((size_t)L + L / 16) * csz <= g_max_lds;
+  if (wave_ok) {
+    // wave-per-line kernel: 4 waves per workgroup, 64/T row pairs per wave
+    const int T = L / wE(L);
+    const int pairs_per_block = 4 * (64 / T);
     const size_t LL = (size_t)L + L / 16;
-    int B = std::max(1, 256 / T);
-    while (B > 1 && 2 * B * LL * csz > g_max_lds) --B;
-    const int nrg = (npairs + B - 1) / B;
-    const size_t lds = 2 * B * LL * csz;
+    const size_t lds = (size_t)pairs_per_block * LL * csz;
+    const int nrg = (npairs + pairs_per_block - 1) / pairs_per_block;
+    const int ngiv = nfg / p->M;
+    const int GC = std::max(1, std::min(ngiv, p->xfused_gc > 0 ? p->xfused_gc : 4));
+    const int nchunk = (ngiv + GC - 1) / GC;
     switch (L) {
-#define FIT_CASE(LLEN)                                                                                         \
-  case LLEN:                                                                                                   \
-    hipLaunchKernelGGL((k_xfused_r<R, LLEN>), dim3((unsigned)((int64_t)nrg * nfg)), dim3(B * reg_T(LLEN)), lds, \
-                       p->stream, (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf,             \
-                       (const R*)p->uphys, usy, usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M,      \
-                       p->G[1], nrows, B, (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, nfg);               \
+#define FIT_CASE(LLEN)                                                                                          \
+  case LLEN:                                                                                                    \
+    hipLaunchKernelGGL((k_xfused_w<R, LLEN>), dim3((unsigned)((int64_t)nrg * nchunk)), dim3(256), lds, p->stream, \
+                       (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf, (const R*)p->uphys, usy,    \
+                       usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M, p->G[1], nrows,              \
+                       (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, ngiv, GC);                              \
     break;
       FIT_REG_LENGTHS(FIT_CASE)
 #undef FIT_CASE
       default: break;
     }
+  } else if (reg_supported(L)) {
+    const int T = reg_T(L);
+    const size_t LL = (size_t)L + L / 16;
   } else {
     int P = 8;
     while (P > 1 && (size_t)3 * L * P * csz > 65536) --P;
@@ -1033,6 +1043,11 @@ int fit_plan_set_option(fit_plan* p, int option, int value) {
       if (value < 0) return fail(FIT_E_ARGUMENT, "giver batch must be >= 0");
       p->giver_batch = value;
       return FIT_OK;
+    case FIT_OPT_XPASS_GIVERS:
+      if (value < 0) return fail(FIT_E_ARGUMENT, "givers per workgroup must be >= 0");
+      p->xfused_gc = value;
+      return FIT_OK;
+    case FIT_OPT_XPASS_BLOCK: p->xfused_block = value != 0; return FIT_OK;
     default: return fail(FIT_E_ARGUMENT, "unknown plan option");
   }
 }

@@ -477,6 +477,102 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
   }
 }
 
+// Fused x pass, wave-per-line form: each wavefront owns 64/T row pairs; a workgroup (4 waves)
+// loads its u rows once and then runs GC givers × M components through
+//   N = Σ_j u_j · c2r(grad_{g,c,j}),  r2c(N) -> half rows kx <= kxb
+// with no workgroup barrier.  Workgroups with the same rows and different giver chunks are
+// adjacent in the launch order (giver chunk fastest) so the u rows are shared through L2.
+template <typename R, int L>
+__global__ void __launch_bounds__(256) k_xfused_w(const cpx<R>* __restrict__ tw, const cpx<R>* __restrict__ grad,
+                                                  int64_t gsy, int64_t gsz, int64_t gsf, const R* __restrict__ u,
+                                                  int64_t usy, int64_t usz, int64_t usf, cpx<R>* __restrict__ out,
+                                                  int64_t osy, int64_t osz, int64_t osf, int nd, int M, int G2,
+                                                  int nrows, R gscale, int kxa, const int4* __restrict__ fband,
+                                                  int kxb, int ngiv, int GC) {
+  constexpr int E = wE(L), T = L / E, LPW = 64 / T, LL = ldsline(L);
+  constexpr bool UREG = E <= 16;                 // keep u in registers for short lines
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* ldsb = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int t = lane % T;
+  const int line = wave * LPW + lane / T;
+  const int nchunk = (ngiv + GC - 1) / GC;
+  const int chunk = blockIdx.x % nchunk;
+  const int pair = (blockIdx.x / nchunk) * (4 * LPW) + line;
+  const int r0 = 2 * pair, r1 = r0 + 1;
+  const bool ok0 = r0 < nrows, ok1 = r1 < nrows;
+  const int y0 = ok0 ? r0 % G2 : 0, z0 = ok0 ? r0 / G2 : 0, y1 = ok1 ? r1 % G2 : 0, z1 = ok1 ? r1 / G2 : 0;
+  const int64_t ro0 = (int64_t)y0 * gsy + (int64_t)z0 * gsz, ro1 = (int64_t)y1 * gsy + (int64_t)z1 * gsz;
+  const int64_t uo0 = (int64_t)y0 * usy + (int64_t)z0 * usz, uo1 = (int64_t)y1 * usy + (int64_t)z1 * usz;
+  const int64_t oo0 = (int64_t)y0 * osy + (int64_t)z0 * osz, oo1 = (int64_t)y1 * osy + (int64_t)z1 * osz;
+  cpx<R>* lds = ldsb + line * LL;
+  const IdxLine idx{0};
+  cpx<R> ur[UREG ? 3 : 1][UREG ? E : 1];
+  if constexpr (UREG) {
+    for (int j = 0; j < nd; ++j) {
+      const R* uj = u + (int64_t)j * usf;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int xx = t + T * e;
+        ur[j][e] = mk<R>(ok0 ? uj[xx + uo0] * gscale : (R)0, ok1 ? uj[xx + uo1] * gscale : (R)0);
+      }
+    }
+  }
+  const int g0 = chunk * GC;
+  const int g1 = min(ngiv, g0 + GC);
+  for (int g = g0; g < g1; ++g) {
+    const int ka = fband ? min(kxa, fband[g].x) : kxa;
+    for (int c = 0; c < M; ++c) {
+      const int fg = g * M + c;
+      cpx<R> acc[E];
+      for (int j = 0; j < nd; ++j) {
+        const cpx<R>* gf = grad + (int64_t)(fg * nd + j) * gsf;
+        cpx<R> v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int k = t + T * e;
+          const int kk = k <= L / 2 ? k : L - k;
+          cpx<R> A = mk<R>((R)0, (R)0), Bv = A;
+          if (kk <= ka) {
+            if (ok0) A = gf[kk + ro0];
+            if (ok1) Bv = gf[kk + ro1];
+          }
+          if (k == 0 || 2 * k == L) { A.y = (R)0; Bv.y = (R)0; }
+          if (k > L / 2) { A.y = -A.y; Bv.y = -Bv.y; }
+          v[e] = mk<R>(A.x - Bv.y, A.y + Bv.x);
+        }
+        wave_fft<L, 1, R>(v, lds, idx, t, tw);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          cpx<R> uu;
+          if constexpr (UREG) {
+            uu = ur[j][e];
+          } else {
+            const R* uj = u + (int64_t)j * usf;
+            const int xx = t + T * e;
+            uu = mk<R>(ok0 ? uj[xx + uo0] * gscale : (R)0, ok1 ? uj[xx + uo1] * gscale : (R)0);
+          }
+          const cpx<R> tt = mk<R>(uu.x * v[e].x, uu.y * v[e].y);
+          acc[e] = j == 0 ? tt : cadd(acc[e], tt);
+        }
+      }
+      wave_fft<L, -1, R>(acc, lds, idx, t, tw);
+#pragma unroll
+      for (int e = 0; e < E; ++e) lds[idx(t + T * e)] = acc[e];
+      wave_lds_sync();
+      cpx<R>* of = out + (int64_t)fg * osf;
+      for (int k = t; k <= kxb; k += T) {
+        const cpx<R> Zk = lds[idx(k)];
+        const cpx<R> Zm = lds[idx((L - k) % L)];
+        const cpx<R> A = mk<R>((R)0.5 * (Zk.x + Zm.x), (R)0.5 * (Zk.y - Zm.y));
+        const cpx<R> Bv = mk<R>((R)0.5 * (Zk.y + Zm.y), (R)-0.5 * (Zk.x - Zm.x));
+        if (ok0) of[k + oo0] = A;
+        if (ok1) of[k + oo1] = Bv;
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Fused last forward pass + transfer density + shell sums (non-padded plans, where N̂ is
 // Hermitian): for the M component lines of giver g at TX adjacent columns, N̂_c = scale·FFT(line)

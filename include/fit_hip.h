@@ -70,8 +70,11 @@ int64_t fit_plan_device_bytes(const fit_plan* plan);
 /* Plan options (no reference counterpart: execution knobs of this backend).
  *   FIT_OPT_PLAIN_PIPELINE = 1: value 1 runs the plain dataflow (unfused, unpruned passes, one
  *     giver at a time, N̂ through HBM) -- the comparison baseline; 0 (default) the fused one.
- *   FIT_OPT_GIVER_BATCH = 2: cap on giver shells per batched launch (0 = device memory budget). */
-enum { FIT_OPT_PLAIN_PIPELINE = 1, FIT_OPT_GIVER_BATCH = 2 };
+ *   FIT_OPT_GIVER_BATCH = 2: cap on giver shells per batched launch (0 = device memory budget).
+ *   FIT_OPT_XPASS_GIVERS = 3: givers per workgroup in the fused contiguous-axis pass (0 = 4).
+ *   FIT_OPT_XPASS_BLOCK = 4: value 1 selects the workgroup-barrier form of that pass. */
+enum { FIT_OPT_PLAIN_PIPELINE = 1, FIT_OPT_GIVER_BATCH = 2, FIT_OPT_XPASS_GIVERS = 3,
+       FIT_OPT_XPASS_BLOCK = 4 };
 int fit_plan_set_option(fit_plan* plan, int option, int value);
 
 /*

@@ -266,9 +266,16 @@ def test_device_arrays_roundtrip(fit, oracle):
     assert rel(r.transfer_spectrum, T) < 1e-5
 
 
-@pytest.mark.parametrize("ns,dea,inv,batch", [((16, 16, 16), "23", "ke", 0), ((16, 16, 16), "23", "helicity", 3),
-                                              ((24, 20), "none", "enstrophy", 2), ((12, 12, 12), "none", "ke", 5)])
-def test_fused_pipeline_equals_plain(fit, oracle, ns, dea, inv, batch):
+@pytest.mark.parametrize("ns,dea,inv,batch,xopt", [((16, 16, 16), "23", "ke", 0, None),
+                                                   ((16, 16, 16), "23", "helicity", 3, None),
+                                                   ((24, 20), "none", "enstrophy", 2, None),
+                                                   ((12, 12, 12), "none", "ke", 5, None),
+                                                   ((256, 24, 20), "23", "ke", 0, None),
+                                                   ((256, 24, 20), "23", "ke", 0, "block"),
+                                                   ((512, 12, 10), "23", "helicity", 4, 3),
+                                                   ((384, 16), "none", "ke", 0, 2),
+                                                   ((768, 12), "23", "enstrophy", 0, 1)])
+def test_fused_pipeline_equals_plain(fit, oracle, ns, dea, inv, batch, xopt):
     """The fused / pruned / batched giver loop against the plain one (same plan, FP64) and the
     oracle, with a giver-batch cap that splits the givers into several launches."""
     say("case fused vs plain", ns, dea, inv, batch)
@@ -279,8 +286,13 @@ def test_fused_pipeline_equals_plain(fit, oracle, ns, dea, inv, batch):
     I = {"ke": fit.KineticEnergy(), "helicity": fit.Helicity(), "enstrophy": fit.Enstrophy()}[inv]
     from fit_amd.grid import shell_setup
     c, e, sidx = shell_setup(ks, fit.CustomBinning(range(0, 7)), None)
+    from fit_amd import _lib
     with fit.Plan(ns, nd, nd, ks, deal, np.float64, sidx, len(c)) as plan:
         plan.set_giver_batch(batch)
+        if xopt == "block":
+            plan.set_option(_lib.OPT_XPASS_BLOCK, 1)
+        elif xopt is not None:
+            plan.set_option(_lib.OPT_XPASS_GIVERS, xopt)
         Tf, netf, _ = plan.shell_to_shell(I, u)
         plan.set_plain(True)
         Tp, netp, _ = plan.shell_to_shell(I, u)
