@@ -406,7 +406,8 @@ __global__ void __launch_bounds__(1024) k_col_r(LD ld, ST st, const cpx<R>* __re
   }
 }
 
-// Fused x pass (see k_xfused) with the register FFT: B row pairs per workgroup, T threads each.
+// Fused x pass (see k_xfused) with the register FFT: B row pairs per workgroup, T threads each,
+// one LDS exchange buffer with workgroup barriers (fallback when the wave form's LDS does not fit).
 template <typename R, int L>
 __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw, const cpx<R>* __restrict__ grad,
                                                   int64_t gsy, int64_t gsz, int64_t gsf, const R* __restrict__ u,
@@ -417,7 +418,6 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
   constexpr int E = rp_E(L), T = L / E, LL = ldsline(L);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* lds0 = reinterpret_cast<cpx<R>*>(smem_raw);
-  cpx<R>* lds1 = lds0 + B * LL;
   const int line = threadIdx.x / T, t = threadIdx.x % T;
   const int fg = blockIdx.x % nfg;
   const int pair = (blockIdx.x / nfg) * B + line;
@@ -447,7 +447,7 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
       v[e] = mk<R>(A.x - Bv.y, A.y + Bv.x);
     }
     __syncthreads();
-    reg_fft<L, 1, true, R>(v, lds0, lds1, idx, t, tw);
+    reg_fft<L, 1, false, R>(v, lds0, lds0, idx, t, tw);
     const R* uj = u + (int64_t)j * usf;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -459,7 +459,7 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
     }
   }
   __syncthreads();
-  reg_fft<L, -1, true, R>(acc, lds0, lds1, idx, t, tw);
+  reg_fft<L, -1, false, R>(acc, lds0, lds0, idx, t, tw);
   __syncthreads();
 #pragma unroll
   for (int e = 0; e < E; ++e) lds0[idx(t + T * e)] = acc[e];
