@@ -118,6 +118,10 @@ def main():
     ap.add_argument("--givers", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-workers", type=int, default=16)
+    ap.add_argument("--no-fp64-check", action="store_true",
+                    help="skip the one-off FP64 device run the FP32 errors are measured against")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 1 GiB device copy first (reference kernel for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -128,6 +132,13 @@ def main():
     import torch
     fit = load_pkg()
     n, ng = args.n, args.givers
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")     # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("[bench] copy calibration: 1 GiB read + 1 GiB written")
     log(f"[bench] building the {n}³ synthetic field on the device")
     u = synthetic_velocity_gpu(n, 3, torch)
     torch.cuda.synchronize()
@@ -198,8 +209,33 @@ def main():
                      "launches": d["launches"], "avg_launch_ms": d["ms"] / max(d["launches"], 1)},
         "cpu_baseline": None,
     }
+    # FP32 result of the last timed step, compared with one FP64 device run of the same snapshot
+    Pi_ke32, Pi_h32, Tm32, _ = out
     pi_plan.close()
     tm_plan.close()
+    if not args.no_fp64_check:
+        log("[bench] FP64 device reference of the same snapshot")
+        _, _, s_pi = shell_setup(ks, fit.LinearBinning(1.0), fit.IsotropicShells())
+        _, _, s_tm = shell_setup(ks, fit.CustomBinning(range(0, ng + 1)), fit.IsotropicShells())
+        u64 = u.to(torch.complex128)
+        p64 = fit.Plan((n,) * 3, 3, 3, ks, fit.PaddedThreeHalves(), np.float64, s_pi, len(c_pi))
+        N64 = p64.nonlinear_term(u64)
+        _, Pi_ke64 = p64.spectral_flux_with_nhat(KE, u64, N64)
+        _, Pi_h64 = p64.spectral_flux_with_nhat(HEL, u64, N64)
+        p64.close()
+        del N64
+        t64 = fit.Plan((n,) * 3, 3, 3, ks, fit.OrszagTwoThirds(), np.float64, s_tm, len(c_tm))
+        Tm64, _, _ = t64.shell_to_shell(KE, u64)
+        t64.close()
+        del u64
+
+        def nerr(a, b):
+            return float(np.max(np.abs(np.asarray(a) - b)) / np.max(np.abs(b)))
+        result["error_vs_fp64_device"] = {"pi_kinetic_energy": nerr(Pi_ke32, Pi_ke64),
+                                          "pi_helicity": nerr(Pi_h32, Pi_h64),
+                                          "t_nm": nerr(Tm32, Tm64),
+                                          "definition": "max|X32 - X64| / max|X64|"}
+        log("[bench] errors " + json.dumps(result["error_vs_fp64_device"]))
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline sample (FP64 oracle, one giver column)")
         workers = max(1, min(args.cpu_workers, os.cpu_count() or 1))

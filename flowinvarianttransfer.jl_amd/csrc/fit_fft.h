@@ -209,9 +209,8 @@ __device__ __forceinline__ void reg_stage(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* 
     constexpr int r = rp_rad(L, S);
     constexpr int T = L / E;
     constexpr int NB = E / r;
+    constexpr bool LAST = S + 1 == rp_nst(L);
     static_assert(E % r == 0, "radix must divide the points per thread");
-    cpx<R>* lds = (DB && (S & 1)) ? lds1 : lds0;
-    if constexpr (!DB) __syncthreads();   // single buffer: previous readers are done
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
       const int j = t + i * T;
@@ -224,14 +223,25 @@ __device__ __forceinline__ void reg_stage(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* 
         for (int k = 1; k < r; ++k) w[k] = cmul(w[k], twiddle<SIGN, R>(tw, k * m * (L / (NS * r))));
       }
       dft<r, SIGN, R>(w);
-      const int o = (j - m) * r + m;
+      if constexpr (LAST) {
+        // last stage: NS = L/r, so output j + k NS = t + T (i + k NB) is this thread's own slot
 #pragma unroll
-      for (int k = 0; k < r; ++k) lds[idx(o + k * NS)] = w[k];
+        for (int k = 0; k < r; ++k) v[i + k * NB] = w[k];
+      } else {
+        cpx<R>* lds = (DB && (S & 1)) ? lds1 : lds0;
+        if constexpr (!DB) { if (i == 0) __syncthreads(); }  // single buffer: previous readers done
+        const int o = (j - m) * r + m;
+#pragma unroll
+        for (int k = 0; k < r; ++k) lds[idx(o + k * NS)] = w[k];
+      }
     }
-    __syncthreads();
+    if constexpr (!LAST) {
+      cpx<R>* lds = (DB && (S & 1)) ? lds1 : lds0;
+      __syncthreads();
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = lds[idx(t + T * e)];
-    reg_stage<L, E, S + 1, NS * r, SIGN, DB, R, IDX>(v, lds0, lds1, idx, t, tw);
+      for (int e = 0; e < E; ++e) v[e] = lds[idx(t + T * e)];
+      reg_stage<L, E, S + 1, NS * r, SIGN, DB, R, IDX>(v, lds0, lds1, idx, t, tw);
+    }
   }
 }
 
@@ -283,8 +293,9 @@ __device__ __forceinline__ void wave_stage(cpx<R> (&v)[E], cpx<R>* lds, const ID
     constexpr int r = rp_rad(L, S);
     constexpr int T = L / E;
     constexpr int NB = E / r;
+    constexpr bool LAST = S + 1 == rp_nst(L);
     static_assert(E % r == 0, "radix must divide the points per lane");
-    wave_lds_sync();  // reads of the previous stage are complete before overwriting
+    if constexpr (!LAST) wave_lds_sync();  // reads of the previous stage are complete
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
       const int j = t + i * T;
@@ -297,14 +308,21 @@ __device__ __forceinline__ void wave_stage(cpx<R> (&v)[E], cpx<R>* lds, const ID
         for (int k = 1; k < r; ++k) w[k] = cmul(w[k], twiddle<SIGN, R>(tw, k * m * (L / (NS * r))));
       }
       dft<r, SIGN, R>(w);
-      const int o = (j - m) * r + m;
+      if constexpr (LAST) {
 #pragma unroll
-      for (int k = 0; k < r; ++k) lds[idx(o + k * NS)] = w[k];
+        for (int k = 0; k < r; ++k) v[i + k * NB] = w[k];   // own slots: no exchange
+      } else {
+        const int o = (j - m) * r + m;
+#pragma unroll
+        for (int k = 0; k < r; ++k) lds[idx(o + k * NS)] = w[k];
+      }
     }
-    wave_lds_sync();
+    if constexpr (!LAST) {
+      wave_lds_sync();
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = lds[idx(t + T * e)];
-    wave_stage<L, E, S + 1, NS * r, SIGN, R, IDX>(v, lds, idx, t, tw);
+      for (int e = 0; e < E; ++e) v[e] = lds[idx(t + T * e)];
+      wave_stage<L, E, S + 1, NS * r, SIGN, R, IDX>(v, lds, idx, t, tw);
+    }
   }
 }
 

@@ -444,7 +444,8 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
   const int64_t usy = L, usz = (int64_t)L * p->G[1];
   const double bytes = (double)nfg * nrows * (p->nd * (xa * csz) + xb * csz) + (double)p->nd * nrows * L * sizeof(R);
   prof_begin(p, FIT_K_FFT_ROW_INVERSE, bytes);
-  const bool wave_ok = reg_supported(L) && !p->xfused_block &&
+  // wave form for short register footprints (E <= 16); the workgroup form otherwise
+  const bool wave_ok = reg_supported(L) && !p->xfused_block && wE(L) <= 16 &&
                        (size_t)4 * (64 / (L / wE(L))) * ((size_t)L + L / 16) * csz <= g_max_lds;
   if (wave_ok) {
     // wave-per-line kernel: 4 waves per workgroup, 64/T row pairs per wave
@@ -469,8 +470,25 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
       default: break;
     }
   } else if (reg_supported(L)) {
+    // workgroup form: B row pairs per workgroup, single LDS exchange buffer
     const int T = reg_T(L);
     const size_t LL = (size_t)L + L / 16;
+    int B = std::max(1, 256 / T);
+    while (B > 1 && B * LL * csz > g_max_lds) --B;
+    const int nrg = (npairs + B - 1) / B;
+    const size_t lds = B * LL * csz;
+    switch (L) {
+#define FIT_CASE(LLEN)                                                                                         \
+  case LLEN:                                                                                                   \
+    hipLaunchKernelGGL((k_xfused_r<R, LLEN>), dim3((unsigned)((int64_t)nrg * nfg)), dim3(B * reg_T(LLEN)), lds, \
+                       p->stream, (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf,             \
+                       (const R*)p->uphys, usy, usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M,      \
+                       p->G[1], nrows, B, (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, nfg);               \
+    break;
+      FIT_REG_LENGTHS(FIT_CASE)
+#undef FIT_CASE
+      default: break;
+    }
   } else {
     int P = 8;
     while (P > 1 && (size_t)3 * L * P * csz > 65536) --P;

@@ -509,12 +509,14 @@ __global__ void __launch_bounds__(256) k_xfused_w(const cpx<R>* __restrict__ tw,
   const IdxLine idx{0};
   cpx<R> ur[UREG ? 3 : 1][UREG ? E : 1];
   if constexpr (UREG) {
-    for (int j = 0; j < nd; ++j) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
       const R* uj = u + (int64_t)j * usf;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int xx = t + T * e;
-        ur[j][e] = mk<R>(ok0 ? uj[xx + uo0] * gscale : (R)0, ok1 ? uj[xx + uo1] * gscale : (R)0);
+        ur[j][e] = j < nd ? mk<R>(ok0 ? uj[xx + uo0] * gscale : (R)0, ok1 ? uj[xx + uo1] * gscale : (R)0)
+                          : mk<R>((R)0, (R)0);
       }
     }
   }
@@ -525,7 +527,9 @@ __global__ void __launch_bounds__(256) k_xfused_w(const cpx<R>* __restrict__ tw,
     for (int c = 0; c < M; ++c) {
       const int fg = g * M + c;
       cpx<R> acc[E];
-      for (int j = 0; j < nd; ++j) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        if (j >= nd) break;
         const cpx<R>* gf = grad + (int64_t)(fg * nd + j) * gsf;
         cpx<R> v[E];
 #pragma unroll
