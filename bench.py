@@ -80,6 +80,21 @@ def synthetic_velocity_gpu(n, seed, torch):
     return u.permute(3, 2, 1, 0)              # (x, y, z, c) view, column-major memory
 
 
+def measured_traffic(plan_key, kernel_class, n, ng):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated) when one exists for this kernel and workload."""
+    if (plan_key, kernel_class, n, ng) != ("tnm", "fused_x", 512, 48):
+        return None
+    path = os.path.join(ROOT, "profiles", "k_xfused_w_f32_512_tnm48_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split("=")[1])
+    except OSError:
+        return None
+    return None
+
+
 def cpu_baseline(u_dev, n, ngivers, torch, fit, workers):
     """FP64 CPU restatement of the reference's FFT path (c2c, its dataflow) on a bounded sample:
     one giver column of T(n,m) at 512³ (the advecting field's 3 inverse transforms are shared
@@ -120,6 +135,8 @@ def main():
     ap.add_argument("--cpu-workers", type=int, default=16)
     ap.add_argument("--no-fp64-check", action="store_true",
                     help="skip the one-off FP64 device run the FP32 errors are measured against")
+    ap.add_argument("--no-fp64-plain", action="store_true",
+                    help="skip timing the plain (unfused, unpruned, one giver at a time) FP64 form")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device copy first (reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -174,20 +191,23 @@ def main():
         log(f"[bench] step {s} done")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    stats = [pi_plan.profile_stats(), tm_plan.profile_stats()]
+    stats = {"pi": pi_plan.profile_stats(), "tnm": tm_plan.profile_stats()}
     pi_plan.profile(False)
     tm_plan.profile(False)
-    merged = {}
-    for st in stats:
-        for k, v in st.items():
-            m = merged.setdefault(k, {"launches": 0, "ms": 0.0, "alg_bytes": 0.0})
-            for f in m:
-                m[f] += v[f]
-    dom = max(merged, key=lambda k: merged[k]["ms"])
-    d = merged[dom]
+    # dominant kernel = the (plan, kernel class) with the most HIP-event time over the timed steps
+    plan_key, dom = max(((pk, k) for pk in stats for k in stats[pk]), key=lambda pk: stats[pk[0]][pk[1]]["ms"])
+    d = stats[plan_key][dom]
     achieved = d["alg_bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
+    merged = stats
     log("[bench] kernel classes: " + json.dumps(merged))
     ms_step = elapsed * 1e3 / args.steps
+    fx = stats["tnm"]["fused_x"]
+    if fx["launches"]:
+        # 4 complex FFTs of n points per (giver, component, row pair): 3 inverse gradients + 1 forward
+        fft_flops = ng * 3 * (n * n // 2) * 4 * 5.0 * n * math.log2(n)
+        fx_tflops = fft_flops * fx["launches"] / (fx["ms"] * 1e-3) / 1e12
+    else:
+        fx_tflops = None
     result = {
         "metric": METRIC,
         "value": args.steps / elapsed,
@@ -204,10 +224,13 @@ def main():
         "config": {"workload": f"{n}^3 D=3 KE+helicity Pi(K) PaddedThreeHalves ({int(n * 1.5)}^3) "
                                f"+ {ng}-shell T(n,m) OrszagTwoThirds, one snapshot per step",
                    "grid": n, "givers": ng, "pi_shells": len(c_pi)},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "launches": d["launches"], "avg_launch_ms": d["ms"] / max(d["launches"], 1)},
+        "roofline": {"bound": "hbm", "kernel": f"{plan_key}:{dom}", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(plan_key, dom, n, ng),
+                     "launches": d["launches"], "avg_launch_ms": d["ms"] / max(d["launches"], 1),
+                     "alg_bytes_per_launch": d["alg_bytes"] / max(d["launches"], 1)},
         "cpu_baseline": None,
+        "fused_x_fft_tflops": fx_tflops,
     }
     # FP32 result of the last timed step, compared with one FP64 device run of the same snapshot
     Pi_ke32, Pi_h32, Tm32, _ = out
@@ -218,15 +241,33 @@ def main():
         _, _, s_pi = shell_setup(ks, fit.LinearBinning(1.0), fit.IsotropicShells())
         _, _, s_tm = shell_setup(ks, fit.CustomBinning(range(0, ng + 1)), fit.IsotropicShells())
         u64 = u.to(torch.complex128)
-        p64 = fit.Plan((n,) * 3, 3, 3, ks, fit.PaddedThreeHalves(), np.float64, s_pi, len(c_pi))
-        N64 = p64.nonlinear_term(u64)
-        _, Pi_ke64 = p64.spectral_flux_with_nhat(KE, u64, N64)
-        _, Pi_h64 = p64.spectral_flux_with_nhat(HEL, u64, N64)
-        p64.close()
-        del N64
-        t64 = fit.Plan((n,) * 3, 3, 3, ks, fit.OrszagTwoThirds(), np.float64, s_tm, len(c_tm))
-        Tm64, _, _ = t64.shell_to_shell(KE, u64)
-        t64.close()
+
+        def fp64_snapshot(plain):
+            p64 = fit.Plan((n,) * 3, 3, 3, ks, fit.PaddedThreeHalves(), np.float64, s_pi, len(c_pi))
+            t64 = fit.Plan((n,) * 3, 3, 3, ks, fit.OrszagTwoThirds(), np.float64, s_tm, len(c_tm))
+            p64.set_plain(plain)
+            t64.set_plain(plain)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            N64 = p64.nonlinear_term(u64)
+            _, pk = p64.spectral_flux_with_nhat(KE, u64, N64)
+            _, ph = p64.spectral_flux_with_nhat(HEL, u64, N64)
+            del N64
+            tm, _, _ = t64.shell_to_shell(KE, u64)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            p64.close()
+            t64.close()
+            return pk, ph, tm, dt
+
+        Pi_ke64, Pi_h64, Tm64, t_fused64 = fp64_snapshot(False)
+        result["fp64_fused_gpu"] = {"value": 1.0 / t_fused64, "unit": "snapshots/s",
+                                    "note": "same kernels in FP64, one untimed-warmup-free snapshot"}
+        if not args.no_fp64_plain:
+            log("[bench] plain FP64 device form (unfused, unpruned, one giver per pass)")
+            _, _, _, t_plain64 = fp64_snapshot(True)
+            result["fp64_plain_gpu"] = {"value": 1.0 / t_plain64, "unit": "snapshots/s",
+                                        "speedup_of_fp32_fused": (args.steps / elapsed) * t_plain64}
         del u64
 
         def nerr(a, b):

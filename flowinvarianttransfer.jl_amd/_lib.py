@@ -14,7 +14,8 @@ DEALIAS_NONE, DEALIAS_TWOTHIRDS, DEALIAS_PADDED = 0, 1, 2
 KINETIC_ENERGY, HELICITY, ENSTROPHY, PASSIVE_SCALAR = 0, 1, 2, 3
 PTR_HOST, PTR_DEVICE = 0, 1
 OPT_PLAIN_PIPELINE, OPT_GIVER_BATCH, OPT_XPASS_GIVERS, OPT_XPASS_BLOCK = 1, 2, 3, 4
-K_CLASSES = ("fft_column", "fft_row_inverse", "fft_row_forward", "density_shell", "reduce", "other")
+K_CLASSES = ("fft_column", "fft_row_inverse", "fft_row_forward", "density_shell", "reduce", "other",
+             "fused_x")
 
 # every entry point of include/fit_hip.h with its ctypes signature
 _P = ctypes.c_void_p

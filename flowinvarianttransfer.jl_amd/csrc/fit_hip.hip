@@ -443,7 +443,7 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
   const int64_t osy = xb, osz = (int64_t)xb * p->G[1], osf = osz * p->G[2];
   const int64_t usy = L, usz = (int64_t)L * p->G[1];
   const double bytes = (double)nfg * nrows * (p->nd * (xa * csz) + xb * csz) + (double)p->nd * nrows * L * sizeof(R);
-  prof_begin(p, FIT_K_FFT_ROW_INVERSE, bytes);
+  prof_begin(p, FIT_K_FUSED_X, bytes);
   // wave form for short register footprints (E <= 16); the workgroup form otherwise
   const bool wave_ok = reg_supported(L) && !p->xfused_block && wE(L) <= 16 &&
                        (size_t)4 * (64 / (L / wE(L))) * ((size_t)L + L / 16) * csz <= g_max_lds;

@@ -149,7 +149,8 @@ enum {
   FIT_K_DENSITY_SHELL = 3,   /* density + shell partial sums */
   FIT_K_REDUCE = 4,          /* fixed-order FP64 partial reduction */
   FIT_K_OTHER = 5,
-  FIT_K_COUNT = 6
+  FIT_K_FUSED_X = 6,         /* fused contiguous-axis pass: c2r of gradients, product, r2c */
+  FIT_K_COUNT = 7
 };
 
 /* Launch count, summed kernel time (ms) and summed algorithmic HBM bytes for one class. */
