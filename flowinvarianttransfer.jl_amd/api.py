@@ -249,7 +249,7 @@ class Plan:
 
 
 _PLANS = OrderedDict()
-_MAX_PLANS = 4
+_MAX_PLANS = 12
 
 
 def _plan_for(ns, M, D, ks, dealiasing, precision, device, binning=None, geometry=None,
@@ -469,3 +469,38 @@ def calculate_partial_fluxes(velocity_hat, ks, *, decomposition=HelicalDecomposi
 def calculate_helical_partial_fluxes(velocity_hat, ks, **kw):
     """SpectralFlux.jl:295-296."""
     return calculate_partial_fluxes(velocity_hat, ks, decomposition=HelicalDecomposition(), **kw)
+
+
+# ---------------------------------------------------------------------------------------------
+# batch axis (the reference's mpi_batch_map semantics on one GPU)
+# ---------------------------------------------------------------------------------------------
+
+def batch_map(f, items):
+    """Apply f to every item and return the results in input order -- the collation contract of
+    the reference's batch axis (MPIExt.jl:14-23, :39-68), executed on one device."""
+    return [f(x) for x in items]
+
+
+def calculate_ensemble_fluxes(velocities, ks_list, scalars=None, *, binning=None,
+                              dealiasing=OrszagTwoThirds(), spectral=HIPSpectral(),
+                              invariant=KineticEnergy(), geometry=None):
+    """Spectral fluxes of an ensemble of snapshots of possibly different sizes, in input order.
+
+    velocities[j]: (ns_j..., D) array; ks_list[j]: its wavenumber vectors; scalars[j] (optional):
+    a scalar advected by velocities[j].  Returns a list of (flux of `invariant`, scalar-variance
+    flux or None).  Plans are cached per grid, so snapshots of equal size reuse device workspaces."""
+    if scalars is not None and len(scalars) != len(velocities):
+        raise ValueError("scalars must match velocities one to one")
+    if len(ks_list) != len(velocities):
+        raise ValueError("ks_list must match velocities one to one")
+
+    def one(j):
+        ke = calculate_spectral_flux(velocities[j], ks_list[j], binning=binning, dealiasing=dealiasing,
+                                     invariant=invariant, spectral=spectral, geometry=geometry)
+        sc = None
+        if scalars is not None:
+            sc = calculate_scalar_flux(velocities[j], scalars[j], ks_list[j], binning=binning,
+                                       dealiasing=dealiasing, spectral=spectral, geometry=geometry)
+        return ke, sc
+
+    return batch_map(one, range(len(velocities)))

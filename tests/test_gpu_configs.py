@@ -1,0 +1,172 @@
+"""Full-size GPU checks of the benchmark configurations, through size-independent properties and
+an FP64 device run of the same kernels (itself pinned to the oracle on small grids):
+
+  C2  256³ kinetic-energy T(k), Π(K) and the 32-shell T(n,m) (2/3), FP32 vs FP64
+  C5  32³ mode-to-mode S(k|p) over all 32768² pairs: antisymmetry, Σ_p S = T(k), shell sums =
+      T(n,m), and columns against the oracle's closed form
+  C5' 256³ helical 8-channel partial fluxes: the channels sum to the total flux
+  C4  an uneven 2D ensemble (scalar + kinetic-energy fluxes, results in input order)
+"""
+import math
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def nerr(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def device_field(oracle, ns, D, seed, spectrum="k-5/3"):
+    import torch
+    u = oracle.synthetic_field(ns, D, seed=seed, spectrum=spectrum)
+    rev = tuple(reversed(range(u.ndim)))
+    return torch.from_numpy(np.ascontiguousarray(u.transpose(rev))).to("cuda").permute(rev), u
+
+
+def test_c2_256_flux_and_shell_to_shell(fit, oracle):
+    import torch
+    n = 256
+    say("case C2", n)
+    ud, _ = device_field(oracle, (n,) * 3, 3, seed=2)
+    ks = fit.wavenumber_grid((n,) * 3, (2 * math.pi,) * 3)
+    u32 = ud.to(torch.complex64)
+    t0 = time.perf_counter()
+    r32 = fit.calculate_spectral_flux(u32, ks, binning=fit.LinearBinning(1.0))
+    s32 = fit.calculate_shell_to_shell_transfer(u32, ks, binning=fit.CustomBinning(range(0, 33)))
+    torch.cuda.synchronize()
+    say("  FP32 Π + 32-giver T(n,m) s:", time.perf_counter() - t0)
+    r64 = fit.calculate_spectral_flux(ud, ks, binning=fit.LinearBinning(1.0))
+    s64 = fit.calculate_shell_to_shell_transfer(ud, ks, binning=fit.CustomBinning(range(0, 33)))
+    assert len(r32.flux) == 222
+    eT, eP, eS = nerr(r32.transfer_spectrum, r64.transfer_spectrum), nerr(r32.flux, r64.flux), \
+        nerr(s32.transfer_matrix, s64.transfer_matrix)
+    say("  errors T, Π, T(n,m):", eT, eP, eS)
+    assert eT < 1e-5 and eP < 1e-6 and eS < 1e-5
+    # conservation (FP64): Σ_k T = 0; antisymmetry of T(n,m); rows reduce to T(k) on the block
+    T = r64.transfer_spectrum
+    assert abs(T.sum()) < 1e-9 * np.abs(T).sum()
+    nrm = np.sqrt(np.sum(s64.transfer_matrix ** 2))
+    assert s64.max_antisymmetry_error < 1e-12 * nrm
+
+
+def test_c5_mode_to_mode_32(fit, oracle):
+    import torch
+    n = 32
+    say("case C5", n)
+    ud, u = device_field(oracle, (n,) * 3, 3, seed=5)
+    ks = fit.wavenumber_grid((n,) * 3, (2 * math.pi,) * 3)
+    Np = n ** 3
+    t0 = time.perf_counter()
+    r = fit.calculate_mode_to_mode_transfer(ud.to(torch.complex64), ks, force=True)
+    torch.cuda.synchronize()
+    say("  FP32 S(k|p) over", Np * Np, "pairs, s:", time.perf_counter() - t0)
+    S = r.transfer                                            # device (n,n,n,n,n,n) FP32
+    S2 = S.permute(5, 4, 3, 2, 1, 0).reshape(Np, Np)           # [p, k] in memory order
+    # columns against the oracle's closed form (3 givers)
+    for p_lin in (1, 777, Np - 5):
+        p = (p_lin % n, (p_lin // n) % n, p_lin // (n * n))
+        col = S2[p_lin].double().cpu().numpy().reshape((n, n, n), order="F")
+        ref = closed_form_column(oracle, u, ks, p)
+        assert nerr(col, ref) < 2e-5
+    # antisymmetry S(k|p) = -S(p|k)
+    asym = (S2 + S2.t()).abs().max().item()
+    smax = S2.abs().max().item()
+    say("  antisymmetry", asym / smax)
+    assert asym < 1e-5 * smax
+    # Σ_p S(k|p) shell-summed == T(k) (FP64 device flux) ; shell sums of S == T(n,m)
+    net = r.net_transfer.double().cpu().numpy()
+    from fit_amd.grid import shell_setup
+    c, e, sidx = shell_setup(ks, fit.LinearBinning(1.0), None)
+    T = fit.calculate_spectral_flux(ud, ks, binning=fit.LinearBinning(1.0)).transfer_spectrum
+    netshell = oracle.shell_sum(net, sidx, len(c))
+    assert nerr(netshell, T) < 1e-5
+    s64 = fit.calculate_shell_to_shell_transfer(ud, ks, binning=fit.LinearBinning(1.0))
+    nsh = len(c)
+    lab = torch.from_numpy(sidx.reshape(-1, order="F")).to("cuda")
+    onehot = torch.zeros((nsh, Np), dtype=torch.float64, device="cuda")
+    keep = lab > 0
+    onehot[(lab[keep] - 1), torch.nonzero(keep).squeeze(1)] = 1.0
+    TKQ = onehot @ S2.t().double() @ onehot.t()              # [n (receiver), m (giver)]
+    say("  shell-sum err", nerr(TKQ.cpu().numpy(), s64.transfer_matrix))
+    assert nerr(TKQ.cpu().numpy(), s64.transfer_matrix) < 1e-5
+
+
+def closed_form_column(oracle, u, ks, p):
+    """S(·|p) for one giver from the oracle's closed form (same formula, one column)."""
+    nd = 3
+    ns = u.shape[:3]
+    Np = float(np.prod(ns))
+    keep = oracle.keep_mask(ns)
+    if not keep[p]:
+        return np.zeros(ns)
+    A = []
+    for j in range(nd):
+        g = keep * u[..., j]
+        neg = np.roll(g[::-1, ::-1, ::-1], 1, axis=(0, 1, 2))
+        A.append(0.5 * (g + np.conj(neg)))
+    kidx = np.meshgrid(*[np.arange(n) for n in ns], indexing="ij")
+    pk = [ks[j][p[j]] for j in range(nd)]
+    kmp = tuple((kidx[d] - p[d]) % ns[d] for d in range(nd))
+    kpp = tuple((kidx[d] + p[d]) % ns[d] for d in range(nd))
+    dm = sum(pk[j] * A[j][kmp] for j in range(nd))
+    dp = sum(pk[j] * A[j][kpp] for j in range(nd))
+    tot = np.zeros(ns)
+    for c in range(3):
+        fp = u[p + (c,)]
+        Nc = keep / (2 * Np * Np) * (1j * dm * fp - 1j * dp * np.conj(fp))
+        tot += np.real(np.conj(u[..., c]) * Nc)
+    return tot
+
+
+def test_c5_helical_channels_256(fit, oracle):
+    import torch
+    n = 256
+    say("case C5 helical", n)
+    ud, _ = device_field(oracle, (n,) * 3, 3, seed=6)
+    ks = fit.wavenumber_grid((n,) * 3, (2 * math.pi,) * 3)
+    u32 = ud.to(torch.complex64)
+    t0 = time.perf_counter()
+    hp = fit.calculate_helical_partial_fluxes(u32, ks, binning=fit.LinearBinning(1.0))
+    torch.cuda.synchronize()
+    say("  8 channels s:", time.perf_counter() - t0)
+    tot = fit.calculate_spectral_flux(ud, ks, binning=fit.LinearBinning(1.0))
+    scale = max(np.sqrt(np.sum(np.asarray(c.transfer_spectrum, np.float64) ** 2)) for c in hp["channels"].values())
+    err = np.max(np.abs(hp["total"].transfer_spectrum - tot.transfer_spectrum)) / scale
+    say("  channel-sum err", err)
+    assert len(hp["channels"]) == 8 and err < 1e-5
+
+
+def test_c4_uneven_ensemble(fit, oracle):
+    import torch
+    say("case C4")
+    rng = np.random.default_rng(4)
+    sizes = rng.choice([64, 128, 256, 512], 12)
+    vel, th, kss, refs = [], [], [], []
+    for j, n in enumerate(sizes):
+        ud, u = device_field(oracle, (int(n),) * 2, 2, seed=4000 + j, spectrum="k-3")
+        td, t = device_field(oracle, (int(n),) * 2, 1, seed=9000 + j, spectrum="k-1")
+        vel.append(ud.to(torch.complex64))
+        th.append(td.to(torch.complex64))
+        kss.append(fit.wavenumber_grid((int(n),) * 2, (2 * math.pi,) * 2))
+        if n <= 128:
+            refs.append((j, u, t))
+    t0 = time.perf_counter()
+    out = fit.calculate_ensemble_fluxes(vel, kss, th, binning=fit.LinearBinning(1.0))
+    torch.cuda.synchronize()
+    say("  ensemble of", len(sizes), "s:", time.perf_counter() - t0)
+    assert len(out) == len(sizes)
+    for j, u, t in refs:                                        # input order kept
+        ks = kss[j]
+        _, Tk, Pk = oracle.calculate_spectral_flux(u, ks, oracle.LinearBinning(1.0), "23")
+        _, Ts, Ps = oracle.calculate_scalar_flux(u, t[..., 0], ks, oracle.LinearBinning(1.0), "23")
+        assert nerr(out[j][0].flux, Pk) < 1e-5 and nerr(out[j][1].flux, Ps) < 1e-5
