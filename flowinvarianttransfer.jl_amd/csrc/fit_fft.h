@@ -335,3 +335,99 @@ __device__ __forceinline__ void wave_fft(cpx<R> (&v)[wE(L)], cpx<R>* lds, const 
 }
 
 }  // namespace fit
+
+namespace fit {
+
+// ---------------------------------------------------------------------------------------------
+// Wave-private FFT with the twiddles held in registers.  The twiddles a lane applies depend only
+// on its index t, so a kernel that runs many FFTs per lane loads them once (tw_fill) and every
+// stage reads them at compile-time offsets: twr[off(S) + i*(r-1) + k-1] = W_{NS r}^{k m},
+// m = (t + i T) mod NS, in the forward sign (the inverse uses the conjugate).
+// ---------------------------------------------------------------------------------------------
+constexpr int tw_off(int L, int E, int S) {
+  int off = 0, ns = 1;
+  for (int s = 0; s < S; ++s) {
+    const int r = rp_rad(L, s);
+    if (s >= 1) off += (E / r) * (r - 1);
+    ns *= r;
+  }
+  return off;
+}
+constexpr int tw_count(int L, int E) { return tw_off(L, E, rp_nst(L)) > 0 ? tw_off(L, E, rp_nst(L)) : 1; }
+
+template <int L, int E, int S, int NS, typename R, int NTW>
+__device__ __forceinline__ void tw_fill_stage(cpx<R> (&twr)[NTW], int t, const cpx<R>* __restrict__ tw) {
+  if constexpr (S < rp_nst(L)) {
+    constexpr int r = rp_rad(L, S);
+    constexpr int T = L / E;
+    constexpr int NB = E / r;
+    if constexpr (S >= 1) {
+      constexpr int off = tw_off(L, E, S);
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        const int m = (t + i * T) % NS;
+#pragma unroll
+        for (int k = 1; k < r; ++k) twr[off + i * (r - 1) + k - 1] = tw[k * m * (L / (NS * r))];
+      }
+    }
+    tw_fill_stage<L, E, S + 1, NS * r, R, NTW>(twr, t, tw);
+  }
+}
+
+template <int L, typename R>
+__device__ __forceinline__ void tw_fill(cpx<R> (&twr)[tw_count(L, wE(L))], int t, const cpx<R>* __restrict__ tw) {
+  tw_fill_stage<L, wE(L), 0, 1, R, tw_count(L, wE(L))>(twr, t, tw);
+}
+
+template <int L, int E, int S, int NS, int SIGN, typename R, class IDX, int NTW>
+__device__ __forceinline__ void wave_stage_tw(cpx<R> (&v)[E], cpx<R>* lds, const IDX& idx, int t,
+                                              const cpx<R> (&twr)[NTW]) {
+  if constexpr (S < rp_nst(L)) {
+    constexpr int r = rp_rad(L, S);
+    constexpr int T = L / E;
+    constexpr int NB = E / r;
+    constexpr bool LAST = S + 1 == rp_nst(L);
+    constexpr int off = tw_off(L, E, S);
+    if constexpr (!LAST) wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int j = t + i * T;
+      const int m = j % NS;
+      cpx<R> w[r];
+#pragma unroll
+      for (int k = 0; k < r; ++k) w[k] = v[i + k * NB];
+      if constexpr (S >= 1) {
+#pragma unroll
+        for (int k = 1; k < r; ++k) {
+          cpx<R> tk = twr[off + i * (r - 1) + k - 1];
+          if (SIGN > 0) tk.y = -tk.y;
+          w[k] = cmul(w[k], tk);
+        }
+      }
+      dft<r, SIGN, R>(w);
+      if constexpr (LAST) {
+#pragma unroll
+        for (int k = 0; k < r; ++k) v[i + k * NB] = w[k];
+      } else {
+        const int o = (j - m) * r + m;
+#pragma unroll
+        for (int k = 0; k < r; ++k) lds[idx(o + k * NS)] = w[k];
+      }
+    }
+    if constexpr (!LAST) {
+      wave_lds_sync();
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = lds[idx(t + T * e)];
+      wave_stage_tw<L, E, S + 1, NS * r, SIGN, R, IDX, NTW>(v, lds, idx, t, twr);
+    }
+  }
+}
+
+template <int L, int SIGN, typename R, class IDX>
+__device__ __forceinline__ void wave_fft_tw(cpx<R> (&v)[wE(L)], cpx<R>* lds, const IDX& idx, int t,
+                                            const cpx<R> (&twr)[tw_count(L, wE(L))]) {
+  wave_stage_tw<L, wE(L), 0, 1, SIGN, R, IDX, tw_count(L, wE(L))>(v, lds, idx, t, twr);
+  wave_lds_sync();
+}
+
+}  // namespace fit

@@ -520,6 +520,8 @@ __global__ void __launch_bounds__(256) k_xfused_w(const cpx<R>* __restrict__ tw,
       }
     }
   }
+  cpx<R> twr[tw_count(L, E)];
+  tw_fill<L, R>(twr, t, tw);
   const int g0 = chunk * GC;
   const int g1 = min(ngiv, g0 + GC);
   for (int g = g0; g < g1; ++g) {
@@ -532,20 +534,41 @@ __global__ void __launch_bounds__(256) k_xfused_w(const cpx<R>* __restrict__ tw,
         if (j >= nd) break;
         const cpx<R>* gf = grad + (int64_t)(fg * nd + j) * gsf;
         cpx<R> v[E];
+        if (ka < T) {
+          // band below T: only slot 0 (k = t) and slot E-1 (k = L - T + t, |k| = T - t) can hold
+          // a nonzero bin; unused rows read row 0 (valid memory) and are zeroed by select
+          const cpx<R> z0 = mk<R>((R)0, (R)0);
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-          const int k = t + T * e;
-          const int kk = k <= L / 2 ? k : L - k;
-          cpx<R> A = mk<R>((R)0, (R)0), Bv = A;
-          if (kk <= ka) {
-            if (ok0) A = gf[kk + ro0];
-            if (ok1) Bv = gf[kk + ro1];
+          for (int e = 1; e < E - 1; ++e) v[e] = z0;
+          const int ka0 = t <= ka ? t : 0;
+          const int ka1 = (t >= 1 && T - t <= ka) ? T - t : 0;
+          cpx<R> A0 = gf[ka0 + (ok0 ? ro0 : 0)], B0 = gf[ka0 + (ok1 ? ro1 : 0)];
+          cpx<R> A1 = gf[ka1 + (ok0 ? ro0 : 0)], B1 = gf[ka1 + (ok1 ? ro1 : 0)];
+          const bool u0 = t <= ka, u1 = t >= 1 && T - t <= ka;
+          if (!(u0 && ok0)) A0 = z0;
+          if (!(u0 && ok1)) B0 = z0;
+          if (!(u1 && ok0)) A1 = z0;
+          if (!(u1 && ok1)) B1 = z0;
+          if (t == 0) { A0.y = (R)0; B0.y = (R)0; }
+          // slot E-1 lies above L/2: conjugate of the stored bin T - t
+          v[0] = mk<R>(A0.x - B0.y, A0.y + B0.x);
+          v[E - 1] = mk<R>(A1.x + B1.y, -A1.y + B1.x);
+        } else {
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            const int k = t + T * e;
+            const int kk = k <= L / 2 ? k : L - k;
+            cpx<R> A = mk<R>((R)0, (R)0), Bv = A;
+            if (kk <= ka) {
+              if (ok0) A = gf[kk + ro0];
+              if (ok1) Bv = gf[kk + ro1];
+            }
+            if (k == 0 || 2 * k == L) { A.y = (R)0; Bv.y = (R)0; }
+            if (k > L / 2) { A.y = -A.y; Bv.y = -Bv.y; }
+            v[e] = mk<R>(A.x - Bv.y, A.y + Bv.x);
           }
-          if (k == 0 || 2 * k == L) { A.y = (R)0; Bv.y = (R)0; }
-          if (k > L / 2) { A.y = -A.y; Bv.y = -Bv.y; }
-          v[e] = mk<R>(A.x - Bv.y, A.y + Bv.x);
         }
-        wave_fft<L, 1, R>(v, lds, idx, t, tw);
+        wave_fft_tw<L, 1, R>(v, lds, idx, t, twr);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           cpx<R> uu;
@@ -560,7 +583,7 @@ __global__ void __launch_bounds__(256) k_xfused_w(const cpx<R>* __restrict__ tw,
           acc[e] = j == 0 ? tt : cadd(acc[e], tt);
         }
       }
-      wave_fft<L, -1, R>(acc, lds, idx, t, tw);
+      wave_fft_tw<L, -1, R>(acc, lds, idx, t, twr);
 #pragma unroll
       for (int e = 0; e < E; ++e) lds[idx(t + T * e)] = acc[e];
       wave_lds_sync();
