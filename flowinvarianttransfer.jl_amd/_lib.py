@@ -13,7 +13,7 @@ F32, F64 = 0, 1
 DEALIAS_NONE, DEALIAS_TWOTHIRDS, DEALIAS_PADDED = 0, 1, 2
 KINETIC_ENERGY, HELICITY, ENSTROPHY, PASSIVE_SCALAR = 0, 1, 2, 3
 PTR_HOST, PTR_DEVICE = 0, 1
-OPT_PLAIN_PIPELINE, OPT_GIVER_BATCH, OPT_XPASS_GIVERS, OPT_XPASS_BLOCK = 1, 2, 3, 4
+OPT_PLAIN_PIPELINE, OPT_GIVER_BATCH, OPT_XPASS_GIVERS, OPT_XPASS_BLOCK, OPT_SUBGRID = 1, 2, 3, 4, 5
 K_CLASSES = ("fft_column", "fft_row_inverse", "fft_row_forward", "density_shell", "reduce", "other",
              "fused_x")
 
@@ -30,6 +30,7 @@ SIGNATURES = {
     "fit_plan_destroy": (_I, [_P]),
     "fit_plan_device_bytes": (ctypes.c_int64, [_P]),
     "fit_plan_set_option": (_I, [_P, _I, _I]),
+    "fit_plan_subgrid": (_I, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "fit_nonlinear_term": (_I, [_P, _P, _P, _P, _I, _I]),
     "fit_transfer_density": (_I, [_P, _I, _P, _P, _P, _I, _I]),
     "fit_spectral_flux": (_I, [_P, _I, _P, _P, _D, _D, _I, _I]),

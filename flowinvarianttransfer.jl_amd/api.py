@@ -155,6 +155,19 @@ class Plan:
     def set_giver_batch(self, n):
         self.set_option(_lib.OPT_GIVER_BATCH, int(n))
 
+    def set_subgrid(self, on=True):
+        """Use (default) or bypass the band-limited sub-grid of the giver loop."""
+        self.set_option(_lib.OPT_SUBGRID, int(bool(on)))
+
+    @property
+    def subgrid(self):
+        """Grid of the band-limited giver loop, or None when the plan has none attached."""
+        dims = (ctypes.c_int64 * 3)()
+        rc = self.lib.fit_plan_subgrid(self.handle, dims)
+        if rc < 0:
+            _lib.check(1)
+        return tuple(int(d) for d in dims[:self.nd]) if rc == 1 else None
+
     @property
     def device_bytes(self):
         return int(self.lib.fit_plan_device_bytes(self.handle))

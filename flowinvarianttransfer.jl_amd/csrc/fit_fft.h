@@ -277,7 +277,7 @@ namespace fit {
 // have completed (s_waitcnt lgkmcnt(0)) before its lanes read them, plus a code-motion fence.
 // ---------------------------------------------------------------------------------------------
 constexpr int wE(int L) {
-  return L == 256 ? 8 : L == 384 ? 24 : L == 512 ? 8 : L == 768 ? 24 : L == 1024 ? 16
+  return L == 128 ? 8 : L == 192 ? 24 : L == 256 ? 8 : L == 384 ? 24 : L == 512 ? 8 : L == 768 ? 24 : L == 1024 ? 16
        : L == 1536 ? 24 : L == 2048 ? 32 : L == 3072 ? 48 : 0;
 }
 

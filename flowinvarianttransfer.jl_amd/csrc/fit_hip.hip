@@ -105,6 +105,16 @@ struct fit_plan {
   int giver_batch = 0;           // cap on givers per batch (0 = memory budget only)
   int xfused_gc = 0;             // givers per workgroup in the wave-per-line x pass (0 = 4)
   int xfused_block = 0;          // 1: use the workgroup-barrier x pass instead
+  // band-limited sub-grid for the giver loop: when every shell lies in |k_d| <= R_d with
+  // 4 R_d < n_d, T(n,m) only involves advecting modes |q_d| <= 2 R_d and the exact convolution
+  // is evaluated alias-free on a grid of at least 4 R_d + 1 points per axis
+  fit_plan* sub = nullptr;       // child plan on that grid (no dealiasing, same shells)
+  int sub_q[3] = {0, 0, 0};      // folded box of input modes copied into the child
+  double sub_tscale = 1.0;       // (Np_sub / Np)^2: N̂ carries 1/Np^2 of the grid it is taken on
+  void* sub_f = nullptr; size_t sub_f_bytes = 0;
+  void* sub_a = nullptr; size_t sub_a_bytes = 0;
+  bool own_stream = true;
+  int use_sub = 1;                // FIT_OPT_SUBGRID
   // profiling
   bool prof = false;
   std::vector<ProfRec> recs;
@@ -154,16 +164,20 @@ void prof_end(fit_plan* p) {
   p->nrec++;
 }
 
+// launches recorded by the plan and by its sub-grid child are reported together
 void prof_collect(fit_plan* p) {
   if (!p->prof) return;
-  for (size_t i = 0; i < p->nrec; ++i) {
-    float t = 0.f;
-    hipEventElapsedTime(&t, p->recs[i].a, p->recs[i].b);
-    p->launches[p->recs[i].cls] += 1;
-    p->ms[p->recs[i].cls] += t;
-    p->abytes[p->recs[i].cls] += p->recs[i].bytes;
+  for (fit_plan* s : {p, p->sub}) {
+    if (!s || !s->prof) continue;
+    for (size_t i = 0; i < s->nrec; ++i) {
+      float t = 0.f;
+      hipEventElapsedTime(&t, s->recs[i].a, s->recs[i].b);
+      p->launches[s->recs[i].cls] += 1;
+      p->ms[s->recs[i].cls] += t;
+      p->abytes[s->recs[i].cls] += s->recs[i].bytes;
+    }
+    s->nrec = 0;
   }
-  p->nrec = 0;
 }
 
 template <typename R> Grid grid_of(const fit_plan* p) {
@@ -190,7 +204,7 @@ constexpr int kThreads = 256;
 Band full_band(int G) { return Band{G, G}; }
 
 // lengths with a compile-time register FFT (others use the generic LDS engine)
-#define FIT_REG_LENGTHS(X) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048) X(3072)
+#define FIT_REG_LENGTHS(X) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048) X(3072)
 constexpr int reg_T(int L) { return L / (((L & (L - 1)) == 0) ? 8 : 24); }
 bool reg_supported(int L) {
   switch (L) {
@@ -817,6 +831,46 @@ int s2s_impl(fit_plan* p, int inv, const void* f, const void* a, std::vector<dou
   return s2s_batched<R>(p, inv, f, Tm);
 }
 
+// Giver loop on the band-limited sub-grid.  With receivers and givers in |k_d| <= R_d, every
+// triad k = p + q that T(n,m) sums has |q_d| <= 2 R_d, and on a grid of L_d > |q|+|p|+|k| points
+// the product of those modes is alias-free at the receivers, so the child grid yields the same
+// convolution the full grid does (FFTWExt.jl:234-274).  N̂ = conv/Np² on either grid, hence the
+// (Np_sub/Np)² rescale of the shell sums.
+template <typename R>
+int s2s_sub(fit_plan* p, int inv, const void* f, const void* a, std::vector<double>& Tm) {
+  fit_plan* c = p->sub;
+  const size_t csz = sizeof(cpx<R>);
+  auto gather = [&](const void* src, int ncomp, void** dst, size_t* have) -> int {
+    int r = grow(p, dst, have, (size_t)ncomp * c->Np * csz);
+    if (r) return r;
+    prof_begin(p, FIT_K_OTHER, (double)ncomp * c->Np * csz * 2.0);
+    hipLaunchKernelGGL(k_subgrid_gather<R>, dim3(2048), dim3(256), 0, p->stream, (const cpx<R>*)src, p->n[0],
+                       p->n[1], p->n[2], p->Np, (cpx<R>*)*dst, c->n[0], c->n[1], c->n[2], c->Np, p->sub_q[0],
+                       p->sub_q[1], p->sub_q[2], ncomp);
+    prof_end(p);
+    FIT_HIP(hipGetLastError());
+    return FIT_OK;
+  };
+  int rc = gather(f, p->M, &p->sub_f, &p->sub_f_bytes);
+  if (rc) return rc;
+  const void* ca = p->sub_f;
+  if (a != f) {
+    if ((rc = gather(a, p->D, &p->sub_a, &p->sub_a_bytes))) return rc;
+    ca = p->sub_a;
+  }
+  c->prof = p->prof;
+  if ((rc = s2s_impl<R>(c, inv, p->sub_f, ca, Tm))) return rc;
+  FIT_HIP(hipStreamSynchronize(p->stream));
+  for (double& v : Tm) v *= p->sub_tscale;
+  return FIT_OK;
+}
+
+template <typename R>
+int s2s_entry(fit_plan* p, int inv, const void* f, const void* a, std::vector<double>& Tm) {
+  if (p->sub && p->use_sub && !p->plain) return s2s_sub<R>(p, inv, f, a, Tm);
+  return s2s_impl<R>(p, inv, f, a, Tm);
+}
+
 template <typename R>
 int m2m_impl(fit_plan* p, int inv, const void* f, const void* a, void* S, void* net, int io_dtype,
                     int ptr_kind) {
@@ -883,16 +937,81 @@ int m2m_impl(fit_plan* p, int inv, const void* f, const void* a, void* S, void* 
 // =============================================================================================
 // C ABI
 // =============================================================================================
-extern "C" {
+// Plan construction (the caller holds the creation lock).  allow_sub: may attach a band-limited
+// sub-grid child for the giver loop (children never nest).
+static int create_plan(fit_plan** out, int nd, const int64_t* ns, int M, int D, int compute_dtype, int dealias,
+                       const double* const* ks, const int64_t* shell_idx, int n_shells, int device,
+                       bool allow_sub);
 
-const char* fit_last_error(void) { return g_last_error.c_str(); }
+// smallest transform length of the form 2^a or 3*2^a (at least 8) that is >= need
+static int subgrid_len(int need) {
+  int best = 1 << 30;
+  for (int b = 8; b < (1 << 24); b *= 2)
+    if (b >= need) { best = std::min(best, b); break; }
+  for (int b = 12; b < (1 << 24); b *= 2)
+    if (b >= need) { best = std::min(best, b); break; }
+  return best;
+}
 
-const char* fit_version(void) { return "fit_hip 0.1 (gfx950)"; }
+// Attach the band-limited sub-grid child when every shell lies in |k_d| <= R_d with 4 R_d < n_d
+// and the child grid is at most half the transform grid.  Child: no dealiasing, the API grid's
+// wavenumbers at the same folded indices, the shell map restricted to the box |k_d| <= R_d.
+static int setup_subgrid(fit_plan* p, const double* const* ks, const std::vector<int>& sh) {
+  if (p->nd < 2 || p->nsh <= 0 || !p->shell_symmetric) return FIT_OK;
+  int R[3] = {0, 0, 0}, Q[3] = {0, 0, 0}, Ls[3] = {1, 1, 1};
+  double ratio = 1.0;
+  for (int d = 0; d < p->nd; ++d) {
+    R[d] = std::max(p->band_rx[d], 0);
+    if (4 * R[d] >= p->n[d]) return FIT_OK;
+    Q[d] = 2 * R[d];
+    if (p->dealias == FIT_DEALIAS_TWOTHIRDS) Q[d] = std::min(Q[d], p->band_in[d]);
+    const int need = Q[d] + 2 * R[d] + 1;  // <= 4 R_d + 1 <= n_d
+    Ls[d] = subgrid_len(need);
+    if (Ls[d] > p->n[d]) Ls[d] = p->n[d];
+    ratio *= (double)Ls[d] / (double)p->G[d];
+  }
+  if (ratio > 0.5) return FIT_OK;
+  std::vector<double> kc[3];
+  const double* kp[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < p->nd; ++d) {
+    kc[d].resize(Ls[d]);
+    for (int i = 0; i < Ls[d]; ++i) {
+      const int k = 2 * i <= Ls[d] ? i : i - Ls[d];
+      kc[d][i] = ks[d][k >= 0 ? k : k + p->n[d]];
+    }
+    kp[d] = kc[d].data();
+  }
+  const int64_t npc = (int64_t)Ls[0] * Ls[1] * Ls[2];
+  std::vector<int64_t> shc((size_t)npc, 0);
+  for (int64_t o = 0; o < npc; ++o) {
+    const int i = (int)(o % Ls[0]), j = (int)((o / Ls[0]) % Ls[1]), l = (int)(o / ((int64_t)Ls[0] * Ls[1]));
+    const int ki = 2 * i <= Ls[0] ? i : i - Ls[0];
+    const int kj = 2 * j <= Ls[1] ? j : j - Ls[1];
+    const int kl = 2 * l <= Ls[2] ? l : l - Ls[2];
+    if (std::abs(ki) > R[0] || std::abs(kj) > R[1] || std::abs(kl) > R[2]) continue;
+    const int si = ki >= 0 ? ki : ki + p->n[0], sj = kj >= 0 ? kj : kj + p->n[1], sl = kl >= 0 ? kl : kl + p->n[2];
+    shc[(size_t)o] = sh[(size_t)(si + (int64_t)p->n[0] * (sj + (int64_t)p->n[1] * sl))];
+  }
+  const int64_t nsc[3] = {Ls[0], Ls[1], Ls[2]};
+  fit_plan* c = nullptr;
+  int rc = create_plan(&c, p->nd, nsc, p->M, p->D, p->dtype, FIT_DEALIAS_NONE, kp, shc.data(), p->nsh, p->device,
+                       false);
+  if (rc) return rc;
+  // inputs beyond the copied box are zero: prune the child's input band to it
+  for (int d = 0; d < p->nd; ++d) c->band_in[d] = std::min(c->band_in[d], Q[d]);
+  (void)hipStreamDestroy(c->stream);
+  c->stream = p->stream;
+  c->own_stream = false;
+  p->sub = c;
+  for (int d = 0; d < 3; ++d) p->sub_q[d] = Q[d];
+  const double r = (double)c->Np / (double)p->Np;
+  p->sub_tscale = r * r;
+  return FIT_OK;
+}
 
-int fit_plan_create(fit_plan** out, int nd, const int64_t* ns, int M, int D, int compute_dtype, int dealias,
-                    const double* const* ks, const int64_t* shell_idx, int n_shells, int device) {
-  std::lock_guard<std::mutex> lock(g_create_mutex);
-  g_last_error.clear();
+static int create_plan(fit_plan** out, int nd, const int64_t* ns, int M, int D, int compute_dtype, int dealias,
+                       const double* const* ks, const int64_t* shell_idx, int n_shells, int device,
+                       bool allow_sub) {
   if (!out) return fail(FIT_E_ARGUMENT, "plan out-pointer is NULL");
   *out = nullptr;
   if (nd < 1 || nd > 3) return fail(FIT_E_ARGUMENT, "nd must be 1, 2 or 3");
@@ -1015,6 +1134,7 @@ int fit_plan_create(fit_plan** out, int nd, const int64_t* ns, int M, int D, int
       if ((rc = dalloc(p, (void**)&p->shh, (size_t)Hn * sizeof(int)))) return bail(rc);
       hipMemcpy(p->shh, shh.data(), (size_t)Hn * sizeof(int), hipMemcpyHostToDevice);
     }
+    if (allow_sub && (rc = setup_subgrid(p, ks, sh))) return bail(rc);
   }
   if ((rc = dalloc(p, &p->work, (size_t)p->Hp * cs))) return bail(rc);
   if ((rc = dalloc(p, &p->uphys, (size_t)nd * p->Gp * rs))) return bail(rc);
@@ -1030,10 +1150,25 @@ int fit_plan_create(fit_plan** out, int nd, const int64_t* ns, int M, int D, int
   return FIT_OK;
 }
 
+extern "C" {
+
+const char* fit_last_error(void) { return g_last_error.c_str(); }
+
+const char* fit_version(void) { return "fit_hip 0.1 (gfx950)"; }
+
+int fit_plan_create(fit_plan** out, int nd, const int64_t* ns, int M, int D, int compute_dtype, int dealias,
+                    const double* const* ks, const int64_t* shell_idx, int n_shells, int device) {
+  std::lock_guard<std::mutex> lock(g_create_mutex);
+  g_last_error.clear();
+  return create_plan(out, nd, ns, M, D, compute_dtype, dealias, ks, shell_idx, n_shells, device, true);
+}
+
 int fit_plan_destroy(fit_plan* p) {
   if (!p) return FIT_OK;
   hipSetDevice(p->device);
   if (p->stream) hipStreamSynchronize(p->stream);
+  if (p->sub) { fit_plan_destroy(p->sub); p->sub = nullptr; }
+  dfree(p->sub_f); dfree(p->sub_a);
   for (int d = 0; d < 3; ++d) { dfree(p->tw[d]); dfree(p->ks[d]); }
   void* sh = p->shell; dfree(sh);
   dfree(p->work); dfree(p->uphys); dfree(p->nphys); dfree(p->nhat);
@@ -1046,15 +1181,29 @@ int fit_plan_destroy(fit_plan* p) {
   { void* q = p->dcolG; dfree(q); }
   void* dc = p->dcol; dfree(dc);
   for (auto& r : p->recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
-  if (p->stream) hipStreamDestroy(p->stream);
+  if (p->stream && p->own_stream) hipStreamDestroy(p->stream);
   delete p;
   return FIT_OK;
 }
 
-int64_t fit_plan_device_bytes(const fit_plan* p) { return p ? p->bytes : 0; }
+
+int fit_plan_subgrid(const fit_plan* p, int64_t* dims) {
+  if (!p) { fail(FIT_E_ARGUMENT, "null plan"); return -1; }
+  if (!p->sub) return 0;
+  if (dims)
+    for (int d = 0; d < 3; ++d) dims[d] = p->sub->n[d];
+  return 1;
+}
+
+int64_t fit_plan_device_bytes(const fit_plan* p) { return p ? p->bytes + (p->sub ? p->sub->bytes : 0) : 0; }
 
 int fit_plan_set_option(fit_plan* p, int option, int value) {
   if (!p) return fail(FIT_E_ARGUMENT, "null plan");
+  // pipeline-shape options apply to the sub-grid child as well
+  if (p->sub && option != FIT_OPT_PLAIN_PIPELINE && option != FIT_OPT_SUBGRID) {
+    int rc = fit_plan_set_option(p->sub, option, value);
+    if (rc) return rc;
+  }
   switch (option) {
     case FIT_OPT_PLAIN_PIPELINE: p->plain = value != 0; return FIT_OK;
     case FIT_OPT_GIVER_BATCH:
@@ -1066,6 +1215,7 @@ int fit_plan_set_option(fit_plan* p, int option, int value) {
       p->xfused_gc = value;
       return FIT_OK;
     case FIT_OPT_XPASS_BLOCK: p->xfused_block = value != 0; return FIT_OK;
+    case FIT_OPT_SUBGRID: p->use_sub = value != 0; return FIT_OK;
     default: return fail(FIT_E_ARGUMENT, "unknown plan option");
   }
 }
@@ -1077,6 +1227,7 @@ int fit_profile_enable(fit_plan* p, int on) {
   p->prof = on != 0;
   p->nrec = 0;
   for (int i = 0; i < FIT_K_COUNT; ++i) { p->launches[i] = 0; p->ms[i] = 0; p->abytes[i] = 0; }
+  if (p->sub) return fit_profile_enable(p->sub, on);
   return FIT_OK;
 }
 
@@ -1161,7 +1312,7 @@ int fit_shell_to_shell(fit_plan* p, int inv, const void* f_hat, const void* adv_
   if ((rc = stage_fa(p, f_hat, adv_hat, io_dtype, ptr_kind, &f, &a))) return rc;
   const int nsh = p->nsh;
   std::vector<double> Tm((size_t)nsh * nsh, 0.0);
-  rc = p->dtype == FIT_F64 ? s2s_impl<double>(p, inv, f, a, Tm) : s2s_impl<float>(p, inv, f, a, Tm);
+  rc = p->dtype == FIT_F64 ? s2s_entry<double>(p, inv, f, a, Tm) : s2s_entry<float>(p, inv, f, a, Tm);
   if (rc) return rc;
   if ((rc = finish(p))) return rc;
   if (Tmat) std::memcpy(Tmat, Tm.data(), Tm.size() * sizeof(double));

@@ -831,6 +831,35 @@ __global__ void k_convert(const A* __restrict__ in, B* __restrict__ out, int64_t
 }
 
 // ---------------------------------------------------------------------------------------------
+// Band-limited sub-grid: copy the folded box |k_d| <= q_d of C full-complex components from an
+// API grid (n0, n1, n2) into a smaller transform grid (L0, L1, L2), zeros elsewhere.  Both grids
+// use FFT index order (k >= 0 first, then the negative wavenumbers).
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ int sub_wavenumber(int i, int L) { return 2 * i <= L ? i : i - L; }
+
+template <typename R>
+__global__ void k_subgrid_gather(const cpx<R>* __restrict__ src, int n0, int n1, int n2, int64_t snp,
+                                 cpx<R>* __restrict__ dst, int L0, int L1, int L2, int64_t dnp, int q0, int q1,
+                                 int q2, int C) {
+  const int64_t total = dnp * C;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    const int c = (int)(e / dnp);
+    const int64_t o = e - (int64_t)c * dnp;
+    const int i = (int)(o % L0);
+    const int64_t r = o / L0;
+    const int j = (int)(r % L1), l = (int)(r / L1);
+    const int ki = sub_wavenumber(i, L0), kj = sub_wavenumber(j, L1), kl = sub_wavenumber(l, L2);
+    cpx<R> v = mk<R>((R)0, (R)0);
+    if (abs(ki) <= q0 && abs(kj) <= q1 && abs(kl) <= q2) {
+      const int si = ki >= 0 ? ki : ki + n0, sj = kj >= 0 ? kj : kj + n1, sl = kl >= 0 ? kl : kl + n2;
+      v = src[(int64_t)c * snp + si + (int64_t)n0 * (sj + (int64_t)n1 * sl)];
+    }
+    dst[e] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Mode-to-mode triads.  For giver p (single-mode field entering ifft through its Hermitian part):
 //   N̂_p(k) = keep(k) keep(p) / (2 Np²) Σ_j p_j [ i f̂(p) A_j(k-p) - i conj(f̂(p)) A_j(k+p) ],
 //   A_j = Hermitian part of keep·â_j,  S(k|p) = Re Σ_c conj(e_c(k)) N̂_{p,c}(k)

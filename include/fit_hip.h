@@ -72,10 +72,19 @@ int64_t fit_plan_device_bytes(const fit_plan* plan);
  *     giver at a time, N̂ through HBM) -- the comparison baseline; 0 (default) the fused one.
  *   FIT_OPT_GIVER_BATCH = 2: cap on giver shells per batched launch (0 = device memory budget).
  *   FIT_OPT_XPASS_GIVERS = 3: givers per workgroup in the fused contiguous-axis pass (0 = 4).
- *   FIT_OPT_XPASS_BLOCK = 4: value 1 selects the workgroup-barrier form of that pass. */
+ *   FIT_OPT_XPASS_BLOCK = 4: value 1 selects the workgroup-barrier form of that pass.
+ *   FIT_OPT_SUBGRID = 5: value 0 keeps the giver loop on the plan's own grid even when a
+ *     band-limited sub-grid is attached (1, default: use it). */
 enum { FIT_OPT_PLAIN_PIPELINE = 1, FIT_OPT_GIVER_BATCH = 2, FIT_OPT_XPASS_GIVERS = 3,
-       FIT_OPT_XPASS_BLOCK = 4 };
+       FIT_OPT_XPASS_BLOCK = 4, FIT_OPT_SUBGRID = 5 };
 int fit_plan_set_option(fit_plan* plan, int option, int value);
+
+/* Band-limited sub-grid of the giver loop (no reference counterpart).  When every shell lies in
+ * |k_d| <= R_d with 4 R_d < n_d, fit_plan_create attaches a grid of L_d >= 4 R_d + 1 points per
+ * axis on which fit_shell_to_shell evaluates the same triad sums alias-free (ShellToShell
+ * results equal the full-grid ones in exact arithmetic).  Returns 1 and writes L_d to dims[0..2]
+ * when such a grid is attached, 0 otherwise; negative on a NULL plan. */
+int fit_plan_subgrid(const fit_plan* plan, int64_t* dims);
 
 /*
  * N̂ = keep·FFT[(u_adv·∇) f]/Np into N_hat (ns..., M).  Replaces
