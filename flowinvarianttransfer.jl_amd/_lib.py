@@ -13,7 +13,7 @@ F32, F64 = 0, 1
 DEALIAS_NONE, DEALIAS_TWOTHIRDS, DEALIAS_PADDED = 0, 1, 2
 KINETIC_ENERGY, HELICITY, ENSTROPHY, PASSIVE_SCALAR = 0, 1, 2, 3
 PTR_HOST, PTR_DEVICE = 0, 1
-OPT_PLAIN_PIPELINE, OPT_GIVER_BATCH, OPT_XPASS_GIVERS, OPT_XPASS_BLOCK, OPT_SUBGRID = 1, 2, 3, 4, 5
+OPT_PLAIN_PIPELINE, OPT_GIVER_BATCH, OPT_XPASS_GIVERS, OPT_XPASS_BLOCK, OPT_SUBGRID, OPT_COL_TILE = 1, 2, 3, 4, 5, 6
 K_CLASSES = ("fft_column", "fft_row_inverse", "fft_row_forward", "density_shell", "reduce", "other",
              "fused_x")
 

@@ -166,7 +166,7 @@ __device__ __forceinline__ bool dealiased1(int i, int n) {
 
 // padded-grid index -> original-grid index (inverse of the fftshift embedding of
 // FFTWExt.jl:134-149); false when the padded bin is outside the embedded block.
-__device__ __forceinline__ bool pad_pre(int i, int n, int G, int& o) {
+__host__ __device__ __forceinline__ bool pad_pre(int i, int n, int G, int& o) {
   if (G == n) { o = i; return true; }
   if (i < (n + 1) / 2) { o = i; return true; }
   if (i >= G - n / 2) { o = n - (G - i); return true; }
@@ -181,35 +181,40 @@ namespace fit {
 // Register-resident Stockham FFT for compile-time lengths L = 2^a (E = 8 points per thread) or
 // 3·2^a (E = 24).  A line is held by T = L/E threads; thread t owns points t + T·e (e < E) before
 // and after every stage, so only the Stockham output scatter goes through LDS (double-buffered:
-// one barrier per stage).  Radix plan: 8s, then 4 / 2, then 3.
+// one barrier per stage).  Radix plan: 8s (when 8 divides E), then 4s (when 4 divides E), 2s,
+// then 3s; every radix divides E.
 // ---------------------------------------------------------------------------------------------
-constexpr int rp_nst(int L) {
+constexpr int rp_nst(int L, int E) {
   int n = 0, r = L;
-  while (r % 8 == 0) { r /= 8; ++n; }
-  while (r % 4 == 0) { r /= 4; ++n; }
+  while (r % 8 == 0 && E % 8 == 0) { r /= 8; ++n; }
+  while (r % 4 == 0 && E % 4 == 0) { r /= 4; ++n; }
   while (r % 2 == 0) { r /= 2; ++n; }
   while (r % 3 == 0) { r /= 3; ++n; }
   return n;
 }
-constexpr int rp_rad(int L, int s) {
+constexpr int rp_rad(int L, int E, int s) {
   int n = 0, r = L;
-  while (r % 8 == 0) { if (n == s) return 8; r /= 8; ++n; }
-  while (r % 4 == 0) { if (n == s) return 4; r /= 4; ++n; }
+  while (r % 8 == 0 && E % 8 == 0) { if (n == s) return 8; r /= 8; ++n; }
+  while (r % 4 == 0 && E % 4 == 0) { if (n == s) return 4; r /= 4; ++n; }
   while (r % 2 == 0) { if (n == s) return 2; r /= 2; ++n; }
   while (r % 3 == 0) { if (n == s) return 3; r /= 3; ++n; }
   return 1;
 }
 constexpr bool rp_pow2(int L) { return L > 0 && (L & (L - 1)) == 0; }
 constexpr int rp_E(int L) { return rp_pow2(L) ? 8 : 24; }
+// points per thread for kernels that keep two lines in registers: radix-4 stages for 3·2^a
+constexpr int gE(int L) { return rp_pow2(L) ? 8 : 12; }
+// points per thread of the workgroup-form fused x pass (measured: radix-8 stages win from 768 up)
+constexpr int xE(int L) { return rp_pow2(L) ? 8 : (L <= 384 ? 12 : 24); }
 
 template <int L, int E, int S, int NS, int SIGN, bool DB, typename R, class IDX>
 __device__ __forceinline__ void reg_stage(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* lds1, const IDX& idx, int t,
                                           const cpx<R>* __restrict__ tw) {
-  if constexpr (S < rp_nst(L)) {
-    constexpr int r = rp_rad(L, S);
+  if constexpr (S < rp_nst(L, E)) {
+    constexpr int r = rp_rad(L, E, S);
     constexpr int T = L / E;
     constexpr int NB = E / r;
-    constexpr bool LAST = S + 1 == rp_nst(L);
+    constexpr bool LAST = S + 1 == rp_nst(L, E);
     static_assert(E % r == 0, "radix must divide the points per thread");
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
@@ -253,6 +258,12 @@ __device__ __forceinline__ void reg_fft(cpx<R> (&v)[rp_E(L)], cpx<R>* lds0, cpx<
                                         const cpx<R>* __restrict__ tw) {
   reg_stage<L, rp_E(L), 0, 1, SIGN, DB, R, IDX>(v, lds0, lds1, idx, t, tw);
 }
+// the same with an explicit number of points per thread (T = L/E threads per line)
+template <int L, int E, int SIGN, bool DB, typename R, class IDX>
+__device__ __forceinline__ void reg_fft_e(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* lds1, const IDX& idx, int t,
+                                          const cpx<R>* __restrict__ tw) {
+  reg_stage<L, E, 0, 1, SIGN, DB, R, IDX>(v, lds0, lds1, idx, t, tw);
+}
 
 // line-major LDS index with one pad slot every 16 points (breaks power-of-two strides)
 struct IdxLine {
@@ -289,11 +300,11 @@ __device__ __forceinline__ void wave_lds_sync() {
 template <int L, int E, int S, int NS, int SIGN, typename R, class IDX>
 __device__ __forceinline__ void wave_stage(cpx<R> (&v)[E], cpx<R>* lds, const IDX& idx, int t,
                                            const cpx<R>* __restrict__ tw) {
-  if constexpr (S < rp_nst(L)) {
-    constexpr int r = rp_rad(L, S);
+  if constexpr (S < rp_nst(L, E)) {
+    constexpr int r = rp_rad(L, E, S);
     constexpr int T = L / E;
     constexpr int NB = E / r;
-    constexpr bool LAST = S + 1 == rp_nst(L);
+    constexpr bool LAST = S + 1 == rp_nst(L, E);
     static_assert(E % r == 0, "radix must divide the points per lane");
     if constexpr (!LAST) wave_lds_sync();  // reads of the previous stage are complete
 #pragma unroll
@@ -347,18 +358,18 @@ namespace fit {
 constexpr int tw_off(int L, int E, int S) {
   int off = 0, ns = 1;
   for (int s = 0; s < S; ++s) {
-    const int r = rp_rad(L, s);
+    const int r = rp_rad(L, E, s);
     if (s >= 1) off += (E / r) * (r - 1);
     ns *= r;
   }
   return off;
 }
-constexpr int tw_count(int L, int E) { return tw_off(L, E, rp_nst(L)) > 0 ? tw_off(L, E, rp_nst(L)) : 1; }
+constexpr int tw_count(int L, int E) { return tw_off(L, E, rp_nst(L, E)) > 0 ? tw_off(L, E, rp_nst(L, E)) : 1; }
 
 template <int L, int E, int S, int NS, typename R, int NTW>
 __device__ __forceinline__ void tw_fill_stage(cpx<R> (&twr)[NTW], int t, const cpx<R>* __restrict__ tw) {
-  if constexpr (S < rp_nst(L)) {
-    constexpr int r = rp_rad(L, S);
+  if constexpr (S < rp_nst(L, E)) {
+    constexpr int r = rp_rad(L, E, S);
     constexpr int T = L / E;
     constexpr int NB = E / r;
     if constexpr (S >= 1) {
@@ -382,11 +393,11 @@ __device__ __forceinline__ void tw_fill(cpx<R> (&twr)[tw_count(L, wE(L))], int t
 template <int L, int E, int S, int NS, int SIGN, typename R, class IDX, int NTW>
 __device__ __forceinline__ void wave_stage_tw(cpx<R> (&v)[E], cpx<R>* lds, const IDX& idx, int t,
                                               const cpx<R> (&twr)[NTW]) {
-  if constexpr (S < rp_nst(L)) {
-    constexpr int r = rp_rad(L, S);
+  if constexpr (S < rp_nst(L, E)) {
+    constexpr int r = rp_rad(L, E, S);
     constexpr int T = L / E;
     constexpr int NB = E / r;
-    constexpr bool LAST = S + 1 == rp_nst(L);
+    constexpr bool LAST = S + 1 == rp_nst(L, E);
     constexpr int off = tw_off(L, E, S);
     if constexpr (!LAST) wave_lds_sync();
 #pragma unroll

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -80,6 +81,7 @@ struct fit_plan {
   void* uphys = nullptr;   // nd physical advecting components
   void* nphys = nullptr;   // one physical product
   void* nhat = nullptr;    // M full original-grid N̂
+  void* nhat_dst = nullptr;  // caller's device N̂ array when written in place of nhat
   void* in_f = nullptr;    // staged primary field (M comps)
   void* in_a = nullptr;    // staged advecting field (D comps)
   void* in_c = nullptr;    // staged carrier / extra input (M comps)
@@ -105,6 +107,11 @@ struct fit_plan {
   int giver_batch = 0;           // cap on givers per batch (0 = memory budget only)
   int xfused_gc = 0;             // givers per workgroup in the wave-per-line x pass (0 = 4)
   int xfused_block = 0;          // 1: use the workgroup-barrier x pass instead
+  int col_tx = 0;                // FIT_OPT_COL_TILE (0: automatic)
+  // shared-transform gradients: wavenumbers odd under k -> -k on every populated bin
+  bool commute = false;
+  void* kt[2] = {nullptr, nullptr};  // transform-grid wavenumbers of x and y (compute precision)
+  bool grads_commuted = false;       // the last gradient passes produced slot 0 = h
   // band-limited sub-grid for the giver loop: when every shell lies in |k_d| <= R_d with
   // 4 R_d < n_d, T(n,m) only involves advecting modes |q_d| <= 2 R_d and the exact convolution
   // is evaluated alias-free on a grid of at least 4 R_d + 1 points per axis
@@ -206,6 +213,8 @@ Band full_band(int G) { return Band{G, G}; }
 // lengths with a compile-time register FFT (others use the generic LDS engine)
 #define FIT_REG_LENGTHS(X) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048) X(3072)
 constexpr int reg_T(int L) { return L / (((L & (L - 1)) == 0) ? 8 : 24); }
+constexpr int two_T(int L) { return L / gE(L); }  // threads per line of the two-line kernels
+constexpr int xblk_T(int L) { return L / xE(L); }  // threads per line of the workgroup x pass
 bool reg_supported(int L) {
   switch (L) {
 #define FIT_CASE(LL) case LL:
@@ -230,8 +239,11 @@ int launch_col(fit_plan* p, LD ld, ST st, int axis, int nfields, int nx, Band ob
   prof_begin(p, FIT_K_FFT_COLUMN, (double)nx * nob * (rows_in + rows_out) * csz * nfields);
   if (!p->plain && reg_supported(L)) {
     const int T = reg_T(L);
-    int TX = std::min(16, 1024 / T);
-    while (TX > 1 && (size_t)L * TX * csz > g_max_lds) TX /= 2;
+    // columns per workgroup: 16 (128-byte rows in FP32) within 64 KB of LDS, or the plan's cap
+    // within the 160 KB a workgroup may hold
+    int TX = std::min(p->col_tx > 0 ? p->col_tx : 16, 1024 / T);
+    const size_t lds_cap = p->col_tx > 0 ? (size_t)160 * 1024 : g_max_lds;
+    while (TX > 1 && (size_t)L * TX * csz > lds_cap) TX /= 2;
     dim3 grid((nx + TX - 1) / TX, nob, nfields);
     const size_t lds = (size_t)L * TX * csz;
     switch (L) {
@@ -275,8 +287,26 @@ int launch_row_inv(fit_plan* p, LD ld, ST st, int nfields, double extra_bytes_pe
   const int nin = std::min(kxa + 1, p->h1);
   prof_begin(p, FIT_K_FFT_ROW_INVERSE,
              ((double)nin * csz + (double)L * sizeof(R) + extra_bytes_per_row) * nrows * nfields);
-  hipLaunchKernelGGL((k_row_inv<R, LD, ST>), grid, dim3(kThreads), lds, p->stream, ld, st, p->fd[0],
-                     (const cpx<R>*)p->tw[0], p->G[1], nrows, P, kxa);
+  if (!p->plain && reg_supported(L)) {
+    const int T = reg_T(L);
+    const size_t LL = (size_t)L + L / 16;
+    int B = std::max(1, kThreads / T);
+    while (B > 1 && B * LL * csz > g_max_lds) --B;
+    dim3 rgrid((npairs + B - 1) / B, 1, nfields);
+    switch (L) {
+#define FIT_CASE(LLEN)                                                                                      \
+  case LLEN:                                                                                                \
+    hipLaunchKernelGGL((k_row_inv_r<R, LLEN, LD, ST>), rgrid, dim3(B * reg_T(LLEN)), B * LL * csz, p->stream, \
+                       ld, st, (const cpx<R>*)p->tw[0], p->G[1], nrows, B, kxa);                             \
+    break;
+      FIT_REG_LENGTHS(FIT_CASE)
+#undef FIT_CASE
+      default: break;
+    }
+  } else {
+    hipLaunchKernelGGL((k_row_inv<R, LD, ST>), grid, dim3(kThreads), lds, p->stream, ld, st, p->fd[0],
+                       (const cpx<R>*)p->tw[0], p->G[1], nrows, P, kxa);
+  }
   prof_end(p);
   FIT_HIP(hipGetLastError());
   return FIT_OK;
@@ -335,7 +365,7 @@ template <typename R>
 int forward_field(fit_plan* p, int c, double scale) {
   Grid g = grid_of<R>(p);
   LdReal<R> lr{(const R*)p->nphys, (int64_t)p->G[0], (int64_t)p->G[0] * p->G[1], 0};
-  StNhat<R> sn{g, (cpx<R>*)p->nhat + (int64_t)c * p->Np, 0, p->dealias == FIT_DEALIAS_TWOTHIRDS,
+  StNhat<R> sn{g, (cpx<R>*)(p->nhat_dst ? p->nhat_dst : p->nhat) + (int64_t)c * p->Np, 0, p->dealias == FIT_DEALIAS_TWOTHIRDS,
                (R)scale};
   int rc;
   if (p->nd == 1) return launch_row_fwd<R>(p, lr, sn, 1, p->h1);
@@ -416,6 +446,73 @@ int grow(fit_plan* p, void** buf, size_t* have, size_t need) {
 // x-extent of compact storage for a folded band on the half grid
 int xext(fit_plan* p, int a) { return std::min(a, p->G[0] / 2) + 1; }
 
+// shared-transform gradient passes apply: exact for this plan, register lengths on every axis
+bool commute_ok(const fit_plan* p) {
+  if (!p->commute || p->plain || p->nd < 2) return false;
+  for (int d = 0; d < p->nd; ++d)
+    if (!reg_supported(p->G[d])) return false;
+  return true;
+}
+
+// First and (3D) y passes of the shared-transform gradients of ng*M fields into work2.
+template <typename R>
+int grads_shared(fit_plan* p, const void* f, int giver0, int ng, const int* bi, const int4* fband, int xa,
+                 StHalf<R> st) {
+  const size_t csz = sizeof(cpx<R>);
+  const int nd = p->nd, ngc = ng * p->M;
+  const int ax = nd - 1;
+  LdSpectral<R> ls = spectral_loader<R>(p, f, 0, 0);
+  {
+    const int L = p->G[ax];
+    const int T = two_T(L);
+    int TX = std::min(16, 512 / T);
+    while (TX > 1 && (size_t)L * TX * csz > g_max_lds) TX /= 2;
+    const Band ob = nd == 3 ? Band{bi[1], p->G[1]} : full_band(1);
+    const Band ib = Band{bi[ax], L};
+    const Band wb = full_band(L);
+    const int nob = ob.count();
+    dim3 grid((xa + TX - 1) / TX, nob, ngc);
+    prof_begin(p, FIT_K_FFT_COLUMN, (double)ngc * xa * nob * (2.0 * ib.count() + 2.0 * wb.count()) * csz);
+    switch (L) {
+#define FIT_CASE(LL)                                                                                         \
+  case LL:                                                                                                   \
+    hipLaunchKernelGGL((k_grad1_r<R, LL>), grid, dim3(two_T(LL) * TX), (size_t)L * TX * csz, p->stream, ls,  \
+                       p->M, giver0, st, (const cpx<R>*)p->tw[ax], xa, TX, ax, ob, ib, wb, fband, p->G[1]);  \
+    break;
+      FIT_REG_LENGTHS(FIT_CASE)
+#undef FIT_CASE
+      default: break;
+    }
+    prof_end(p);
+    FIT_HIP(hipGetLastError());
+  }
+  if (nd == 3) {
+    const int L = p->G[1];
+    const int T = two_T(L);
+    int TX = std::min(16, 512 / T);
+    while (TX > 1 && (size_t)L * TX * csz > g_max_lds) TX /= 2;
+    const Band ob = full_band(p->G[2]);
+    const Band ib = Band{bi[1], L};
+    const Band wb = full_band(L);
+    dim3 grid((xa + TX - 1) / TX, ob.count(), ngc);
+    prof_begin(p, FIT_K_FFT_COLUMN, (double)ngc * xa * ob.count() * (2.0 * ib.count() + 3.0 * wb.count()) * csz);
+    switch (L) {
+#define FIT_CASE(LL)                                                                                         \
+  case LL:                                                                                                   \
+    hipLaunchKernelGGL((k_grad2_r<R, LL>), grid, dim3(two_T(LL) * TX), (size_t)L * TX * csz, p->stream,      \
+                       st.p, st.sy, st.sz, st.sf, (const R*)p->kt[1], (const cpx<R>*)p->tw[1], xa, TX, ob,   \
+                       ib, wb, fband, p->M);                                                                 \
+    break;
+      FIT_REG_LENGTHS(FIT_CASE)
+#undef FIT_CASE
+      default: break;
+    }
+    prof_end(p);
+    FIT_HIP(hipGetLastError());
+  }
+  return FIT_OK;
+}
+
 // Inverse passes (all but x) of ng*M*nd gradient fields (giver g, component c, direction j) of
 // f̂ into work2 (compact, pitch xa).  giver0 = 0: no shell filter.  fband: per-giver bounds.
 template <typename R>
@@ -425,9 +522,15 @@ int v2_inverse_grads(fit_plan* p, const void* f, int giver0, int ng, const int* 
   const size_t csz = sizeof(cpx<R>);
   int rc = grow(p, &p->work2, &p->work2_bytes, (size_t)nf * xa * p->G[1] * p->G[2] * csz);
   if (rc) return rc;
-  LdSpectralB<R> ls{spectral_loader<R>(p, f, 0, 0), p->M, giver0};
   const int64_t sy = xa, sz = (int64_t)xa * p->G[1], sf = sz * p->G[2];
   StHalf<R> st{(cpx<R>*)p->work2, sy, sz, sf};
+  p->grads_commuted = false;
+  if (commute_ok(p)) {
+    if ((rc = grads_shared<R>(p, f, giver0, ng, bi, fband, xa, st))) return rc;
+    p->grads_commuted = true;
+    return FIT_OK;
+  }
+  LdSpectralB<R> ls{spectral_loader<R>(p, f, 0, 0), p->M, giver0};
   LdHalf<R> ld{(const cpx<R>*)p->work2, sy, sz, sf};
   const int fdiv = p->M * p->nd;
   if (p->nd == 3) {
@@ -457,6 +560,8 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
   const int64_t osy = xb, osz = (int64_t)xb * p->G[1], osf = osz * p->G[2];
   const int64_t usy = L, usz = (int64_t)L * p->G[1];
   const double bytes = (double)nfg * nrows * (p->nd * (xa * csz) + xb * csz) + (double)p->nd * nrows * L * sizeof(R);
+  const R* kxt = p->grads_commuted ? (const R*)p->kt[0] : nullptr;
+  if (p->grads_commuted && !reg_supported(L)) return fail(FIT_E_UNSUPPORTED, "shared-transform gradients need a register x pass");
   prof_begin(p, FIT_K_FUSED_X, bytes);
   // wave form for short register footprints (E <= 16); the workgroup form otherwise
   const bool wave_ok = reg_supported(L) && !p->xfused_block && wE(L) <= 16 &&
@@ -477,7 +582,7 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
     hipLaunchKernelGGL((k_xfused_w<R, LLEN>), dim3((unsigned)((int64_t)nrg * nchunk)), dim3(256), lds, p->stream, \
                        (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf, (const R*)p->uphys, usy,    \
                        usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M, p->G[1], nrows,              \
-                       (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, ngiv, GC);                              \
+                       (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, ngiv, GC, kxt);                         \
     break;
       FIT_REG_LENGTHS(FIT_CASE)
 #undef FIT_CASE
@@ -485,7 +590,7 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
     }
   } else if (reg_supported(L)) {
     // workgroup form: B row pairs per workgroup, single LDS exchange buffer
-    const int T = reg_T(L);
+    const int T = xblk_T(L);
     const size_t LL = (size_t)L + L / 16;
     int B = std::max(1, 256 / T);
     while (B > 1 && B * LL * csz > g_max_lds) --B;
@@ -494,10 +599,10 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
     switch (L) {
 #define FIT_CASE(LLEN)                                                                                         \
   case LLEN:                                                                                                   \
-    hipLaunchKernelGGL((k_xfused_r<R, LLEN>), dim3((unsigned)((int64_t)nrg * nfg)), dim3(B * reg_T(LLEN)), lds, \
+    hipLaunchKernelGGL((k_xfused_r<R, LLEN>), dim3((unsigned)((int64_t)nrg * nfg)), dim3(B * xblk_T(LLEN)), lds, \
                        p->stream, (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf,             \
                        (const R*)p->uphys, usy, usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M,      \
-                       p->G[1], nrows, B, (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, nfg);               \
+                       p->G[1], nrows, B, (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, nfg, kxt);          \
     break;
       FIT_REG_LENGTHS(FIT_CASE)
 #undef FIT_CASE
@@ -527,9 +632,10 @@ int v2_forward_nhat(fit_plan* p, const int* bo) {
   Grid g = grid_of<R>(p);
   const bool trunc = p->dealias == FIT_DEALIAS_TWOTHIRDS;
   const double scale = (double)p->Gp / ((double)p->Np * (double)p->Np);
-  StNhat<R> sn{g, (cpx<R>*)p->nhat, p->Np, trunc, (R)scale};
+  void* dst = p->nhat_dst ? p->nhat_dst : p->nhat;
+  StNhat<R> sn{g, (cpx<R>*)dst, p->Np, trunc, (R)scale};
   int rc;
-  if (trunc) FIT_HIP(hipMemsetAsync(p->nhat, 0, (size_t)p->M * p->Np * sizeof(cpx<R>), p->stream));
+  if (trunc) FIT_HIP(hipMemsetAsync(dst, 0, (size_t)p->M * p->Np * sizeof(cpx<R>), p->stream));
   if (p->nd == 3) {
     if ((rc = launch_col<R, -1>(p, ld, st, 1, p->M, xb, full_band(p->G[2]), full_band(p->G[1]),
                                 Band{bo[1], p->G[1]})))
@@ -691,7 +797,20 @@ int stage_fa(fit_plan* p, const void* f_hat, const void* adv_hat, int io_dtype, 
 
 template <typename R>
 int nonlinear_term_impl(fit_plan* p, const void* f, const void* a) {
-  int rc = compute_u<R>(p, a);
+  int rc;
+  if (a == f && commute_ok(p)) {
+    // self-advection: u_j = Re ifft(keep f̂_j) is the slot-0 transform of component j, so only
+    // its x pass remains (the y and z passes are shared with the gradients)
+    if ((rc = v2_inverse_grads<R>(p, f, 0, 1, p->band_in, nullptr))) return rc;
+    const int xa = xext(p, p->band_in[0]);
+    const int64_t gsf = (int64_t)xa * p->G[1] * p->G[2];
+    LdHalf<R> ld{(const cpx<R>*)p->work2, (int64_t)xa, (int64_t)xa * p->G[1], (int64_t)p->nd * gsf};
+    StReal<R> sr{(R*)p->uphys, (int64_t)p->G[0], (int64_t)p->G[0] * p->G[1], p->Gp, (R)(1.0 / (double)p->Gp)};
+    if ((rc = launch_row_inv<R>(p, ld, sr, p->nd, 0.0, xa - 1))) return rc;
+    if ((rc = v2_xfused<R>(p, 1, p->band_in, nullptr, p->band_in))) return rc;
+    return v2_forward_nhat<R>(p, p->band_in);
+  }
+  rc = compute_u<R>(p, a);
   if (rc) return rc;
   if (p->plain || p->nd == 1) return nonlinear_from_u<R>(p, f, 0);
   return v2_nonlinear<R>(p, f);
@@ -937,6 +1056,44 @@ int m2m_impl(fit_plan* p, int inv, const void* f, const void* a, void* S, void* 
 // =============================================================================================
 // C ABI
 // =============================================================================================
+// Shared-transform gradients are exact when ks is odd (ks[0] = 0, ks[n-i] = -ks[i] off the
+// Nyquist index) and no self-conjugate Nyquist bin is populated: Orszag 2/3 masks it, the padded
+// embedding leaves its mirror empty, odd lengths have none, and a sub-grid copies no mode that far.
+static int setup_commute(fit_plan* p, const double* const* ks, bool nyquist_zero) {
+  p->commute = false;
+  if (p->nd < 2) return FIT_OK;
+  bool odd_all = true;
+  for (int d = 0; d < p->nd; ++d) {
+    const int n = p->n[d];
+    if (ks[d][0] != 0.0) return FIT_OK;
+    for (int i = 1; 2 * i < n; ++i)
+      if (ks[d][n - i] != -ks[d][i]) return FIT_OK;
+    odd_all = odd_all && (n % 2 == 1);
+  }
+  if (!(nyquist_zero || odd_all || p->dealias == FIT_DEALIAS_PADDED || p->dealias == FIT_DEALIAS_TWOTHIRDS))
+    return FIT_OK;
+  for (int d = 0; d < 2; ++d) {
+    const int G = p->G[d], n = p->n[d];
+    std::vector<double> kt((size_t)G, 0.0);
+    for (int i = 0; i < G; ++i) {
+      int o = 0, om = 0;
+      const int mi = i == 0 ? 0 : G - i;
+      if (pad_pre(i, n, G, o)) kt[i] = ks[d][o];
+      else if (pad_pre(mi, n, G, om)) kt[i] = -ks[d][om];
+    }
+    int rc = dalloc(p, &p->kt[d], (size_t)G * p->rsz);
+    if (rc) return rc;
+    if (p->dtype == FIT_F64) {
+      FIT_HIP(hipMemcpy(p->kt[d], kt.data(), (size_t)G * 8, hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> kf(kt.begin(), kt.end());
+      FIT_HIP(hipMemcpy(p->kt[d], kf.data(), (size_t)G * 4, hipMemcpyHostToDevice));
+    }
+  }
+  p->commute = true;
+  return FIT_OK;
+}
+
 // Plan construction (the caller holds the creation lock).  allow_sub: may attach a band-limited
 // sub-grid child for the giver loop (children never nest).
 static int create_plan(fit_plan** out, int nd, const int64_t* ns, int M, int D, int compute_dtype, int dealias,
@@ -999,6 +1156,8 @@ static int setup_subgrid(fit_plan* p, const double* const* ks, const std::vector
   if (rc) return rc;
   // inputs beyond the copied box are zero: prune the child's input band to it
   for (int d = 0; d < p->nd; ++d) c->band_in[d] = std::min(c->band_in[d], Q[d]);
+  dfree(c->kt[0]); dfree(c->kt[1]);
+  if ((rc = setup_commute(c, kp, true))) { fit_plan_destroy(c); return rc; }
   (void)hipStreamDestroy(c->stream);
   c->stream = p->stream;
   c->own_stream = false;
@@ -1079,6 +1238,7 @@ static int create_plan(fit_plan** out, int nd, const int64_t* ns, int M, int D, 
       hipMemcpy(p->ks[d], kf.data(), (size_t)nk * rs, hipMemcpyHostToDevice);
     }
   }
+  if ((rc = setup_commute(p, ks, false))) return bail(rc);
   // folded support of the spectral inputs: kept box (2/3), all modes otherwise (the embedded
   // block on the padded grid has the same folded extent n/2)
   for (int d = 0; d < 3; ++d) {
@@ -1170,6 +1330,7 @@ int fit_plan_destroy(fit_plan* p) {
   if (p->sub) { fit_plan_destroy(p->sub); p->sub = nullptr; }
   dfree(p->sub_f); dfree(p->sub_a);
   for (int d = 0; d < 3; ++d) { dfree(p->tw[d]); dfree(p->ks[d]); }
+  dfree(p->kt[0]); dfree(p->kt[1]);
   void* sh = p->shell; dfree(sh);
   dfree(p->work); dfree(p->uphys); dfree(p->nphys); dfree(p->nhat);
   dfree(p->in_f); dfree(p->in_a); dfree(p->in_c); dfree(p->out_tmp);
@@ -1216,6 +1377,11 @@ int fit_plan_set_option(fit_plan* p, int option, int value) {
       return FIT_OK;
     case FIT_OPT_XPASS_BLOCK: p->xfused_block = value != 0; return FIT_OK;
     case FIT_OPT_SUBGRID: p->use_sub = value != 0; return FIT_OK;
+    case FIT_OPT_COL_TILE:
+      if (value < 0 || value > 64 || (value & (value - 1)) != 0)
+        return fail(FIT_E_ARGUMENT, "column tile must be 0 or a power of two <= 64");
+      p->col_tx = value;
+      return FIT_OK;
     default: return fail(FIT_E_ARGUMENT, "unknown plan option");
   }
 }
@@ -1246,9 +1412,22 @@ int fit_nonlinear_term(fit_plan* p, const void* f_hat, const void* adv_hat, void
   if (!N_hat) return fail(FIT_E_ARGUMENT, "N_hat is NULL");
   const void *f, *a;
   if ((rc = stage_fa(p, f_hat, adv_hat, io_dtype, ptr_kind, &f, &a))) return rc;
+  // a device output of the compute precision that does not overlap the inputs is written by the
+  // last pass directly (no staging copy)
+  const size_t cb = csize(p->dtype);
+  auto disjoint = [](const void* x, size_t xb, const void* y, size_t yb) {
+    const uintptr_t x0 = (uintptr_t)x, y0 = (uintptr_t)y;
+    return x0 + xb <= y0 || y0 + yb <= x0;
+  };
+  const size_t nb = (size_t)p->M * p->Np * cb;
+  const bool direct = ptr_kind == FIT_PTR_DEVICE && io_dtype == p->dtype &&
+                      disjoint(N_hat, nb, f, (size_t)p->M * p->Np * cb) &&
+                      disjoint(N_hat, nb, a, (size_t)p->D * p->Np * cb);
+  p->nhat_dst = direct ? N_hat : nullptr;
   rc = p->dtype == FIT_F64 ? nonlinear_term_impl<double>(p, f, a) : nonlinear_term_impl<float>(p, f, a);
+  p->nhat_dst = nullptr;
   if (rc) return rc;
-  if ((rc = stage_out_real(p, p->nhat, N_hat, 2 * (int64_t)p->M * p->Np, io_dtype, ptr_kind))) return rc;
+  if (!direct && (rc = stage_out_real(p, p->nhat, N_hat, 2 * (int64_t)p->M * p->Np, io_dtype, ptr_kind))) return rc;
   return finish(p);
 }
 

@@ -89,6 +89,17 @@ template <typename R> struct LdSpectral {
     }
     return v;
   }
+  // masked source value and the wavenumber of axis dk at its original index (0 outside)
+  __device__ __forceinline__ cpx<R> Graw(int x, int y, int z, int comp, int dk, R& k) const {
+    int ox, oy, oz;
+    k = (R)0;
+    if (!g.pre(x, y, z, ox, oy, oz)) return mk<R>((R)0, (R)0);
+    if (trunc && g.dealiased(ox, oy, oz)) return mk<R>((R)0, (R)0);
+    const int64_t o = g.oidx(ox, oy, oz);
+    if (giver > 0 && shell[o] != giver) return mk<R>((R)0, (R)0);
+    k = dk == 0 ? ks0[ox] : (dk == 1 ? ks1[oy] : ks2[oz]);
+    return src[o + comp * sf];
+  }
   __device__ __forceinline__ cpx<R> operator()(int x, int y, int z, int f) const {
     const int comp = f / src_per_field_div;
     const int gj = gradj_per_field ? (f % g.nd) : gradj;
@@ -255,6 +266,43 @@ __global__ void __launch_bounds__(256) k_row_inv(LD ld, ST st, FftDesc d, const 
   }
 }
 
+// The same inverse pass with the register FFT: B row pairs per workgroup, T = L/E threads each
+// (thread t holds points t + T e), one LDS exchange buffer per line.
+template <typename R, int L, class LD, class ST>
+__global__ void __launch_bounds__(256) k_row_inv_r(LD ld, ST st, const cpx<R>* __restrict__ tw, int G2,
+                                                   int nrows, int B, int kxa) {
+  constexpr int E = rp_E(L), T = L / E, LL = ldsline(L);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* lds0 = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int line = threadIdx.x / T, t = threadIdx.x % T;
+  const int f = blockIdx.z;
+  const int pair = blockIdx.x * B + line;
+  const int r0 = 2 * pair, r1 = r0 + 1;
+  const bool ok0 = r0 < nrows, ok1 = r1 < nrows;
+  const int y0 = ok0 ? r0 % G2 : 0, z0 = ok0 ? r0 / G2 : 0, y1 = ok1 ? r1 % G2 : 0, z1 = ok1 ? r1 / G2 : 0;
+  cpx<R> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = t + T * e;
+    const int kk = k <= L / 2 ? k : L - k;
+    cpx<R> A = mk<R>((R)0, (R)0), Bv = A;
+    if (kk <= kxa) {
+      if (ok0) A = ld(kk, y0, z0, f);
+      if (ok1) Bv = ld(kk, y1, z1, f);
+    }
+    if (k == 0 || 2 * k == L) { A.y = (R)0; Bv.y = (R)0; }
+    if (k > L / 2) { A.y = -A.y; Bv.y = -Bv.y; }
+    v[e] = mk<R>(A.x - Bv.y, A.y + Bv.x);
+  }
+  reg_fft<L, 1, false, R>(v, lds0, lds0, IdxLine{line * LL}, t, tw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int x = t + T * e;
+    if (ok0) st(x, y0, z0, f, v[e].x);
+    if (ok1) st(x, y1, z1, f, v[e].y);
+  }
+}
+
 // Contiguous-axis forward pass (r2c) on row pairs: z = a + ib, Z = FFT(z),
 // A(k) = (Z(k) + conj Z(L-k))/2,  B(k) = (Z(k) - conj Z(L-k))/(2i),  k = 0..L/2.
 template <typename R, class LD, class ST>
@@ -406,6 +454,155 @@ __global__ void __launch_bounds__(1024) k_col_r(LD ld, ST st, const cpx<R>* __re
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Gradient inverse passes with shared transforms.  The Hermitian part of i k_d·f̂ equals
+// i k_t(d)·h, h the Hermitian part of f̂, whenever the wavenumbers are odd under k -> -k on every
+// bin the input reaches (no populated self-conjugate Nyquist bin); k_t is the transform-grid
+// wavenumber (the original one inside the embedded block, minus the mirror's outside it).  As
+// i k_t(d) is constant along every other axis it commutes with their transforms, so per field
+// f_c the passes run
+//   first pass (last axis a = nd-1):  slot 0 <- F_a^-1 h,   slot a <- F_a^-1 [exact i k_a f̂]
+//   y pass (nd = 3):                  slot 0 <- F_y^-1 slot 0,  slot 1 <- F_y^-1 [i k_t(y) slot 0],
+//                                     slot 2 <- F_y^-1 slot 2
+// and the x pass multiplies slot 0 by i k_t(x): one source load and nd transforms per pass
+// instead of nd loads of the source.  Fields are (g, c, j) = (g M + c) nd + j as before.
+// Two lines in flight per thread: at most 512 threads per workgroup keeps them in registers.
+// ---------------------------------------------------------------------------------------------
+template <typename R, int L>
+__global__ void __launch_bounds__(512) k_grad1_r(LdSpectral<R> ls, int M, int giver0, StHalf<R> st,
+                                                  const cpx<R>* __restrict__ tw, int nx, int TX, int axis,
+                                                  Band ob, Band ib, Band wb, const int4* __restrict__ fband,
+                                                  int Gother) {
+  constexpr int E = gE(L), T = L / E;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int xl = threadIdx.x % TX, t = threadIdx.x / TX;
+  const int x0 = blockIdx.x * TX;
+  const int o = ob.at(blockIdx.y);
+  const int gc = blockIdx.z;
+  const int g = gc / M, c = gc % M;
+  if (fband) {
+    const int4 fb = fband[g];
+    if (x0 > fb.x) return;
+    if (axis == 2) {
+      if (!Band{fb.y, Gother}.has(o)) return;
+      ib = Band{fb.z, L};
+    } else {
+      ib = Band{fb.y, L};
+    }
+  }
+  LdSpectral<R> s = ls;
+  s.giver = giver0 > 0 ? giver0 + g : 0;
+  const int nd = s.g.nd;
+  const int x = x0 + xl;
+  const int mx = x == 0 ? 0 : s.g.G1 - x;
+  cpx<R> v0[E], v1[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = t + T * e;
+    v0[e] = mk<R>((R)0, (R)0);
+    v1[e] = v0[e];
+    if (x < nx && ib.has(i)) {
+      const int yy = axis == 1 ? i : o, zz = axis == 1 ? o : i;
+      const int my = yy == 0 ? 0 : s.g.G2 - yy, mz = zz == 0 ? 0 : s.g.G3 - zz;
+      R ka, kb;
+      const cpx<R> a = s.Graw(x, yy, zz, c, axis, ka);
+      const cpx<R> b = s.Graw(mx, my, mz, c, axis, kb);
+      v0[e] = mk<R>((R)0.5 * (a.x + b.x), (R)0.5 * (a.y - b.y));
+      v1[e] = mk<R>((R)-0.5 * (ka * a.y + kb * b.y), (R)0.5 * (ka * a.x - kb * b.x));
+    }
+  }
+  const int f0 = gc * nd, fa = gc * nd + nd - 1;
+  reg_fft_e<L, E, 1, false, R>(v0, lds, lds, IdxCol{xl, TX}, t, tw);
+  if (x < nx) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = t + T * e;
+      if (wb.has(i)) {
+        if (axis == 1) st(x, i, o, f0, v0[e]);
+        else st(x, o, i, f0, v0[e]);
+      }
+    }
+  }
+  __syncthreads();
+  reg_fft_e<L, E, 1, false, R>(v1, lds, lds, IdxCol{xl, TX}, t, tw);
+  if (x < nx) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = t + T * e;
+      if (wb.has(i)) {
+        if (axis == 1) st(x, i, o, fa, v1[e]);
+        else st(x, o, i, fa, v1[e]);
+      }
+    }
+  }
+}
+
+// y pass of the shared-transform gradients (nd = 3), in place on the (g, c, j) fields.
+template <typename R, int L>
+__global__ void __launch_bounds__(512) k_grad2_r(cpx<R>* __restrict__ buf, int64_t sy, int64_t sz, int64_t sf,
+                                                  const R* __restrict__ kyt, const cpx<R>* __restrict__ tw,
+                                                  int nx, int TX, Band ob, Band ib, Band wb,
+                                                  const int4* __restrict__ fband, int M) {
+  constexpr int E = gE(L), T = L / E;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int xl = threadIdx.x % TX, t = threadIdx.x / TX;
+  const int x0 = blockIdx.x * TX;
+  const int o = ob.at(blockIdx.y);
+  const int gc = blockIdx.z;
+  if (fband) {
+    const int4 fb = fband[gc / M];
+    if (x0 > fb.x) return;
+    ib = Band{fb.y, L};
+  }
+  const int x = x0 + xl;
+  cpx<R>* f0 = buf + (int64_t)(gc * 3) * sf + x + o * sz;
+  cpx<R>* f1 = f0 + sf;
+  cpx<R>* f2 = f1 + sf;
+  cpx<R> v0[E], v1[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = t + T * e;
+    v0[e] = mk<R>((R)0, (R)0);
+    if (x < nx && ib.has(i)) v0[e] = f0[i * sy];
+    const R k = kyt[i];
+    v1[e] = mk<R>(-k * v0[e].y, k * v0[e].x);
+  }
+  reg_fft_e<L, E, 1, false, R>(v0, lds, lds, IdxCol{xl, TX}, t, tw);
+  if (x < nx) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = t + T * e;
+      if (wb.has(i)) f0[i * sy] = v0[e];
+    }
+  }
+  __syncthreads();
+  reg_fft_e<L, E, 1, false, R>(v1, lds, lds, IdxCol{xl, TX}, t, tw);
+  if (x < nx) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = t + T * e;
+      if (wb.has(i)) f1[i * sy] = v1[e];
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = t + T * e;
+    v0[e] = mk<R>((R)0, (R)0);
+    if (x < nx && ib.has(i)) v0[e] = f2[i * sy];
+  }
+  __syncthreads();
+  reg_fft_e<L, E, 1, false, R>(v0, lds, lds, IdxCol{xl, TX}, t, tw);
+  if (x < nx) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = t + T * e;
+      if (wb.has(i)) f2[i * sy] = v0[e];
+    }
+  }
+}
+
 // Fused x pass (see k_xfused) with the register FFT: B row pairs per workgroup, T threads each,
 // one LDS exchange buffer with workgroup barriers (fallback when the wave form's LDS does not fit).
 template <typename R, int L>
@@ -414,8 +611,9 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
                                                   int64_t usy, int64_t usz, int64_t usf, cpx<R>* __restrict__ out,
                                                   int64_t osy, int64_t osz, int64_t osf, int nd, int M, int G2,
                                                   int nrows, int B, R gscale, int kxa,
-                                                  const int4* __restrict__ fband, int kxb, int nfg) {
-  constexpr int E = rp_E(L), T = L / E, LL = ldsline(L);
+                                                  const int4* __restrict__ fband, int kxb, int nfg,
+                                                  const R* __restrict__ kxt) {
+  constexpr int E = xE(L), T = L / E, LL = ldsline(L);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* lds0 = reinterpret_cast<cpx<R>*>(smem_raw);
   const int line = threadIdx.x / T, t = threadIdx.x % T;
@@ -442,12 +640,17 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
         if (ok0) A = gf[kk + ro0];
         if (ok1) Bv = gf[kk + ro1];
       }
+      if (kxt && j == 0) {  // slot 0 holds h: the x derivative is i k_t(x) h
+        const R kx = kxt[kk];
+        A = mk<R>(-kx * A.y, kx * A.x);
+        Bv = mk<R>(-kx * Bv.y, kx * Bv.x);
+      }
       if (k == 0 || 2 * k == L) { A.y = (R)0; Bv.y = (R)0; }
       if (k > L / 2) { A.y = -A.y; Bv.y = -Bv.y; }
       v[e] = mk<R>(A.x - Bv.y, A.y + Bv.x);
     }
     __syncthreads();
-    reg_fft<L, 1, false, R>(v, lds0, lds0, idx, t, tw);
+    reg_fft_e<L, E, 1, false, R>(v, lds0, lds0, idx, t, tw);
     const R* uj = u + (int64_t)j * usf;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -459,7 +662,7 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
     }
   }
   __syncthreads();
-  reg_fft<L, -1, false, R>(acc, lds0, lds0, idx, t, tw);
+  reg_fft_e<L, E, -1, false, R>(acc, lds0, lds0, idx, t, tw);
   __syncthreads();
 #pragma unroll
   for (int e = 0; e < E; ++e) lds0[idx(t + T * e)] = acc[e];
@@ -488,9 +691,9 @@ __global__ void __launch_bounds__(256) k_xfused_w(const cpx<R>* __restrict__ tw,
                                                   int64_t usy, int64_t usz, int64_t usf, cpx<R>* __restrict__ out,
                                                   int64_t osy, int64_t osz, int64_t osf, int nd, int M, int G2,
                                                   int nrows, R gscale, int kxa, const int4* __restrict__ fband,
-                                                  int kxb, int ngiv, int GC) {
+                                                  int kxb, int ngiv, int GC, const R* __restrict__ kxt) {
   constexpr int E = wE(L), T = L / E, LPW = 64 / T, LL = ldsline(L);
-  constexpr bool UREG = E <= 16;                 // keep u in registers for short lines
+  constexpr bool UREG = E <= 8;                  // keep u in registers for short lines
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* ldsb = reinterpret_cast<cpx<R>*>(smem_raw);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -549,6 +752,13 @@ __global__ void __launch_bounds__(256) k_xfused_w(const cpx<R>* __restrict__ tw,
           if (!(u0 && ok1)) B0 = z0;
           if (!(u1 && ok0)) A1 = z0;
           if (!(u1 && ok1)) B1 = z0;
+          if (kxt && j == 0) {  // slot 0 holds h: the x derivative is i k_t(x) h
+            const R k0 = kxt[ka0], k1 = kxt[ka1];
+            A0 = mk<R>(-k0 * A0.y, k0 * A0.x);
+            B0 = mk<R>(-k0 * B0.y, k0 * B0.x);
+            A1 = mk<R>(-k1 * A1.y, k1 * A1.x);
+            B1 = mk<R>(-k1 * B1.y, k1 * B1.x);
+          }
           if (t == 0) { A0.y = (R)0; B0.y = (R)0; }
           // slot E-1 lies above L/2: conjugate of the stored bin T - t
           v[0] = mk<R>(A0.x - B0.y, A0.y + B0.x);
@@ -562,6 +772,11 @@ __global__ void __launch_bounds__(256) k_xfused_w(const cpx<R>* __restrict__ tw,
             if (kk <= ka) {
               if (ok0) A = gf[kk + ro0];
               if (ok1) Bv = gf[kk + ro1];
+            }
+            if (kxt && j == 0) {
+              const R kx = kxt[kk];
+              A = mk<R>(-kx * A.y, kx * A.x);
+              Bv = mk<R>(-kx * Bv.y, kx * Bv.x);
             }
             if (k == 0 || 2 * k == L) { A.y = (R)0; Bv.y = (R)0; }
             if (k > L / 2) { A.y = -A.y; Bv.y = -Bv.y; }

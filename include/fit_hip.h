@@ -74,9 +74,11 @@ int64_t fit_plan_device_bytes(const fit_plan* plan);
  *   FIT_OPT_XPASS_GIVERS = 3: givers per workgroup in the fused contiguous-axis pass (0 = 4).
  *   FIT_OPT_XPASS_BLOCK = 4: value 1 selects the workgroup-barrier form of that pass.
  *   FIT_OPT_SUBGRID = 5: value 0 keeps the giver loop on the plan's own grid even when a
- *     band-limited sub-grid is attached (1, default: use it). */
+ *     band-limited sub-grid is attached (1, default: use it).
+ *   FIT_OPT_COL_TILE = 6: columns per workgroup of the strided-axis passes (power of two; 0 =
+ *     automatic: 16 within 64 KB of LDS). */
 enum { FIT_OPT_PLAIN_PIPELINE = 1, FIT_OPT_GIVER_BATCH = 2, FIT_OPT_XPASS_GIVERS = 3,
-       FIT_OPT_XPASS_BLOCK = 4, FIT_OPT_SUBGRID = 5 };
+       FIT_OPT_XPASS_BLOCK = 4, FIT_OPT_SUBGRID = 5, FIT_OPT_COL_TILE = 6 };
 int fit_plan_set_option(fit_plan* plan, int option, int value);
 
 /* Band-limited sub-grid of the giver loop (no reference counterpart).  When every shell lies in

@@ -18,7 +18,7 @@ def say(*a):
     print(*a, flush=True)
 
 
-SHAPES = [(256, 6), (384, 6), (512, 4), (768, 4), (6, 256), (6, 384), (4, 512), (4, 768),
+SHAPES = [(128, 6), (192, 6), (6, 128), (6, 192), (4, 4, 128), (4, 4, 192), (256, 6), (384, 6), (512, 4), (768, 4), (6, 256), (6, 384), (4, 512), (4, 768),
           (4, 4, 256), (4, 4, 384), (4, 4, 512), (4, 4, 768), (1024, 4), (4, 1024), (1536, 2), (2, 1536)]
 
 
