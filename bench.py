@@ -80,6 +80,85 @@ def synthetic_velocity_gpu(n, seed, torch):
     return u.permute(3, 2, 1, 0)              # (x, y, z, c) view, column-major memory
 
 
+def synthetic_2d_gpu(n, D, seed, spectrum, solenoidal, torch):
+    """Seeded Hermitian 2D field (n, n, D) built in spectral space on the device: complex
+    Gaussian g, û = (g(k) + conj g(-k))/2, per-mode |û|² reshaped to k^-3 (spectrum="k-3") or
+    k^-1 ("k-1"), Leray projection when solenoidal, DC and Nyquist lines zeroed, Σ|û|² = 1.
+    Returned as a column-major complex64 view."""
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    shape = (D, n, n)                          # C order (c, y, x) == Julia (x, y, c)
+    g = torch.complex(torch.randn(shape, generator=gen, device="cuda"),
+                      torch.randn(shape, generator=gen, device="cuda"))
+    neg = torch.roll(torch.flip(g, dims=(1, 2)), shifts=(1, 1), dims=(1, 2))
+    u = 0.5 * (g + neg.conj())
+    del g, neg
+    i = torch.arange(n, device="cuda")
+    k1 = torch.where(i <= n // 2, i, i - n).to(torch.float32)
+    kx = k1.view(1, n)
+    ky = k1.view(n, 1)
+    kk = torch.sqrt(kx * kx + ky * ky)
+    ksafe = torch.where(kk > 0, kk, torch.ones_like(kk))
+    p = {"k-3": -3.0, "k-1": -1.0}[spectrum]
+    amp = torch.where(kk > 0, ksafe ** (0.5 * p), torch.zeros_like(kk))
+    for c in range(D):
+        a = u[c].abs()
+        u[c] = torch.where(a > 0, u[c] / torch.where(a > 0, a, torch.ones_like(a)), u[c]) * amp
+    if solenoidal:
+        div = kx * u[0] + ky * u[1]
+        k2 = ksafe * ksafe
+        u[0] -= kx * div / k2
+        u[1] -= ky * div / k2
+    h = n // 2
+    u[:, :, h] = 0
+    u[:, h, :] = 0
+    u[:, 0, 0] = 0
+    u /= torch.sqrt(torch.sum(u.abs() ** 2))
+    return u.permute(2, 1, 0)                  # (x, y, c) view, column-major memory
+
+
+def ensemble_bench(fit, torch, steps):
+    """Uneven 2D ensemble: 64 snapshots with sizes drawn by seed 4 from {256², 512², 1024², 2048²},
+    each a solenoidal k^-3 velocity (seed 4000+j) advecting a k^-1 scalar; kinetic-energy and
+    scalar-variance fluxes of every snapshot, one captured graph per replay."""
+    import numpy as np
+    rng = np.random.default_rng(4)
+    sizes = [int(n) for n in rng.choice([256, 512, 1024, 2048], 64)]
+    us = [synthetic_2d_gpu(n, 2, 4000 + j, "k-3", True, torch) for j, n in enumerate(sizes)]
+    ths = [synthetic_2d_gpu(n, 1, 5000 + j, "k-1", False, torch)[..., 0] for j, n in enumerate(sizes)]
+    kss = [fit.wavenumber_grid((n, n), (2 * math.pi,) * 2) for n in sizes]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens = fit.Ensemble([(n, n) for n in sizes], kss, with_scalar=True, binning=fit.LinearBinning(1.0))
+    t_setup = time.perf_counter() - t0
+    ens.raw_fluxes(us, ths)                    # eager run + graph capture
+    ens.raw_fluxes(us, ths)                    # first replay
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = ens.raw_fluxes(us, ths)
+    elapsed = time.perf_counter() - t0
+    # per-kernel-class device time of one eager pass
+    ens.profile(True)
+    ens.raw_fluxes(us, ths)
+    stats = ens.profile_stats()
+    ens.profile(False)
+    # dense algorithmic bytes (each logical transform reads its input and writes its output once:
+    # 4·Np + 8·H per transform, 11 transforms per snapshot, plus the inputs and the densities)
+    alg = 0.0
+    for n in sizes:
+        Np, H = n * n, (n // 2 + 1) * n
+        alg += 11 * (4.0 * Np + 8.0 * H) + 3 * 8.0 * Np + 2 * (2 + 1) * 8.0 * H
+    dev_ms = sum(v["ms"] for v in stats.values())
+    ens.close()
+    return {"workload": "64 2D snapshots, sizes drawn by seed 4 from {256,512,1024,2048}^2 "
+                        f"({sizes.count(256)}/{sizes.count(512)}/{sizes.count(1024)}/{sizes.count(2048)}); "
+                        "kinetic-energy + passive-scalar Π(K) per snapshot, OrszagTwoThirds, FP32",
+            "value": steps / elapsed, "unit": "ensembles/s", "ms_per_ensemble": elapsed * 1e3 / steps,
+            "groups": 4, "setup_s": t_setup,
+            "dense_alg_bytes": alg, "dense_alg_GBs": alg / (elapsed / steps) / 1e9,
+            "eager_device_ms": dev_ms, "kernel_classes": stats}
+
+
 def measured_traffic(plan_key, kernel_class, n, ng):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, copy-calibrated) when one exists for this kernel and workload."""
@@ -137,6 +216,9 @@ def main():
                     help="skip the one-off FP64 device run the FP32 errors are measured against")
     ap.add_argument("--no-fp64-plain", action="store_true",
                     help="skip timing the plain (unfused, unpruned, one giver at a time) FP64 form")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the uneven 2D ensemble line (64 snapshots, graph replays)")
+    ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device copy first (reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -288,6 +370,11 @@ def main():
                                              "definition": "max|X - X64plain| / max|X64plain|"}
             log("[bench] errors vs plain " + json.dumps(result["error_vs_fp64_plain"]))
         del u64
+    if not args.no_ensemble:
+        log("[bench] uneven 2D ensemble (64 snapshots)")
+        torch.cuda.empty_cache()
+        result["ensemble"] = ensemble_bench(fit, torch, args.ensemble_steps)
+        log(f"[bench] ensemble: {result['ensemble']['ms_per_ensemble']:.2f} ms per ensemble")
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline sample (FP64 oracle, one giver column)")
         workers = max(1, min(args.cpu_workers, os.cpu_count() or 1))

@@ -4,7 +4,7 @@ Public names mirror the reference (src/FlowInvariantTransfer.jl exports for this
 computation runs in libfit_hip.so (hand-written HIP kernels), loaded through ctypes.
 """
 from ._lib import DimensionMismatch, LIB_PATH, load_library
-from .api import (Plan, batch_map, calculate_ensemble_fluxes, calculate_helical_partial_fluxes, calculate_mode_to_mode_transfer,
+from .api import (Ensemble, Plan, batch_map, calculate_ensemble_fluxes, calculate_helical_partial_fluxes, calculate_mode_to_mode_transfer,
                   calculate_partial_fluxes, calculate_scalar_flux,
                   calculate_scalar_mode_to_mode_transfer, calculate_scalar_shell_to_shell_transfer,
                   calculate_shell_to_shell_transfer, calculate_spectral_flux, clear_plan_cache,

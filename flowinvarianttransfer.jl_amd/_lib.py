@@ -14,6 +14,7 @@ DEALIAS_NONE, DEALIAS_TWOTHIRDS, DEALIAS_PADDED = 0, 1, 2
 KINETIC_ENERGY, HELICITY, ENSTROPHY, PASSIVE_SCALAR = 0, 1, 2, 3
 PTR_HOST, PTR_DEVICE = 0, 1
 OPT_PLAIN_PIPELINE, OPT_GIVER_BATCH, OPT_XPASS_GIVERS, OPT_XPASS_BLOCK, OPT_SUBGRID, OPT_COL_TILE = 1, 2, 3, 4, 5, 6
+ENS_OPT_GRAPH, ENS_OPT_BATCH = 1, 2
 K_CLASSES = ("fft_column", "fft_row_inverse", "fft_row_forward", "density_shell", "reduce", "other",
              "fused_x")
 
@@ -41,6 +42,17 @@ SIGNATURES = {
     "fit_profile_enable": (_I, [_P, _I]),
     "fit_profile_query": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int64), _D, _D]),
     "fit_plan_stream": (_P, [_P]),
+    "fit_ensemble_create": (_I, [ctypes.POINTER(_P), _I, _I, ctypes.POINTER(ctypes.c_int64), _I, _I,
+                                 ctypes.POINTER(ctypes.POINTER(ctypes.c_double)),
+                                 ctypes.POINTER(ctypes.POINTER(ctypes.c_int64)), ctypes.POINTER(_I), _I,
+                                 _I]),
+    "fit_ensemble_destroy": (_I, [_P]),
+    "fit_ensemble_groups": (_I, [_P]),
+    "fit_ensemble_set_option": (_I, [_P, _I, _I]),
+    "fit_ensemble_flux": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), _D, _D, _D, _D, _I, _I]),
+    "fit_ensemble_profile_enable": (_I, [_P, _I]),
+    "fit_ensemble_profile_query": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int64), _D, _D]),
+    "fit_ensemble_stream": (_P, [_P]),
 }
 
 

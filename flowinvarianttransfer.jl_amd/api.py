@@ -17,8 +17,8 @@ from . import _lib
 from .grid import as_component_field, shell_setup
 from .types import (AbstractDealiasing, AbstractInvariant, HelicalDecomposition, HIPExecution,
                     HIPSpectral, KineticEnergy, ModeToModeTriadResult, NoDecomposition,
-                    OrszagTwoThirds, PassiveScalar, ShellMagnitude, ShellToShellResult,
-                    SpectralFluxResult)
+                    OrszagTwoThirds, PaddedThreeHalves, PassiveScalar, ShellMagnitude,
+                    ShellToShellResult, SpectralFluxResult)
 
 try:  # torch is only plumbing for device arrays
     import torch
@@ -290,6 +290,8 @@ def _plan_for(ns, M, D, ks, dealiasing, precision, device, binning=None, geometr
 def clear_plan_cache():
     while _PLANS:
         _PLANS.popitem()[1].close()
+    while _ENSEMBLES:
+        _ENSEMBLES.popitem()[1].close()
 
 
 def _shape_info(f_hat, ks, advecting_hat):
@@ -494,18 +496,194 @@ def batch_map(f, items):
     return [f(x) for x in items]
 
 
+class Ensemble:
+    """Device-resident ensemble of snapshots (fit_ensemble): the reference's batch axis
+    (mpi_batch_map, MPIExt.jl:39-68) with the kinetic-energy flux and, optionally, the scalar
+    flux of every snapshot.  Snapshots on the same grid run batched in the same launches and the
+    whole ensemble replays as one captured HIP graph.
+
+    shapes: per-snapshot grid sizes; ks_list: per-snapshot wavenumber vectors."""
+
+    def __init__(self, shapes, ks_list, *, with_scalar=False, binning=None,
+                 dealiasing=OrszagTwoThirds(), precision=np.float32, geometry=None, device=0):
+        self.lib = _lib.load_library()
+        self.shapes = [tuple(int(n) for n in s) for s in shapes]
+        if len(self.shapes) != len(ks_list) or not self.shapes:
+            raise ValueError("ks_list must match shapes one to one (and be non-empty)")
+        self.nd = len(self.shapes[0])
+        if any(len(s) != self.nd for s in self.shapes):
+            raise _lib.DimensionMismatch("every snapshot of an ensemble has the same dimension")
+        self.io = _lib.F64 if np.dtype(precision) == np.float64 else _lib.F32
+        self.with_scalar = bool(with_scalar)
+        geometry = geometry if geometry is not None else ShellMagnitude(None)
+        n = len(self.shapes)
+        self._ks = []
+        self._shells = []
+        self.centers = []
+        cache = {}
+        for s, ks in zip(self.shapes, ks_list):
+            kk = [np.ascontiguousarray(np.asarray(k, dtype=np.float64)) for k in ks]
+            if len(kk) != self.nd or any(len(k) != m for k, m in zip(kk, s)):
+                raise _lib.DimensionMismatch("ks must hold nd vectors of lengths ns")
+            key = (s, tuple(k.tobytes() for k in kk))
+            if key not in cache:
+                c, _, sidx = shell_setup(kk, binning, geometry)
+                cache[key] = (c, np.asfortranarray(np.asarray(sidx, dtype=np.int64)))
+            c, sidx = cache[key]
+            self._ks.extend(kk)
+            self._shells.append(sidx)
+            self.centers.append(c)
+        self.n_shells = [len(c) for c in self.centers]
+        nsarr = (ctypes.c_int64 * (n * self.nd))(*[m for s in self.shapes for m in s])
+        ksp = (ctypes.POINTER(ctypes.c_double) * len(self._ks))(*[_dptr(k) for k in self._ks])
+        shp = (ctypes.POINTER(ctypes.c_int64) * n)(
+            *[a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)) for a in self._shells])
+        nsh = (ctypes.c_int * n)(*self.n_shells)
+        self.handle = ctypes.c_void_p()
+        rc = self.lib.fit_ensemble_create(ctypes.byref(self.handle), n, self.nd, nsarr, self.io,
+                                          dealiasing.code, ksp, shp, nsh, int(self.with_scalar),
+                                          device)
+        _lib.check(rc)
+        self.offsets = np.concatenate([[0], np.cumsum(self.n_shells)]).astype(np.int64)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.fit_ensemble_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def groups(self):
+        return int(self.lib.fit_ensemble_groups(self.handle))
+
+    def set_option(self, option, value):
+        _lib.check(self.lib.fit_ensemble_set_option(self.handle, int(option), int(value)))
+
+    def set_graph(self, on=True):
+        self.set_option(_lib.ENS_OPT_GRAPH, int(bool(on)))
+
+    def set_batch(self, n):
+        self.set_option(_lib.ENS_OPT_BATCH, int(n))
+
+    def stream(self):
+        return self.lib.fit_ensemble_stream(self.handle)
+
+    def profile(self, on=True):
+        _lib.check(self.lib.fit_ensemble_profile_enable(self.handle, int(on)))
+
+    def profile_stats(self):
+        out = {}
+        for i, name in enumerate(_lib.K_CLASSES):
+            n = ctypes.c_int64()
+            ms = ctypes.c_double()
+            by = ctypes.c_double()
+            _lib.check(self.lib.fit_ensemble_profile_query(self.handle, i, ctypes.byref(n),
+                                                           ctypes.byref(ms), ctypes.byref(by)))
+            out[name] = {"launches": n.value, "ms": ms.value, "alg_bytes": by.value}
+        return out
+
+    def raw_fluxes(self, velocities, scalars=None):
+        """-> (T_ke, Pi_ke, T_sc, Pi_sc) concatenated over snapshots (offsets in self.offsets)."""
+        n = len(self.shapes)
+        if len(velocities) != n or (self.with_scalar and (scalars is None or len(scalars) != n)):
+            raise ValueError("one velocity (and one scalar) per snapshot")
+        keep = []
+        up = (ctypes.c_void_p * n)()
+        tp = (ctypes.c_void_p * n)()
+        kind = None
+        for j in range(n):
+            s = self.shapes[j]
+            if tuple(int(m) for m in velocities[j].shape) != s + (self.nd,):
+                raise _lib.DimensionMismatch(f"snapshot {j}: velocity must be (ns..., {self.nd})")
+            p, k, kd = _marshal(velocities[j], self.io)
+            keep.append(k)
+            up[j] = p
+            kinds = {kd}
+            if self.with_scalar:
+                th = scalars[j]
+                if tuple(int(m) for m in th.shape[:self.nd]) != s:
+                    raise _lib.DimensionMismatch(f"snapshot {j}: scalar grid differs from the velocity's")
+                p, k, kd2 = _marshal(th, self.io)
+                keep.append(k)
+                tp[j] = p
+                kinds.add(kd2)
+            if kind is None:
+                kind = kd
+            kinds.add(kind)
+            if len(kinds) != 1:
+                raise ValueError("all ensemble arrays must be host arrays or all device arrays")
+        tot = int(self.offsets[-1])
+        T, Pi = np.zeros(tot), np.zeros(tot)
+        Ts, Ps = np.zeros(tot), np.zeros(tot)
+        _lib.check(self.lib.fit_ensemble_flux(self.handle, up, tp if self.with_scalar else None,
+                                              _dptr(T), _dptr(Pi), _dptr(Ts), _dptr(Ps), self.io,
+                                              kind))
+        return T, Pi, Ts, Ps
+
+    def fluxes(self, velocities, scalars=None):
+        """Per snapshot, in input order: (kinetic-energy SpectralFluxResult, scalar-variance
+        SpectralFluxResult or None)."""
+        T, Pi, Ts, Ps = self.raw_fluxes(velocities, scalars)
+        out = []
+        for j in range(len(self.shapes)):
+            ft = _real_dtype(velocities[j])
+            a, b = self.offsets[j], self.offsets[j + 1]
+            c = _ft_cast(self.centers[j], ft)
+            ke = SpectralFluxResult(c, _ft_cast(T[a:b], ft), _ft_cast(Pi[a:b], ft))
+            sc = SpectralFluxResult(c, _ft_cast(Ts[a:b], ft), _ft_cast(Ps[a:b], ft)) \
+                if self.with_scalar else None
+            out.append((ke, sc))
+        return out
+
+
 def calculate_ensemble_fluxes(velocities, ks_list, scalars=None, *, binning=None,
                               dealiasing=OrszagTwoThirds(), spectral=HIPSpectral(),
                               invariant=KineticEnergy(), geometry=None):
-    """Spectral fluxes of an ensemble of snapshots of possibly different sizes, in input order.
+    """Spectral fluxes of an ensemble of snapshots of possibly different sizes, in input order
+    (the collation contract of mpi_batch_map, MPIExt.jl:14-23, :39-68).
 
     velocities[j]: (ns_j..., D) array; ks_list[j]: its wavenumber vectors; scalars[j] (optional):
     a scalar advected by velocities[j].  Returns a list of (flux of `invariant`, scalar-variance
-    flux or None).  Plans are cached per grid, so snapshots of equal size reuse device workspaces."""
+    flux or None).  Kinetic-energy ensembles of 2D / 3D snapshots (D = nd, no padding) run as one
+    batched, graph-captured fit_ensemble; anything else runs snapshot by snapshot on cached
+    plans."""
+    _check_spectral(spectral)
     if scalars is not None and len(scalars) != len(velocities):
         raise ValueError("scalars must match velocities one to one")
     if len(ks_list) != len(velocities):
         raise ValueError("ks_list must match velocities one to one")
+    if not velocities:
+        return []
+    nd = len(ks_list[0])
+    batched = (isinstance(invariant, KineticEnergy) and nd in (2, 3)
+               and not isinstance(dealiasing, PaddedThreeHalves)
+               and all(len(k) == nd and int(v.shape[nd]) == nd and len(v.shape) == nd + 1
+                       for v, k in zip(velocities, ks_list))
+               and len({_real_dtype(v) for v in velocities}) == 1
+               and len({_is_device(v) for v in velocities}) == 1)
+    if batched:
+        ft = _real_dtype(velocities[0])
+        shapes = [tuple(int(m) for m in v.shape[:nd]) for v in velocities]
+        key = (tuple(shapes), tuple(tuple(np.asarray(k, dtype=np.float64).tobytes() for k in ks)
+                                    for ks in ks_list),
+               type(dealiasing).__name__, np.dtype(spectral.compute_dtype(ft)).name, spectral.device,
+               repr(binning), repr(geometry), scalars is not None)
+        ens = _ENSEMBLES.get(key)
+        if ens is None:
+            ens = Ensemble(shapes, ks_list, with_scalar=scalars is not None, binning=binning,
+                           dealiasing=dealiasing, precision=spectral.compute_dtype(ft),
+                           geometry=geometry, device=spectral.device)
+            _ENSEMBLES.clear()
+            _ENSEMBLES[key] = ens
+        sc = None
+        if scalars is not None:
+            sc = [as_component_field(s, nd) for s in scalars]
+        return ens.fluxes(velocities, sc)
 
     def one(j):
         ke = calculate_spectral_flux(velocities[j], ks_list[j], binning=binning, dealiasing=dealiasing,
@@ -517,3 +695,6 @@ def calculate_ensemble_fluxes(velocities, ks_list, scalars=None, *, binning=None
         return ke, sc
 
     return batch_map(one, range(len(velocities)))
+
+
+_ENSEMBLES = {}

@@ -389,6 +389,9 @@ template <typename R> LdSpectral<R> spectral_loader(fit_plan* p, const void* src
   ls.ks2 = (const R*)p->ks[2];
   ls.shell = p->shell;
   ls.giver = giver;
+  ls.srcs = nullptr;
+  ls.cps = 1;
+  ls.gstride = 0;
   return ls;
 }
 
@@ -457,11 +460,16 @@ bool commute_ok(const fit_plan* p) {
 // First and (3D) y passes of the shared-transform gradients of ng*M fields into work2.
 template <typename R>
 int grads_shared(fit_plan* p, const void* f, int giver0, int ng, const int* bi, const int4* fband, int xa,
-                 StHalf<R> st) {
+                 StHalf<R> st, const void* const* srcs) {
   const size_t csz = sizeof(cpx<R>);
   const int nd = p->nd, ngc = ng * p->M;
   const int ax = nd - 1;
   LdSpectral<R> ls = spectral_loader<R>(p, f, 0, 0);
+  if (srcs) {
+    ls.srcs = (const cpx<R>* const*)srcs;
+    ls.cps = p->M;
+    ls.gstride = p->M;
+  }
   {
     const int L = p->G[ax];
     const int T = two_T(L);
@@ -515,8 +523,10 @@ int grads_shared(fit_plan* p, const void* f, int giver0, int ng, const int* bi, 
 
 // Inverse passes (all but x) of ng*M*nd gradient fields (giver g, component c, direction j) of
 // f̂ into work2 (compact, pitch xa).  giver0 = 0: no shell filter.  fband: per-giver bounds.
+// srcs (ensembles): device table of ng snapshot arrays (M components each); group g reads srcs[g].
 template <typename R>
-int v2_inverse_grads(fit_plan* p, const void* f, int giver0, int ng, const int* bi, const int4* fband) {
+int v2_inverse_grads(fit_plan* p, const void* f, int giver0, int ng, const int* bi, const int4* fband,
+                     const void* const* srcs = nullptr) {
   const int nf = ng * p->M * p->nd;
   const int xa = xext(p, bi[0]);
   const size_t csz = sizeof(cpx<R>);
@@ -526,11 +536,16 @@ int v2_inverse_grads(fit_plan* p, const void* f, int giver0, int ng, const int* 
   StHalf<R> st{(cpx<R>*)p->work2, sy, sz, sf};
   p->grads_commuted = false;
   if (commute_ok(p)) {
-    if ((rc = grads_shared<R>(p, f, giver0, ng, bi, fband, xa, st))) return rc;
+    if ((rc = grads_shared<R>(p, f, giver0, ng, bi, fband, xa, st, srcs))) return rc;
     p->grads_commuted = true;
     return FIT_OK;
   }
   LdSpectralB<R> ls{spectral_loader<R>(p, f, 0, 0), p->M, giver0};
+  if (srcs) {
+    ls.base.srcs = (const cpx<R>* const*)srcs;
+    ls.base.cps = p->M;
+    ls.base.gstride = p->M;
+  }
   LdHalf<R> ld{(const cpx<R>*)p->work2, sy, sz, sf};
   const int fdiv = p->M * p->nd;
   if (p->nd == 3) {
@@ -545,8 +560,12 @@ int v2_inverse_grads(fit_plan* p, const void* f, int giver0, int ng, const int* 
 }
 
 // Fused x pass: ng*M field groups -> forward-x half rows kx <= bo[0] in nxbuf (pitch xb+1).
+// u: physical advecting velocity (nd components, pitch Gp) of group 0; ugs: its stride per group
+// (0: one velocity for every group, as for the giver shells; ensembles: nd*Gp).
 template <typename R>
-int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* bo) {
+int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* bo, const void* u = nullptr,
+              int64_t ugs = 0) {
+  if (!u) u = p->uphys;
   const int nfg = ng * p->M;
   const int xa = xext(p, bi[0]);
   const int xb = xext(p, bo[0]);
@@ -574,15 +593,15 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
     const size_t lds = (size_t)pairs_per_block * LL * csz;
     const int nrg = (npairs + pairs_per_block - 1) / pairs_per_block;
     const int ngiv = nfg / p->M;
-    const int GC = std::max(1, std::min(ngiv, p->xfused_gc > 0 ? p->xfused_gc : 4));
+    const int GC = ugs ? 1 : std::max(1, std::min(ngiv, p->xfused_gc > 0 ? p->xfused_gc : 4));
     const int nchunk = (ngiv + GC - 1) / GC;
     switch (L) {
 #define FIT_CASE(LLEN)                                                                                          \
   case LLEN:                                                                                                    \
     hipLaunchKernelGGL((k_xfused_w<R, LLEN>), dim3((unsigned)((int64_t)nrg * nchunk)), dim3(256), lds, p->stream, \
-                       (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf, (const R*)p->uphys, usy,    \
+                       (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf, (const R*)u, usy,          \
                        usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M, p->G[1], nrows,              \
-                       (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, ngiv, GC, kxt);                         \
+                       (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, ngiv, GC, kxt, ugs);                    \
     break;
       FIT_REG_LENGTHS(FIT_CASE)
 #undef FIT_CASE
@@ -601,8 +620,8 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
   case LLEN:                                                                                                   \
     hipLaunchKernelGGL((k_xfused_r<R, LLEN>), dim3((unsigned)((int64_t)nrg * nfg)), dim3(B * xblk_T(LLEN)), lds, \
                        p->stream, (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf,             \
-                       (const R*)p->uphys, usy, usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M,      \
-                       p->G[1], nrows, B, (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, nfg, kxt);          \
+                       (const R*)u, usy, usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M,            \
+                       p->G[1], nrows, B, (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, nfg, kxt, ugs);     \
     break;
       FIT_REG_LENGTHS(FIT_CASE)
 #undef FIT_CASE
@@ -614,8 +633,8 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
     const int nrg = (npairs + P - 1) / P;
     hipLaunchKernelGGL(k_xfused<R>, dim3((unsigned)((int64_t)nrg * nfg)), dim3(kThreads), (size_t)3 * L * P * csz,
                        p->stream, p->fd[0], (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf,
-                       (const R*)p->uphys, usy, usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M, p->G[1],
-                       nrows, P, (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, nfg);
+                       (const R*)u, usy, usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M, p->G[1],
+                       nrows, P, (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, nfg, ugs);
   }
   prof_end(p);
   FIT_HIP(hipGetLastError());
@@ -852,6 +871,78 @@ int s2s_plain(fit_plan* p, int inv, const void* f, std::vector<double>& Tm) {
   return FIT_OK;
 }
 
+// Tail of a batched pass for gb field groups already through the fused x pass (nxbuf, pitch
+// xext(band_rx)): the remaining forward pass(es), the pair density against eh (+ g*ehgs for group
+// g) and FP64 shell bins, reduced in fixed order into out (gb rows of nsh doubles, device).
+template <typename R>
+int batched_lastdens(fit_plan* p, int gb, const void* eh, int64_t ehgs, double* out) {
+  const int nsh = p->nsh;
+  const int* bo = p->band_rx;
+  const size_t csz = sizeof(cpx<R>);
+  const int n1h = p->n[0] / 2 + 1;
+  const int64_t Hn = (int64_t)n1h * p->n[1] * p->n[2];
+  const int lax = p->nd == 3 ? 2 : 1;
+  const int L = p->G[lax];
+  int TX = 16;
+  while (TX > 1 && ((size_t)2 * L * TX * csz + (size_t)nsh * 8 > 65536 || TX * L > 32 * kThreads)) TX /= 2;
+  const int xb = std::max(1, xext(p, bo[0]));
+  const Band ob = p->nd == 3 ? Band{bo[1], p->G[1]} : full_band(1);
+  const Band wb = p->nd == 3 ? Band{bo[2], p->G[2]} : Band{bo[1], p->G[1]};
+  const int gx = (xb + TX - 1) / TX, gy = ob.count();
+  const int nblk = gx * gy;
+  int rc;
+  if ((rc = grow(p, (void**)&p->partials2, &p->partials2_bytes, (size_t)gb * nblk * nsh * sizeof(double)))) return rc;
+  const double scale = 1.0 / (double)p->Np;
+  const int64_t sy = xb, sz = (int64_t)xb * p->G[1], sf = sz * p->G[2];
+  if (p->nd == 3) {
+    LdHalf<R> ld{(const cpx<R>*)p->nxbuf, sy, sz, sf};
+    StHalf<R> st{(cpx<R>*)p->nxbuf, sy, sz, sf};
+    if ((rc = launch_col<R, -1>(p, ld, st, 1, gb * p->M, xb, full_band(p->G[2]), full_band(p->G[1]),
+                                Band{bo[1], p->G[1]})))
+      return rc;
+  }
+  prof_begin(p, FIT_K_DENSITY_SHELL,
+             (double)gb * p->M * xb * gy * L * csz + (double)gb * p->M * xb * gy * wb.count() * csz);
+  hipLaunchKernelGGL(k_lastdens<R>, dim3(gx, gy, gb), dim3(kThreads), (size_t)2 * L * TX * csz + (size_t)nsh * 8,
+                     p->stream, p->fd[lax], (const cpx<R>*)p->tw[lax], (const cpx<R>*)p->nxbuf, sy, sz, sf, xb, TX,
+                     lax, ob, wb, p->M, (R)scale, (const cpx<R>*)eh, (int64_t)n1h, (int64_t)n1h * p->n[1], Hn,
+                     p->shh, nsh, p->n[0], p->partials2, ehgs);
+  prof_end(p);
+  FIT_HIP(hipGetLastError());
+  prof_begin(p, FIT_K_REDUCE, (double)gb * nblk * nsh * 8.0);
+  hipLaunchKernelGGL(k_reduce_batched, dim3((nsh + 127) / 128, gb), dim3(128), 0, p->stream, p->partials2, nblk,
+                     nsh, gb, out);
+  prof_end(p);
+  FIT_HIP(hipGetLastError());
+  return FIT_OK;
+}
+
+// Hermitian part of the masked effective carrier on the receiving band, for nb snapshots
+// (srcs: device table of their arrays, or nullptr for the single array f) at eh + b*M*Hn.
+template <typename R>
+int carrier_half(fit_plan* p, int inv, const void* f, const void* const* srcs, int nb, void* eh) {
+  const int* bo = p->band_rx;
+  const size_t csz = sizeof(cpx<R>);
+  const int n1h = p->n[0] / 2 + 1;
+  const int64_t Hn = (int64_t)n1h * p->n[1] * p->n[2];
+  Grid g = grid_of<R>(p);
+  prof_begin(p, FIT_K_OTHER, (double)nb * Hn * p->M * csz * 3.0);
+  hipLaunchKernelGGL(k_carrier_half<R>, dim3(srcs ? 256 : 2048, nb), dim3(256), 0, p->stream, g, inv, p->M,
+                     p->dealias == FIT_DEALIAS_TWOTHIRDS ? 1 : 0, (const cpx<R>*)f, p->Np, (const R*)p->ks[0],
+                     (const R*)p->ks[1], (const R*)p->ks[2], Band{bo[1], p->n[1]}, Band{bo[2], p->n[2]}, bo[0],
+                     (cpx<R>*)eh, Hn, (const cpx<R>* const*)srcs, (int64_t)p->M * Hn);
+  prof_end(p);
+  FIT_HIP(hipGetLastError());
+  return FIT_OK;
+}
+
+// Bytes of batched workspace one field group (giver or snapshot) needs in the pruned pipeline.
+template <typename R>
+double batch_bytes_per_group(fit_plan* p, int xa_max) {
+  const int xb = std::max(1, xext(p, p->band_rx[0]));
+  return ((double)p->M * p->nd * xa_max + (double)p->M * xb) * p->G[1] * p->G[2] * sizeof(cpx<R>);
+}
+
 // Batched giver loop: every giver of a batch goes through each pruned pass in one launch; the
 // last forward pass computes the pair density and the shell sums in place (no N̂ in HBM).
 template <typename R>
@@ -864,16 +955,7 @@ int s2s_batched(fit_plan* p, int inv, const void* f, std::vector<double>& Tm) {
   int rc;
   // carrier Hermitian part on the receiving band
   if ((rc = grow(p, &p->eh, &p->eh_bytes, (size_t)p->M * Hn * csz))) return rc;
-  {
-    Grid g = grid_of<R>(p);
-    prof_begin(p, FIT_K_OTHER, (double)Hn * p->M * csz * 3.0);
-    hipLaunchKernelGGL(k_carrier_half<R>, dim3(2048), dim3(256), 0, p->stream, g, inv, p->M,
-                       p->dealias == FIT_DEALIAS_TWOTHIRDS ? 1 : 0, (const cpx<R>*)f, p->Np, (const R*)p->ks[0],
-                       (const R*)p->ks[1], (const R*)p->ks[2], Band{bo[1], p->n[1]}, Band{bo[2], p->n[2]},
-                       bo[0], (cpx<R>*)p->eh, Hn);
-    prof_end(p);
-    FIT_HIP(hipGetLastError());
-  }
+  if ((rc = carrier_half<R>(p, inv, f, nullptr, 1, p->eh))) return rc;
   // batch size from the device memory budget
   int bmax[3] = {-1, -1, -1};
   for (int s = 0; s < nsh; ++s) {
@@ -882,25 +964,13 @@ int s2s_batched(fit_plan* p, int inv, const void* f, std::vector<double>& Tm) {
     bmax[2] = std::max(bmax[2], p->shell_band[s].z);
   }
   const int xa_max = std::max(1, xext(p, bmax[0]));
-  const int xb = std::max(1, xext(p, bo[0]));
-  const double per_giver = ((double)p->M * p->nd * xa_max + (double)p->M * xb) * p->G[1] * p->G[2] * csz;
+  const double per_giver = batch_bytes_per_group<R>(p, xa_max);
   size_t freeb = 0, totalb = 0;
   FIT_HIP(hipMemGetInfo(&freeb, &totalb));
   const double budget = 0.6 * ((double)freeb + (double)p->work2_bytes + (double)p->nxbuf_bytes);
   int ng = (int)std::max(1.0, std::min((double)nsh, budget / per_giver));
   if (p->giver_batch > 0) ng = std::min(ng, p->giver_batch);
-  // last-pass launch geometry
-  const int lax = p->nd == 3 ? 2 : 1;
-  const int L = p->G[lax];
-  int TX = 16;
-  while (TX > 1 && ((size_t)2 * L * TX * csz + (size_t)nsh * 8 > 65536 || TX * L > 32 * kThreads)) TX /= 2;
-  const Band ob = p->nd == 3 ? Band{bo[1], p->G[1]} : full_band(1);
-  const Band wb = p->nd == 3 ? Band{bo[2], p->G[2]} : Band{bo[1], p->G[1]};
-  const int gx = (xb + TX - 1) / TX, gy = ob.count();
-  const int nblk = gx * gy;
-  if ((rc = grow(p, (void**)&p->partials2, &p->partials2_bytes, (size_t)ng * nblk * nsh * sizeof(double)))) return rc;
   if ((rc = grow(p, (void**)&p->dcolG, &p->dcolG_bytes, (size_t)ng * nsh * sizeof(double)))) return rc;
-  const double scale = 1.0 / (double)p->Np;
   for (int m0 = 1; m0 <= nsh; m0 += ng) {
     const int gb = std::min(ng, nsh - m0 + 1);
     int bi[3] = {-1, -1, -1};
@@ -913,27 +983,7 @@ int s2s_batched(fit_plan* p, int inv, const void* f, std::vector<double>& Tm) {
     const int4* fb = p->d_shell_band + (m0 - 1);
     if ((rc = v2_inverse_grads<R>(p, f, m0, gb, bi, fb))) return rc;
     if ((rc = v2_xfused<R>(p, gb, bi, fb, bo))) return rc;
-    const int64_t sy = xb, sz = (int64_t)xb * p->G[1], sf = sz * p->G[2];
-    if (p->nd == 3) {
-      LdHalf<R> ld{(const cpx<R>*)p->nxbuf, sy, sz, sf};
-      StHalf<R> st{(cpx<R>*)p->nxbuf, sy, sz, sf};
-      if ((rc = launch_col<R, -1>(p, ld, st, 1, gb * p->M, xb, full_band(p->G[2]), full_band(p->G[1]),
-                                  Band{bo[1], p->G[1]})))
-        return rc;
-    }
-    prof_begin(p, FIT_K_DENSITY_SHELL,
-               (double)gb * p->M * xb * gy * L * csz + (double)gb * p->M * xb * gy * wb.count() * csz);
-    hipLaunchKernelGGL(k_lastdens<R>, dim3(gx, gy, gb), dim3(kThreads), (size_t)2 * L * TX * csz + (size_t)nsh * 8,
-                       p->stream, p->fd[lax], (const cpx<R>*)p->tw[lax], (const cpx<R>*)p->nxbuf, sy, sz, sf, xb, TX,
-                       lax, ob, wb, p->M, (R)scale, (const cpx<R>*)p->eh, (int64_t)n1h, (int64_t)n1h * p->n[1], Hn,
-                       p->shh, nsh, p->n[0], p->partials2);
-    prof_end(p);
-    FIT_HIP(hipGetLastError());
-    prof_begin(p, FIT_K_REDUCE, (double)gb * nblk * nsh * 8.0);
-    hipLaunchKernelGGL(k_reduce_batched, dim3((nsh + 127) / 128, gb), dim3(128), 0, p->stream, p->partials2, nblk,
-                       nsh, gb, p->dcolG);
-    prof_end(p);
-    FIT_HIP(hipGetLastError());
+    if ((rc = batched_lastdens<R>(p, gb, p->eh, 0, p->dcolG))) return rc;
     FIT_HIP(hipMemcpyAsync(Tm.data() + (size_t)(m0 - 1) * nsh, p->dcolG, (size_t)gb * nsh * sizeof(double),
                            hipMemcpyDeviceToHost, p->stream));
     FIT_HIP(hipStreamSynchronize(p->stream));
@@ -1310,6 +1360,167 @@ static int create_plan(fit_plan** out, int nd, const int64_t* ns, int M, int D, 
   return FIT_OK;
 }
 
+// =============================================================================================
+// Ensembles of snapshots: the batch axis of the reference (mpi_batch_map, MPIExt.jl:39-68) with
+// f = calculate_spectral_flux + calculate_scalar_flux (SpectralFlux.jl:49-94, :203-215) per
+// snapshot, on one GPU.  Snapshots on the same grid (size, wavenumbers and shell map) form a
+// group served by one plan; every pass of a group runs a batch of its snapshots in one launch
+// (batch index = the field-group index the giver loop uses) and reads each snapshot through a
+// device table of array pointers, so the caller's arrays are used in place.  Per batch: the
+// advecting velocities (shared by both fluxes), the kinetic-energy N̂ fused with its density and
+// shell sums, then the same for the scalar.  The sequence for all groups is captured once into
+// a HIP graph and replayed by later calls, which only rewrite the pointer tables.
+// =============================================================================================
+namespace {
+
+struct EnsGroup {
+  fit_plan* p = nullptr;          // the group's grid, shells and workspaces (M = D = nd)
+  std::vector<int> snaps;         // snapshot indices, input order
+  void** d_tab = nullptr;         // 2B device pointers: velocities, then scalars
+  std::vector<void*> h_tab;       // host copy of the table (outlives the async upload)
+  void* ub = nullptr; size_t ub_bytes = 0;    // physical velocities of one batch (ng, nd, Gp)
+  void* ehk = nullptr; size_t ehk_bytes = 0;  // kinetic-energy carrier, Hermitian part (ng, nd, Hn)
+  void* ehs = nullptr; size_t ehs_bytes = 0;  // scalar carrier (ng, Hn)
+  double* rows = nullptr;         // (2, B, nsh): kinetic-energy rows, then scalar rows
+  std::vector<double> h_rows;
+  void* stage = nullptr; size_t stage_bytes = 0;  // inputs staged from the host / other dtype
+  int ng = 0;                     // snapshots per batch (device memory budget)
+};
+
+}  // namespace
+
+struct fit_ensemble {
+  int n = 0, nd = 0, dtype = 0, dealias = 0, device = 0, with_scalar = 0;
+  hipStream_t stream = nullptr;
+  std::vector<EnsGroup> groups;
+  std::vector<int> nsh;           // per snapshot
+  std::vector<int64_t> off;       // per snapshot: first row of its shells in the outputs
+  int use_graph = 1;
+  int batch_cap = 0;
+  bool prof = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_io = -1;              // (ptr_kind, io_dtype) the graph was captured for
+};
+
+namespace {
+
+// restores a plan's component count when a batch runs the scalar part with M = 1
+struct MGuard {
+  fit_plan* p; int m;
+  MGuard(fit_plan* q, int mm) : p(q), m(q->M) { q->M = mm; }
+  ~MGuard() { p->M = m; }
+};
+
+// u_j = Re ifft(keep û_j) for the gb snapshots tabled at tu: fields (b, j) -> ub (b*nd + j)*Gp
+template <typename R>
+int ens_velocity(fit_plan* p, const void* const* tu, int gb, void* ub) {
+  const int nd = p->nd, nf = gb * nd;
+  const int* bi = p->band_in;
+  const int xa = xext(p, bi[0]);
+  const size_t csz = sizeof(cpx<R>);
+  int rc = grow(p, &p->work2, &p->work2_bytes, (size_t)nf * xa * p->G[1] * p->G[2] * csz);
+  if (rc) return rc;
+  const int64_t sy = xa, sz = (int64_t)xa * p->G[1], sf = sz * p->G[2];
+  StHalf<R> st{(cpx<R>*)p->work2, sy, sz, sf};
+  LdHalf<R> ld{(const cpx<R>*)p->work2, sy, sz, sf};
+  LdSpectral<R> ls = spectral_loader<R>(p, nullptr, -1, 0);
+  ls.srcs = (const cpx<R>* const*)tu;
+  ls.cps = nd;
+  if (nd == 3) {
+    if ((rc = launch_col<R, 1>(p, ls, st, 2, nf, xa, Band{bi[1], p->G[1]}, Band{bi[2], p->G[2]}, full_band(p->G[2]))))
+      return rc;
+    if ((rc = launch_col<R, 1>(p, ld, st, 1, nf, xa, full_band(p->G[2]), Band{bi[1], p->G[1]}, full_band(p->G[1]))))
+      return rc;
+  } else {
+    if ((rc = launch_col<R, 1>(p, ls, st, 1, nf, xa, full_band(1), Band{bi[1], p->G[1]}, full_band(p->G[1]))))
+      return rc;
+  }
+  StReal<R> sr{(R*)ub, (int64_t)p->G[0], (int64_t)p->G[0] * p->G[1], p->Gp, (R)(1.0 / (double)p->Gp)};
+  return launch_row_inv<R>(p, ld, sr, nf, 0.0, xa - 1);
+}
+
+// one batch [b0, b0 + gb) of a group: velocities, kinetic-energy rows, scalar rows
+template <typename R>
+int ens_batch(fit_ensemble* e, EnsGroup& G, int b0, int gb) {
+  fit_plan* p = G.p;
+  const int nd = p->nd, nsh = p->nsh;
+  const int B = (int)G.snaps.size();
+  const int64_t Hn = (int64_t)(p->n[0] / 2 + 1) * p->n[1] * p->n[2];
+  const void* const* tu = (const void* const*)(G.d_tab + b0);
+  const void* const* tt = (const void* const*)(G.d_tab + B + b0);
+  int rc;
+  if ((rc = ens_velocity<R>(p, tu, gb, G.ub))) return rc;
+  const int64_t ugs = (int64_t)nd * p->Gp;
+  {
+    MGuard mg(p, nd);
+    if ((rc = carrier_half<R>(p, FIT_KINETIC_ENERGY, nullptr, tu, gb, G.ehk))) return rc;
+    if ((rc = v2_inverse_grads<R>(p, nullptr, 0, gb, p->band_in, nullptr, tu))) return rc;
+    if ((rc = v2_xfused<R>(p, gb, p->band_in, nullptr, p->band_rx, G.ub, ugs))) return rc;
+    if ((rc = batched_lastdens<R>(p, gb, G.ehk, (int64_t)nd * Hn, G.rows + (size_t)b0 * nsh))) return rc;
+  }
+  if (e->with_scalar) {
+    MGuard mg(p, 1);
+    if ((rc = carrier_half<R>(p, FIT_PASSIVE_SCALAR, nullptr, tt, gb, G.ehs))) return rc;
+    if ((rc = v2_inverse_grads<R>(p, nullptr, 0, gb, p->band_in, nullptr, tt))) return rc;
+    if ((rc = v2_xfused<R>(p, gb, p->band_in, nullptr, p->band_rx, G.ub, ugs))) return rc;
+    if ((rc = batched_lastdens<R>(p, gb, G.ehs, Hn, G.rows + (size_t)(B + b0) * nsh))) return rc;
+  }
+  return FIT_OK;
+}
+
+// batch size of a group from the device memory budget (first run only: the graph reuses it)
+template <typename R>
+int ens_batch_size(fit_ensemble* e, EnsGroup& G) {
+  if (G.ng > 0) return FIT_OK;
+  fit_plan* p = G.p;
+  const int nd = p->nd;
+  const int B = (int)G.snaps.size();
+  const size_t csz = sizeof(cpx<R>);
+  const int64_t Hn = (int64_t)(p->n[0] / 2 + 1) * p->n[1] * p->n[2];
+  const int xa = xext(p, p->band_in[0]);
+  double per = 0.0;
+  {
+    MGuard mg(p, nd);
+    per = batch_bytes_per_group<R>(p, xa);
+  }
+  per += (double)nd * p->Gp * sizeof(R) + (double)(nd + 1) * Hn * csz;
+  size_t freeb = 0, totalb = 0;
+  FIT_HIP(hipMemGetInfo(&freeb, &totalb));
+  int ng = (int)std::max(1.0, std::min((double)B, 0.5 * (double)freeb / per));
+  if (e->batch_cap > 0) ng = std::min(ng, e->batch_cap);
+  G.ng = ng;
+  int rc;
+  if ((rc = grow(p, &G.ub, &G.ub_bytes, (size_t)ng * nd * p->Gp * sizeof(R)))) return rc;
+  if ((rc = grow(p, &G.ehk, &G.ehk_bytes, (size_t)ng * nd * Hn * csz))) return rc;
+  if (e->with_scalar && (rc = grow(p, &G.ehs, &G.ehs_bytes, (size_t)ng * Hn * csz))) return rc;
+  return FIT_OK;
+}
+
+template <typename R>
+int ens_run(fit_ensemble* e) {
+  for (EnsGroup& G : e->groups) {
+    int rc = ens_batch_size<R>(e, G);
+    if (rc) return rc;
+    const int B = (int)G.snaps.size();
+    for (int b0 = 0; b0 < B; b0 += G.ng)
+      if ((rc = ens_batch<R>(e, G, b0, std::min(G.ng, B - b0)))) return rc;
+  }
+  return FIT_OK;
+}
+
+int ens_run_any(fit_ensemble* e) { return e->dtype == FIT_F64 ? ens_run<double>(e) : ens_run<float>(e); }
+
+void ens_drop_graph(fit_ensemble* e) {
+  if (e->exec) hipGraphExecDestroy(e->exec);
+  if (e->graph) hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+  e->graph_io = -1;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* fit_last_error(void) { return g_last_error.c_str(); }
@@ -1566,5 +1777,253 @@ int fit_helical_decompose(fit_plan* p, const void* u_hat, void* u_plus, void* u_
   if ((rc = stage_out_real(p, p->in_c, u_minus, 2 * (int64_t)p->M * Np, io_dtype, ptr_kind))) return rc;
   return finish(p);
 }
+
+
+// ---------------------------------------------------------------------------------------------
+// ensembles
+// ---------------------------------------------------------------------------------------------
+
+int fit_ensemble_create(fit_ensemble** out, int n_snap, int nd, const int64_t* ns, int compute_dtype,
+                        int dealias, const double* const* ks, const int64_t* const* shell_idx,
+                        const int* n_shells, int with_scalar, int device) {
+  std::lock_guard<std::mutex> lock(g_create_mutex);
+  g_last_error.clear();
+  if (!out) return fail(FIT_E_ARGUMENT, "ensemble out-pointer is NULL");
+  *out = nullptr;
+  if (n_snap < 1) return fail(FIT_E_ARGUMENT, "an ensemble needs at least one snapshot");
+  if (nd != 2 && nd != 3) return fail(FIT_E_UNSUPPORTED, "ensembles run 2D or 3D snapshots");
+  if (!ns || !ks || !shell_idx || !n_shells) return fail(FIT_E_ARGUMENT, "ns / ks / shell_idx / n_shells is NULL");
+  if (dealias == FIT_DEALIAS_PADDED)
+    return fail(FIT_E_UNSUPPORTED, "ensembles run NoDealiasing or OrszagTwoThirds (use a plan per snapshot for padding)");
+  fit_ensemble* e = new fit_ensemble();
+  e->n = n_snap; e->nd = nd; e->dtype = compute_dtype; e->dealias = dealias; e->device = device;
+  e->with_scalar = with_scalar != 0;
+  auto bail = [&](int code) {
+    std::string msg = g_last_error;
+    fit_ensemble_destroy(e);
+    g_last_error = msg;
+    return code;
+  };
+  hipError_t he = hipSetDevice(device);
+  if (he != hipSuccess) { delete e; return fail(FIT_E_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(he)); }
+  he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) { delete e; return fail(FIT_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(he)); }
+  e->nsh.resize(n_snap);
+  e->off.resize(n_snap);
+  int64_t acc = 0;
+  for (int j = 0; j < n_snap; ++j) {
+    if (n_shells[j] < 1 || !shell_idx[j]) return bail(fail(FIT_E_ARGUMENT, "every snapshot needs a shell map"));
+    e->nsh[j] = n_shells[j];
+    e->off[j] = acc;
+    acc += n_shells[j];
+  }
+  // group snapshots with the same grid: sizes, wavenumbers and shell map all equal
+  std::vector<int> rep;  // representative snapshot of each group
+  for (int j = 0; j < n_snap; ++j) {
+    int64_t np = 1;
+    for (int d = 0; d < nd; ++d) np *= ns[(size_t)j * nd + d];
+    int gi = -1;
+    for (size_t g = 0; g < rep.size() && gi < 0; ++g) {
+      const int r = rep[g];
+      bool same = n_shells[r] == n_shells[j];
+      for (int d = 0; d < nd && same; ++d) same = ns[(size_t)r * nd + d] == ns[(size_t)j * nd + d];
+      for (int d = 0; d < nd && same; ++d)
+        same = std::memcmp(ks[(size_t)r * nd + d], ks[(size_t)j * nd + d], ns[(size_t)j * nd + d] * sizeof(double)) == 0;
+      if (same) same = std::memcmp(shell_idx[r], shell_idx[j], (size_t)np * sizeof(int64_t)) == 0;
+      if (same) gi = (int)g;
+    }
+    if (gi < 0) {
+      gi = (int)rep.size();
+      rep.push_back(j);
+      e->groups.emplace_back();
+    }
+    e->groups[gi].snaps.push_back(j);
+  }
+  for (size_t g = 0; g < e->groups.size(); ++g) {
+    EnsGroup& G = e->groups[g];
+    const int r = rep[g];
+    int rc = create_plan(&G.p, nd, ns + (size_t)r * nd, nd, nd, compute_dtype, dealias, ks + (size_t)r * nd,
+                         shell_idx[r], n_shells[r], device, false);
+    if (rc) return bail(rc);
+    if (!G.p->shell_symmetric) return bail(fail(FIT_E_UNSUPPORTED, "ensemble shells must be symmetric under k -> -k"));
+    hipStreamDestroy(G.p->stream);
+    G.p->stream = e->stream;
+    G.p->own_stream = false;
+    const int B = (int)G.snaps.size();
+    if ((rc = dalloc(G.p, (void**)&G.d_tab, (size_t)2 * B * sizeof(void*)))) return bail(rc);
+    if ((rc = dalloc(G.p, (void**)&G.rows, (size_t)2 * B * G.p->nsh * sizeof(double)))) return bail(rc);
+    G.h_tab.assign((size_t)2 * B, nullptr);
+    G.h_rows.assign((size_t)2 * B * G.p->nsh, 0.0);
+  }
+  *out = e;
+  return FIT_OK;
+}
+
+int fit_ensemble_destroy(fit_ensemble* e) {
+  if (!e) return FIT_OK;
+  hipSetDevice(e->device);
+  if (e->stream) hipStreamSynchronize(e->stream);
+  ens_drop_graph(e);
+  for (EnsGroup& G : e->groups) {
+    if (!G.p) continue;
+    dfree(G.ub); dfree(G.ehk); dfree(G.ehs); dfree(G.stage);
+    { void* q = G.d_tab; dfree(q); }
+    { void* q = G.rows; dfree(q); }
+    fit_plan_destroy(G.p);
+  }
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
+  return FIT_OK;
+}
+
+int fit_ensemble_set_option(fit_ensemble* e, int option, int value) {
+  if (!e) return fail(FIT_E_ARGUMENT, "null ensemble");
+  switch (option) {
+    case FIT_ENS_OPT_GRAPH:
+      e->use_graph = value != 0;
+      if (!e->use_graph) ens_drop_graph(e);
+      return FIT_OK;
+    case FIT_ENS_OPT_BATCH:
+      if (value < 0) return fail(FIT_E_ARGUMENT, "batch cap must be >= 0");
+      e->batch_cap = value;
+      for (EnsGroup& G : e->groups) G.ng = 0;
+      ens_drop_graph(e);
+      return FIT_OK;
+    default:
+      return fail(FIT_E_ARGUMENT, "unknown ensemble option");
+  }
+}
+
+int fit_ensemble_groups(const fit_ensemble* e) { return e ? (int)e->groups.size() : -1; }
+
+int fit_ensemble_flux(fit_ensemble* e, const void* const* u_hat, const void* const* theta_hat, double* T_ke,
+                      double* Pi_ke, double* T_sc, double* Pi_sc, int io_dtype, int ptr_kind) {
+  if (!e) return fail(FIT_E_ARGUMENT, "null ensemble");
+  FIT_HIP(hipSetDevice(e->device));
+  if (!u_hat) return fail(FIT_E_ARGUMENT, "u_hat is NULL");
+  if (e->with_scalar && !theta_hat) return fail(FIT_E_ARGUMENT, "this ensemble was created with a scalar: theta_hat is NULL");
+  if (io_dtype != FIT_F32 && io_dtype != FIT_F64) return fail(FIT_E_ARGUMENT, "io_dtype must be FIT_F32 or FIT_F64");
+  if (ptr_kind != FIT_PTR_HOST && ptr_kind != FIT_PTR_DEVICE) return fail(FIT_E_ARGUMENT, "bad ptr_kind");
+  for (int j = 0; j < e->n; ++j)
+    if (!u_hat[j] || (e->with_scalar && !theta_hat[j])) return fail(FIT_E_ARGUMENT, "NULL snapshot array");
+  const size_t cb = csize(e->dtype), ib = csize(io_dtype);
+  const bool direct = ptr_kind == FIT_PTR_DEVICE && io_dtype == e->dtype;
+  // pointer tables (staging host / other-precision inputs into the group's stage buffer)
+  for (EnsGroup& G : e->groups) {
+    fit_plan* p = G.p;
+    const int B = (int)G.snaps.size();
+    const int nc = e->nd + (e->with_scalar ? 1 : 0);
+    if (!direct) {
+      int rc = grow(p, &G.stage, &G.stage_bytes, (size_t)B * nc * p->Np * cb);
+      if (rc) return rc;
+    }
+    void* tmp = nullptr;
+    for (int b = 0; b < B; ++b) {
+      const int j = G.snaps[b];
+      for (int part = 0; part < (e->with_scalar ? 2 : 1); ++part) {
+        const void* src = part == 0 ? u_hat[j] : theta_hat[j];
+        const int ncomp = part == 0 ? e->nd : 1;
+        if (direct) {
+          G.h_tab[(size_t)part * B + b] = const_cast<void*>(src);
+          continue;
+        }
+        char* dst = (char*)G.stage + ((size_t)b * nc + (part == 0 ? 0 : e->nd)) * p->Np * cb;
+        G.h_tab[(size_t)part * B + b] = dst;
+        const int64_t nvals = (int64_t)ncomp * p->Np;
+        if (io_dtype == e->dtype) {
+          FIT_HIP(hipMemcpyAsync(dst, src, nvals * cb, hipMemcpyHostToDevice, e->stream));
+          continue;
+        }
+        const void* dsrc = src;
+        if (ptr_kind == FIT_PTR_HOST) {
+          if (!tmp) FIT_HIP(hipMalloc(&tmp, (size_t)nc * p->Np * ib));
+          FIT_HIP(hipMemcpyAsync(tmp, src, nvals * ib, hipMemcpyHostToDevice, e->stream));
+          dsrc = tmp;
+        }
+        if (io_dtype == FIT_F64)
+          hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(256), 0, e->stream, (const double*)dsrc,
+                             (float*)dst, 2 * nvals);
+        else
+          hipLaunchKernelGGL((k_convert<float, double>), dim3(1024), dim3(256), 0, e->stream, (const float*)dsrc,
+                             (double*)dst, 2 * nvals);
+        FIT_HIP(hipGetLastError());
+        if (tmp) FIT_HIP(hipStreamSynchronize(e->stream));
+      }
+    }
+    if (tmp) hipFree(tmp);
+    FIT_HIP(hipMemcpyAsync(G.d_tab, G.h_tab.data(), G.h_tab.size() * sizeof(void*), hipMemcpyHostToDevice, e->stream));
+  }
+  const int io_key = ptr_kind * 2 + io_dtype;
+  if (e->exec && e->graph_io == io_key && !e->prof) {
+    FIT_HIP(hipGraphLaunch(e->exec, e->stream));
+  } else {
+    int rc = ens_run_any(e);  // allocates every workspace the graph will reuse
+    if (rc) return rc;
+    if (e->use_graph && !e->prof) {
+      ens_drop_graph(e);
+      FIT_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      rc = ens_run_any(e);
+      hipGraph_t g = nullptr;
+      hipError_t ce = hipStreamEndCapture(e->stream, &g);
+      if (rc) { if (g) hipGraphDestroy(g); return rc; }
+      FIT_HIP(ce);
+      e->graph = g;
+      FIT_HIP(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+      e->graph_io = io_key;
+    }
+  }
+  for (EnsGroup& G : e->groups)
+    FIT_HIP(hipMemcpyAsync(G.h_rows.data(), G.rows, G.h_rows.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           e->stream));
+  FIT_HIP(hipStreamSynchronize(e->stream));
+  for (EnsGroup& G : e->groups) {
+    prof_collect(G.p);
+    const int B = (int)G.snaps.size(), nsh = G.p->nsh;
+    for (int b = 0; b < B; ++b) {
+      const int j = G.snaps[b];
+      for (int part = 0; part < (e->with_scalar ? 2 : 1); ++part) {
+        double* T = part == 0 ? T_ke : T_sc;
+        double* Pi = part == 0 ? Pi_ke : Pi_sc;
+        const double* h = G.h_rows.data() + ((size_t)part * B + b) * nsh;
+        double a = 0.0;
+        for (int s = 0; s < nsh; ++s) {
+          a += h[s];  // Π = +cumsum(T) (SpectralFlux.jl:172-177)
+          if (T) T[e->off[j] + s] = h[s];
+          if (Pi) Pi[e->off[j] + s] = a;
+        }
+      }
+    }
+  }
+  return FIT_OK;
+}
+
+int fit_ensemble_profile_enable(fit_ensemble* e, int on) {
+  if (!e) return fail(FIT_E_ARGUMENT, "null ensemble");
+  e->prof = on != 0;
+  for (EnsGroup& G : e->groups) {
+    int rc = fit_profile_enable(G.p, on);
+    if (rc) return rc;
+  }
+  return FIT_OK;
+}
+
+int fit_ensemble_profile_query(const fit_ensemble* e, int cls, int64_t* launches, double* total_ms,
+                               double* alg_bytes) {
+  if (!e) return fail(FIT_E_ARGUMENT, "null ensemble");
+  if (cls < 0 || cls >= FIT_K_COUNT) return fail(FIT_E_ARGUMENT, "bad kernel class");
+  int64_t l = 0;
+  double ms = 0, by = 0;
+  for (const EnsGroup& G : e->groups) {
+    l += G.p->launches[cls];
+    ms += G.p->ms[cls];
+    by += G.p->abytes[cls];
+  }
+  if (launches) *launches = l;
+  if (total_ms) *total_ms = ms;
+  if (alg_bytes) *alg_bytes = by;
+  return FIT_OK;
+}
+
+void* fit_ensemble_stream(const fit_ensemble* e) { return e ? (void*)e->stream : nullptr; }
 
 }  // extern "C"

@@ -75,6 +75,13 @@ template <typename R> struct LdSpectral {
   int src_per_field_div;            // component index = f / src_per_field_div
   const R* ks0; const R* ks1; const R* ks2;
   const int* shell; int giver;
+  // ensembles: component `comp` of a batch lives in snapshot comp / cps at srcs[comp / cps] +
+  // (comp % cps)*sf (srcs == nullptr: one array at src); batched field groups g read component
+  // c + g*gstride (gstride = 0: every group reads the same field, as the giver shells do)
+  const cpx<R>* const* srcs; int cps; int gstride;
+  __device__ __forceinline__ const cpx<R>* base(int comp) const {
+    return srcs ? srcs[comp / cps] + (int64_t)(comp % cps) * sf : src + (int64_t)comp * sf;
+  }
   __device__ __forceinline__ cpx<R> G(int x, int y, int z, int comp, int gj) const {
     int ox, oy, oz;
     cpx<R> z0 = mk<R>((R)0, (R)0);
@@ -82,7 +89,7 @@ template <typename R> struct LdSpectral {
     if (trunc && g.dealiased(ox, oy, oz)) return z0;
     const int64_t o = g.oidx(ox, oy, oz);
     if (giver > 0 && shell[o] != giver) return z0;
-    cpx<R> v = src[o + comp * sf];
+    cpx<R> v = base(comp)[o];
     if (gj >= 0) {
       const R k = gj == 0 ? ks0[ox] : (gj == 1 ? ks1[oy] : ks2[oz]);
       v = mk<R>(-k * v.y, k * v.x);
@@ -98,7 +105,7 @@ template <typename R> struct LdSpectral {
     const int64_t o = g.oidx(ox, oy, oz);
     if (giver > 0 && shell[o] != giver) return mk<R>((R)0, (R)0);
     k = dk == 0 ? ks0[ox] : (dk == 1 ? ks1[oy] : ks2[oz]);
-    return src[o + comp * sf];
+    return base(comp)[o];
   }
   __device__ __forceinline__ cpx<R> operator()(int x, int y, int z, int f) const {
     const int comp = f / src_per_field_div;
@@ -124,11 +131,12 @@ template <typename R> struct LdSpectralB {
     const int g = f / (nd * M);
     LdSpectral<R> s = base;
     s.giver = giver0 > 0 ? giver0 + g : 0;
-    const cpx<R> a = s.G(x, y, z, c, j);
+    const int comp = c + g * s.gstride;
+    const cpx<R> a = s.G(x, y, z, comp, j);
     const int mx = x == 0 ? 0 : s.g.G1 - x;
     const int my = y == 0 ? 0 : s.g.G2 - y;
     const int mz = z == 0 ? 0 : s.g.G3 - z;
-    const cpx<R> b = s.G(mx, my, mz, c, j);
+    const cpx<R> b = s.G(mx, my, mz, comp, j);
     return mk<R>((R)0.5 * (a.x + b.x), (R)0.5 * (a.y - b.y));
   }
 };
@@ -352,7 +360,7 @@ __global__ void __launch_bounds__(256) k_xfused(FftDesc d, const cpx<R>* __restr
                                                 int64_t usf, cpx<R>* __restrict__ out, int64_t osy, int64_t osz,
                                                 int64_t osf, int nd, int M, int G2, int nrows, int P,
                                                 R gscale, int kxa, const int4* __restrict__ fband, int kxb,
-                                                int nfg) {
+                                                int nfg, int64_t ugs) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int L = d.L;
   cpx<R>* a = reinterpret_cast<cpx<R>*>(smem_raw);
@@ -360,6 +368,7 @@ __global__ void __launch_bounds__(256) k_xfused(FftDesc d, const cpx<R>* __restr
   cpx<R>* acc = b + (size_t)L * P;
   // field group fastest: consecutive workgroups share the same u rows (L2 reuse)
   const int fg = blockIdx.x % nfg;
+  u += (int64_t)(fg / M) * ugs;  // ensembles: each batch index advects with its own velocity
   const int pair0 = (blockIdx.x / nfg) * P;
   const int tid = threadIdx.x, nth = blockDim.x;
   const int ka = fband ? min(kxa, fband[fg / M].x) : kxa;
@@ -506,8 +515,8 @@ __global__ void __launch_bounds__(512) k_grad1_r(LdSpectral<R> ls, int M, int gi
       const int yy = axis == 1 ? i : o, zz = axis == 1 ? o : i;
       const int my = yy == 0 ? 0 : s.g.G2 - yy, mz = zz == 0 ? 0 : s.g.G3 - zz;
       R ka, kb;
-      const cpx<R> a = s.Graw(x, yy, zz, c, axis, ka);
-      const cpx<R> b = s.Graw(mx, my, mz, c, axis, kb);
+      const cpx<R> a = s.Graw(x, yy, zz, c + g * s.gstride, axis, ka);
+      const cpx<R> b = s.Graw(mx, my, mz, c + g * s.gstride, axis, kb);
       v0[e] = mk<R>((R)0.5 * (a.x + b.x), (R)0.5 * (a.y - b.y));
       v1[e] = mk<R>((R)-0.5 * (ka * a.y + kb * b.y), (R)0.5 * (ka * a.x - kb * b.x));
     }
@@ -612,12 +621,13 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
                                                   int64_t osy, int64_t osz, int64_t osf, int nd, int M, int G2,
                                                   int nrows, int B, R gscale, int kxa,
                                                   const int4* __restrict__ fband, int kxb, int nfg,
-                                                  const R* __restrict__ kxt) {
+                                                  const R* __restrict__ kxt, int64_t ugs) {
   constexpr int E = xE(L), T = L / E, LL = ldsline(L);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* lds0 = reinterpret_cast<cpx<R>*>(smem_raw);
   const int line = threadIdx.x / T, t = threadIdx.x % T;
   const int fg = blockIdx.x % nfg;
+  u += (int64_t)(fg / M) * ugs;
   const int pair = (blockIdx.x / nfg) * B + line;
   const int r0 = 2 * pair, r1 = r0 + 1;
   const bool ok0 = r0 < nrows, ok1 = r1 < nrows;
@@ -691,7 +701,8 @@ __global__ void __launch_bounds__(256) k_xfused_w(const cpx<R>* __restrict__ tw,
                                                   int64_t usy, int64_t usz, int64_t usf, cpx<R>* __restrict__ out,
                                                   int64_t osy, int64_t osz, int64_t osf, int nd, int M, int G2,
                                                   int nrows, R gscale, int kxa, const int4* __restrict__ fband,
-                                                  int kxb, int ngiv, int GC, const R* __restrict__ kxt) {
+                                                  int kxb, int ngiv, int GC, const R* __restrict__ kxt,
+                                                  int64_t ugs) {
   constexpr int E = wE(L), T = L / E, LPW = 64 / T, LL = ldsline(L);
   constexpr bool UREG = E <= 8;                  // keep u in registers for short lines
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -702,6 +713,8 @@ __global__ void __launch_bounds__(256) k_xfused_w(const cpx<R>* __restrict__ tw,
   const int nchunk = (ngiv + GC - 1) / GC;
   const int chunk = blockIdx.x % nchunk;
   const int pair = (blockIdx.x / nchunk) * (4 * LPW) + line;
+  // ensembles (ugs != 0, launched with GC = 1): the chunk's batch index has its own velocity
+  u += (int64_t)(chunk * GC) * ugs;
   const int r0 = 2 * pair, r1 = r0 + 1;
   const bool ok0 = r0 < nrows, ok1 = r1 < nrows;
   const int y0 = ok0 ? r0 % G2 : 0, z0 = ok0 ? r0 / G2 : 0, y1 = ok1 ? r1 % G2 : 0, z1 = ok1 ? r1 / G2 : 0;
@@ -828,7 +841,8 @@ __global__ void __launch_bounds__(256) k_lastdens(FftDesc d, const cpx<R>* __res
                                                   int64_t sf, int nx, int TX, int axis, Band ob, Band wb, int M,
                                                   R scale, const cpx<R>* __restrict__ eh, int64_t ehsy,
                                                   int64_t ehsz, int64_t ehsf, const int* __restrict__ shh,
-                                                  int nsh, int n1, double* __restrict__ partials) {
+                                                  int nsh, int n1, double* __restrict__ partials,
+                                                  int64_t ehgs) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int L = d.L;
   cpx<R>* a = reinterpret_cast<cpx<R>*>(smem_raw);
@@ -837,6 +851,7 @@ __global__ void __launch_bounds__(256) k_lastdens(FftDesc d, const cpx<R>* __res
   const int x0 = blockIdx.x * TX;
   const int o = ob.at(blockIdx.y);
   const int g = blockIdx.z;
+  eh += (int64_t)g * ehgs;  // ensembles: each batch index carries its own carrier
   const int tid = threadIdx.x, nth = blockDim.x;
   for (int s = tid; s < nsh; s += nth) bins[s] = 0.0;
   constexpr int QMAX = 32;  // TX*L <= QMAX*256 per launch
@@ -891,10 +906,14 @@ __global__ void __launch_bounds__(256) k_lastdens(FftDesc d, const cpx<R>* __res
 // Hermitian part of the masked effective carrier on the original half grid (x <= n1/2), only on
 // the output band: eh_c(k) = keep(k)·(e_c(k) + conj e_c(-k))/2, e from Invariants.jl:40-104 as
 // t = Re Σ_c conj(e_c) N̂_c (KE/scalar: ĉ; helicity: i k×û; enstrophy: i k×(i k×û), 2D form).
+// Ensembles: blockIdx.y = snapshot b reads srcs[b] (when srcs != nullptr) and writes eh + b*ehbs.
 template <typename R>
 __global__ void k_carrier_half(Grid g, int inv, int M, int trunc, const cpx<R>* __restrict__ f, int64_t fsf,
                                const R* __restrict__ ks0, const R* __restrict__ ks1, const R* __restrict__ ks2,
-                               Band by, Band bz, int xb, cpx<R>* __restrict__ eh, int64_t ehsf) {
+                               Band by, Band bz, int xb, cpx<R>* __restrict__ eh, int64_t ehsf,
+                               const cpx<R>* const* __restrict__ srcs, int64_t ehbs) {
+  if (srcs) f = srcs[blockIdx.y];
+  eh += (int64_t)blockIdx.y * ehbs;
   const int n1h = g.n1 / 2 + 1;
   const int64_t Hn = (int64_t)n1h * g.n2 * g.n3;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;

@@ -147,6 +147,55 @@ int fit_mode_to_mode(fit_plan* plan, int invariant, const void* f_hat, const voi
 int fit_helical_decompose(fit_plan* plan, const void* u_hat, void* u_plus, void* u_minus,
                           int io_dtype, int ptr_kind);
 
+/* ---- ensembles of snapshots ---------------------------------------------------------------- */
+
+typedef struct fit_ensemble fit_ensemble;
+
+/*
+ * Ensemble of n_snap snapshots, possibly of different sizes, for the kinetic-energy flux and
+ * (with_scalar != 0) the passive-scalar flux of every snapshot.  Replaces the reference's batch
+ * axis mpi_batch_map(f, snapshots) (ext/FlowInvariantTransferMPIExt.jl:39-68) with f =
+ * calculate_spectral_flux (SpectralFlux.jl:49-94) + calculate_scalar_flux (SpectralFlux.jl:203-215)
+ * per snapshot, results collated in input order.  Snapshots on the same grid (sizes, wavenumbers
+ * and shell map equal) are batched into the same launches; the whole ensemble is captured into
+ * one HIP graph on the first call and replayed afterwards.
+ *   nd in {2, 3}; ns: n_snap*nd sizes (snapshot j: ns[j*nd .. j*nd+nd-1]); velocities have D = nd
+ *   components; ks: n_snap*nd arrays (snapshot j, axis d at ks[j*nd + d]); shell_idx[j]: Np_j
+ *   Int64 shell numbers (as fit_plan_create); n_shells[j] >= 1; dealias NONE or TWOTHIRDS.
+ */
+int fit_ensemble_create(fit_ensemble** ens, int n_snap, int nd, const int64_t* ns, int compute_dtype,
+                        int dealias, const double* const* ks, const int64_t* const* shell_idx,
+                        const int* n_shells, int with_scalar, int device);
+
+int fit_ensemble_destroy(fit_ensemble* ens);
+
+/* Number of same-grid groups the snapshots were sorted into. */
+int fit_ensemble_groups(const fit_ensemble* ens);
+
+/* Ensemble options (no reference counterpart):
+ *   FIT_ENS_OPT_GRAPH = 1: 1 (default) capture the ensemble into a HIP graph, 0 run eagerly.
+ *   FIT_ENS_OPT_BATCH = 2: cap on snapshots per batched launch (0 = device memory budget). */
+enum { FIT_ENS_OPT_GRAPH = 1, FIT_ENS_OPT_BATCH = 2 };
+int fit_ensemble_set_option(fit_ensemble* ens, int option, int value);
+
+/*
+ * Fluxes of every snapshot.  u_hat[j]: (ns_j..., nd) velocity; theta_hat[j]: (ns_j...) scalar
+ * (ignored, may be NULL, without a scalar).  Outputs are host arrays of Σ_j n_shells[j] doubles,
+ * snapshot j's shells at offset Σ_{i<j} n_shells[i]: T_ke / Pi_ke (kinetic energy, Π = cumsum T),
+ * T_sc / Pi_sc (scalar variance); any output may be NULL.  ptr_kind applies to every snapshot.
+ */
+int fit_ensemble_flux(fit_ensemble* ens, const void* const* u_hat, const void* const* theta_hat, double* T_ke,
+                      double* Pi_ke, double* T_sc, double* Pi_sc, int io_dtype, int ptr_kind);
+
+/* Per-kernel-class timing of the ensemble (as fit_profile_*, summed over its groups); while
+ * enabled, calls run eagerly instead of replaying the graph. */
+int fit_ensemble_profile_enable(fit_ensemble* ens, int on);
+int fit_ensemble_profile_query(const fit_ensemble* ens, int kernel_class, int64_t* launches, double* total_ms,
+                               double* alg_bytes);
+
+/* The HIP stream the ensemble launches on (as void*: hipStream_t). */
+void* fit_ensemble_stream(const fit_ensemble* ens);
+
 /* ---- instrumentation (no reference counterpart: measurement hooks for the benchmark) ---- */
 
 /* Enable (1) / disable (0) per-kernel HIP-event timing on the plan's stream; resets counters. */
