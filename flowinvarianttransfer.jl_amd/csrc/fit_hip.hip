@@ -615,13 +615,17 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
     while (B > 1 && B * LL * csz > g_max_lds) --B;
     const int nrg = (npairs + B - 1) / B;
     const size_t lds = B * LL * csz;
+    // field groups per workgroup: every component of up to xfused_gc givers (or one snapshot)
+    const int ngiv = nfg / p->M;
+    const int GCM = p->M * (ugs ? 1 : std::max(1, std::min(ngiv, p->xfused_gc > 0 ? p->xfused_gc : 4)));
+    const int nchunk = (nfg + GCM - 1) / GCM;
     switch (L) {
 #define FIT_CASE(LLEN)                                                                                         \
   case LLEN:                                                                                                   \
-    hipLaunchKernelGGL((k_xfused_r<R, LLEN>), dim3((unsigned)((int64_t)nrg * nfg)), dim3(B * xblk_T(LLEN)), lds, \
-                       p->stream, (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf,             \
+    hipLaunchKernelGGL((k_xfused_r<R, LLEN>), dim3((unsigned)((int64_t)nrg * nchunk)), dim3(B * xblk_T(LLEN)), \
+                       lds, p->stream, (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf,        \
                        (const R*)u, usy, usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M,            \
-                       p->G[1], nrows, B, (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, nfg, kxt, ugs);     \
+                       p->G[1], nrows, B, (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, nfg, kxt, ugs, GCM); \
     break;
       FIT_REG_LENGTHS(FIT_CASE)
 #undef FIT_CASE

@@ -614,6 +614,8 @@ __global__ void __launch_bounds__(512) k_grad2_r(cpx<R>* __restrict__ buf, int64
 
 // Fused x pass (see k_xfused) with the register FFT: B row pairs per workgroup, T threads each,
 // one LDS exchange buffer with workgroup barriers (fallback when the wave form's LDS does not fit).
+// Each workgroup runs GCM consecutive field groups of its row block, so the u rows it reads stay
+// in its own XCD's L2 instead of being fetched again by neighbouring workgroups on other XCDs.
 template <typename R, int L>
 __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw, const cpx<R>* __restrict__ grad,
                                                   int64_t gsy, int64_t gsz, int64_t gsf, const R* __restrict__ u,
@@ -621,72 +623,76 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
                                                   int64_t osy, int64_t osz, int64_t osf, int nd, int M, int G2,
                                                   int nrows, int B, R gscale, int kxa,
                                                   const int4* __restrict__ fband, int kxb, int nfg,
-                                                  const R* __restrict__ kxt, int64_t ugs) {
+                                                  const R* __restrict__ kxt, int64_t ugs, int GCM) {
   constexpr int E = xE(L), T = L / E, LL = ldsline(L);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* lds0 = reinterpret_cast<cpx<R>*>(smem_raw);
   const int line = threadIdx.x / T, t = threadIdx.x % T;
-  const int fg = blockIdx.x % nfg;
-  u += (int64_t)(fg / M) * ugs;
-  const int pair = (blockIdx.x / nfg) * B + line;
+  const int nchunk = (nfg + GCM - 1) / GCM;
+  const int chunk = blockIdx.x % nchunk;
+  const int pair = (blockIdx.x / nchunk) * B + line;
   const int r0 = 2 * pair, r1 = r0 + 1;
   const bool ok0 = r0 < nrows, ok1 = r1 < nrows;
   const int64_t ro0 = ok0 ? (int64_t)(r0 % G2) * gsy + (int64_t)(r0 / G2) * gsz : 0;
   const int64_t ro1 = ok1 ? (int64_t)(r1 % G2) * gsy + (int64_t)(r1 / G2) * gsz : 0;
   const int64_t uo0 = ok0 ? (int64_t)(r0 % G2) * usy + (int64_t)(r0 / G2) * usz : 0;
   const int64_t uo1 = ok1 ? (int64_t)(r1 % G2) * usy + (int64_t)(r1 / G2) * usz : 0;
-  const IdxLine idx{line * LL};
-  const int ka = fband ? min(kxa, fband[fg / M].x) : kxa;
-  cpx<R> acc[E];
-  for (int j = 0; j < nd; ++j) {
-    const cpx<R>* gf = grad + (int64_t)(fg * nd + j) * gsf;
-    cpx<R> v[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int k = t + T * e;
-      const int kk = k <= L / 2 ? k : L - k;
-      cpx<R> A = mk<R>((R)0, (R)0), Bv = A;
-      if (kk <= ka) {
-        if (ok0) A = gf[kk + ro0];
-        if (ok1) Bv = gf[kk + ro1];
-      }
-      if (kxt && j == 0) {  // slot 0 holds h: the x derivative is i k_t(x) h
-        const R kx = kxt[kk];
-        A = mk<R>(-kx * A.y, kx * A.x);
-        Bv = mk<R>(-kx * Bv.y, kx * Bv.x);
-      }
-      if (k == 0 || 2 * k == L) { A.y = (R)0; Bv.y = (R)0; }
-      if (k > L / 2) { A.y = -A.y; Bv.y = -Bv.y; }
-      v[e] = mk<R>(A.x - Bv.y, A.y + Bv.x);
-    }
-    __syncthreads();
-    reg_fft_e<L, E, 1, false, R>(v, lds0, lds0, idx, t, tw);
-    const R* uj = u + (int64_t)j * usf;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int xx = t + T * e;
-      const R u0 = ok0 ? uj[xx + uo0] : (R)0;
-      const R u1 = ok1 ? uj[xx + uo1] : (R)0;
-      const cpx<R> tt = mk<R>(u0 * (v[e].x * gscale), u1 * (v[e].y * gscale));
-      acc[e] = j == 0 ? tt : cadd(acc[e], tt);
-    }
-  }
-  __syncthreads();
-  reg_fft_e<L, E, -1, false, R>(acc, lds0, lds0, idx, t, tw);
-  __syncthreads();
-#pragma unroll
-  for (int e = 0; e < E; ++e) lds0[idx(t + T * e)] = acc[e];
-  __syncthreads();
-  cpx<R>* of = out + (int64_t)fg * osf;
   const int64_t oo0 = ok0 ? (int64_t)(r0 % G2) * osy + (int64_t)(r0 / G2) * osz : 0;
   const int64_t oo1 = ok1 ? (int64_t)(r1 % G2) * osy + (int64_t)(r1 / G2) * osz : 0;
-  for (int k = t; k <= kxb; k += T) {
-    const cpx<R> Zk = lds0[idx(k)];
-    const cpx<R> Zm = lds0[idx((L - k) % L)];
-    const cpx<R> A = mk<R>((R)0.5 * (Zk.x + Zm.x), (R)0.5 * (Zk.y - Zm.y));
-    const cpx<R> Bv = mk<R>((R)0.5 * (Zk.y + Zm.y), (R)-0.5 * (Zk.x - Zm.x));
-    if (ok0) of[k + oo0] = A;
-    if (ok1) of[k + oo1] = Bv;
+  const IdxLine idx{line * LL};
+  const int fg1 = min(nfg, (chunk + 1) * GCM);
+  for (int fg = chunk * GCM; fg < fg1; ++fg) {
+    const R* ug = u + (int64_t)(fg / M) * ugs;  // ensembles: each batch index has its own velocity
+    const int ka = fband ? min(kxa, fband[fg / M].x) : kxa;
+    cpx<R> acc[E];
+    for (int j = 0; j < nd; ++j) {
+      const cpx<R>* gf = grad + (int64_t)(fg * nd + j) * gsf;
+      cpx<R> v[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int k = t + T * e;
+        const int kk = k <= L / 2 ? k : L - k;
+        cpx<R> A = mk<R>((R)0, (R)0), Bv = A;
+        if (kk <= ka) {
+          if (ok0) A = gf[kk + ro0];
+          if (ok1) Bv = gf[kk + ro1];
+        }
+        if (kxt && j == 0) {  // slot 0 holds h: the x derivative is i k_t(x) h
+          const R kx = kxt[kk];
+          A = mk<R>(-kx * A.y, kx * A.x);
+          Bv = mk<R>(-kx * Bv.y, kx * Bv.x);
+        }
+        if (k == 0 || 2 * k == L) { A.y = (R)0; Bv.y = (R)0; }
+        if (k > L / 2) { A.y = -A.y; Bv.y = -Bv.y; }
+        v[e] = mk<R>(A.x - Bv.y, A.y + Bv.x);
+      }
+      __syncthreads();  // the previous transform / output loop is done with the LDS buffer
+      reg_fft_e<L, E, 1, false, R>(v, lds0, lds0, idx, t, tw);
+      const R* uj = ug + (int64_t)j * usf;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int xx = t + T * e;
+        const R u0 = ok0 ? uj[xx + uo0] : (R)0;
+        const R u1 = ok1 ? uj[xx + uo1] : (R)0;
+        const cpx<R> tt = mk<R>(u0 * (v[e].x * gscale), u1 * (v[e].y * gscale));
+        acc[e] = j == 0 ? tt : cadd(acc[e], tt);
+      }
+    }
+    __syncthreads();
+    reg_fft_e<L, E, -1, false, R>(acc, lds0, lds0, idx, t, tw);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) lds0[idx(t + T * e)] = acc[e];
+    __syncthreads();
+    cpx<R>* of = out + (int64_t)fg * osf;
+    for (int k = t; k <= kxb; k += T) {
+      const cpx<R> Zk = lds0[idx(k)];
+      const cpx<R> Zm = lds0[idx((L - k) % L)];
+      const cpx<R> A = mk<R>((R)0.5 * (Zk.x + Zm.x), (R)0.5 * (Zk.y - Zm.y));
+      const cpx<R> Bv = mk<R>((R)0.5 * (Zk.y + Zm.y), (R)-0.5 * (Zk.x - Zm.x));
+      if (ok0) of[k + oo0] = A;
+      if (ok1) of[k + oo1] = Bv;
+    }
   }
 }
 

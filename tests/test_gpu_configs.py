@@ -146,27 +146,76 @@ def test_c5_helical_channels_256(fit, oracle):
     assert len(hp["channels"]) == 8 and err < 1e-5
 
 
+def device_field_2d(n, D, seed, power, solenoidal):
+    """Seeded Hermitian 2D field built on the device: complex Gaussian g, (g(k) + conj g(-k))/2,
+    per-mode |û|² ∝ k^power, Leray projection when solenoidal, DC and Nyquist lines zeroed,
+    Σ|û|² = 1; (n, n, D) column-major complex128 view."""
+    import torch
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    g = torch.complex(torch.randn((D, n, n), generator=gen, device="cuda", dtype=torch.float64),
+                      torch.randn((D, n, n), generator=gen, device="cuda", dtype=torch.float64))
+    u = 0.5 * (g + torch.roll(torch.flip(g, dims=(1, 2)), shifts=(1, 1), dims=(1, 2)).conj())
+    i = torch.arange(n, device="cuda")
+    k1 = torch.where(i <= n // 2, i, i - n).to(torch.float64)
+    kx, ky = k1.view(1, n), k1.view(n, 1)
+    kk = torch.sqrt(kx * kx + ky * ky)
+    ks = torch.where(kk > 0, kk, torch.ones_like(kk))
+    amp = torch.where(kk > 0, ks ** (0.5 * power), torch.zeros_like(kk))
+    u = torch.where(u.abs() > 0, u / torch.where(u.abs() > 0, u.abs(), torch.ones_like(u.abs())), u) * amp
+    if solenoidal:
+        div = kx * u[0] + ky * u[1]
+        u = torch.stack([u[0] - kx * div / (ks * ks), u[1] - ky * div / (ks * ks)])
+    u[:, :, n // 2] = 0
+    u[:, n // 2, :] = 0
+    u[:, 0, 0] = 0
+    u = u / torch.sqrt(torch.sum(u.abs() ** 2))
+    return u.permute(2, 1, 0)
+
+
 def test_c4_uneven_ensemble(fit, oracle):
+    """The C4 ensemble at full size: 64 2D snapshots with sizes drawn by seed 4 from
+    {256, 512, 1024, 2048}², k^-3 velocity + k^-1 scalar, FP32 batched graph vs an FP64 run of the
+    same ensemble (itself pinned to the oracle here on the 256² snapshots), conservation of both
+    fluxes, input order kept."""
     import torch
     say("case C4")
     rng = np.random.default_rng(4)
-    sizes = rng.choice([64, 128, 256, 512], 12)
-    vel, th, kss, refs = [], [], [], []
-    for j, n in enumerate(sizes):
-        ud, u = device_field(oracle, (int(n),) * 2, 2, seed=4000 + j, spectrum="k-3")
-        td, t = device_field(oracle, (int(n),) * 2, 1, seed=9000 + j, spectrum="k-1")
-        vel.append(ud.to(torch.complex64))
-        th.append(td.to(torch.complex64))
-        kss.append(fit.wavenumber_grid((int(n),) * 2, (2 * math.pi,) * 2))
-        if n <= 128:
-            refs.append((j, u, t))
+    sizes = [int(n) for n in rng.choice([256, 512, 1024, 2048], 64)]
+    vel = [device_field_2d(n, 2, 4000 + j, -3.0, True) for j, n in enumerate(sizes)]
+    th = [device_field_2d(n, 1, 5000 + j, -1.0, False)[..., 0] for j, n in enumerate(sizes)]
+    kss = [fit.wavenumber_grid((n, n), (2 * math.pi,) * 2) for n in sizes]
+    v32 = [v.to(torch.complex64) for v in vel]
+    t32 = [t.to(torch.complex64) for t in th]
     t0 = time.perf_counter()
-    out = fit.calculate_ensemble_fluxes(vel, kss, th, binning=fit.LinearBinning(1.0))
+    out = fit.calculate_ensemble_fluxes(v32, kss, t32, binning=fit.LinearBinning(1.0))
     torch.cuda.synchronize()
-    say("  ensemble of", len(sizes), "s:", time.perf_counter() - t0)
-    assert len(out) == len(sizes)
-    for j, u, t in refs:                                        # input order kept
+    say("  FP32 ensemble of", len(sizes), "(eager + capture) s:", time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    out = fit.calculate_ensemble_fluxes(v32, kss, t32, binning=fit.LinearBinning(1.0))
+    say("  FP32 ensemble replay s:", time.perf_counter() - t0)
+    o64 = fit.calculate_ensemble_fluxes(vel, kss, th, binning=fit.LinearBinning(1.0))
+    assert len(out) == len(sizes) == len(o64)
+    worst = 0.0
+    for j, n in enumerate(sizes):
+        ke, sc = out[j]
+        ke64, sc64 = o64[j]
+        assert len(ke.flux) == int(math.sqrt(2) * n / 2) + 1
+        worst = max(worst, nerr(ke.flux, ke64.flux), nerr(sc.flux, sc64.flux),
+                    nerr(ke.transfer_spectrum, ke64.transfer_spectrum))
+        for T in (ke64.transfer_spectrum, sc64.transfer_spectrum):       # conservation (FP64)
+            assert abs(T.sum()) < 1e-9 * np.abs(T).sum()
+    say("  FP32 vs FP64 worst normwise error:", worst)
+    assert worst < 1e-5
+    checked = 0
+    for j, n in enumerate(sizes):                                # FP64 run vs oracle, in order
+        if n != 256 or checked == 3:
+            continue
+        u = vel[j].permute(2, 1, 0).cpu().numpy().transpose(2, 1, 0)
+        t = th[j].permute(1, 0).cpu().numpy().transpose(1, 0)
         ks = kss[j]
         _, Tk, Pk = oracle.calculate_spectral_flux(u, ks, oracle.LinearBinning(1.0), "23")
-        _, Ts, Ps = oracle.calculate_scalar_flux(u, t[..., 0], ks, oracle.LinearBinning(1.0), "23")
+        _, Ts, Ps = oracle.calculate_scalar_flux(u, t, ks, oracle.LinearBinning(1.0), "23")
+        assert nerr(o64[j][0].flux, Pk) < 1e-11 and nerr(o64[j][1].flux, Ps) < 1e-11
         assert nerr(out[j][0].flux, Pk) < 1e-5 and nerr(out[j][1].flux, Ps) < 1e-5
+        checked += 1
+    assert checked == 3
