@@ -14,6 +14,7 @@ DEALIAS_NONE, DEALIAS_TWOTHIRDS, DEALIAS_PADDED = 0, 1, 2
 KINETIC_ENERGY, HELICITY, ENSTROPHY, PASSIVE_SCALAR = 0, 1, 2, 3
 PTR_HOST, PTR_DEVICE = 0, 1
 OPT_PLAIN_PIPELINE, OPT_GIVER_BATCH, OPT_XPASS_GIVERS, OPT_XPASS_BLOCK, OPT_SUBGRID, OPT_COL_TILE = 1, 2, 3, 4, 5, 6
+OPT_XPASS_SELF = 7
 ENS_OPT_GRAPH, ENS_OPT_BATCH = 1, 2
 K_CLASSES = ("fft_column", "fft_row_inverse", "fft_row_forward", "density_shell", "reduce", "other",
              "fused_x")
@@ -39,6 +40,7 @@ SIGNATURES = {
     "fit_shell_to_shell": (_I, [_P, _I, _P, _P, _D, _D, _D, _I, _I, _I]),
     "fit_mode_to_mode": (_I, [_P, _I, _P, _P, _P, _P, _I, _I]),
     "fit_helical_decompose": (_I, [_P, _P, _P, _P, _I, _I]),
+    "fit_band_to_band": (_I, [_P, _I, _D, _I, ctypes.c_double, _D, _P, _P, _D, _D, _D, _I, _I]),
     "fit_profile_enable": (_I, [_P, _I]),
     "fit_profile_query": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int64), _D, _D]),
     "fit_plan_stream": (_P, [_P]),

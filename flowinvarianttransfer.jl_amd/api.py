@@ -14,11 +14,11 @@ from collections import OrderedDict
 import numpy as np
 
 from . import _lib
-from .grid import as_component_field, shell_setup
-from .types import (AbstractDealiasing, AbstractInvariant, HelicalDecomposition, HIPExecution,
-                    HIPSpectral, KineticEnergy, ModeToModeTriadResult, NoDecomposition,
-                    OrszagTwoThirds, PaddedThreeHalves, PassiveScalar, ShellMagnitude,
-                    ShellToShellResult, SpectralFluxResult)
+from .grid import as_component_field, shell_coordinate, shell_setup
+from .types import (AbstractDealiasing, AbstractInvariant, BandToBandResult, HelicalDecomposition,
+                    HIPExecution, HIPSpectral, KineticEnergy, ModeToModeTriadResult,
+                    NoDecomposition, OrszagTwoThirds, PaddedThreeHalves, PassiveScalar,
+                    ShellMagnitude, ShellToShellResult, SmoothBands, SpectralFluxResult)
 
 try:  # torch is only plumbing for device arrays
     import torch
@@ -424,6 +424,37 @@ def calculate_mode_to_mode_transfer(velocity_hat, ks, *, invariant=KineticEnergy
     if not _is_device(S):
         net, S = net.astype(ft), S.astype(ft)
     return ModeToModeTriadResult(invariant, ks, net, S)
+
+
+def calculate_band_to_band_transfer(velocity_hat, ks, *, bands, dealiasing=OrszagTwoThirds(),
+                                    invariant=KineticEnergy(), spectral=HIPSpectral(),
+                                    advecting_hat=None, geometry=None):
+    """BandTransfer.jl:66-113 -- smooth band-to-band transfer T(n,m) between the graded bands of
+    `bands` (SmoothBands): one nonlinear term of w_m ⊙ û per band, band-weighted density sums on
+    the device.  Returns BandToBandResult(centers, transfer_matrix, net_transfer,
+    max_antisymmetry_error)."""
+    _check_spectral(spectral)
+    if not isinstance(bands, SmoothBands):
+        raise ValueError(f"bands must be SmoothBands, got {bands!r}")
+    cen = np.asarray(bands.centers, dtype=np.float64)
+    if cen.size == 0 or np.any(cen <= 0) or np.any(np.diff(cen) <= 0):
+        raise ValueError("SmoothBands centers must be positive and increasing")
+    nd, ns, M, D = _shape_info(velocity_hat, ks, advecting_hat)
+    _check_invariant(invariant, nd, M)
+    ft = _real_dtype(velocity_hat)
+    geometry = geometry if geometry is not None else ShellMagnitude(None)
+    plan = _plan_for(ns, M, D, ks, dealiasing, spectral.compute_dtype(ft), spectral.device)
+    kc = np.asfortranarray(shell_coordinate(geometry, ks).astype(np.float64))
+    fp, ap, keep, kind = plan._fa(velocity_hat, advecting_hat)
+    nb = cen.size
+    T = np.zeros((nb, nb), order="F")
+    net = np.zeros(nb)
+    asym = np.zeros(1)
+    _lib.check(plan.lib.fit_band_to_band(plan.handle, invariant.code, _dptr(cen), nb,
+                                         float(bands.logwidth), _dptr(kc), fp, ap, _dptr(T),
+                                         _dptr(net), _dptr(asym), plan.io, kind))
+    return BandToBandResult(_ft_cast(cen, ft), np.asfortranarray(T.astype(ft)), _ft_cast(net, ft),
+                            ft(asym[0]))
 
 
 def calculate_scalar_mode_to_mode_transfer(velocity_hat, scalar_hat, ks, **kw):

@@ -206,6 +206,8 @@ constexpr int rp_E(int L) { return rp_pow2(L) ? 8 : 24; }
 constexpr int gE(int L) { return rp_pow2(L) ? 8 : 12; }
 // points per thread of the workgroup-form fused x pass (measured: radix-8 stages win from 768 up)
 constexpr int xE(int L) { return rp_pow2(L) ? 8 : (L <= 384 ? 12 : 24); }
+// points per thread of the self-advection fused x pass (three velocity lines stay in registers)
+constexpr int sE(int L) { return rp_pow2(L) ? 8 : 12; }
 
 template <int L, int E, int S, int NS, int SIGN, bool DB, typename R, class IDX>
 __device__ __forceinline__ void reg_stage(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* lds1, const IDX& idx, int t,

@@ -108,6 +108,7 @@ struct fit_plan {
   int xfused_gc = 0;             // givers per workgroup in the wave-per-line x pass (0 = 4)
   int xfused_block = 0;          // 1: use the workgroup-barrier x pass instead
   int col_tx = 0;                // FIT_OPT_COL_TILE (0: automatic)
+  int xself = 1;                 // FIT_OPT_XPASS_SELF: self-advection x pass forms u in registers
   // shared-transform gradients: wavenumbers odd under k -> -k on every populated bin
   bool commute = false;
   void* kt[2] = {nullptr, nullptr};  // transform-grid wavenumbers of x and y (compute precision)
@@ -448,6 +449,9 @@ int grow(fit_plan* p, void** buf, size_t* have, size_t need) {
 
 // x-extent of compact storage for a folded band on the half grid
 int xext(fit_plan* p, int a) { return std::min(a, p->G[0] / 2) + 1; }
+// row pitch of compact storage: the extent rounded up to 16 elements, so that every row starts
+// on a 128-byte line and column tiles never straddle two lines
+int xpitch(int xe) { return (xe + 15) / 16 * 16; }
 
 // shared-transform gradient passes apply: exact for this plan, register lengths on every axis
 bool commute_ok(const fit_plan* p) {
@@ -522,17 +526,17 @@ int grads_shared(fit_plan* p, const void* f, int giver0, int ng, const int* bi, 
 }
 
 // Inverse passes (all but x) of ng*M*nd gradient fields (giver g, component c, direction j) of
-// f̂ into work2 (compact, pitch xa).  giver0 = 0: no shell filter.  fband: per-giver bounds.
+// f̂ into work2 (compact, pitch xpitch(xa)).  giver0 = 0: no shell filter.  fband: per-giver bounds.
 // srcs (ensembles): device table of ng snapshot arrays (M components each); group g reads srcs[g].
 template <typename R>
 int v2_inverse_grads(fit_plan* p, const void* f, int giver0, int ng, const int* bi, const int4* fband,
                      const void* const* srcs = nullptr) {
   const int nf = ng * p->M * p->nd;
-  const int xa = xext(p, bi[0]);
+  const int xa = xext(p, bi[0]), pa = xpitch(xa);
   const size_t csz = sizeof(cpx<R>);
-  int rc = grow(p, &p->work2, &p->work2_bytes, (size_t)nf * xa * p->G[1] * p->G[2] * csz);
+  int rc = grow(p, &p->work2, &p->work2_bytes, (size_t)nf * pa * p->G[1] * p->G[2] * csz);
   if (rc) return rc;
-  const int64_t sy = xa, sz = (int64_t)xa * p->G[1], sf = sz * p->G[2];
+  const int64_t sy = pa, sz = (int64_t)pa * p->G[1], sf = sz * p->G[2];
   StHalf<R> st{(cpx<R>*)p->work2, sy, sz, sf};
   p->grads_commuted = false;
   if (commute_ok(p)) {
@@ -559,7 +563,7 @@ int v2_inverse_grads(fit_plan* p, const void* f, int giver0, int ng, const int* 
                           fband, fdiv);
 }
 
-// Fused x pass: ng*M field groups -> forward-x half rows kx <= bo[0] in nxbuf (pitch xb+1).
+// Fused x pass: ng*M field groups -> forward-x half rows kx <= bo[0] in nxbuf (pitch xpitch(xb)).
 // u: physical advecting velocity (nd components, pitch Gp) of group 0; ugs: its stride per group
 // (0: one velocity for every group, as for the giver shells; ensembles: nd*Gp).
 template <typename R>
@@ -567,16 +571,16 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
               int64_t ugs = 0) {
   if (!u) u = p->uphys;
   const int nfg = ng * p->M;
-  const int xa = xext(p, bi[0]);
-  const int xb = xext(p, bo[0]);
+  const int xa = xext(p, bi[0]), pa = xpitch(xa);
+  const int xb = xext(p, bo[0]), pb = xpitch(xb);
   const size_t csz = sizeof(cpx<R>);
-  int rc = grow(p, &p->nxbuf, &p->nxbuf_bytes, (size_t)nfg * xb * p->G[1] * p->G[2] * csz);
+  int rc = grow(p, &p->nxbuf, &p->nxbuf_bytes, (size_t)nfg * pb * p->G[1] * p->G[2] * csz);
   if (rc) return rc;
   const int L = p->G[0];
   const int nrows = p->G[1] * p->G[2];
   const int npairs = (nrows + 1) / 2;
-  const int64_t gsy = xa, gsz = (int64_t)xa * p->G[1], gsf = gsz * p->G[2];
-  const int64_t osy = xb, osz = (int64_t)xb * p->G[1], osf = osz * p->G[2];
+  const int64_t gsy = pa, gsz = (int64_t)pa * p->G[1], gsf = gsz * p->G[2];
+  const int64_t osy = pb, osz = (int64_t)pb * p->G[1], osf = osz * p->G[2];
   const int64_t usy = L, usz = (int64_t)L * p->G[1];
   const double bytes = (double)nfg * nrows * (p->nd * (xa * csz) + xb * csz) + (double)p->nd * nrows * L * sizeof(R);
   const R* kxt = p->grads_commuted ? (const R*)p->kt[0] : nullptr;
@@ -645,11 +649,51 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
   return FIT_OK;
 }
 
+// Fused x pass of a self-advected field whose gradients went through the shared-transform
+// passes: u_j is formed in registers from slot 0 (no u in HBM), then N_c for every component.
+template <typename R>
+int v2_xfused_self(fit_plan* p, const int* bi, const int* bo) {
+  const int nfg = p->M;
+  const int xa = xext(p, bi[0]), pa = xpitch(xa);
+  const int xb = xext(p, bo[0]), pb = xpitch(xb);
+  const size_t csz = sizeof(cpx<R>);
+  int rc = grow(p, &p->nxbuf, &p->nxbuf_bytes, (size_t)nfg * pb * p->G[1] * p->G[2] * csz);
+  if (rc) return rc;
+  const int L = p->G[0];
+  const int nrows = p->G[1] * p->G[2];
+  const int npairs = (nrows + 1) / 2;
+  const int64_t gsy = pa, gsz = (int64_t)pa * p->G[1], gsf = gsz * p->G[2];
+  const int64_t osy = pb, osz = (int64_t)pb * p->G[1], osf = osz * p->G[2];
+  const double bytes = (double)nrows * (((double)p->nd + (double)nfg * p->nd) * xa * csz + (double)nfg * xb * csz);
+  prof_begin(p, FIT_K_FUSED_X, bytes);
+  switch (L) {
+#define FIT_CASE(LLEN)                                                                                          \
+  case LLEN: {                                                                                                  \
+    constexpr int T = LLEN / sE(LLEN);                                                                          \
+    const int B = std::max(1, kThreads / T);                                                                    \
+    const size_t lds = (size_t)B * ldsline(LLEN) * csz;                                                         \
+    const int nrg = (npairs + B - 1) / B;                                                                       \
+    hipLaunchKernelGGL((k_xfused_self<R, LLEN, sE(LLEN)>), dim3((unsigned)nrg), dim3(B * T), lds, p->stream,   \
+                       (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf, (cpx<R>*)p->nxbuf, osy, \
+                       osz, osf, p->nd, p->M, p->G[1], nrows, B, (R)(1.0 / (double)p->Gp), xa - 1, xb - 1,     \
+                       (const R*)p->kt[0]);                                                                     \
+    break;                                                                                                      \
+  }
+    FIT_REG_LENGTHS(FIT_CASE)
+#undef FIT_CASE
+    default:
+      return fail(FIT_E_UNSUPPORTED, "self-advection x pass needs a register FFT length");
+  }
+  prof_end(p);
+  FIT_HIP(hipGetLastError());
+  return FIT_OK;
+}
+
 // Remaining forward passes of nxbuf (M fields, one field group per component) into full N̂.
 template <typename R>
 int v2_forward_nhat(fit_plan* p, const int* bo) {
-  const int xb = xext(p, bo[0]);
-  const int64_t sy = xb, sz = (int64_t)xb * p->G[1], sf = sz * p->G[2];
+  const int xb = xext(p, bo[0]), pb = xpitch(xb);
+  const int64_t sy = pb, sz = (int64_t)pb * p->G[1], sf = sz * p->G[2];
   LdHalf<R> ld{(const cpx<R>*)p->nxbuf, sy, sz, sf};
   StHalf<R> st{(cpx<R>*)p->nxbuf, sy, sz, sf};
   Grid g = grid_of<R>(p);
@@ -823,11 +867,16 @@ int nonlinear_term_impl(fit_plan* p, const void* f, const void* a) {
   int rc;
   if (a == f && commute_ok(p)) {
     // self-advection: u_j = Re ifft(keep f̂_j) is the slot-0 transform of component j, so only
-    // its x pass remains (the y and z passes are shared with the gradients)
+    // its x pass remains (the y and z passes are shared with the gradients); the fused x pass
+    // forms it in registers
     if ((rc = v2_inverse_grads<R>(p, f, 0, 1, p->band_in, nullptr))) return rc;
-    const int xa = xext(p, p->band_in[0]);
-    const int64_t gsf = (int64_t)xa * p->G[1] * p->G[2];
-    LdHalf<R> ld{(const cpx<R>*)p->work2, (int64_t)xa, (int64_t)xa * p->G[1], (int64_t)p->nd * gsf};
+    if (p->xself) {
+      if ((rc = v2_xfused_self<R>(p, p->band_in, p->band_in))) return rc;
+      return v2_forward_nhat<R>(p, p->band_in);
+    }
+    const int xa = xext(p, p->band_in[0]), pa = xpitch(xa);
+    const int64_t gsf = (int64_t)pa * p->G[1] * p->G[2];
+    LdHalf<R> ld{(const cpx<R>*)p->work2, (int64_t)pa, (int64_t)pa * p->G[1], (int64_t)p->nd * gsf};
     StReal<R> sr{(R*)p->uphys, (int64_t)p->G[0], (int64_t)p->G[0] * p->G[1], p->Gp, (R)(1.0 / (double)p->Gp)};
     if ((rc = launch_row_inv<R>(p, ld, sr, p->nd, 0.0, xa - 1))) return rc;
     if ((rc = v2_xfused<R>(p, 1, p->band_in, nullptr, p->band_in))) return rc;
@@ -876,7 +925,7 @@ int s2s_plain(fit_plan* p, int inv, const void* f, std::vector<double>& Tm) {
 }
 
 // Tail of a batched pass for gb field groups already through the fused x pass (nxbuf, pitch
-// xext(band_rx)): the remaining forward pass(es), the pair density against eh (+ g*ehgs for group
+// xpitch(xext(band_rx))): the remaining forward pass(es), the pair density against eh (+ g*ehgs for group
 // g) and FP64 shell bins, reduced in fixed order into out (gb rows of nsh doubles, device).
 template <typename R>
 int batched_lastdens(fit_plan* p, int gb, const void* eh, int64_t ehgs, double* out) {
@@ -897,7 +946,8 @@ int batched_lastdens(fit_plan* p, int gb, const void* eh, int64_t ehgs, double* 
   int rc;
   if ((rc = grow(p, (void**)&p->partials2, &p->partials2_bytes, (size_t)gb * nblk * nsh * sizeof(double)))) return rc;
   const double scale = 1.0 / (double)p->Np;
-  const int64_t sy = xb, sz = (int64_t)xb * p->G[1], sf = sz * p->G[2];
+  const int pb = xpitch(xb);
+  const int64_t sy = pb, sz = (int64_t)pb * p->G[1], sf = sz * p->G[2];
   if (p->nd == 3) {
     LdHalf<R> ld{(const cpx<R>*)p->nxbuf, sy, sz, sf};
     StHalf<R> st{(cpx<R>*)p->nxbuf, sy, sz, sf};
@@ -944,7 +994,7 @@ int carrier_half(fit_plan* p, int inv, const void* f, const void* const* srcs, i
 template <typename R>
 double batch_bytes_per_group(fit_plan* p, int xa_max) {
   const int xb = std::max(1, xext(p, p->band_rx[0]));
-  return ((double)p->M * p->nd * xa_max + (double)p->M * xb) * p->G[1] * p->G[2] * sizeof(cpx<R>);
+  return ((double)p->M * p->nd * xpitch(xa_max) + (double)p->M * xpitch(xb)) * p->G[1] * p->G[2] * sizeof(cpx<R>);
 }
 
 // Batched giver loop: every giver of a batch goes through each pruned pass in one launch; the
@@ -1042,6 +1092,47 @@ template <typename R>
 int s2s_entry(fit_plan* p, int inv, const void* f, const void* a, std::vector<double>& Tm) {
   if (p->sub && p->use_sub && !p->plain) return s2s_sub<R>(p, inv, f, a, Tm);
   return s2s_impl<R>(p, inv, f, a, Tm);
+}
+
+// Smooth band-to-band transfer (BandTransfer.jl:66-113): for every band m, the nonlinear term of
+// w_m ⊙ f̂ advected by the full velocity (fused pipeline, u computed once), then the band sums
+// Σ_k w_n(k) t(k) of its density for every n in one pass.  kc: per-mode band coordinate.
+template <typename R>
+int b2b_impl(fit_plan* p, int inv, const void* f, const void* a, const R* kc, const double* cen, int nb,
+             double inv2s2, std::vector<double>& Tm) {
+  int rc = compute_u<R>(p, a);
+  if (rc) return rc;
+  const size_t cs = sizeof(cpx<R>);
+  if (!p->in_c && (rc = dalloc(p, &p->in_c, (size_t)std::max(p->M, p->D) * p->Np * cs))) return rc;
+  const int nblk = p->nblk_density;
+  if ((rc = grow(p, (void**)&p->partials2, &p->partials2_bytes, (size_t)nblk * nb * sizeof(double)))) return rc;
+  if ((rc = grow(p, (void**)&p->dcolG, &p->dcolG_bytes, (size_t)nb * sizeof(double)))) return rc;
+  Grid g = grid_of<R>(p);
+  p->nhat_dst = nullptr;
+  for (int m = 0; m < nb; ++m) {
+    prof_begin(p, FIT_K_OTHER, (double)p->Np * (2.0 * p->M * cs + sizeof(R)));
+    hipLaunchKernelGGL(k_band_field<R>, dim3(1024), dim3(256), 0, p->stream, p->Np, p->M, (const cpx<R>*)f, p->Np,
+                       kc, cen, nb, inv2s2, m, (cpx<R>*)p->in_c);
+    prof_end(p);
+    FIT_HIP(hipGetLastError());
+    rc = (p->plain || p->nd == 1) ? nonlinear_from_u<R>(p, p->in_c, 0) : v2_nonlinear<R>(p, p->in_c);
+    if (rc) return rc;
+    prof_begin(p, FIT_K_DENSITY_SHELL, (double)p->Np * (2.0 * p->M * cs + sizeof(R)));
+    hipLaunchKernelGGL(k_band_density<R>, dim3(nblk), dim3(kThreads), (size_t)nb * sizeof(double), p->stream, g, inv,
+                       p->M, (const cpx<R>*)f, p->Np, (const cpx<R>*)p->nhat, p->Np, (const R*)p->ks[0],
+                       (const R*)p->ks[1], (const R*)p->ks[2], kc, cen, nb, inv2s2, p->partials2);
+    prof_end(p);
+    FIT_HIP(hipGetLastError());
+    prof_begin(p, FIT_K_REDUCE, (double)nblk * nb * 8.0);
+    hipLaunchKernelGGL(k_reduce_partials, dim3((nb + 127) / 128), dim3(128), 0, p->stream, p->partials2, nblk, nb,
+                       p->dcolG);
+    prof_end(p);
+    FIT_HIP(hipGetLastError());
+    FIT_HIP(hipMemcpyAsync(Tm.data() + (size_t)m * nb, p->dcolG, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost,
+                           p->stream));
+    FIT_HIP(hipStreamSynchronize(p->stream));
+  }
+  return FIT_OK;
 }
 
 template <typename R>
@@ -1421,11 +1512,11 @@ template <typename R>
 int ens_velocity(fit_plan* p, const void* const* tu, int gb, void* ub) {
   const int nd = p->nd, nf = gb * nd;
   const int* bi = p->band_in;
-  const int xa = xext(p, bi[0]);
+  const int xa = xext(p, bi[0]), pa = xpitch(xa);
   const size_t csz = sizeof(cpx<R>);
-  int rc = grow(p, &p->work2, &p->work2_bytes, (size_t)nf * xa * p->G[1] * p->G[2] * csz);
+  int rc = grow(p, &p->work2, &p->work2_bytes, (size_t)nf * pa * p->G[1] * p->G[2] * csz);
   if (rc) return rc;
-  const int64_t sy = xa, sz = (int64_t)xa * p->G[1], sf = sz * p->G[2];
+  const int64_t sy = pa, sz = (int64_t)pa * p->G[1], sf = sz * p->G[2];
   StHalf<R> st{(cpx<R>*)p->work2, sy, sz, sf};
   LdHalf<R> ld{(const cpx<R>*)p->work2, sy, sz, sf};
   LdSpectral<R> ls = spectral_loader<R>(p, nullptr, -1, 0);
@@ -1597,6 +1688,7 @@ int fit_plan_set_option(fit_plan* p, int option, int value) {
         return fail(FIT_E_ARGUMENT, "column tile must be 0 or a power of two <= 64");
       p->col_tx = value;
       return FIT_OK;
+    case FIT_OPT_XPASS_SELF: p->xself = value != 0; return FIT_OK;
     default: return fail(FIT_E_ARGUMENT, "unknown plan option");
   }
 }
@@ -1782,6 +1874,64 @@ int fit_helical_decompose(fit_plan* p, const void* u_hat, void* u_plus, void* u_
   return finish(p);
 }
 
+
+
+int fit_band_to_band(fit_plan* p, int inv, const double* centers, int n_bands, double logwidth,
+                     const double* k_coord, const void* f_hat, const void* adv_hat, double* Tmat, double* net,
+                     double* max_asym, int io_dtype, int ptr_kind) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if ((rc = check_invariant(p, inv, p->M))) return rc;
+  if (!centers || !k_coord) return fail(FIT_E_ARGUMENT, "centers / k_coord is NULL");
+  if (n_bands < 1 || n_bands > 4096) return fail(FIT_E_ARGUMENT, "SmoothBands needs 1..4096 centers");
+  if (!(logwidth > 0.0)) return fail(FIT_E_ARGUMENT, "SmoothBands: logwidth must be positive");
+  for (int n = 0; n < n_bands; ++n)
+    if (!(centers[n] > 0.0)) return fail(FIT_E_ARGUMENT, "SmoothBands: centers must be positive");
+  const void *f, *a;
+  if ((rc = stage_fa(p, f_hat, adv_hat, io_dtype, ptr_kind, &f, &a))) return rc;
+  void *kc = nullptr, *cen = nullptr;
+  if ((rc = dalloc(p, &kc, (size_t)p->Np * p->rsz))) return rc;
+  if ((rc = dalloc(p, &cen, (size_t)n_bands * sizeof(double)))) { dfree(kc); return rc; }
+  auto release = [&]() {
+    hipStreamSynchronize(p->stream);
+    p->bytes -= (int64_t)p->Np * p->rsz + (int64_t)n_bands * sizeof(double);
+    dfree(kc);
+    dfree(cen);
+  };
+  hipError_t he = hipMemcpy(cen, centers, (size_t)n_bands * sizeof(double), hipMemcpyHostToDevice);
+  if (he == hipSuccess) {
+    if (p->dtype == FIT_F64) {
+      he = hipMemcpy(kc, k_coord, (size_t)p->Np * sizeof(double), hipMemcpyHostToDevice);
+    } else {
+      std::vector<float> k32((size_t)p->Np);
+      for (int64_t i = 0; i < p->Np; ++i) k32[i] = (float)k_coord[i];
+      he = hipMemcpy(kc, k32.data(), (size_t)p->Np * sizeof(float), hipMemcpyHostToDevice);
+    }
+  }
+  if (he != hipSuccess) { release(); return fail(FIT_E_DEVICE, std::string("band upload: ") + hipGetErrorString(he)); }
+  std::vector<double> Tm((size_t)n_bands * n_bands, 0.0);
+  const double inv2s2 = 1.0 / (2.0 * logwidth * logwidth);
+  rc = p->dtype == FIT_F64
+           ? b2b_impl<double>(p, inv, f, a, (const double*)kc, (const double*)cen, n_bands, inv2s2, Tm)
+           : b2b_impl<float>(p, inv, f, a, (const float*)kc, (const double*)cen, n_bands, inv2s2, Tm);
+  release();
+  if (rc) return rc;
+  if ((rc = finish(p))) return rc;
+  const int nb = n_bands;
+  double asym = 0.0;
+  for (int n = 0; n < nb; ++n) {
+    double acc = 0.0;
+    for (int m = 0; m < nb; ++m) {
+      const double v = Tm[n + (size_t)nb * m];   // T(n, m): receiver band n, giver band m
+      if (Tmat) Tmat[n + (size_t)nb * m] = v;
+      acc += v;
+      asym = std::max(asym, std::fabs(v + Tm[m + (size_t)nb * n]));
+    }
+    if (net) net[n] = acc;
+  }
+  if (max_asym) *max_asym = asym;
+  return FIT_OK;
+}
 
 // ---------------------------------------------------------------------------------------------
 // ensembles

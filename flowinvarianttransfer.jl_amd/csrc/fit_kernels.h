@@ -34,6 +34,22 @@ struct Grid {
   }
 };
 
+// Workgroups go to the 8 XCDs round-robin by linear id and each XCD has its own L2.  This remaps
+// the linear id so that each XCD runs one contiguous range of logical tiles (x tile fastest):
+// neighbouring column tiles, which share 128-byte lines, then run on the same XCD close in time
+// instead of each XCD fetching whole lines for half of them.
+__device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz) {
+  constexpr unsigned X = 8;
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned n = gx * gy * gridDim.z;
+  const unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned q = n / X, r = n % X, xcd = b % X, loc = b / X;
+  const unsigned l = xcd < r ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
+  bx = (int)(l % gx);
+  by = (int)((l / gx) % gy);
+  bz = (int)(l / (gx * gy));
+}
+
 // Folded-index band {i : min(i, G - i) <= a} of one axis (all of it when 2a+1 >= G); used to
 // prune FFT passes to the spectral support of their input and the modes their output feeds.
 struct Band {
@@ -208,9 +224,11 @@ __global__ void __launch_bounds__(256) k_col(LD ld, ST st, FftDesc d, const cpx<
   cpx<R>* a = reinterpret_cast<cpx<R>*>(smem_raw);
   cpx<R>* b = a + (size_t)d.L * TX;
   const int L = d.L;
-  const int x0 = blockIdx.x * TX;
-  const int o = ob.at(blockIdx.y);
-  const int f = blockIdx.z;
+  int bx, by, bz;
+  xcd_tile(bx, by, bz);
+  const int x0 = bx * TX;
+  const int o = ob.at(by);
+  const int f = bz;
   if (fband) {
     const int4 fb = fband[f / fdiv];
     if (x0 > fb.x) return;
@@ -428,10 +446,12 @@ __global__ void __launch_bounds__(1024) k_col_r(LD ld, ST st, const cpx<R>* __re
   constexpr int E = rp_E(L), T = L / E;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  int bx, by, bz;
+  xcd_tile(bx, by, bz);
   const int xl = threadIdx.x % TX, t = threadIdx.x / TX;
-  const int x0 = blockIdx.x * TX;
-  const int o = ob.at(blockIdx.y);
-  const int f = blockIdx.z;
+  const int x0 = bx * TX;
+  const int o = ob.at(by);
+  const int f = bz;
   if (fband) {
     const int4 fb = fband[f / fdiv];
     if (x0 > fb.x) return;
@@ -485,10 +505,12 @@ __global__ void __launch_bounds__(512) k_grad1_r(LdSpectral<R> ls, int M, int gi
   constexpr int E = gE(L), T = L / E;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  int bx, by, bz;
+  xcd_tile(bx, by, bz);
   const int xl = threadIdx.x % TX, t = threadIdx.x / TX;
-  const int x0 = blockIdx.x * TX;
-  const int o = ob.at(blockIdx.y);
-  const int gc = blockIdx.z;
+  const int x0 = bx * TX;
+  const int o = ob.at(by);
+  const int gc = bz;
   const int g = gc / M, c = gc % M;
   if (fband) {
     const int4 fb = fband[g];
@@ -556,10 +578,12 @@ __global__ void __launch_bounds__(512) k_grad2_r(cpx<R>* __restrict__ buf, int64
   constexpr int E = gE(L), T = L / E;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  int bx, by, bz;
+  xcd_tile(bx, by, bz);
   const int xl = threadIdx.x % TX, t = threadIdx.x / TX;
-  const int x0 = blockIdx.x * TX;
-  const int o = ob.at(blockIdx.y);
-  const int gc = blockIdx.z;
+  const int x0 = bx * TX;
+  const int o = ob.at(by);
+  const int gc = bz;
   if (fband) {
     const int4 fb = fband[gc / M];
     if (x0 > fb.x) return;
@@ -685,6 +709,101 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
     for (int e = 0; e < E; ++e) lds0[idx(t + T * e)] = acc[e];
     __syncthreads();
     cpx<R>* of = out + (int64_t)fg * osf;
+    for (int k = t; k <= kxb; k += T) {
+      const cpx<R> Zk = lds0[idx(k)];
+      const cpx<R> Zm = lds0[idx((L - k) % L)];
+      const cpx<R> A = mk<R>((R)0.5 * (Zk.x + Zm.x), (R)0.5 * (Zk.y - Zm.y));
+      const cpx<R> Bv = mk<R>((R)0.5 * (Zk.y + Zm.y), (R)-0.5 * (Zk.x - Zm.x));
+      if (ok0) of[k + oo0] = A;
+      if (ok1) of[k + oo1] = Bv;
+    }
+  }
+}
+
+// Fused x pass of a self-advected field (advecting = advected, shared-transform gradients):
+// slot 0 of component j already holds h_j after the y/z inverse passes, so the advecting
+// velocity u_j = c2r(h_j) is formed here, per row pair, instead of being stored to HBM and read
+// back.  Phase 1: u_j for j < nd into registers (E points per thread, the two rows of the pair
+// in the real / imaginary parts).  Phase 2: for each component c, N = Σ_j u_j · c2r(∂_j f_c) with
+// ∂_x f_c = c2r(i k_t(x) h_c), then r2c of N into the half rows kx <= kxb.  u and N never touch
+// HBM.  B row pairs per workgroup, T = L/E threads each, one LDS exchange buffer.
+template <typename R, int L, int E>
+__global__ void __launch_bounds__(256) k_xfused_self(const cpx<R>* __restrict__ tw, const cpx<R>* __restrict__ grad,
+                                                     int64_t gsy, int64_t gsz, int64_t gsf, cpx<R>* __restrict__ out,
+                                                     int64_t osy, int64_t osz, int64_t osf, int nd, int M, int G2,
+                                                     int nrows, int B, R gscale, int kxa, int kxb,
+                                                     const R* __restrict__ kxt) {
+  constexpr int T = L / E, LL = ldsline(L);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<R>* lds0 = reinterpret_cast<cpx<R>*>(smem_raw);
+  const int line = threadIdx.x / T, t = threadIdx.x % T;
+  const int pair = blockIdx.x * B + line;
+  const int r0 = 2 * pair, r1 = r0 + 1;
+  const bool ok0 = r0 < nrows, ok1 = r1 < nrows;
+  const int64_t ro0 = ok0 ? (int64_t)(r0 % G2) * gsy + (int64_t)(r0 / G2) * gsz : 0;
+  const int64_t ro1 = ok1 ? (int64_t)(r1 % G2) * gsy + (int64_t)(r1 / G2) * gsz : 0;
+  const int64_t oo0 = ok0 ? (int64_t)(r0 % G2) * osy + (int64_t)(r0 / G2) * osz : 0;
+  const int64_t oo1 = ok1 ? (int64_t)(r1 % G2) * osy + (int64_t)(r1 / G2) * osz : 0;
+  const IdxLine idx{line * LL};
+  // packed half rows (kx <= kxa) of field f, optionally times i k_t(x), extended to L points
+  auto load = [&](const cpx<R>* gf, bool dx, cpx<R> (&v)[E]) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = t + T * e;
+      const int kk = k <= L / 2 ? k : L - k;
+      cpx<R> A = mk<R>((R)0, (R)0), Bv = A;
+      if (kk <= kxa) {
+        if (ok0) A = gf[kk + ro0];
+        if (ok1) Bv = gf[kk + ro1];
+      }
+      if (dx) {
+        const R kx = kxt[kk];
+        A = mk<R>(-kx * A.y, kx * A.x);
+        Bv = mk<R>(-kx * Bv.y, kx * Bv.x);
+      }
+      if (k == 0 || 2 * k == L) { A.y = (R)0; Bv.y = (R)0; }
+      if (k > L / 2) { A.y = -A.y; Bv.y = -Bv.y; }
+      v[e] = mk<R>(A.x - Bv.y, A.y + Bv.x);
+    }
+  };
+  cpx<R> ur[3][E];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    if (j < nd) {
+      cpx<R> v[E];
+      load(grad + (int64_t)(j * nd) * gsf, false, v);
+      __syncthreads();  // the previous transform is done with the LDS buffer
+      reg_fft_e<L, E, 1, false, R>(v, lds0, lds0, idx, t, tw);
+#pragma unroll
+      for (int e = 0; e < E; ++e) ur[j][e] = mk<R>(v[e].x * gscale, v[e].y * gscale);
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) ur[j][e] = mk<R>((R)0, (R)0);
+    }
+  }
+  for (int c = 0; c < M; ++c) {
+    cpx<R> acc[E];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      if (j < nd) {
+        cpx<R> v[E];
+        load(grad + (int64_t)(c * nd + j) * gsf, j == 0, v);
+        __syncthreads();
+        reg_fft_e<L, E, 1, false, R>(v, lds0, lds0, idx, t, tw);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const cpx<R> tt = mk<R>(ur[j][e].x * (v[e].x * gscale), ur[j][e].y * (v[e].y * gscale));
+          acc[e] = j == 0 ? tt : cadd(acc[e], tt);
+        }
+      }
+    }
+    __syncthreads();
+    reg_fft_e<L, E, -1, false, R>(acc, lds0, lds0, idx, t, tw);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) lds0[idx(t + T * e)] = acc[e];
+    __syncthreads();
+    cpx<R>* of = out + (int64_t)c * osf;
     for (int k = t; k <= kxb; k += T) {
       const cpx<R> Zk = lds0[idx(k)];
       const cpx<R> Zm = lds0[idx((L - k) % L)];
@@ -1050,6 +1169,71 @@ __global__ void __launch_bounds__(256) k_density(Grid g, int inv, int M, const c
     __syncthreads();
     for (int s = threadIdx.x; s < nsh; s += blockDim.x) partials[(int64_t)blockIdx.x * nsh + s] = bins[s];
   }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Smooth band-to-band transfer (BandTransfer.jl:20-44, :66-113): graded band weights
+// w_n(κ) = exp(-(ln(κ/c_n))²/(2σ²)) / Σ_n' exp(-(ln(κ/c_n'))²/(2σ²)), 0 for κ <= 0, in FP64.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ double band_sum(double kap, const double* __restrict__ cen, int nb, double inv2s2) {
+  double s = 0.0;
+  for (int n = 0; n < nb; ++n) {
+    const double r = log(kap / cen[n]);
+    s += exp(-r * r * inv2s2);
+  }
+  return s;
+}
+
+// out_c = w_m(κ) f_c on the full original grid: the band-m giver field
+template <typename R>
+__global__ void k_band_field(int64_t Np, int M, const cpx<R>* __restrict__ f, int64_t fsf, const R* __restrict__ kc,
+                             const double* __restrict__ cen, int nb, double inv2s2, int m, cpx<R>* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < Np; o += stride) {
+    const double kap = (double)kc[o];
+    double w = 0.0;
+    if (kap > 0.0) {
+      const double sum = band_sum(kap, cen, nb, inv2s2);
+      const double r = log(kap / cen[m]);
+      if (sum > 0.0) w = exp(-r * r * inv2s2) / sum;
+    }
+    for (int c = 0; c < M; ++c) out[o + c * Np] = cscale(f[o + c * fsf], (R)w);
+  }
+}
+
+// band sums Σ_k w_n(k) t(k) of the density t = Re Σ_c conj(e_c) N̂_c over the full grid, for all
+// nb bands: FP64 LDS bins, one row of nb partials per workgroup (reduced in fixed order later)
+template <typename R>
+__global__ void __launch_bounds__(256) k_band_density(Grid g, int inv, int M, const cpx<R>* __restrict__ car,
+                                                      int64_t csf, const cpx<R>* __restrict__ nh, int64_t nsf,
+                                                      const R* __restrict__ ks0, const R* __restrict__ ks1,
+                                                      const R* __restrict__ ks2, const R* __restrict__ kc,
+                                                      const double* __restrict__ cen, int nb, double inv2s2,
+                                                      double* __restrict__ partials) {
+  extern __shared__ double dbins[];
+  for (int s = threadIdx.x; s < nb; s += blockDim.x) dbins[s] = 0.0;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < g.Np; o += stride) {
+    const double kap = (double)kc[o];
+    if (!(kap > 0.0)) continue;
+    const int ix = (int)(o % g.n1);
+    const int iy = (int)((o / g.n1) % g.n2);
+    const int iz = (int)(o / ((int64_t)g.n1 * g.n2));
+    const R kx = ks0[ix];
+    const R ky = g.nd >= 2 ? ks1[iy] : (R)0;
+    const R kz = g.nd >= 3 ? ks2[iz] : (R)0;
+    const double t = (double)density_at<R>(inv, g.nd, M, car, csf, nh, nsf, o, kx, ky, kz);
+    if (t == 0.0) continue;
+    const double sum = band_sum(kap, cen, nb, inv2s2);
+    if (!(sum > 0.0)) continue;
+    for (int n = 0; n < nb; ++n) {
+      const double r = log(kap / cen[n]);
+      atomicAdd(&dbins[n], exp(-r * r * inv2s2) / sum * t);
+    }
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < nb; s += blockDim.x) partials[(int64_t)blockIdx.x * nb + s] = dbins[s];
 }
 
 // out[s] = Σ_b partials[b][s], b in increasing order (deterministic FP64 reduction)

@@ -161,6 +161,28 @@ class ShellToShellResult:
 
 
 @dataclass
+class BandToBandResult:
+    """The NamedTuple calculate_band_to_band_transfer returns (BandTransfer.jl:110-112)."""
+    centers: np.ndarray
+    transfer_matrix: np.ndarray
+    net_transfer: np.ndarray
+    max_antisymmetry_error: float
+
+
+@dataclass(frozen=True)
+class SmoothBands:
+    """Graded spectral bands for band-to-band transfer (Types.jl:414-430): log-Gaussian weights
+    about `centers` (increasing, > 0) of width `logwidth` in ln κ, renormalised to a partition of
+    unity."""
+    centers: Tuple[float, ...]
+    logwidth: float = 0.6
+
+    def __post_init__(self):
+        object.__setattr__(self, "centers", tuple(float(c) for c in self.centers))
+        object.__setattr__(self, "logwidth", float(self.logwidth))
+
+
+@dataclass
 class ModeToModeTriadResult:
     invariant: AbstractInvariant
     ks: tuple

@@ -76,9 +76,11 @@ int64_t fit_plan_device_bytes(const fit_plan* plan);
  *   FIT_OPT_SUBGRID = 5: value 0 keeps the giver loop on the plan's own grid even when a
  *     band-limited sub-grid is attached (1, default: use it).
  *   FIT_OPT_COL_TILE = 6: columns per workgroup of the strided-axis passes (power of two; 0 =
- *     automatic: 16 within 64 KB of LDS). */
+ *     automatic: 16 within 64 KB of LDS).
+ *   FIT_OPT_XPASS_SELF = 7: 1 (default) forms the advecting velocity of a self-advected field in
+ *     registers inside the fused contiguous-axis pass; 0 stores it to device memory first. */
 enum { FIT_OPT_PLAIN_PIPELINE = 1, FIT_OPT_GIVER_BATCH = 2, FIT_OPT_XPASS_GIVERS = 3,
-       FIT_OPT_XPASS_BLOCK = 4, FIT_OPT_SUBGRID = 5, FIT_OPT_COL_TILE = 6 };
+       FIT_OPT_XPASS_BLOCK = 4, FIT_OPT_SUBGRID = 5, FIT_OPT_COL_TILE = 6, FIT_OPT_XPASS_SELF = 7 };
 int fit_plan_set_option(fit_plan* plan, int option, int value);
 
 /* Band-limited sub-grid of the giver loop (no reference counterpart).  When every shell lies in
@@ -139,6 +141,19 @@ int fit_shell_to_shell(fit_plan* plan, int invariant, const void* f_hat, const v
  */
 int fit_mode_to_mode(fit_plan* plan, int invariant, const void* f_hat, const void* adv_hat,
                      void* S, void* net, int io_dtype, int ptr_kind);
+
+/*
+ * Smooth band-to-band transfer T(n,m) between graded spectral bands (SmoothBands: log-Gaussian
+ * weights w_n(κ) about centers[n] of width logwidth in ln κ, renormalised to a partition of unity,
+ * 0 at κ <= 0).  Replaces BandTransfer.calculate_band_to_band_transfer (BandTransfer.jl:66-113):
+ * for every band m the nonlinear term of w_m ⊙ f̂ advected by adv_hat (or f̂), and
+ * T(n,m) = Σ_k w_n(k) t_m(k).  k_coord: host array of Np doubles, the band coordinate of every
+ * mode (shell_coordinate, ShellBinning.jl:19-36).  Tmat: n_bands² doubles (column-major,
+ * Tmat[n + n_bands*m] into band n from band m), net: n_bands doubles, *max_asym = max|T+Tᵀ|.
+ */
+int fit_band_to_band(fit_plan* plan, int invariant, const double* centers, int n_bands, double logwidth,
+                     const double* k_coord, const void* f_hat, const void* adv_hat, double* Tmat, double* net,
+                     double* max_asym, int io_dtype, int ptr_kind);
 
 /*
  * Helical (Craya-Herring) split of a 3D velocity into u+ and u- (each (ns..., 3)).

@@ -27,6 +27,9 @@ Parity pins (what checks each function; see tests/test_oracle.py):
                                  reductions S -> T(n,m) -> T(k), and the independent closed-form
                                  triad sum (mode_to_mode_closed_form).
   * helical decomposition     -- reconstruction, orthonormality, divergence, helicity split.
+  * band-to-band transfer     -- the DirectSum twin, the bilinear identity T = W S Wᵀ with the
+                                 independent mode-to-mode closed form S, partition of unity of
+                                 the weights, antisymmetry / conservation / net = row sums.
 No function of this module is "parity unpinned".
 """
 from __future__ import annotations
@@ -654,6 +657,51 @@ def mode_to_mode_closed_form(u_hat, ks, invariant="ke", dealiasing="23", advecti
             tot += np.real(np.conj(e[..., c]) * Nc)
         S[(Ellipsis,) + p] = tot
     return S
+
+
+# ---------------------------------------------------------------------------------------------
+# Smooth band-to-band transfer T(K,Q)
+# ---------------------------------------------------------------------------------------------
+
+@dataclass(frozen=True)
+class SmoothBands:            # Types.jl:414-430
+    centers: tuple
+    logwidth: float = 0.6
+
+
+def band_weights(centers, logwidth, k_coord):
+    """BandTransfer.jl:20-44 -- w_n(κ) = exp(-(ln(κ/c_n))²/(2σ²)) renormalised over the bands to a
+    partition of unity; modes with κ <= 0 (and modes whose weights all underflow) get weight 0."""
+    nb = len(centers)
+    pos = k_coord > 0
+    kk = np.where(pos, k_coord, 1.0)
+    W = np.zeros((nb,) + k_coord.shape)
+    for n, c in enumerate(centers):
+        r = np.log(kk / float(c))
+        W[n] = np.where(pos, np.exp(-(r * r) / (2.0 * float(logwidth) ** 2)), 0.0)
+    ssum = W.sum(axis=0)
+    return np.where(ssum > 0, W / np.where(ssum > 0, ssum, 1.0), W)
+
+
+def calculate_band_to_band_transfer(u_hat, ks, bands, dealiasing="23", invariant="ke",
+                                    spectral="FFTBackend", advecting_hat=None, geometry=None,
+                                    workers=1):
+    """BandTransfer.jl:66-113 -- T(n,m) = Σ_k w_n(k) t_m(k), t_m the density of the nonlinear term
+    of the band-m field w_m ⊙ û advected by the full velocity.  Returns
+    (centers, T, net = Σ_m T(n,m), max|T + Tᵀ|)."""
+    if advecting_hat is None:
+        advecting_hat = u_hat
+    kc = shell_coordinate(geometry, ks)
+    W = band_weights(bands.centers, bands.logwidth, kc)
+    nb = len(bands.centers)
+    T = np.zeros((nb, nb))
+    for m in range(nb):
+        fm = W[m][..., None] * u_hat
+        N = compute_nonlinear_term(fm, ks, dealiasing, spectral, advecting_hat, workers)
+        d = transfer_density(invariant, u_hat, N, ks)
+        for n in range(nb):
+            T[n, m] = float(np.sum(W[n] * d))
+    return np.asarray(bands.centers, dtype=float), T, T.sum(axis=1), float(np.max(np.abs(T + T.T)))
 
 
 # ---------------------------------------------------------------------------------------------

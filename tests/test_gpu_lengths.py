@@ -40,3 +40,36 @@ def test_register_lengths_nonlinear(fit, oracle, ns, prec, xblock):
     e = rel(got, ref)
     say("  err", e)
     assert e < (1e-11 if prec == "f64" else 2e-5)
+
+
+# every axis a register length: the shared-transform gradient passes and the self-advection
+# fused x pass (u formed in registers) are used; xself = 0 stores u to device memory instead
+SELF_CASES = [((128, 192), "23"), ((192, 128), "23"), ((256, 384), "23"), ((384, 256), "23"),
+              ((512, 128), "23"), ((768, 128), "23"), ((1024, 128), "23"), ((1536, 128), "23"),
+              ((2048, 128), "23"), ((3072, 128), "23"), ((128, 3072), "23"), ((128, 128, 128), "23"),
+              ((128, 128), "pad"), ((256, 128), "pad"), ((512, 128), "pad"), ((1024, 128), "pad"),
+              ((2048, 128), "pad"), ((128, 128, 128), "pad")]
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("xself", [1, 0])
+@pytest.mark.parametrize("ns,dea", SELF_CASES)
+def test_self_advection_register_lengths(fit, oracle, ns, dea, xself, prec):
+    from fit_amd import _lib
+    say("case self-advection lengths", ns, dea, xself, prec)
+    G = [n * 3 // 2 if dea == "pad" else n for n in ns]
+    if prec == "f64" and max(G) > 2048:
+        pytest.skip("FP64 transforms longer than 2048 exceed the plan's 64 KB LDS line limit")
+    rng = np.random.default_rng(sum(ns) + 7)
+    nd = len(ns)
+    ks = oracle.wavenumber_grid(ns, (2 * math.pi,) * nd)
+    u = rng.standard_normal(ns + (nd,)) + 1j * rng.standard_normal(ns + (nd,))
+    ref = oracle.compute_nonlinear_term(u, ks, dea)
+    deal = {"23": fit.OrszagTwoThirds(), "pad": fit.PaddedThreeHalves()}[dea]
+    dt = np.float64 if prec == "f64" else np.float32
+    with fit.Plan(ns, nd, nd, ks, deal, dt) as plan:
+        plan.set_option(_lib.OPT_XPASS_SELF, xself)
+        got = plan.nonlinear_term(u)
+    e = rel(got, ref)
+    say("  err", e)
+    assert e < (1e-11 if prec == "f64" else 2e-5)

@@ -395,3 +395,51 @@ def test_synthetic_field_properties(oracle):
     # Hermitian: u(-k) = conj u(k)
     neg = np.roll(u[::-1, ::-1, ::-1], 1, axis=(0, 1, 2))
     assert np.max(np.abs(neg - np.conj(u))) < 1e-15
+
+
+# ----------------------------------------------------------------------------- band-to-band
+
+def test_band_weights_partition_of_unity(oracle):
+    ks = oracle.wavenumber_grid((12, 10), (2 * math.pi,) * 2)
+    kc = oracle.shell_coordinate(None, ks)
+    W = oracle.band_weights((1.0, 2.0, 4.0), 0.5, kc)
+    s = W.sum(axis=0)
+    assert np.allclose(s[kc > 0], 1.0, atol=1e-14) and np.all(W[:, kc == 0] == 0)
+    # a mode at a band centre carries that band's largest share
+    i = np.argmin(np.abs(kc - 2.0).reshape(-1) + (kc == 0).reshape(-1))
+    assert np.argmax(W.reshape(3, -1)[:, i]) == 1
+
+
+@pytest.mark.parametrize("dea,inv", [("23", "ke"), ("none", "ke"), ("23", "enstrophy")])
+def test_band_to_band_equals_w_s_wt(oracle, dea, inv):
+    """T(n,m) = Σ_k Σ_p w_n(k) w_m(p) S(k|p): N̂ is linear in the advected field, so the band
+    transfer is the band-weighted mode-to-mode matrix, S from the index-arithmetic closed form."""
+    ns = (12, 12)
+    ks = oracle.wavenumber_grid(ns, (2 * math.pi,) * 2)
+    u, _ = oracle.streamfunction_velocity_2d(12, 31)
+    bands = oracle.SmoothBands((1.0, 2.0, 3.0), 0.5)
+    _, T, net, asym = oracle.calculate_band_to_band_transfer(u, ks, bands, dea, inv)
+    assert np.max(np.abs(T)) > 1e-9
+    S = oracle.mode_to_mode_closed_form(u, ks, inv, dea)
+    W = oracle.band_weights(bands.centers, bands.logwidth, oracle.shell_coordinate(None, ks))
+    Np = 144
+    S2 = S.reshape(Np, Np, order="F")          # [k, p], both column-major linear indices
+    Wf = W.reshape(3, -1, order="F")
+    ref = Wf @ S2 @ Wf.T
+    assert rel(T, ref) < 1e-12
+    assert rel(net, T.sum(axis=1)) < 1e-14
+
+
+def test_band_to_band_direct_sum_and_properties(oracle):
+    """runtests.jl:661-681: antisymmetric, conserves, net = Σ_m T(n,m), Σ net = 0; the FFT path
+    equals the DirectSum twin."""
+    ns = (16, 16)
+    ks = oracle.wavenumber_grid(ns, (2 * math.pi,) * 2)
+    u, _ = oracle.streamfunction_velocity_2d(16, 71)
+    bands = oracle.SmoothBands((1.0, 2.0, 3.0, 4.0), 0.5)
+    _, T, net, asym = oracle.calculate_band_to_band_transfer(u, ks, bands, "23")
+    Tn = np.sqrt(np.sum(T ** 2))
+    assert Tn > 0 and asym < 1e-10 * Tn and abs(T.sum()) < 1e-10 * Tn
+    assert abs(net.sum()) < 1e-10 * Tn
+    _, Td, _, _ = oracle.calculate_band_to_band_transfer(u, ks, bands, "23", spectral="DirectSumBackend")
+    assert rel(T, Td) < 1e-12
