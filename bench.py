@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the uneven 2D ensemble line (64 snapshots, graph replays)")
     ap.add_argument("--ensemble-steps", type=int, default=10)
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="run only the uneven 2D ensemble (for profiling it on its own)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device copy first (reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -230,6 +232,11 @@ def main():
     import numpy as np
     import torch
     fit = load_pkg()
+    if args.ensemble_only:
+        ens = ensemble_bench(fit, torch, args.ensemble_steps)
+        print(json.dumps({"metric": "uneven 2D ensemble fluxes", "value": ens["value"], "unit": ens["unit"],
+                          "ensemble": ens}), flush=True)
+        return
     n, ng = args.n, args.givers
     if args.copy_calibration:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")     # 1 GiB

@@ -108,7 +108,7 @@ struct fit_plan {
   int xfused_gc = 0;             // givers per workgroup in the wave-per-line x pass (0 = 4)
   int xfused_block = 0;          // 1: use the workgroup-barrier x pass instead
   int col_tx = 0;                // FIT_OPT_COL_TILE (0: automatic)
-  int xself = 1;                 // FIT_OPT_XPASS_SELF: self-advection x pass forms u in registers
+  int xself = 0;                 // FIT_OPT_XPASS_SELF: self-advection x pass forms u in registers
   // shared-transform gradients: wavenumbers odd under k -> -k on every populated bin
   bool commute = false;
   void* kt[2] = {nullptr, nullptr};  // transform-grid wavenumbers of x and y (compute precision)
@@ -936,8 +936,11 @@ int batched_lastdens(fit_plan* p, int gb, const void* eh, int64_t ehgs, double* 
   const int64_t Hn = (int64_t)n1h * p->n[1] * p->n[2];
   const int lax = p->nd == 3 ? 2 : 1;
   const int L = p->G[lax];
+  // columns per workgroup: up to 16 (128-byte rows) within the 160 KB a workgroup may hold; long
+  // columns (2048-point y lines of 2D ensembles) keep at least two lines per workgroup this way
   int TX = 16;
-  while (TX > 1 && ((size_t)2 * L * TX * csz + (size_t)nsh * 8 > 65536 || TX * L > 32 * kThreads)) TX /= 2;
+  while (TX > 1 && ((size_t)2 * L * TX * csz + (size_t)nsh * 8 > (size_t)160 * 1024 || TX * L > 32 * kThreads))
+    TX /= 2;
   const int xb = std::max(1, xext(p, bo[0]));
   const Band ob = p->nd == 3 ? Band{bo[1], p->G[1]} : full_band(1);
   const Band wb = p->nd == 3 ? Band{bo[2], p->G[2]} : Band{bo[1], p->G[1]};

@@ -973,9 +973,11 @@ __global__ void __launch_bounds__(256) k_lastdens(FftDesc d, const cpx<R>* __res
   cpx<R>* a = reinterpret_cast<cpx<R>*>(smem_raw);
   cpx<R>* b = a + (size_t)L * TX;
   double* bins = reinterpret_cast<double*>(b + (size_t)L * TX);
-  const int x0 = blockIdx.x * TX;
-  const int o = ob.at(blockIdx.y);
-  const int g = blockIdx.z;
+  int bx, by, bz;
+  xcd_tile(bx, by, bz);
+  const int x0 = bx * TX;
+  const int o = ob.at(by);
+  const int g = bz;
   eh += (int64_t)g * ehgs;  // ensembles: each batch index carries its own carrier
   const int tid = threadIdx.x, nth = blockDim.x;
   for (int s = tid; s < nsh; s += nth) bins[s] = 0.0;
@@ -1024,7 +1026,7 @@ __global__ void __launch_bounds__(256) k_lastdens(FftDesc d, const cpx<R>* __res
     }
   }
   __syncthreads();
-  const int64_t blk = (int64_t)g * gridDim.x * gridDim.y + (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+  const int64_t blk = (int64_t)g * gridDim.x * gridDim.y + (int64_t)by * gridDim.x + bx;
   for (int s = tid; s < nsh; s += nth) partials[blk * nsh + s] = bins[s];
 }
 
