@@ -261,8 +261,7 @@ def main():
 
     def step():
         N = pi_plan.nonlinear_term(u)
-        T_ke, Pi_ke = pi_plan.spectral_flux_with_nhat(KE, u, N)
-        T_h, Pi_h = pi_plan.spectral_flux_with_nhat(HEL, u, N)
+        (T_ke, Pi_ke), (T_h, Pi_h) = pi_plan.spectral_fluxes_with_nhat((KE, HEL), u, N)
         Tm, net, asym = tm_plan.shell_to_shell(KE, u)
         return Pi_ke, Pi_h, Tm, asym
 

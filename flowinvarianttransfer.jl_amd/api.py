@@ -232,6 +232,19 @@ class Plan:
                                                         _dptr(T), _dptr(Pi), self.io, kind))
         return T, Pi
 
+    def spectral_fluxes_with_nhat(self, invariants, c_hat, N_hat):
+        """[(T, Π)] for several invariants from one carrier and N̂ in one density pass."""
+        cp, ck, kind = _marshal(c_hat, self.io)
+        npp, nk, _ = _marshal(N_hat, self.io)
+        q = len(invariants)
+        T = np.zeros(q * self.n_shells)
+        Pi = np.zeros(q * self.n_shells)
+        codes = (ctypes.c_int * q)(*[inv.code for inv in invariants])
+        _lib.check(self.lib.fit_spectral_fluxes_with_nhat(self.handle, q, codes, cp, npp, _dptr(T),
+                                                          _dptr(Pi), self.io, kind))
+        n = self.n_shells
+        return [(T[i * n:(i + 1) * n], Pi[i * n:(i + 1) * n]) for i in range(q)]
+
     def shell_to_shell(self, invariant, f, adv=None, verify=True):
         fp, ap, keep, kind = self._fa(f, adv)
         n = self.n_shells

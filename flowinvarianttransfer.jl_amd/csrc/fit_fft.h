@@ -204,8 +204,9 @@ constexpr bool rp_pow2(int L) { return L > 0 && (L & (L - 1)) == 0; }
 constexpr int rp_E(int L) { return rp_pow2(L) ? 8 : 24; }
 // points per thread for kernels that keep two lines in registers: radix-4 stages for 3·2^a
 constexpr int gE(int L) { return rp_pow2(L) ? 8 : 12; }
-// points per thread of the workgroup-form fused x pass (measured: radix-8 stages win from 768 up)
-constexpr int xE(int L) { return rp_pow2(L) ? 8 : (L <= 384 ? 12 : 24); }
+// points per thread of the workgroup-form fused x pass (radix-4 stages for 3·2^a: at 768 they
+// halve the registers of the radix-8 form and run 10% faster)
+constexpr int xE(int L) { return rp_pow2(L) ? 8 : 12; }
 // points per thread of the self-advection fused x pass (three velocity lines stay in registers)
 constexpr int sE(int L) { return rp_pow2(L) ? 8 : 12; }
 
@@ -255,16 +256,29 @@ __device__ __forceinline__ void reg_stage(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* 
 // v[e] = x[t + T e] in, X[t + T e] out (natural order).  DB: lds0 / lds1 are two exchange
 // buffers (one barrier per stage); otherwise lds0 is used alone (two barriers per stage).  The
 // caller must place a barrier between two calls that share the buffers.
+// For the long 3·2^a lengths the twiddle base is passed through an empty asm so that the compiler
+// cannot hoist the twiddle loads of a transform out of the loops around it: kernels that run
+// several transforms per line would otherwise keep every twiddle of every stage live in registers
+// for their whole duration (256 VGPRs plus AGPR spill, one wave per SIMD, in the fused x pass at
+// 768).  Shorter lines keep the hoisted twiddles (measured faster at 192).
+template <int L>
+__device__ __forceinline__ void tw_opaque(const void*& tw) {
+  if constexpr (L >= 768 && !rp_pow2(L)) asm volatile("" : "+s"(tw));
+}
 template <int L, int SIGN, bool DB, typename R, class IDX>
 __device__ __forceinline__ void reg_fft(cpx<R> (&v)[rp_E(L)], cpx<R>* lds0, cpx<R>* lds1, const IDX& idx, int t,
-                                        const cpx<R>* __restrict__ tw) {
-  reg_stage<L, rp_E(L), 0, 1, SIGN, DB, R, IDX>(v, lds0, lds1, idx, t, tw);
+                                        const cpx<R>* tw) {
+  const void* tv = tw;
+  tw_opaque<L>(tv);
+  reg_stage<L, rp_E(L), 0, 1, SIGN, DB, R, IDX>(v, lds0, lds1, idx, t, (const cpx<R>*)tv);
 }
 // the same with an explicit number of points per thread (T = L/E threads per line)
 template <int L, int E, int SIGN, bool DB, typename R, class IDX>
 __device__ __forceinline__ void reg_fft_e(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* lds1, const IDX& idx, int t,
-                                          const cpx<R>* __restrict__ tw) {
-  reg_stage<L, E, 0, 1, SIGN, DB, R, IDX>(v, lds0, lds1, idx, t, tw);
+                                          const cpx<R>* tw) {
+  const void* tv = tw;
+  tw_opaque<L>(tv);
+  reg_stage<L, E, 0, 1, SIGN, DB, R, IDX>(v, lds0, lds1, idx, t, (const cpx<R>*)tv);
 }
 
 // line-major LDS index with one pad slot every 16 points (breaks power-of-two strides)

@@ -733,7 +733,7 @@ int density_launch(fit_plan* p, int inv, const void* car, const void* nh, void* 
   const size_t csz = sizeof(cpx<R>);
   prof_begin(p, FIT_K_DENSITY_SHELL,
              (double)p->Np * (2.0 * p->M * csz + (shells ? 4.0 : 0.0) + (t_out ? sizeof(R) : 0.0)));
-  hipLaunchKernelGGL(k_density<R>, dim3(nblk), dim3(kThreads), lds, p->stream, g, inv, p->M,
+  hipLaunchKernelGGL(k_density<R>, dim3(nblk), dim3(kThreads), lds, p->stream, g, int4{inv, 0, 0, 0}, 1, p->M,
                      (const cpx<R>*)car, p->Np, (const cpx<R>*)nh, p->Np, (const R*)p->ks[0],
                      (const R*)p->ks[1], (const R*)p->ks[2], shells ? p->shell : nullptr, nsh,
                      p->partials, (R*)t_out);
@@ -746,6 +746,32 @@ int density_launch(fit_plan* p, int inv, const void* car, const void* nh, void* 
     prof_end(p);
     FIT_HIP(hipGetLastError());
   }
+  return FIT_OK;
+}
+
+// shell sums of ninv (<= 4) invariants in one density pass over carrier and N̂: rows of nsh
+// doubles, invariant q at dcolG + q*nsh
+template <typename R>
+int density_multi(fit_plan* p, int ninv, const int* invs, const void* car, const void* nh) {
+  Grid g = grid_of<R>(p);
+  const int nsh = p->nsh, nb = ninv * nsh;
+  const int nblk = p->nblk_density;
+  const size_t csz = sizeof(cpx<R>);
+  int rc;
+  if ((rc = grow(p, (void**)&p->partials2, &p->partials2_bytes, (size_t)nblk * nb * sizeof(double)))) return rc;
+  if ((rc = grow(p, (void**)&p->dcolG, &p->dcolG_bytes, (size_t)nb * sizeof(double)))) return rc;
+  const int4 iv{invs[0], ninv > 1 ? invs[1] : 0, ninv > 2 ? invs[2] : 0, ninv > 3 ? invs[3] : 0};
+  prof_begin(p, FIT_K_DENSITY_SHELL, (double)p->Np * (2.0 * p->M * csz + 4.0));
+  hipLaunchKernelGGL(k_density<R>, dim3(nblk), dim3(kThreads), (size_t)nb * sizeof(double), p->stream, g, iv, ninv,
+                     p->M, (const cpx<R>*)car, p->Np, (const cpx<R>*)nh, p->Np, (const R*)p->ks[0],
+                     (const R*)p->ks[1], (const R*)p->ks[2], p->shell, nsh, p->partials2, (R*)nullptr);
+  prof_end(p);
+  FIT_HIP(hipGetLastError());
+  prof_begin(p, FIT_K_REDUCE, (double)nblk * nb * 8.0 + nb * 8.0);
+  hipLaunchKernelGGL(k_reduce_partials, dim3((nb + 127) / 128), dim3(128), 0, p->stream, p->partials2, nblk, nb,
+                     p->dcolG);
+  prof_end(p);
+  FIT_HIP(hipGetLastError());
   return FIT_OK;
 }
 
@@ -1789,6 +1815,38 @@ int fit_spectral_flux_with_nhat(fit_plan* p, int inv, const void* c_hat, const v
                            : density_launch<float>(p, inv, c, n, nullptr, true);
   if (rc) return rc;
   return spectral_flux_tail(p, T, Pi);
+}
+
+int fit_spectral_fluxes_with_nhat(fit_plan* p, int n_inv, const int* invariants, const void* c_hat,
+                                  const void* N_hat, double* T, double* Pi, int io_dtype, int ptr_kind) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (p->nsh <= 0) return fail(FIT_E_ARGUMENT, "plan has no shells");
+  if (!invariants || n_inv < 1 || n_inv > 4) return fail(FIT_E_ARGUMENT, "1 to 4 invariants per call");
+  for (int q = 0; q < n_inv; ++q)
+    if ((rc = check_invariant(p, invariants[q], p->M))) return rc;
+  if ((size_t)n_inv * p->nsh * sizeof(double) > (size_t)160 * 1024)
+    return fail(FIT_E_UNSUPPORTED, "n_inv * n_shells exceeds the shell bins one workgroup holds");
+  const void *c, *n;
+  if ((rc = stage_in(p, c_hat, p->M, io_dtype, ptr_kind, &p->in_f, &c))) return rc;
+  if ((rc = stage_in(p, N_hat, p->M, io_dtype, ptr_kind, &p->in_c, &n))) return rc;
+  rc = p->dtype == FIT_F64 ? density_multi<double>(p, n_inv, invariants, c, n)
+                           : density_multi<float>(p, n_inv, invariants, c, n);
+  if (rc) return rc;
+  const int nb = n_inv * p->nsh;
+  std::vector<double> h((size_t)nb);
+  FIT_HIP(hipMemcpyAsync(h.data(), p->dcolG, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  if ((rc = finish(p))) return rc;
+  for (int q = 0; q < n_inv; ++q) {
+    double acc = 0.0;
+    for (int s = 0; s < p->nsh; ++s) {
+      const double v = h[(size_t)q * p->nsh + s];
+      acc += v;  // Π = +cumsum(T) (SpectralFlux.jl:172-177)
+      if (T) T[(size_t)q * p->nsh + s] = v;
+      if (Pi) Pi[(size_t)q * p->nsh + s] = acc;
+    }
+  }
+  return FIT_OK;
 }
 
 int fit_shell_to_shell(fit_plan* p, int inv, const void* f_hat, const void* adv_hat, double* Tmat, double* net,

@@ -1143,14 +1143,17 @@ __device__ __forceinline__ R density_at(int inv, int nd, int M, const cpx<R>* ca
 // density over the full original grid; optional per-mode output t_out; optional shell partial
 // sums (FP64 LDS bins, one row of nsh partials per workgroup, reduced later in fixed order).
 template <typename R>
-__global__ void __launch_bounds__(256) k_density(Grid g, int inv, int M, const cpx<R>* __restrict__ car,
+__global__ void __launch_bounds__(256) k_density(Grid g, int4 invs, int ninv, int M, const cpx<R>* __restrict__ car,
                                                  int64_t csf, const cpx<R>* __restrict__ nh, int64_t nsf,
                                                  const R* __restrict__ ks0, const R* __restrict__ ks1,
                                                  const R* __restrict__ ks2, const int* __restrict__ shell,
                                                  int nsh, double* __restrict__ partials, R* __restrict__ t_out) {
+  // ninv invariants (codes invs.x, .y, .z, .w) from one read of carrier and N̂: bins q*nsh + s;
+  // t_out (per-mode output) holds the first invariant's density
   extern __shared__ double dbins[];
   double* bins = dbins;
-  for (int s = threadIdx.x; s < nsh; s += blockDim.x) bins[s] = 0.0;
+  const int nb = ninv * nsh;
+  for (int s = threadIdx.x; s < nb; s += blockDim.x) bins[s] = 0.0;
   __syncthreads();
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < g.Np; o += stride) {
@@ -1160,16 +1163,17 @@ __global__ void __launch_bounds__(256) k_density(Grid g, int inv, int M, const c
     const R kx = ks0[ix];
     const R ky = g.nd >= 2 ? ks1[iy] : (R)0;
     const R kz = g.nd >= 3 ? ks2[iz] : (R)0;
-    const R t = density_at<R>(inv, g.nd, M, car, csf, nh, nsf, o, kx, ky, kz);
-    if (t_out) t_out[o] = t;
-    if (shell) {
-      const int s = shell[o];
-      if (s > 0) atomicAdd(&bins[s - 1], (double)t);
+    const int s = shell ? shell[o] : 0;
+    for (int q = 0; q < ninv; ++q) {
+      const int inv = q == 0 ? invs.x : (q == 1 ? invs.y : (q == 2 ? invs.z : invs.w));
+      const R t = density_at<R>(inv, g.nd, M, car, csf, nh, nsf, o, kx, ky, kz);
+      if (q == 0 && t_out) t_out[o] = t;
+      if (s > 0) atomicAdd(&bins[q * nsh + s - 1], (double)t);
     }
   }
   if (shell) {
     __syncthreads();
-    for (int s = threadIdx.x; s < nsh; s += blockDim.x) partials[(int64_t)blockIdx.x * nsh + s] = bins[s];
+    for (int s = threadIdx.x; s < nb; s += blockDim.x) partials[(int64_t)blockIdx.x * nb + s] = bins[s];
   }
 }
 

@@ -124,6 +124,16 @@ int fit_spectral_flux_with_nhat(fit_plan* plan, int invariant, const void* c_hat
                                 double* T, double* Pi, int io_dtype, int ptr_kind);
 
 /*
+ * T and Pi of n_inv (1..4) invariants from one carrier and one N̂ in a single density pass over
+ * them.  The reference evaluates _calculate_spectral_flux_with_N̂! (SpectralFlux.jl:148-180) once
+ * per invariant; this is that tail for several invariants sharing N̂ (as calculate_spectral_flux
+ * with a decomposition shares it, SpectralFlux.jl:96-122).  invariants: n_inv codes; T, Pi: host
+ * arrays of n_inv*n_shells doubles, invariant q's shells at offset q*n_shells.
+ */
+int fit_spectral_fluxes_with_nhat(fit_plan* plan, int n_inv, const int* invariants, const void* c_hat,
+                                  const void* N_hat, double* T, double* Pi, int io_dtype, int ptr_kind);
+
+/*
  * Shell-to-shell matrix T(n,m) over the plan's shells (every shell is a giver).  Replaces
  * ShellToShellTransfer._calculate_shell_to_shell! (ShellToShellTransfer.jl:152-159), i.e. the
  * FFT giver loop FFTWExt._shell_to_shell_fft! (FFTWExt.jl:209-295) and the KernelAbstractions

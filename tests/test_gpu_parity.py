@@ -319,3 +319,30 @@ def test_fused_nonlinear_equals_plain(fit, oracle, ns, dea):
         Np_ = plan.nonlinear_term(u)
     ref = oracle.compute_nonlinear_term(u, ks, dea)
     assert rel(Nf, ref) < TOL64 and rel(Np_, ref) < TOL64
+
+
+@pytest.mark.parametrize("ns,dea,invs", [((12, 12, 12), "pad", ("ke", "helicity", "enstrophy")),
+                                         ((16, 16, 16), "23", ("ke", "helicity")),
+                                         ((24, 20), "none", ("enstrophy", "ke"))])
+def test_several_invariants_one_density_pass(fit, oracle, ns, dea, invs):
+    """fit_spectral_fluxes_with_nhat: several invariants' T, Π from one N̂ in one pass equal the
+    oracle's per-invariant fluxes (and the one-invariant entry point)."""
+    say("case multi-invariant density", ns, dea, invs)
+    nd = len(ns)
+    ks = ks_of(oracle, ns)
+    u = oracle.synthetic_field(ns, nd, seed=21) if nd == 3 else rand_c(np.random.default_rng(5), ns + (2,))
+    deal = {"none": fit.NoDealiasing(), "23": fit.OrszagTwoThirds(), "pad": fit.PaddedThreeHalves()}[dea]
+    I = {"ke": fit.KineticEnergy(), "helicity": fit.Helicity(), "enstrophy": fit.Enstrophy()}
+    from fit_amd.grid import shell_setup
+    c, e, sidx = shell_setup(ks, fit.LinearBinning(1.0), None)
+    for prec, tol in ((np.float64, TOL64), (np.float32, 1e-5)):
+        with fit.Plan(ns, nd, nd, ks, deal, prec, sidx, len(c)) as plan:
+            N = plan.nonlinear_term(u)
+            multi = plan.spectral_fluxes_with_nhat([I[i] for i in invs], u, N)
+            for inv, (T, Pi) in zip(invs, multi):
+                _, To, Po = oracle.calculate_spectral_flux(u, ks, oracle.LinearBinning(1.0), dea, inv)
+                Ts, Ps = plan.spectral_flux_with_nhat(I[inv], u, N)
+                e1, e2 = rel(T, To), rel(Pi, Po)
+                say("  ", prec.__name__, inv, e1, e2)
+                assert e1 < tol and e2 < tol
+                assert rel(T, Ts) < 1e-12 and rel(Pi, Ps) < 1e-12
