@@ -160,14 +160,14 @@ def ensemble_bench(fit, torch, steps):
 
 
 def measured_traffic(plan_key, kernel_class, n, ng):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE, copy-calibrated) when one exists for this kernel and workload."""
-    if (plan_key, kernel_class, n, ng) != ("tnm", "fused_x", 512, 48):
-        return None
-    path = os.path.join(ROOT, "profiles", "k_xfused_w_f32_512_tnm48_pmc.md")
+    """HBM bytes per launch of the dominant kernel class, from the committed rocprofv3 PMC summary
+    of this workload (FETCH_SIZE + WRITE_SIZE of the class's kernels, calibrated on a 1 GiB copy),
+    or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", f"snapshot{n}_tnm{ng}_pmc_traffic.md")
+    key = f"traffic_bytes_per_launch[{plan_key}:{kernel_class}]"
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch"):
+            if line.startswith(key):
                 return float(line.split("=")[1])
     except OSError:
         return None

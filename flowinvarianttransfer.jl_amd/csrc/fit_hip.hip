@@ -965,7 +965,8 @@ int batched_lastdens(fit_plan* p, int gb, const void* eh, int64_t ehgs, double* 
   // columns per workgroup: up to 16 (128-byte rows) within the 160 KB a workgroup may hold; long
   // columns (2048-point y lines of 2D ensembles) keep at least two lines per workgroup this way
   int TX = 16;
-  while (TX > 1 && ((size_t)2 * L * TX * csz + (size_t)nsh * 8 > (size_t)160 * 1024 || TX * L > 32 * kThreads))
+  // (TX*L <= 16*256 keeps the per-thread accumulators at 16: about 160 VGPRs, three waves per SIMD)
+  while (TX > 1 && ((size_t)2 * L * TX * csz + (size_t)nsh * 8 > (size_t)160 * 1024 || TX * L > 16 * kThreads))
     TX /= 2;
   const int xb = std::max(1, xext(p, bo[0]));
   const Band ob = p->nd == 3 ? Band{bo[1], p->G[1]} : full_band(1);
@@ -986,10 +987,18 @@ int batched_lastdens(fit_plan* p, int gb, const void* eh, int64_t ehgs, double* 
   }
   prof_begin(p, FIT_K_DENSITY_SHELL,
              (double)gb * p->M * xb * gy * L * csz + (double)gb * p->M * xb * gy * wb.count() * csz);
-  hipLaunchKernelGGL(k_lastdens<R>, dim3(gx, gy, gb), dim3(kThreads), (size_t)2 * L * TX * csz + (size_t)nsh * 8,
-                     p->stream, p->fd[lax], (const cpx<R>*)p->tw[lax], (const cpx<R>*)p->nxbuf, sy, sz, sf, xb, TX,
-                     lax, ob, wb, p->M, (R)scale, (const cpx<R>*)eh, (int64_t)n1h, (int64_t)n1h * p->n[1], Hn,
-                     p->shh, nsh, p->n[0], p->partials2, ehgs);
+  // per-thread accumulators: the smallest register array that covers TX*L points per workgroup
+  const int qneed = (L * TX + kThreads - 1) / kThreads;
+#define FIT_LASTDENS(QQ)                                                                                         \
+  hipLaunchKernelGGL((k_lastdens<R, QQ>), dim3(gx, gy, gb), dim3(kThreads), (size_t)2 * L * TX * csz + (size_t)nsh * 8, \
+                     p->stream, p->fd[lax], (const cpx<R>*)p->tw[lax], (const cpx<R>*)p->nxbuf, sy, sz, sf, xb, TX, \
+                     lax, ob, wb, p->M, (R)scale, (const cpx<R>*)eh, (int64_t)n1h, (int64_t)n1h * p->n[1], Hn,     \
+                     p->shh, nsh, p->n[0], p->partials2, ehgs)
+  if (qneed <= 4) FIT_LASTDENS(4);
+  else if (qneed <= 8) FIT_LASTDENS(8);
+  else if (qneed <= 16) FIT_LASTDENS(16);
+  else FIT_LASTDENS(32);
+#undef FIT_LASTDENS
   prof_end(p);
   FIT_HIP(hipGetLastError());
   prof_begin(p, FIT_K_REDUCE, (double)gb * nblk * nsh * 8.0);

@@ -960,7 +960,7 @@ __global__ void __launch_bounds__(256) k_xfused_w(const cpx<R>* __restrict__ tw,
 // masked effective carrier (the pair k, -k at once: w = 2 off the kx = 0, n1/2 planes), and
 // FP64 LDS shell bins -> one partial row per workgroup.  N̂ never touches HBM.
 // ---------------------------------------------------------------------------------------------
-template <typename R>
+template <typename R, int QMAX>
 __global__ void __launch_bounds__(256) k_lastdens(FftDesc d, const cpx<R>* __restrict__ tw,
                                                   const cpx<R>* __restrict__ src, int64_t sy, int64_t sz,
                                                   int64_t sf, int nx, int TX, int axis, Band ob, Band wb, int M,
@@ -981,8 +981,7 @@ __global__ void __launch_bounds__(256) k_lastdens(FftDesc d, const cpx<R>* __res
   eh += (int64_t)g * ehgs;  // ensembles: each batch index carries its own carrier
   const int tid = threadIdx.x, nth = blockDim.x;
   for (int s = tid; s < nsh; s += nth) bins[s] = 0.0;
-  constexpr int QMAX = 32;  // TX*L <= QMAX*256 per launch
-  R tq[QMAX];
+  R tq[QMAX];  // TX*L <= QMAX*256 per launch: one density accumulator per owned line point
 #pragma unroll
   for (int q = 0; q < QMAX; ++q) tq[q] = (R)0;
   for (int c = 0; c < M; ++c) {
