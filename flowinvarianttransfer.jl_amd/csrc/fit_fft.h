@@ -372,11 +372,10 @@ namespace fit {
 // m = (t + i T) mod NS, in the forward sign (the inverse uses the conjugate).
 // ---------------------------------------------------------------------------------------------
 constexpr int tw_off(int L, int E, int S) {
-  int off = 0, ns = 1;
-  for (int s = 0; s < S; ++s) {
+  int off = 0;
+  for (int s = 1; s < S; ++s) {
     const int r = rp_rad(L, E, s);
-    if (s >= 1) off += (E / r) * (r - 1);
-    ns *= r;
+    off += (E / r) * (r - 1);
   }
   return off;
 }

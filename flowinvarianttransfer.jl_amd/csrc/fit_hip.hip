@@ -108,7 +108,8 @@ struct fit_plan {
   int xfused_gc = 0;             // givers per workgroup in the wave-per-line x pass (0 = 4)
   int xfused_block = 0;          // 1: use the workgroup-barrier x pass instead
   int col_tx = 0;                // FIT_OPT_COL_TILE (0: automatic)
-  int xself = 0;                 // FIT_OPT_XPASS_SELF: self-advection x pass forms u in registers
+  int xself = -1;                // FIT_OPT_XPASS_SELF: self-advection x pass forms u in registers
+                                 // (-1: automatic, on for x transforms of at most 2048 points)
   // shared-transform gradients: wavenumbers odd under k -> -k on every populated bin
   bool commute = false;
   void* kt[2] = {nullptr, nullptr};  // transform-grid wavenumbers of x and y (compute precision)
@@ -896,7 +897,7 @@ int nonlinear_term_impl(fit_plan* p, const void* f, const void* a) {
     // its x pass remains (the y and z passes are shared with the gradients); the fused x pass
     // forms it in registers
     if ((rc = v2_inverse_grads<R>(p, f, 0, 1, p->band_in, nullptr))) return rc;
-    if (p->xself) {
+    if (p->xself > 0 || (p->xself < 0 && p->G[0] <= 2048)) {
       if ((rc = v2_xfused_self<R>(p, p->band_in, p->band_in))) return rc;
       return v2_forward_nhat<R>(p, p->band_in);
     }
@@ -1726,7 +1727,7 @@ int fit_plan_set_option(fit_plan* p, int option, int value) {
         return fail(FIT_E_ARGUMENT, "column tile must be 0 or a power of two <= 64");
       p->col_tx = value;
       return FIT_OK;
-    case FIT_OPT_XPASS_SELF: p->xself = value != 0; return FIT_OK;
+    case FIT_OPT_XPASS_SELF: p->xself = value < 0 ? -1 : (value != 0); return FIT_OK;
     default: return fail(FIT_E_ARGUMENT, "unknown plan option");
   }
 }

@@ -52,7 +52,7 @@ SELF_CASES = [((128, 192), "23"), ((192, 128), "23"), ((256, 384), "23"), ((384,
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-@pytest.mark.parametrize("xself", [1, 0])
+@pytest.mark.parametrize("xself", [1, 0, -1])
 @pytest.mark.parametrize("ns,dea", SELF_CASES)
 def test_self_advection_register_lengths(fit, oracle, ns, dea, xself, prec):
     from fit_amd import _lib
@@ -68,6 +68,29 @@ def test_self_advection_register_lengths(fit, oracle, ns, dea, xself, prec):
     deal = {"23": fit.OrszagTwoThirds(), "pad": fit.PaddedThreeHalves()}[dea]
     dt = np.float64 if prec == "f64" else np.float32
     with fit.Plan(ns, nd, nd, ks, deal, dt) as plan:
+        plan.set_option(_lib.OPT_XPASS_SELF, xself)
+        got = plan.nonlinear_term(u)
+    e = rel(got, ref)
+    say("  err", e)
+    assert e < (1e-11 if prec == "f64" else 2e-5)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("xself", [1, 0, -1])
+@pytest.mark.parametrize("dea", ["23", "pad"])
+def test_self_advection_three_components_on_a_plane(fit, oracle, dea, xself, prec):
+    """A 2D grid carrying three velocity components (M = 3 > nd = 2): the first two advect all
+    three, (u·∇)f_c with u = (f_1, f_2), as the oracle's self-advection does."""
+    from fit_amd import _lib
+    say("case self-advection 2D3C", dea, xself, prec)
+    ns = (256, 128)
+    rng = np.random.default_rng(91)
+    ks = oracle.wavenumber_grid(ns, (2 * math.pi,) * 2)
+    u = rng.standard_normal(ns + (3,)) + 1j * rng.standard_normal(ns + (3,))
+    ref = oracle.compute_nonlinear_term(u, ks, dea)
+    deal = {"23": fit.OrszagTwoThirds(), "pad": fit.PaddedThreeHalves()}[dea]
+    dt = np.float64 if prec == "f64" else np.float32
+    with fit.Plan(ns, 3, 2, ks, deal, dt) as plan:
         plan.set_option(_lib.OPT_XPASS_SELF, xself)
         got = plan.nonlinear_term(u)
     e = rel(got, ref)
