@@ -272,7 +272,7 @@ __device__ __forceinline__ void reg_fft(cpx<R> (&v)[rp_E(L)], cpx<R>* lds0, cpx<
   tw_opaque<L>(tv);
   reg_stage<L, rp_E(L), 0, 1, SIGN, DB, R, IDX>(v, lds0, lds1, idx, t, (const cpx<R>*)tv);
 }
-// the same with an explicit number of points per thread (T = L/E threads per line)
+// The same with an explicit number of points per thread (T = L/E threads per line).
 template <int L, int E, int SIGN, bool DB, typename R, class IDX>
 __device__ __forceinline__ void reg_fft_e(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* lds1, const IDX& idx, int t,
                                           const cpx<R>* tw) {
@@ -280,6 +280,24 @@ __device__ __forceinline__ void reg_fft_e(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* 
   tw_opaque<L>(tv);
   reg_stage<L, E, 0, 1, SIGN, DB, R, IDX>(v, lds0, lds1, idx, t, (const cpx<R>*)tv);
 }
+// For kernels that run several transforms per lane: FENCE passes the lane index through an empty
+// asm first.  Every LDS and twiddle offset of a stage derives from it, and without the fence the
+// compiler computes them once and keeps all of them live across every transform of the kernel
+// (about 100 VGPRs at E = 12: 171 instead of 90 in the two-line gradient pass at 768, two waves
+// per SIMD instead of five).  Recomputing them costs a few integer operations per stage, which
+// is a loss where the fence does not raise the occupancy.
+template <int L, int E, int SIGN, bool FENCE, typename R, class IDX>
+__device__ __forceinline__ void reg_fft_m(cpx<R> (&v)[E], cpx<R>* lds, const IDX& idx, int t, const cpx<R>* tw) {
+  if constexpr (FENCE) asm volatile("" : "+v"(t));
+  reg_fft_e<L, E, SIGN, false, R, IDX>(v, lds, lds, idx, t, tw);
+}
+// Fused x passes, by measurement on MI355X (FP32 flux of a self-advected field, both forms):
+// the gradient-product pass runs fenced above 192 (3.6 vs 4.3 ms at 384, 27.0 vs 27.9 ms at
+// 768, 0.50 vs 0.54 ms at 2048, equal elsewhere); the self-advection pass, which holds three
+// velocity lines, only at 384, 512 and 3072 (at 768 the fence leaves it at two waves per SIMD
+// and costs 1.7 ms of 27).
+template <int L> constexpr bool xfence_r() { return L > 192; }
+template <int L> constexpr bool xfence_s() { return L == 384 || L == 512 || L == 3072; }
 
 // line-major LDS index with one pad slot every 16 points (breaks power-of-two strides)
 struct IdxLine {

@@ -108,8 +108,7 @@ struct fit_plan {
   int xfused_gc = 0;             // givers per workgroup in the wave-per-line x pass (0 = 4)
   int xfused_block = 0;          // 1: use the workgroup-barrier x pass instead
   int col_tx = 0;                // FIT_OPT_COL_TILE (0: automatic)
-  int xself = -1;                // FIT_OPT_XPASS_SELF: self-advection x pass forms u in registers
-                                 // (-1: automatic, on for x transforms of at most 2048 points)
+  int xself = 1;                 // FIT_OPT_XPASS_SELF: self-advection x pass forms u in registers
   // shared-transform gradients: wavenumbers odd under k -> -k on every populated bin
   bool commute = false;
   void* kt[2] = {nullptr, nullptr};  // transform-grid wavenumbers of x and y (compute precision)
@@ -462,6 +461,16 @@ bool commute_ok(const fit_plan* p) {
   return true;
 }
 
+// Columns per workgroup of the two-line gradient passes: 16 within 512 threads and 64 KB of
+// LDS, or the plan's column tile within 1024 threads and 160 KB.
+int grad_tile(const fit_plan* p, int L, size_t csz) {
+  const int T = two_T(L);
+  int TX = p->col_tx > 0 ? std::min(p->col_tx, 1024 / T) : std::min(16, 512 / T);
+  const size_t cap = p->col_tx > 0 ? (size_t)160 * 1024 : g_max_lds;
+  while (TX > 1 && (size_t)L * TX * csz > cap) TX /= 2;
+  return TX;
+}
+
 // First and (3D) y passes of the shared-transform gradients of ng*M fields into work2.
 template <typename R>
 int grads_shared(fit_plan* p, const void* f, int giver0, int ng, const int* bi, const int4* fband, int xa,
@@ -477,9 +486,7 @@ int grads_shared(fit_plan* p, const void* f, int giver0, int ng, const int* bi, 
   }
   {
     const int L = p->G[ax];
-    const int T = two_T(L);
-    int TX = std::min(16, 512 / T);
-    while (TX > 1 && (size_t)L * TX * csz > g_max_lds) TX /= 2;
+    const int TX = grad_tile(p, L, csz);
     const Band ob = nd == 3 ? Band{bi[1], p->G[1]} : full_band(1);
     const Band ib = Band{bi[ax], L};
     const Band wb = full_band(L);
@@ -501,9 +508,7 @@ int grads_shared(fit_plan* p, const void* f, int giver0, int ng, const int* bi, 
   }
   if (nd == 3) {
     const int L = p->G[1];
-    const int T = two_T(L);
-    int TX = std::min(16, 512 / T);
-    while (TX > 1 && (size_t)L * TX * csz > g_max_lds) TX /= 2;
+    const int TX = grad_tile(p, L, csz);
     const Band ob = full_band(p->G[2]);
     const Band ib = Band{bi[1], L};
     const Band wb = full_band(L);
@@ -897,7 +902,7 @@ int nonlinear_term_impl(fit_plan* p, const void* f, const void* a) {
     // its x pass remains (the y and z passes are shared with the gradients); the fused x pass
     // forms it in registers
     if ((rc = v2_inverse_grads<R>(p, f, 0, 1, p->band_in, nullptr))) return rc;
-    if (p->xself > 0 || (p->xself < 0 && p->G[0] <= 2048)) {
+    if (p->xself) {
       if ((rc = v2_xfused_self<R>(p, p->band_in, p->band_in))) return rc;
       return v2_forward_nhat<R>(p, p->band_in);
     }
@@ -1727,7 +1732,7 @@ int fit_plan_set_option(fit_plan* p, int option, int value) {
         return fail(FIT_E_ARGUMENT, "column tile must be 0 or a power of two <= 64");
       p->col_tx = value;
       return FIT_OK;
-    case FIT_OPT_XPASS_SELF: p->xself = value < 0 ? -1 : (value != 0); return FIT_OK;
+    case FIT_OPT_XPASS_SELF: p->xself = value != 0; return FIT_OK;
     default: return fail(FIT_E_ARGUMENT, "unknown plan option");
   }
 }

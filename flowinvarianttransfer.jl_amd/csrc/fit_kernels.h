@@ -544,7 +544,7 @@ __global__ void __launch_bounds__(512) k_grad1_r(LdSpectral<R> ls, int M, int gi
     }
   }
   const int f0 = gc * nd, fa = gc * nd + nd - 1;
-  reg_fft_e<L, E, 1, false, R>(v0, lds, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, R>(v0, lds, IdxCol{xl, TX}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -556,7 +556,7 @@ __global__ void __launch_bounds__(512) k_grad1_r(LdSpectral<R> ls, int M, int gi
     }
   }
   __syncthreads();
-  reg_fft_e<L, E, 1, false, R>(v1, lds, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, R>(v1, lds, IdxCol{xl, TX}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -602,7 +602,7 @@ __global__ void __launch_bounds__(512) k_grad2_r(cpx<R>* __restrict__ buf, int64
     const R k = kyt[i];
     v1[e] = mk<R>(-k * v0[e].y, k * v0[e].x);
   }
-  reg_fft_e<L, E, 1, false, R>(v0, lds, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, R>(v0, lds, IdxCol{xl, TX}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -611,7 +611,7 @@ __global__ void __launch_bounds__(512) k_grad2_r(cpx<R>* __restrict__ buf, int64
     }
   }
   __syncthreads();
-  reg_fft_e<L, E, 1, false, R>(v1, lds, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, R>(v1, lds, IdxCol{xl, TX}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -626,7 +626,7 @@ __global__ void __launch_bounds__(512) k_grad2_r(cpx<R>* __restrict__ buf, int64
     if (x < nx && ib.has(i)) v0[e] = f2[i * sy];
   }
   __syncthreads();
-  reg_fft_e<L, E, 1, false, R>(v0, lds, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, R>(v0, lds, IdxCol{xl, TX}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -691,7 +691,7 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
         v[e] = mk<R>(A.x - Bv.y, A.y + Bv.x);
       }
       __syncthreads();  // the previous transform / output loop is done with the LDS buffer
-      reg_fft_e<L, E, 1, false, R>(v, lds0, lds0, idx, t, tw);
+      reg_fft_m<L, E, 1, xfence_r<L>(), R>(v, lds0, idx, t, tw);
       const R* uj = ug + (int64_t)j * usf;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
@@ -703,7 +703,7 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
       }
     }
     __syncthreads();
-    reg_fft_e<L, E, -1, false, R>(acc, lds0, lds0, idx, t, tw);
+    reg_fft_m<L, E, -1, xfence_r<L>(), R>(acc, lds0, idx, t, tw);
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) lds0[idx(t + T * e)] = acc[e];
@@ -773,7 +773,7 @@ __global__ void __launch_bounds__(256) k_xfused_self(const cpx<R>* __restrict__ 
       cpx<R> v[E];
       load(grad + (int64_t)(j * nd) * gsf, false, v);
       __syncthreads();  // the previous transform is done with the LDS buffer
-      reg_fft_e<L, E, 1, false, R>(v, lds0, lds0, idx, t, tw);
+      reg_fft_m<L, E, 1, xfence_s<L>(), R>(v, lds0, idx, t, tw);
 #pragma unroll
       for (int e = 0; e < E; ++e) ur[j][e] = mk<R>(v[e].x * gscale, v[e].y * gscale);
     } else {
@@ -789,7 +789,7 @@ __global__ void __launch_bounds__(256) k_xfused_self(const cpx<R>* __restrict__ 
         cpx<R> v[E];
         load(grad + (int64_t)(c * nd + j) * gsf, j == 0, v);
         __syncthreads();
-        reg_fft_e<L, E, 1, false, R>(v, lds0, lds0, idx, t, tw);
+        reg_fft_m<L, E, 1, xfence_s<L>(), R>(v, lds0, idx, t, tw);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           const cpx<R> tt = mk<R>(ur[j][e].x * (v[e].x * gscale), ur[j][e].y * (v[e].y * gscale));
@@ -798,7 +798,7 @@ __global__ void __launch_bounds__(256) k_xfused_self(const cpx<R>* __restrict__ 
       }
     }
     __syncthreads();
-    reg_fft_e<L, E, -1, false, R>(acc, lds0, lds0, idx, t, tw);
+    reg_fft_m<L, E, -1, xfence_s<L>(), R>(acc, lds0, idx, t, tw);
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) lds0[idx(t + T * e)] = acc[e];

@@ -77,10 +77,9 @@ int64_t fit_plan_device_bytes(const fit_plan* plan);
  *     band-limited sub-grid is attached (1, default: use it).
  *   FIT_OPT_COL_TILE = 6: columns per workgroup of the strided-axis passes (power of two; 0 =
  *     automatic: 16 within 64 KB of LDS).
- *   FIT_OPT_XPASS_SELF = 7: 1 forms the advecting velocity of a self-advected field in registers
- *     inside the fused contiguous-axis pass; 0 stores it to device memory first; -1 (default)
- *     forms it in registers for x transforms of at most 2048 points (2.6 ms of 29 ms saved on a
- *     512³ 3/2-padded flux) and stores it for the 3072-point line, where the store is faster. */
+ *   FIT_OPT_XPASS_SELF = 7: 1 (default) forms the advecting velocity of a self-advected field in
+ *     registers inside the fused contiguous-axis pass; 0 stores it to device memory first (one
+ *     more pass: 2.6 ms more on the 29 ms flux of a 512³ field with 3/2 padding). */
 enum { FIT_OPT_PLAIN_PIPELINE = 1, FIT_OPT_GIVER_BATCH = 2, FIT_OPT_XPASS_GIVERS = 3,
        FIT_OPT_XPASS_BLOCK = 4, FIT_OPT_SUBGRID = 5, FIT_OPT_COL_TILE = 6, FIT_OPT_XPASS_SELF = 7 };
 int fit_plan_set_option(fit_plan* plan, int option, int value);

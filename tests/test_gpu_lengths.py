@@ -52,7 +52,7 @@ SELF_CASES = [((128, 192), "23"), ((192, 128), "23"), ((256, 384), "23"), ((384,
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-@pytest.mark.parametrize("xself", [1, 0, -1])
+@pytest.mark.parametrize("xself", [1, 0])
 @pytest.mark.parametrize("ns,dea", SELF_CASES)
 def test_self_advection_register_lengths(fit, oracle, ns, dea, xself, prec):
     from fit_amd import _lib
@@ -76,7 +76,7 @@ def test_self_advection_register_lengths(fit, oracle, ns, dea, xself, prec):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-@pytest.mark.parametrize("xself", [1, 0, -1])
+@pytest.mark.parametrize("xself", [1, 0])
 @pytest.mark.parametrize("dea", ["23", "pad"])
 def test_self_advection_three_components_on_a_plane(fit, oracle, dea, xself, prec):
     """A 2D grid carrying three velocity components (M = 3 > nd = 2): the first two advect all
