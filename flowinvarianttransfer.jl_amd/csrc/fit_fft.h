@@ -210,7 +210,19 @@ constexpr int xE(int L) { return rp_pow2(L) ? 8 : 12; }
 // points per thread of the self-advection fused x pass (three velocity lines stay in registers)
 constexpr int sE(int L) { return rp_pow2(L) ? 8 : 12; }
 
-template <int L, int E, int S, int NS, int SIGN, bool DB, typename R, class IDX>
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): this wave's LDS accesses are complete
+  __builtin_amdgcn_wave_barrier();      // no code motion across this point
+}
+// Exchange barrier: the workgroup's, or (WAVE) only this wave's when every line the wave
+// touches is held by lanes of that wave alone (T divides 64 and lines are lane-contiguous).
+template <bool WAVE>
+__device__ __forceinline__ void lds_sync() {
+  if constexpr (WAVE) wave_lds_sync();
+  else __syncthreads();
+}
+
+template <int L, int E, int S, int NS, int SIGN, bool DB, typename R, class IDX, bool WAVE = false>
 __device__ __forceinline__ void reg_stage(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* lds1, const IDX& idx, int t,
                                           const cpx<R>* __restrict__ tw) {
   if constexpr (S < rp_nst(L, E)) {
@@ -237,7 +249,7 @@ __device__ __forceinline__ void reg_stage(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* 
         for (int k = 0; k < r; ++k) v[i + k * NB] = w[k];
       } else {
         cpx<R>* lds = (DB && (S & 1)) ? lds1 : lds0;
-        if constexpr (!DB) { if (i == 0) __syncthreads(); }  // single buffer: previous readers done
+        if constexpr (!DB) { if (i == 0) lds_sync<WAVE>(); }  // single buffer: previous readers done
         const int o = (j - m) * r + m;
 #pragma unroll
         for (int k = 0; k < r; ++k) lds[idx(o + k * NS)] = w[k];
@@ -245,10 +257,10 @@ __device__ __forceinline__ void reg_stage(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* 
     }
     if constexpr (!LAST) {
       cpx<R>* lds = (DB && (S & 1)) ? lds1 : lds0;
-      __syncthreads();
+      lds_sync<WAVE>();
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = lds[idx(t + T * e)];
-      reg_stage<L, E, S + 1, NS * r, SIGN, DB, R, IDX>(v, lds0, lds1, idx, t, tw);
+      reg_stage<L, E, S + 1, NS * r, SIGN, DB, R, IDX, WAVE>(v, lds0, lds1, idx, t, tw);
     }
   }
 }
@@ -286,11 +298,16 @@ __device__ __forceinline__ void reg_fft_e(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* 
 // (about 100 VGPRs at E = 12: 171 instead of 90 in the two-line gradient pass at 768, two waves
 // per SIMD instead of five).  Recomputing them costs a few integer operations per stage, which
 // is a loss where the fence does not raise the occupancy.
-template <int L, int E, int SIGN, bool FENCE, typename R, class IDX>
+// WAVE: the exchanges synchronise the wave only (see lds_sync).
+template <int L, int E, int SIGN, bool FENCE, bool WAVE, typename R, class IDX>
 __device__ __forceinline__ void reg_fft_m(cpx<R> (&v)[E], cpx<R>* lds, const IDX& idx, int t, const cpx<R>* tw) {
   if constexpr (FENCE) asm volatile("" : "+v"(t));
-  reg_fft_e<L, E, SIGN, false, R, IDX>(v, lds, lds, idx, t, tw);
+  const void* tv = tw;
+  tw_opaque<L>(tv);
+  reg_stage<L, E, 0, 1, SIGN, false, R, IDX, WAVE>(v, lds, lds, idx, t, (const cpx<R>*)tv);
 }
+// Row-pair lines of T = L/E consecutive lanes are wave-private when T divides the wave size.
+template <int L, int E> constexpr bool wave_lines() { return (L / E) <= 64 && 64 % (L / E) == 0; }
 // Fused x passes, by measurement on MI355X (FP32 flux of a self-advected field, both forms):
 // the gradient-product pass runs fenced above 192 (3.6 vs 4.3 ms at 384, 27.0 vs 27.9 ms at
 // 768, 0.50 vs 0.54 ms at 2048, equal elsewhere); the self-advection pass, which holds three
@@ -326,10 +343,6 @@ constexpr int wE(int L) {
        : L == 1536 ? 24 : L == 2048 ? 32 : L == 3072 ? 48 : 0;
 }
 
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): this wave's LDS accesses are complete
-  __builtin_amdgcn_wave_barrier();      // no code motion across this point
-}
 
 template <int L, int E, int S, int NS, int SIGN, typename R, class IDX>
 __device__ __forceinline__ void wave_stage(cpx<R> (&v)[E], cpx<R>* lds, const IDX& idx, int t,

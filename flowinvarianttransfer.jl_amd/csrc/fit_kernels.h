@@ -544,7 +544,7 @@ __global__ void __launch_bounds__(512) k_grad1_r(LdSpectral<R> ls, int M, int gi
     }
   }
   const int f0 = gc * nd, fa = gc * nd + nd - 1;
-  reg_fft_m<L, E, 1, true, R>(v0, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, false, R>(v0, lds, IdxCol{xl, TX}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -556,7 +556,7 @@ __global__ void __launch_bounds__(512) k_grad1_r(LdSpectral<R> ls, int M, int gi
     }
   }
   __syncthreads();
-  reg_fft_m<L, E, 1, true, R>(v1, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, false, R>(v1, lds, IdxCol{xl, TX}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -602,7 +602,7 @@ __global__ void __launch_bounds__(512) k_grad2_r(cpx<R>* __restrict__ buf, int64
     const R k = kyt[i];
     v1[e] = mk<R>(-k * v0[e].y, k * v0[e].x);
   }
-  reg_fft_m<L, E, 1, true, R>(v0, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, false, R>(v0, lds, IdxCol{xl, TX}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -611,7 +611,7 @@ __global__ void __launch_bounds__(512) k_grad2_r(cpx<R>* __restrict__ buf, int64
     }
   }
   __syncthreads();
-  reg_fft_m<L, E, 1, true, R>(v1, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, false, R>(v1, lds, IdxCol{xl, TX}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -626,7 +626,7 @@ __global__ void __launch_bounds__(512) k_grad2_r(cpx<R>* __restrict__ buf, int64
     if (x < nx && ib.has(i)) v0[e] = f2[i * sy];
   }
   __syncthreads();
-  reg_fft_m<L, E, 1, true, R>(v0, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, false, R>(v0, lds, IdxCol{xl, TX}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -649,6 +649,7 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
                                                   const int4* __restrict__ fband, int kxb, int nfg,
                                                   const R* __restrict__ kxt, int64_t ugs, int GCM) {
   constexpr int E = xE(L), T = L / E, LL = ldsline(L);
+  constexpr bool WV = wave_lines<L, E>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* lds0 = reinterpret_cast<cpx<R>*>(smem_raw);
   const int line = threadIdx.x / T, t = threadIdx.x % T;
@@ -690,8 +691,8 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
         if (k > L / 2) { A.y = -A.y; Bv.y = -Bv.y; }
         v[e] = mk<R>(A.x - Bv.y, A.y + Bv.x);
       }
-      __syncthreads();  // the previous transform / output loop is done with the LDS buffer
-      reg_fft_m<L, E, 1, xfence_r<L>(), R>(v, lds0, idx, t, tw);
+      lds_sync<WV>();  // the previous transform / output loop is done with the LDS buffer
+      reg_fft_m<L, E, 1, xfence_r<L>(), WV, R>(v, lds0, idx, t, tw);
       const R* uj = ug + (int64_t)j * usf;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
@@ -702,12 +703,12 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
         acc[e] = j == 0 ? tt : cadd(acc[e], tt);
       }
     }
-    __syncthreads();
-    reg_fft_m<L, E, -1, xfence_r<L>(), R>(acc, lds0, idx, t, tw);
-    __syncthreads();
+    lds_sync<WV>();
+    reg_fft_m<L, E, -1, xfence_r<L>(), WV, R>(acc, lds0, idx, t, tw);
+    lds_sync<WV>();
 #pragma unroll
     for (int e = 0; e < E; ++e) lds0[idx(t + T * e)] = acc[e];
-    __syncthreads();
+    lds_sync<WV>();
     cpx<R>* of = out + (int64_t)fg * osf;
     for (int k = t; k <= kxb; k += T) {
       const cpx<R> Zk = lds0[idx(k)];
@@ -734,6 +735,7 @@ __global__ void __launch_bounds__(256) k_xfused_self(const cpx<R>* __restrict__ 
                                                      int nrows, int B, R gscale, int kxa, int kxb,
                                                      const R* __restrict__ kxt) {
   constexpr int T = L / E, LL = ldsline(L);
+  constexpr bool WV = wave_lines<L, E>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* lds0 = reinterpret_cast<cpx<R>*>(smem_raw);
   const int line = threadIdx.x / T, t = threadIdx.x % T;
@@ -772,8 +774,8 @@ __global__ void __launch_bounds__(256) k_xfused_self(const cpx<R>* __restrict__ 
     if (j < nd) {
       cpx<R> v[E];
       load(grad + (int64_t)(j * nd) * gsf, false, v);
-      __syncthreads();  // the previous transform is done with the LDS buffer
-      reg_fft_m<L, E, 1, xfence_s<L>(), R>(v, lds0, idx, t, tw);
+      lds_sync<WV>();  // the previous transform is done with the LDS buffer
+      reg_fft_m<L, E, 1, xfence_s<L>(), WV, R>(v, lds0, idx, t, tw);
 #pragma unroll
       for (int e = 0; e < E; ++e) ur[j][e] = mk<R>(v[e].x * gscale, v[e].y * gscale);
     } else {
@@ -788,8 +790,8 @@ __global__ void __launch_bounds__(256) k_xfused_self(const cpx<R>* __restrict__ 
       if (j < nd) {
         cpx<R> v[E];
         load(grad + (int64_t)(c * nd + j) * gsf, j == 0, v);
-        __syncthreads();
-        reg_fft_m<L, E, 1, xfence_s<L>(), R>(v, lds0, idx, t, tw);
+        lds_sync<WV>();
+        reg_fft_m<L, E, 1, xfence_s<L>(), WV, R>(v, lds0, idx, t, tw);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           const cpx<R> tt = mk<R>(ur[j][e].x * (v[e].x * gscale), ur[j][e].y * (v[e].y * gscale));
@@ -797,12 +799,12 @@ __global__ void __launch_bounds__(256) k_xfused_self(const cpx<R>* __restrict__ 
         }
       }
     }
-    __syncthreads();
-    reg_fft_m<L, E, -1, xfence_s<L>(), R>(acc, lds0, idx, t, tw);
-    __syncthreads();
+    lds_sync<WV>();
+    reg_fft_m<L, E, -1, xfence_s<L>(), WV, R>(acc, lds0, idx, t, tw);
+    lds_sync<WV>();
 #pragma unroll
     for (int e = 0; e < E; ++e) lds0[idx(t + T * e)] = acc[e];
-    __syncthreads();
+    lds_sync<WV>();
     cpx<R>* of = out + (int64_t)c * osf;
     for (int k = t; k <= kxb; k += T) {
       const cpx<R> Zk = lds0[idx(k)];
