@@ -12,6 +12,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fit_hip.h"
@@ -109,6 +110,7 @@ struct fit_plan {
   int xfused_block = 0;          // 1: use the workgroup-barrier x pass instead
   int col_tx = 0;                // FIT_OPT_COL_TILE (0: automatic)
   int xself = 1;                 // FIT_OPT_XPASS_SELF: self-advection x pass forms u in registers
+  int xstage = 1;                // FIT_OPT_XPASS_STAGE: FP32 self pass stages the next line by LDS DMA
   // shared-transform gradients: wavenumbers odd under k -> -k on every populated bin
   bool commute = false;
   void* kt[2] = {nullptr, nullptr};  // transform-grid wavenumbers of x and y (compute precision)
@@ -672,6 +674,23 @@ int v2_xfused_self(fit_plan* p, const int* bi, const int* bo) {
   const int64_t osy = pb, osz = (int64_t)pb * p->G[1], osf = osz * p->G[2];
   const double bytes = (double)nrows * (((double)p->nd + (double)nfg * p->nd) * xa * csz + (double)nfg * xb * csz);
   prof_begin(p, FIT_K_FUSED_X, bytes);
+  if constexpr (std::is_same<R, float>::value) {
+    // one line pair per wave (T = 64): the LDS-staged form, 4 pairs per workgroup
+    if (p->xstage && (L == 512 || L == 768)) {
+      const int nrg = (npairs + 3) / 4;
+#define FIT_STG(LLEN)                                                                                         \
+  hipLaunchKernelGGL((k_xfused_self_stg<LLEN, sE(LLEN)>), dim3((unsigned)nrg), dim3(256), 0, p->stream,        \
+                     (const cpx<float>*)p->tw[0], (const cpx<float>*)p->work2, gsy, gsz, gsf,                 \
+                     (cpx<float>*)p->nxbuf, osy, osz, osf, p->nd, p->M, p->G[1], nrows,                       \
+                     (float)(1.0 / (double)p->Gp), xa - 1, xb - 1, (const float*)p->kt[0])
+      if (L == 512) FIT_STG(512);
+      else FIT_STG(768);
+#undef FIT_STG
+      prof_end(p);
+      FIT_HIP(hipGetLastError());
+      return FIT_OK;
+    }
+  }
   switch (L) {
 #define FIT_CASE(LLEN)                                                                                          \
   case LLEN: {                                                                                                  \
@@ -1733,6 +1752,7 @@ int fit_plan_set_option(fit_plan* p, int option, int value) {
       p->col_tx = value;
       return FIT_OK;
     case FIT_OPT_XPASS_SELF: p->xself = value != 0; return FIT_OK;
+    case FIT_OPT_XPASS_STAGE: p->xstage = value != 0; return FIT_OK;
     default: return fail(FIT_E_ARGUMENT, "unknown plan option");
   }
 }

@@ -817,6 +817,149 @@ __global__ void __launch_bounds__(256) k_xfused_self(const cpx<R>* __restrict__ 
   }
 }
 
+// LDS DMA: 16 bytes per lane from global memory into LDS at lds_base + 16·lane.  Issued from asm:
+// the compiler cannot tell a staging buffer from the other LDS arrays, and for the builtin form
+// it waits for the transfer (vmcnt(0)) before every later LDS access, which serialises the
+// prefetch with the transforms it should overlap.  The caller waits (vmcnt) before reading.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ __forceinline__ void lds_dma16(const void* src, unsigned lds_base) {
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src), "s"(lds_base)
+               : "memory", "m0");
+}
+#pragma clang diagnostic pop
+typedef __attribute__((address_space(3))) void lds_void;
+
+// k_xfused_self for FP32 lines held by one full wave (T = 64: L = 512, 768): the next field's
+// two half rows are staged into LDS by DMA while the current line is transformed, so a wave
+// never waits on HBM between its twelve transforms.  Twiddles and k_t(x) are copied into LDS
+// once per workgroup (a global twiddle load would wait behind the staged rows, as vmcnt retires
+// in order).  4 row pairs (waves) per workgroup; exchanges synchronise the wave only.
+template <int L, int E>
+__global__ void __launch_bounds__(256) k_xfused_self_stg(const cpx<float>* __restrict__ tw,
+                                                         const cpx<float>* __restrict__ grad, int64_t gsy,
+                                                         int64_t gsz, int64_t gsf, cpx<float>* __restrict__ out,
+                                                         int64_t osy, int64_t osz, int64_t osf, int nd, int M,
+                                                         int G2, int nrows, float gscale, int kxa, int kxb,
+                                                         const float* __restrict__ kxt) {
+  using R = float;
+  constexpr int T = L / E, LL = ldsline(L), B = 256 / T;
+  constexpr int SROW = L / 2 + 2;  // staged half row: kx <= L/2 plus one pad point (16-byte units)
+  static_assert(T == 64, "one line pair per wave");
+  __shared__ __align__(16) cpx<R> xch[B * LL];
+  __shared__ __align__(16) cpx<R> stg[B * 2 * SROW];
+  __shared__ __align__(16) cpx<R> tws[L];
+  __shared__ R kxs[L / 2 + 1];
+  for (int i = threadIdx.x; i < L; i += blockDim.x) tws[i] = tw[i];
+  for (int i = threadIdx.x; i <= L / 2; i += blockDim.x) kxs[i] = kxt[i];
+  __syncthreads();
+  const int line = threadIdx.x / T, t = threadIdx.x % T;
+  const int pair = blockIdx.x * B + line;
+  const int r0 = 2 * pair, r1 = r0 + 1;
+  const bool ok0 = r0 < nrows, ok1 = r1 < nrows;
+  const int64_t ro0 = ok0 ? (int64_t)(r0 % G2) * gsy + (int64_t)(r0 / G2) * gsz : 0;
+  const int64_t ro1 = ok1 ? (int64_t)(r1 % G2) * gsy + (int64_t)(r1 / G2) * gsz : 0;
+  const int64_t oo0 = ok0 ? (int64_t)(r0 % G2) * osy + (int64_t)(r0 / G2) * osz : 0;
+  const int64_t oo1 = ok1 ? (int64_t)(r1 % G2) * osy + (int64_t)(r1 / G2) * osz : 0;
+  const IdxLine idx{line * LL};
+  cpx<R>* sl = stg + line * 2 * SROW;
+  const int nun = (kxa + 2) / 2;  // 16-byte units (two points) covering kx <= kxa
+  // stage rows r0, r1 of field gf (kx <= kxa; rows start 128-byte aligned, the pitch covers
+  // the odd point of the last unit)
+  auto issue = [&](const cpx<R>* gf) {
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr) {
+      const bool ok = rr ? ok1 : ok0;
+      const cpx<R>* src = gf + (rr ? ro1 : ro0);
+#pragma unroll
+      for (int i = 0; i < (SROW / 2 + 63) / 64; ++i) {
+        const int ui = i * 64 + t;
+        if (ok && ui < nun)
+          lds_dma16(src + 2 * ui,
+                    __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)(lds_void*)(sl + rr * SROW + i * 128)));
+      }
+    }
+  };
+  // the staged rows, extended to L points (times i k_t(x) for dx), packed row pair
+  auto consume = [&](bool dx, cpx<R> (&v)[E]) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the staged rows have landed
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = t + T * e;
+      const int kk = k <= L / 2 ? k : L - k;
+      cpx<R> A = mk<R>((R)0, (R)0), Bv = A;
+      if (kk <= kxa) {
+        if (ok0) A = sl[kk];
+        if (ok1) Bv = sl[SROW + kk];
+      }
+      if (dx) {
+        const R kx = kxs[kk];
+        A = mk<R>(-kx * A.y, kx * A.x);
+        Bv = mk<R>(-kx * Bv.y, kx * Bv.x);
+      }
+      if (k == 0 || 2 * k == L) { A.y = (R)0; Bv.y = (R)0; }
+      if (k > L / 2) { A.y = -A.y; Bv.y = -Bv.y; }
+      v[e] = mk<R>(A.x - Bv.y, A.y + Bv.x);
+    }
+    wave_lds_sync();  // staged reads done before the next DMA overwrites them
+  };
+  auto fft = [&](auto sg, cpx<R>(&v)[E]) {
+    int tt = t;
+    asm volatile("" : "+v"(tt));
+    reg_stage<L, E, 0, 1, decltype(sg)::value, false, R, IdxLine, true>(v, xch, xch, idx, tt, tws);
+  };
+  cpx<R> ur[3][E];
+  issue(grad);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    if (j < nd) {
+      cpx<R> v[E];
+      consume(false, v);
+      issue(grad + (int64_t)(j + 1 < nd ? (j + 1) * nd : 0) * gsf);
+      fft(std::integral_constant<int, 1>{}, v);
+#pragma unroll
+      for (int e = 0; e < E; ++e) ur[j][e] = mk<R>(v[e].x * gscale, v[e].y * gscale);
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) ur[j][e] = mk<R>((R)0, (R)0);
+    }
+  }
+  for (int c = 0; c < M; ++c) {
+    cpx<R> acc[E];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      if (j < nd) {
+        cpx<R> v[E];
+        consume(j == 0, v);
+        const int q = c * nd + j + 1;
+        if (q < M * nd) issue(grad + (int64_t)q * gsf);
+        fft(std::integral_constant<int, 1>{}, v);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const cpx<R> tt = mk<R>(ur[j][e].x * (v[e].x * gscale), ur[j][e].y * (v[e].y * gscale));
+          acc[e] = j == 0 ? tt : cadd(acc[e], tt);
+        }
+      }
+    }
+    wave_lds_sync();
+    fft(std::integral_constant<int, -1>{}, acc);
+    wave_lds_sync();
+#pragma unroll
+    for (int e = 0; e < E; ++e) xch[idx(t + T * e)] = acc[e];
+    wave_lds_sync();
+    cpx<R>* of = out + (int64_t)c * osf;
+    for (int k = t; k <= kxb; k += T) {
+      const cpx<R> Zk = xch[idx(k)];
+      const cpx<R> Zm = xch[idx((L - k) % L)];
+      const cpx<R> A = mk<R>((R)0.5 * (Zk.x + Zm.x), (R)0.5 * (Zk.y - Zm.y));
+      const cpx<R> Bv = mk<R>((R)0.5 * (Zk.y + Zm.y), (R)-0.5 * (Zk.x - Zm.x));
+      if (ok0) of[k + oo0] = A;
+      if (ok1) of[k + oo1] = Bv;
+    }
+  }
+}
+
 // Fused x pass, wave-per-line form: each wavefront owns 64/T row pairs; a workgroup (4 waves)
 // loads its u rows once and then runs GC givers × M components through
 //   N = Σ_j u_j · c2r(grad_{g,c,j}),  r2c(N) -> half rows kx <= kxb

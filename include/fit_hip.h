@@ -79,9 +79,13 @@ int64_t fit_plan_device_bytes(const fit_plan* plan);
  *     automatic: 16 within 64 KB of LDS).
  *   FIT_OPT_XPASS_SELF = 7: 1 (default) forms the advecting velocity of a self-advected field in
  *     registers inside the fused contiguous-axis pass; 0 stores it to device memory first (one
- *     more pass: 2.6 ms more on the 29 ms flux of a 512³ field with 3/2 padding). */
+ *     more pass: 2.6 ms more on the 29 ms flux of a 512³ field with 3/2 padding).
+ *   FIT_OPT_XPASS_STAGE = 8: 1 (default) lets the FP32 self-advection pass at 512- and 768-point
+ *     lines stage the next gradient line into LDS by DMA while the current one is transformed;
+ *     0 loads each line into registers just before its transform. */
 enum { FIT_OPT_PLAIN_PIPELINE = 1, FIT_OPT_GIVER_BATCH = 2, FIT_OPT_XPASS_GIVERS = 3,
-       FIT_OPT_XPASS_BLOCK = 4, FIT_OPT_SUBGRID = 5, FIT_OPT_COL_TILE = 6, FIT_OPT_XPASS_SELF = 7 };
+       FIT_OPT_XPASS_BLOCK = 4, FIT_OPT_SUBGRID = 5, FIT_OPT_COL_TILE = 6, FIT_OPT_XPASS_SELF = 7,
+       FIT_OPT_XPASS_STAGE = 8 };
 int fit_plan_set_option(fit_plan* plan, int option, int value);
 
 /* Band-limited sub-grid of the giver loop (no reference counterpart).  When every shell lies in

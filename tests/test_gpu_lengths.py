@@ -96,3 +96,31 @@ def test_self_advection_three_components_on_a_plane(fit, oracle, dea, xself, pre
     e = rel(got, ref)
     say("  err", e)
     assert e < (1e-11 if prec == "f64" else 2e-5)
+
+
+# LDS-staged FP32 self-advection pass (x lines of 512 and 768 points, one row pair per wave):
+# odd row counts leave the last pair half empty and row-pair counts that are not a multiple of 4
+# leave waves of the last workgroup without rows
+STAGE_CASES = [((512, 42), 2, "pad"), ((512, 128), 2, "23"), ((768, 10), 2, "23"), ((512, 42), 3, "pad"),
+               ((512, 20, 6), 3, "pad"), ((512, 24, 8), 3, "23"), ((768, 6, 5), 3, "23")]
+
+
+@pytest.mark.parametrize("stage", [1, 0])
+@pytest.mark.parametrize("ns,M,dea", STAGE_CASES)
+def test_staged_self_advection_pass(fit, oracle, ns, M, dea, stage):
+    from fit_amd import _lib
+    say("case staged self-advection", ns, M, dea, stage)
+    rng = np.random.default_rng(sum(ns) + M)
+    nd = len(ns)
+    ks = oracle.wavenumber_grid(ns, (2 * math.pi,) * nd)
+    u = rng.standard_normal(ns + (M,)) + 1j * rng.standard_normal(ns + (M,))
+    ref = oracle.compute_nonlinear_term(u, ks, dea)
+    deal = {"23": fit.OrszagTwoThirds(), "pad": fit.PaddedThreeHalves()}[dea]
+    with fit.Plan(ns, M, nd, ks, deal, np.float32) as plan:
+        plan.set_option(_lib.OPT_XPASS_STAGE, stage)
+        got = plan.nonlinear_term(u)
+        again = plan.nonlinear_term(u)
+    e = rel(got, ref)
+    say("  err", e)
+    assert e < 2e-5
+    assert np.array_equal(got, again)
