@@ -16,6 +16,7 @@ PTR_HOST, PTR_DEVICE = 0, 1
 OPT_PLAIN_PIPELINE, OPT_GIVER_BATCH, OPT_XPASS_GIVERS, OPT_XPASS_BLOCK, OPT_SUBGRID, OPT_COL_TILE = 1, 2, 3, 4, 5, 6
 OPT_XPASS_SELF = 7
 OPT_XPASS_STAGE = 8
+OPT_GRAD_SPLIT = 9
 ENS_OPT_GRAPH, ENS_OPT_BATCH = 1, 2
 K_CLASSES = ("fft_column", "fft_row_inverse", "fft_row_forward", "density_shell", "reduce", "other",
              "fused_x")

@@ -111,6 +111,8 @@ struct fit_plan {
   int col_tx = 0;                // FIT_OPT_COL_TILE (0: automatic)
   int xself = 1;                 // FIT_OPT_XPASS_SELF: self-advection x pass forms u in registers
   int xstage = 1;                // FIT_OPT_XPASS_STAGE: FP32 self pass stages the next line by LDS DMA
+  int grad_split = -1;           // FIT_OPT_GRAD_SPLIT: slot 2 of the y gradient pass as a column pass
+                                 // (-1: for y lines shorter than 512 points)
   // shared-transform gradients: wavenumbers odd under k -> -k on every populated bin
   bool commute = false;
   void* kt[2] = {nullptr, nullptr};  // transform-grid wavenumbers of x and y (compute precision)
@@ -514,14 +516,16 @@ int grads_shared(fit_plan* p, const void* f, int giver0, int ng, const int* bi, 
     const Band ob = full_band(p->G[2]);
     const Band ib = Band{bi[1], L};
     const Band wb = full_band(L);
+    const bool split = p->grad_split < 0 ? L < 512 : p->grad_split != 0;
     dim3 grid((xa + TX - 1) / TX, ob.count(), ngc);
-    prof_begin(p, FIT_K_FFT_COLUMN, (double)ngc * xa * ob.count() * (2.0 * ib.count() + 3.0 * wb.count()) * csz);
+    prof_begin(p, FIT_K_FFT_COLUMN,
+               (double)ngc * xa * ob.count() * ((split ? 1.0 : 2.0) * ib.count() + (split ? 2.0 : 3.0) * wb.count()) * csz);
     switch (L) {
 #define FIT_CASE(LL)                                                                                         \
   case LL:                                                                                                   \
     hipLaunchKernelGGL((k_grad2_r<R, LL>), grid, dim3(two_T(LL) * TX), (size_t)L * TX * csz, p->stream,      \
                        st.p, st.sy, st.sz, st.sf, (const R*)p->kt[1], (const cpx<R>*)p->tw[1], xa, TX, ob,   \
-                       ib, wb, fband, p->M);                                                                 \
+                       ib, wb, fband, p->M, split ? 0 : 1);                                                  \
     break;
       FIT_REG_LENGTHS(FIT_CASE)
 #undef FIT_CASE
@@ -529,6 +533,14 @@ int grads_shared(fit_plan* p, const void* f, int giver0, int ng, const int* bi, 
     }
     prof_end(p);
     FIT_HIP(hipGetLastError());
+    if (split) {
+      // slot 2 (the last-axis derivative) needs only its own y transform: the one-line column
+      // pass, which holds twice the waves of the two-line kernel
+      LdHalf<R> l2{st.p + 2 * st.sf, st.sy, st.sz, 3 * st.sf};
+      StHalf<R> s2{st.p + 2 * st.sf, st.sy, st.sz, 3 * st.sf};
+      int rc = launch_col<R, 1>(p, l2, s2, 1, ngc, xa, ob, ib, wb, fband, p->M);
+      if (rc) return rc;
+    }
   }
   return FIT_OK;
 }
@@ -1753,6 +1765,7 @@ int fit_plan_set_option(fit_plan* p, int option, int value) {
       return FIT_OK;
     case FIT_OPT_XPASS_SELF: p->xself = value != 0; return FIT_OK;
     case FIT_OPT_XPASS_STAGE: p->xstage = value != 0; return FIT_OK;
+    case FIT_OPT_GRAD_SPLIT: p->grad_split = value < 0 ? -1 : (value != 0); return FIT_OK;
     default: return fail(FIT_E_ARGUMENT, "unknown plan option");
   }
 }

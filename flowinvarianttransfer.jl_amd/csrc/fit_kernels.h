@@ -574,7 +574,7 @@ template <typename R, int L>
 __global__ void __launch_bounds__(512) k_grad2_r(cpx<R>* __restrict__ buf, int64_t sy, int64_t sz, int64_t sf,
                                                   const R* __restrict__ kyt, const cpx<R>* __restrict__ tw,
                                                   int nx, int TX, Band ob, Band ib, Band wb,
-                                                  const int4* __restrict__ fband, int M) {
+                                                  const int4* __restrict__ fband, int M, int slot2) {
   constexpr int E = gE(L), T = L / E;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
@@ -619,6 +619,7 @@ __global__ void __launch_bounds__(512) k_grad2_r(cpx<R>* __restrict__ buf, int64
       if (wb.has(i)) f1[i * sy] = v1[e];
     }
   }
+  if (!slot2) return;  // slot 2 runs as a plain column pass
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = t + T * e;

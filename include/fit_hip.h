@@ -82,10 +82,14 @@ int64_t fit_plan_device_bytes(const fit_plan* plan);
  *     more pass: 2.6 ms more on the 29 ms flux of a 512³ field with 3/2 padding).
  *   FIT_OPT_XPASS_STAGE = 8: 1 (default) lets the FP32 self-advection pass at 512- and 768-point
  *     lines stage the next gradient line into LDS by DMA while the current one is transformed;
- *     0 loads each line into registers just before its transform. */
+ *     0 loads each line into registers just before its transform.
+ *   FIT_OPT_GRAD_SPLIT = 9: 1 runs the y transform of the last-axis derivative (3D shared-
+ *     transform gradients) as its own one-line column pass; 0 keeps it in the two-line gradient
+ *     kernel after the other two; -1 (default) splits for y lines shorter than 512 points
+ *     (measured: 0.17 ms faster at 192, 0.11 ms slower at 768 on the 512³ snapshot). */
 enum { FIT_OPT_PLAIN_PIPELINE = 1, FIT_OPT_GIVER_BATCH = 2, FIT_OPT_XPASS_GIVERS = 3,
        FIT_OPT_XPASS_BLOCK = 4, FIT_OPT_SUBGRID = 5, FIT_OPT_COL_TILE = 6, FIT_OPT_XPASS_SELF = 7,
-       FIT_OPT_XPASS_STAGE = 8 };
+       FIT_OPT_XPASS_STAGE = 8, FIT_OPT_GRAD_SPLIT = 9 };
 int fit_plan_set_option(fit_plan* plan, int option, int value);
 
 /* Band-limited sub-grid of the giver loop (no reference counterpart).  When every shell lies in

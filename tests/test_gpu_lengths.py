@@ -124,3 +124,25 @@ def test_staged_self_advection_pass(fit, oracle, ns, M, dea, stage):
     say("  err", e)
     assert e < 2e-5
     assert np.array_equal(got, again)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("split", [1, 0])
+@pytest.mark.parametrize("ns,dea", [((128, 96, 64), "pad"), ((96, 128, 64), "23"), ((64, 64, 128), "23")])
+def test_gradient_pass_split(fit, oracle, ns, dea, split, prec):
+    """The y transform of the last-axis derivative as its own column pass (split = 1) or inside
+    the two-line gradient kernel (split = 0): both against the oracle."""
+    from fit_amd import _lib
+    say("case gradient split", ns, dea, split, prec)
+    rng = np.random.default_rng(sum(ns) + split)
+    ks = oracle.wavenumber_grid(ns, (2 * math.pi,) * 3)
+    u = rng.standard_normal(ns + (3,)) + 1j * rng.standard_normal(ns + (3,))
+    ref = oracle.compute_nonlinear_term(u, ks, dea)
+    deal = {"23": fit.OrszagTwoThirds(), "pad": fit.PaddedThreeHalves()}[dea]
+    dt = np.float64 if prec == "f64" else np.float32
+    with fit.Plan(ns, 3, 3, ks, deal, dt) as plan:
+        plan.set_option(_lib.OPT_GRAD_SPLIT, split)
+        got = plan.nonlinear_term(u)
+    e = rel(got, ref)
+    say("  err", e)
+    assert e < (1e-11 if prec == "f64" else 2e-5)
