@@ -465,11 +465,11 @@ bool commute_ok(const fit_plan* p) {
   return true;
 }
 
-// Columns per workgroup of the two-line gradient passes: 16 within 512 threads and 64 KB of
-// LDS, or the plan's column tile within 1024 threads and 160 KB.
+// Columns per workgroup of the two-line gradient passes (512 threads at most): 16 within 64 KB
+// of LDS, or the plan's column tile within 160 KB.
 int grad_tile(const fit_plan* p, int L, size_t csz) {
   const int T = two_T(L);
-  int TX = p->col_tx > 0 ? std::min(p->col_tx, 1024 / T) : std::min(16, 512 / T);
+  int TX = std::min(p->col_tx > 0 ? p->col_tx : 16, 512 / T);
   const size_t cap = p->col_tx > 0 ? (size_t)160 * 1024 : g_max_lds;
   while (TX > 1 && (size_t)L * TX * csz > cap) TX /= 2;
   return TX;
