@@ -1300,20 +1300,24 @@ __global__ void __launch_bounds__(256) k_density(Grid g, int4 invs, int ninv, in
   const int nb = ninv * nsh;
   for (int s = threadIdx.x; s < nb; s += blockDim.x) bins[s] = 0.0;
   __syncthreads();
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < g.Np; o += stride) {
-    const int ix = (int)(o % g.n1);
-    const int iy = (int)((o / g.n1) % g.n2);
-    const int iz = (int)(o / ((int64_t)g.n1 * g.n2));
-    const R kx = ks0[ix];
+  // one x row per iteration: the row's (y, z) and wavenumbers are found once per row instead of
+  // by 64-bit division per mode
+  const int nrow = g.n2 * g.n3;
+  for (int row = blockIdx.x; row < nrow; row += gridDim.x) {
+    const int iy = row % g.n2, iz = row / g.n2;
     const R ky = g.nd >= 2 ? ks1[iy] : (R)0;
     const R kz = g.nd >= 3 ? ks2[iz] : (R)0;
-    const int s = shell ? shell[o] : 0;
-    for (int q = 0; q < ninv; ++q) {
-      const int inv = q == 0 ? invs.x : (q == 1 ? invs.y : (q == 2 ? invs.z : invs.w));
-      const R t = density_at<R>(inv, g.nd, M, car, csf, nh, nsf, o, kx, ky, kz);
-      if (q == 0 && t_out) t_out[o] = t;
-      if (s > 0) atomicAdd(&bins[q * nsh + s - 1], (double)t);
+    const int64_t base = (int64_t)row * g.n1;
+    for (int ix = threadIdx.x; ix < g.n1; ix += blockDim.x) {
+      const int64_t o = base + ix;
+      const R kx = ks0[ix];
+      const int s = shell ? shell[o] : 0;
+      for (int q = 0; q < ninv; ++q) {
+        const int inv = q == 0 ? invs.x : (q == 1 ? invs.y : (q == 2 ? invs.z : invs.w));
+        const R t = density_at<R>(inv, g.nd, M, car, csf, nh, nsf, o, kx, ky, kz);
+        if (q == 0 && t_out) t_out[o] = t;
+        if (s > 0) atomicAdd(&bins[q * nsh + s - 1], (double)t);
+      }
     }
   }
   if (shell) {
