@@ -231,6 +231,13 @@ bool reg_supported(int L) {
 }
 size_t g_max_lds = 65536;  // per-workgroup LDS the launches may use
 
+// Fixed-order reduction of ng groups of nblk partial rows of nsh bins (k_reduce_rows).
+void launch_reduce(fit_plan* p, const double* partials, int nblk, int nsh, int ng, double* out) {
+  if (nsh <= 0 || ng <= 0) return;
+  hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)((nsh + kRedBins - 1) / kRedBins), (unsigned)ng),
+                     dim3(kRedBins * kRedRows), 0, p->stream, partials, nblk, nsh, out);
+}
+
 // Column pass over fields [0, nfields): x < nx, other-index band ob, input band ib, output band wb.
 template <typename R, int SIGN, class LD, class ST>
 int launch_col(fit_plan* p, LD ld, ST st, int axis, int nfields, int nx, Band ob, Band ib, Band wb,
@@ -778,8 +785,7 @@ int density_launch(fit_plan* p, int inv, const void* car, const void* nh, void* 
   FIT_HIP(hipGetLastError());
   if (shells) {
     prof_begin(p, FIT_K_REDUCE, (double)nblk * nsh * 8.0 + nsh * 8.0);
-    hipLaunchKernelGGL(k_reduce_partials, dim3((nsh + 127) / 128), dim3(128), 0, p->stream, p->partials,
-                       nblk, nsh, p->dcol);
+    launch_reduce(p, p->partials, nblk, nsh, 1, p->dcol);
     prof_end(p);
     FIT_HIP(hipGetLastError());
   }
@@ -805,8 +811,7 @@ int density_multi(fit_plan* p, int ninv, const int* invs, const void* car, const
   prof_end(p);
   FIT_HIP(hipGetLastError());
   prof_begin(p, FIT_K_REDUCE, (double)nblk * nb * 8.0 + nb * 8.0);
-  hipLaunchKernelGGL(k_reduce_partials, dim3((nb + 127) / 128), dim3(128), 0, p->stream, p->partials2, nblk, nb,
-                     p->dcolG);
+  launch_reduce(p, p->partials2, nblk, nb, 1, p->dcolG);
   prof_end(p);
   FIT_HIP(hipGetLastError());
   return FIT_OK;
@@ -1039,8 +1044,7 @@ int batched_lastdens(fit_plan* p, int gb, const void* eh, int64_t ehgs, double* 
   prof_end(p);
   FIT_HIP(hipGetLastError());
   prof_begin(p, FIT_K_REDUCE, (double)gb * nblk * nsh * 8.0);
-  hipLaunchKernelGGL(k_reduce_batched, dim3((nsh + 127) / 128, gb), dim3(128), 0, p->stream, p->partials2, nblk,
-                     nsh, gb, out);
+  launch_reduce(p, p->partials2, nblk, nsh, gb, out);
   prof_end(p);
   FIT_HIP(hipGetLastError());
   return FIT_OK;
@@ -1199,8 +1203,7 @@ int b2b_impl(fit_plan* p, int inv, const void* f, const void* a, const R* kc, co
     prof_end(p);
     FIT_HIP(hipGetLastError());
     prof_begin(p, FIT_K_REDUCE, (double)nblk * nb * 8.0);
-    hipLaunchKernelGGL(k_reduce_partials, dim3((nb + 127) / 128), dim3(128), 0, p->stream, p->partials2, nblk, nb,
-                       p->dcolG);
+    launch_reduce(p, p->partials2, nblk, nb, 1, p->dcolG);
     prof_end(p);
     FIT_HIP(hipGetLastError());
     FIT_HIP(hipMemcpyAsync(Tm.data() + (size_t)m * nb, p->dcolG, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost,
@@ -1245,8 +1248,7 @@ int m2m_impl(fit_plan* p, int inv, const void* f, const void* a, void* S, void* 
                        (const cpx<R>*)A, (const cpx<R>*)E, (const R*)p->ks[0], (const R*)p->ks[1], (const R*)p->ks[2],
                        pchunk, (float*)Sd, (double*)netpart);
   prof_end(p);
-  hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((Np + 127) / 128)), dim3(128), 0, p->stream,
-                     (const double*)netpart, nchunk, (int)Np, (double*)netd);
+  launch_reduce(p, (const double*)netpart, nchunk, (int)Np, 1, (double*)netd);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) { cleanup(); return fail(FIT_E_DEVICE, hipGetErrorString(e)); }
   // net in io precision

@@ -1231,15 +1231,28 @@ __global__ void k_carrier_half(Grid g, int inv, int M, int trunc, const cpx<R>* 
 }
 
 // out[g*nsh + s] = Σ_b partials[(g*nblk + b)*nsh + s] in increasing b (deterministic FP64)
-__global__ void k_reduce_batched(const double* __restrict__ partials, int nblk, int nsh, int ng,
-                                 double* __restrict__ out) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = blockIdx.y;
-  if (s >= nsh || g >= ng) return;
+// Fixed-order sum of the nblk rows of partials (row-major nblk × nsh; group g = blockIdx.y at
+// partials + g·nblk·nsh) into out[g·nsh + s].  A workgroup covers 64 bins × 16 row groups: group y
+// adds rows y, y + 16, ... in order, then the 16 group sums are added in order.  The association
+// is fixed by the indices alone, so the result does not depend on scheduling; 16 row groups keep
+// 16 times more loads in flight than one thread per bin.
+constexpr int kRedBins = 64, kRedRows = 16;
+__global__ void __launch_bounds__(kRedBins * kRedRows) k_reduce_rows(const double* __restrict__ partials, int nblk,
+                                                                     int nsh, double* __restrict__ out) {
+  __shared__ double part[kRedRows][kRedBins];
+  const int tx = threadIdx.x % kRedBins, y = threadIdx.x / kRedBins;
+  const int s = blockIdx.x * kRedBins + tx;
+  const double* p = partials + (int64_t)blockIdx.y * nblk * nsh;
   double acc = 0.0;
-  const double* p = partials + (int64_t)g * nblk * nsh;
-  for (int b = 0; b < nblk; ++b) acc += p[(int64_t)b * nsh + s];
-  out[(int64_t)g * nsh + s] = acc;
+  if (s < nsh)
+    for (int b = y; b < nblk; b += kRedRows) acc += p[(int64_t)b * nsh + s];
+  part[y][tx] = acc;
+  __syncthreads();
+  if (y == 0 && s < nsh) {
+    double t = 0.0;
+    for (int k = 0; k < kRedRows; ++k) t += part[k][tx];
+    out[(int64_t)blockIdx.y * nsh + s] = t;
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1392,13 +1405,6 @@ __global__ void __launch_bounds__(256) k_band_density(Grid g, int inv, int M, co
 }
 
 // out[s] = Σ_b partials[b][s], b in increasing order (deterministic FP64 reduction)
-__global__ void k_reduce_partials(const double* __restrict__ partials, int nblk, int nsh, double* __restrict__ out) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsh) return;
-  double acc = 0.0;
-  for (int b = 0; b < nblk; ++b) acc += partials[(int64_t)b * nsh + s];
-  out[s] = acc;
-}
 
 // ---------------------------------------------------------------------------------------------
 // Precision conversion of API arrays
