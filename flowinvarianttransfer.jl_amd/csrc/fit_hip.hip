@@ -253,7 +253,7 @@ int launch_col(fit_plan* p, LD ld, ST st, int axis, int nfields, int nx, Band ob
     const int T = reg_T(L);
     // columns per workgroup: 16 (128-byte rows in FP32) within 64 KB of LDS, or the plan's cap
     // within the 160 KB a workgroup may hold
-    int TX = std::min(p->col_tx > 0 ? p->col_tx : 16, 1024 / T);
+    int TX = std::min(p->col_tx > 0 ? p->col_tx : 16, col_threads<R>() / T);
     const size_t lds_cap = p->col_tx > 0 ? (size_t)160 * 1024 : g_max_lds;
     while (TX > 1 && (size_t)L * TX * csz > lds_cap) TX /= 2;
     dim3 grid((nx + TX - 1) / TX, nob, nfields);

@@ -439,8 +439,11 @@ __global__ void __launch_bounds__(256) k_xfused(FftDesc d, const cpx<R>* __restr
 // ---------------------------------------------------------------------------------------------
 
 // Column pass: thread (t, x) with x fastest; TX columns per workgroup; single LDS buffer.
+// FP64 workgroups stay within 256 threads, so a lane may hold all 24 complex doubles of its line
+// and the FFT temporaries (a 1024-thread bound caps a lane at 128 VGPRs and spills ~100).
+template <typename R> constexpr int col_threads() { return sizeof(R) == 8 ? 256 : 1024; }
 template <typename R, int L, int SIGN, class LD, class ST>
-__global__ void __launch_bounds__(1024) k_col_r(LD ld, ST st, const cpx<R>* __restrict__ tw, int nx, int TX,
+__global__ void __launch_bounds__(col_threads<R>()) k_col_r(LD ld, ST st, const cpx<R>* __restrict__ tw, int nx, int TX,
                                                 int axis, Band ob, Band ib, Band wb,
                                                 const int4* __restrict__ fband, int fdiv, int Gother) {
   constexpr int E = rp_E(L), T = L / E;
