@@ -626,13 +626,19 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
     const int ngiv = nfg / p->M;
     const int GC = ugs ? 1 : std::max(1, std::min(ngiv, p->xfused_gc > 0 ? p->xfused_gc : 4));
     const int nchunk = (ngiv + GC - 1) / GC;
+    // only lengths with at most 16 points per lane are instantiated (the others never qualify)
     switch (L) {
-#define FIT_CASE(LLEN)                                                                                          \
-  case LLEN:                                                                                                    \
-    hipLaunchKernelGGL((k_xfused_w<R, LLEN>), dim3((unsigned)((int64_t)nrg * nchunk)), dim3(256), lds, p->stream, \
-                       (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf, (const R*)u, usy,          \
-                       usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M, p->G[1], nrows,              \
-                       (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, ngiv, GC, kxt, ugs);                    \
+#define FIT_CASE(LLEN)                                                                                            \
+  case LLEN:                                                                                                      \
+    if constexpr (wE(LLEN) <= 16) {                                                                               \
+      hipLaunchKernelGGL((k_xfused_w<R, LLEN>), dim3((unsigned)((int64_t)nrg * nchunk)), dim3(256), lds,          \
+                         p->stream, (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf, (const R*)u, \
+                         usy, usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M, p->G[1], nrows,          \
+                         (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, ngiv, GC, kxt, ugs);                     \
+    } else {                                                                                                      \
+      prof_end(p);                                                                                                \
+      return fail(FIT_E_UNSUPPORTED, "wave-form x pass needs at most 16 points per lane");                        \
+    }                                                                                                             \
     break;
       FIT_REG_LENGTHS(FIT_CASE)
 #undef FIT_CASE
