@@ -1,24 +1,29 @@
 """Build libfit_hip.so (hand-written HIP for gfx950) in-tree with hipcc.
 
+The library is three translation units compiled in parallel and linked into one shared object:
+fit_hip.hip (plan management, staging, C ABI), fit_f32.hip and fit_f64.hip (the pipeline and its
+kernels per compute precision).
+
 Usage:  python flowinvarianttransfer.jl_amd/build_lib.py   (or via __graft_entry__.build())
 """
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
+OBJDIR = os.path.join(LIBDIR, "obj")
 LIB = os.path.join(LIBDIR, "libfit_hip.so")
-SOURCES = ["fit_hip.hip"]
-HEADERS = ["fit_fft.h", "fit_kernels.h"]
+SOURCES = ["fit_f32.hip", "fit_f64.hip", "fit_hip.hip"]
+HEADERS = ["fit_fft.h", "fit_kernels.h", "fit_internal.h", "fit_pipeline.h"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-Wall",
     "-Wno-unused-function",
     "-Wno-unused-variable",
@@ -29,22 +34,42 @@ HIPCC_FLAGS = [
 ]
 
 
+def _quiet(stderr):
+    # the host pass does not know the packed-fp32 feature: harmless, filtered
+    return [l for l in stderr.splitlines() if "packed-fp32-ops" not in l and l.strip()]
+
+
 def build(verbose=True):
-    os.makedirs(LIBDIR, exist_ok=True)
+    os.makedirs(OBJDIR, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+    def compile_one(src):
+        obj = os.path.join(OBJDIR, src.replace(".hip", ".o"))
+        cmd = [hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        return src, obj, res
+
+    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+        results = list(ex.map(compile_one, SOURCES))
+    failed = False
+    for src, _, res in results:
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            failed = True
+        elif verbose and _quiet(res.stderr):
+            sys.stderr.write("\n".join(_quiet(res.stderr)[:40]) + "\n")
+    if failed:
+        raise RuntimeError("hipcc failed building libfit_hip.so")
     tmp = LIB + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp]
+    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *[o for _, o, _ in results], "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libfit_hip.so")
-    if verbose and res.stderr.strip():
-        # warnings (the host pass does not know the packed-fp32 feature: harmless)
-        lines = [l for l in res.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip()]
-        if lines:
-            sys.stderr.write("\n".join(lines[:40]) + "\n")
+        raise RuntimeError("linking libfit_hip.so failed")
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,26 +1,14 @@
-// fit_hip.hip -- plan management, pipeline orchestration and the C ABI of libfit_hip.so.
+// fit_hip.hip -- plan management, staging, profiling and the C ABI of libfit_hip.so.
 //
 // One plan = one grid on one GPU: device-resident twiddles, wavenumbers, shell map and
 // workspaces, one HIP stream.  Every call runs the whole hot path on the device and returns
-// FP64 shell sums (or device/host arrays) to the caller.
-#include <hip/hip_runtime.h>
+// FP64 shell sums (or device/host arrays) to the caller.  The pipeline itself (pass launchers
+// and their kernels) is compiled per precision in fit_f32.hip / fit_f64.hip.
+#include "fit_internal.h"
 
-#include <algorithm>
-#include <cmath>
-#include <cstdio>
-#include <cstdint>
-#include <cstring>
-#include <mutex>
-#include <string>
-#include <type_traits>
-#include <vector>
-
-#include "../../include/fit_hip.h"
-#include "fit_kernels.h"
-
-using namespace fit;
-
-namespace {
+namespace fitx {
+FIT_PIPELINE_EXTERN(float)
+FIT_PIPELINE_EXTERN(double)
 
 thread_local std::string g_last_error;
 std::mutex g_create_mutex;
@@ -29,15 +17,6 @@ int fail(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
 }
-
-#define FIT_HIP(expr)                                                                          \
-  do {                                                                                         \
-    hipError_t e_ = (expr);                                                                    \
-    if (e_ != hipSuccess) {                                                                    \
-      return fail(e_ == hipErrorOutOfMemory ? FIT_E_OUT_OF_MEMORY : FIT_E_DEVICE,              \
-                  std::string(#expr) + ": " + hipGetErrorString(e_));                          \
-    }                                                                                          \
-  } while (0)
 
 // radix plan: as many 8s as possible, then 4 / 2, then 3s and 5s (any order is valid Stockham)
 bool make_desc(int L, FftDesc& d) {
@@ -57,86 +36,6 @@ int padded_len(int n) {  // FFTWExt.jl:128-131
   return ((m - n) % 2 == 0) ? m : m + 1;
 }
 
-struct ProfRec {
-  hipEvent_t a = nullptr, b = nullptr;
-  int cls = 0;
-  double bytes = 0;
-};
-
-}  // namespace
-
-struct fit_plan {
-  int nd = 0, M = 0, D = 0, dtype = 0, dealias = 0, device = 0;
-  int n[3] = {1, 1, 1}, G[3] = {1, 1, 1};
-  int h1 = 1;
-  int64_t Np = 1, Gp = 1, Hp = 1;
-  int nsh = 0;
-  bool shell_symmetric = true;
-  size_t rsz = 4;  // sizeof(R)
-  hipStream_t stream = nullptr;
-  FftDesc fd[3];
-  void* tw[3] = {nullptr, nullptr, nullptr};
-  void* ks[3] = {nullptr, nullptr, nullptr};
-  int* shell = nullptr;
-  void* work = nullptr;    // one half spectrum on the transform grid
-  void* uphys = nullptr;   // nd physical advecting components
-  void* nphys = nullptr;   // one physical product
-  void* nhat = nullptr;    // M full original-grid N̂
-  void* nhat_dst = nullptr;  // caller's device N̂ array when written in place of nhat
-  void* in_f = nullptr;    // staged primary field (M comps)
-  void* in_a = nullptr;    // staged advecting field (D comps)
-  void* in_c = nullptr;    // staged carrier / extra input (M comps)
-  void* out_tmp = nullptr; // staged output
-  size_t out_tmp_bytes = 0;
-  double* partials = nullptr;
-  int nblk_density = 0;
-  double* dcol = nullptr;  // nsh doubles
-  int64_t bytes = 0;
-  // fused / pruned pipeline state
-  int plain = 0;                 // 1: unfused, unpruned, one giver per launch (reference dataflow)
-  int band_in[3] = {0, 0, 0};    // folded support of spectral inputs (keep / embedded block)
-  int band_rx[3] = {0, 0, 0};    // folded support of the receiving shells (∩ output mask)
-  std::vector<int4> shell_band;  // per shell (index s-1): folded support ∩ band_in (-1: empty)
-  int4* d_shell_band = nullptr;
-  int* shh = nullptr;            // shell map on the original half grid (x <= n1/2)
-  void* eh = nullptr;            // Hermitian part of the masked carrier, M comps, half grid
-  void* work2 = nullptr; size_t work2_bytes = 0;
-  void* nxbuf = nullptr; size_t nxbuf_bytes = 0;
-  double* partials2 = nullptr; size_t partials2_bytes = 0;
-  double* dcolG = nullptr; size_t dcolG_bytes = 0;
-  size_t eh_bytes = 0;
-  int giver_batch = 0;           // cap on givers per batch (0 = memory budget only)
-  int xfused_gc = 0;             // givers per workgroup in the wave-per-line x pass (0 = 4)
-  int xfused_block = 0;          // 1: use the workgroup-barrier x pass instead
-  int col_tx = 0;                // FIT_OPT_COL_TILE (0: automatic)
-  int xself = 1;                 // FIT_OPT_XPASS_SELF: self-advection x pass forms u in registers
-  int xstage = 1;                // FIT_OPT_XPASS_STAGE: FP32 self pass stages the next line by LDS DMA
-  int grad_split = -1;           // FIT_OPT_GRAD_SPLIT: slot 2 of the y gradient pass as a column pass
-                                 // (-1: for y lines shorter than 512 points)
-  // shared-transform gradients: wavenumbers odd under k -> -k on every populated bin
-  bool commute = false;
-  void* kt[2] = {nullptr, nullptr};  // transform-grid wavenumbers of x and y (compute precision)
-  bool grads_commuted = false;       // the last gradient passes produced slot 0 = h
-  // band-limited sub-grid for the giver loop: when every shell lies in |k_d| <= R_d with
-  // 4 R_d < n_d, T(n,m) only involves advecting modes |q_d| <= 2 R_d and the exact convolution
-  // is evaluated alias-free on a grid of at least 4 R_d + 1 points per axis
-  fit_plan* sub = nullptr;       // child plan on that grid (no dealiasing, same shells)
-  int sub_q[3] = {0, 0, 0};      // folded box of input modes copied into the child
-  double sub_tscale = 1.0;       // (Np_sub / Np)^2: N̂ carries 1/Np^2 of the grid it is taken on
-  void* sub_f = nullptr; size_t sub_f_bytes = 0;
-  void* sub_a = nullptr; size_t sub_a_bytes = 0;
-  bool own_stream = true;
-  int use_sub = 1;                // FIT_OPT_SUBGRID
-  // profiling
-  bool prof = false;
-  std::vector<ProfRec> recs;
-  size_t nrec = 0;
-  int64_t launches[FIT_K_COUNT] = {0};
-  double ms[FIT_K_COUNT] = {0};
-  double abytes[FIT_K_COUNT] = {0};
-};
-
-namespace {
 
 int dalloc(fit_plan* p, void** ptr, size_t bytes) {
   if (bytes == 0) bytes = 256;
@@ -192,44 +91,8 @@ void prof_collect(fit_plan* p) {
   }
 }
 
-template <typename R> Grid grid_of(const fit_plan* p) {
-  Grid g;
-  g.nd = p->nd;
-  g.n1 = p->n[0]; g.n2 = p->n[1]; g.n3 = p->n[2];
-  g.G1 = p->G[0]; g.G2 = p->G[1]; g.G3 = p->G[2];
-  g.h1 = p->h1;
-  g.Np = p->Np;
-  return g;
-}
 
-// largest power-of-two tile with 2*L*T*sizeof(cpx) <= 64 KiB (ping-pong LDS), capped at cap
-int tile_for(int L, size_t csz, int cap) {
-  int t = cap;
-  while (t > 1 && (size_t)2 * L * t * csz > 65536) t /= 2;
-  return t;
-}
-
-constexpr int kThreads = 256;
-
-// ------------------------------ pass launchers ------------------------------------------------
-
-Band full_band(int G) { return Band{G, G}; }
-
-// lengths with a compile-time register FFT (others use the generic LDS engine)
-#define FIT_REG_LENGTHS(X) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048) X(3072)
-constexpr int reg_T(int L) { return L / (((L & (L - 1)) == 0) ? 8 : 24); }
-constexpr int two_T(int L) { return L / gE(L); }  // threads per line of the two-line kernels
-constexpr int xblk_T(int L) { return L / xE(L); }  // threads per line of the workgroup x pass
-bool reg_supported(int L) {
-  switch (L) {
-#define FIT_CASE(LL) case LL:
-    FIT_REG_LENGTHS(FIT_CASE)
-#undef FIT_CASE
-    return true;
-    default: return false;
-  }
-}
-size_t g_max_lds = 65536;  // per-workgroup LDS the launches may use
+size_t g_max_lds = 65536;
 
 // Fixed-order reduction of ng groups of nblk partial rows of nsh bins (k_reduce_rows).
 void launch_reduce(fit_plan* p, const double* partials, int nblk, int nsh, int ng, double* out) {
@@ -238,216 +101,7 @@ void launch_reduce(fit_plan* p, const double* partials, int nblk, int nsh, int n
                      dim3(kRedBins * kRedRows), 0, p->stream, partials, nblk, nsh, out);
 }
 
-// Column pass over fields [0, nfields): x < nx, other-index band ob, input band ib, output band wb.
-template <typename R, int SIGN, class LD, class ST>
-int launch_col(fit_plan* p, LD ld, ST st, int axis, int nfields, int nx, Band ob, Band ib, Band wb,
-               const int4* fband = nullptr, int fdiv = 1) {
-  const int L = p->G[axis];
-  const size_t csz = sizeof(cpx<R>);
-  const int nob = ob.count();
-  if (nx <= 0 || nob <= 0 || nfields <= 0) return FIT_OK;
-  const double rows_in = ib.count(), rows_out = wb.count();
-  const int gother = axis == 2 ? p->G[1] : p->G[2];
-  prof_begin(p, FIT_K_FFT_COLUMN, (double)nx * nob * (rows_in + rows_out) * csz * nfields);
-  if (!p->plain && reg_supported(L)) {
-    const int T = reg_T(L);
-    // columns per workgroup: 16 (128-byte rows in FP32) within 64 KB of LDS, or the plan's cap
-    // within the 160 KB a workgroup may hold
-    int TX = std::min(p->col_tx > 0 ? p->col_tx : 16, col_threads<R>() / T);
-    const size_t lds_cap = p->col_tx > 0 ? (size_t)160 * 1024 : g_max_lds;
-    while (TX > 1 && (size_t)L * TX * csz > lds_cap) TX /= 2;
-    dim3 grid((nx + TX - 1) / TX, nob, nfields);
-    const size_t lds = (size_t)L * TX * csz;
-    switch (L) {
-#define FIT_CASE(LL)                                                                                       \
-  case LL:                                                                                                 \
-    hipLaunchKernelGGL((k_col_r<R, LL, SIGN, LD, ST>), grid, dim3(reg_T(LL) * TX), lds, p->stream, ld, st, \
-                       (const cpx<R>*)p->tw[axis], nx, TX, axis, ob, ib, wb, fband, fdiv, gother);           \
-    break;
-      FIT_REG_LENGTHS(FIT_CASE)
-#undef FIT_CASE
-      default: break;
-    }
-  } else {
-    const int TX = tile_for(L, csz, 16);
-    dim3 grid((nx + TX - 1) / TX, nob, nfields);
-    const size_t lds = (size_t)2 * L * TX * csz;
-    hipLaunchKernelGGL((k_col<R, SIGN, LD, ST>), grid, dim3(kThreads), lds, p->stream, ld, st, p->fd[axis],
-                       (const cpx<R>*)p->tw[axis], nx, TX, axis, ob, ib, wb, fband, fdiv, gother);
-  }
-  prof_end(p);
-  FIT_HIP(hipGetLastError());
-  return FIT_OK;
-}
 
-template <typename R, int SIGN, class LD, class ST>
-int launch_col_full(fit_plan* p, LD ld, ST st, int axis, int nfields) {
-  const int other = axis == 1 ? p->G[2] : p->G[1];
-  return launch_col<R, SIGN>(p, ld, st, axis, nfields, p->h1, full_band(other), full_band(p->G[axis]),
-                             full_band(p->G[axis]));
-}
-
-template <typename R, class LD, class ST>
-int launch_row_inv(fit_plan* p, LD ld, ST st, int nfields, double extra_bytes_per_row, int kxa) {
-  const int L = p->G[0];
-  const size_t csz = sizeof(cpx<R>);
-  const int P = tile_for(L, csz, 8);
-  const int nrows = p->G[1] * p->G[2];
-  const int npairs = (nrows + 1) / 2;
-  dim3 grid((npairs + P - 1) / P, 1, nfields);
-  const size_t lds = (size_t)2 * L * P * csz;
-  const int nin = std::min(kxa + 1, p->h1);
-  prof_begin(p, FIT_K_FFT_ROW_INVERSE,
-             ((double)nin * csz + (double)L * sizeof(R) + extra_bytes_per_row) * nrows * nfields);
-  if (!p->plain && reg_supported(L)) {
-    const int T = reg_T(L);
-    const size_t LL = (size_t)L + L / 16;
-    int B = std::max(1, kThreads / T);
-    while (B > 1 && B * LL * csz > g_max_lds) --B;
-    dim3 rgrid((npairs + B - 1) / B, 1, nfields);
-    switch (L) {
-#define FIT_CASE(LLEN)                                                                                      \
-  case LLEN:                                                                                                \
-    hipLaunchKernelGGL((k_row_inv_r<R, LLEN, LD, ST>), rgrid, dim3(B * reg_T(LLEN)), B * LL * csz, p->stream, \
-                       ld, st, (const cpx<R>*)p->tw[0], p->G[1], nrows, B, kxa);                             \
-    break;
-      FIT_REG_LENGTHS(FIT_CASE)
-#undef FIT_CASE
-      default: break;
-    }
-  } else {
-    hipLaunchKernelGGL((k_row_inv<R, LD, ST>), grid, dim3(kThreads), lds, p->stream, ld, st, p->fd[0],
-                       (const cpx<R>*)p->tw[0], p->G[1], nrows, P, kxa);
-  }
-  prof_end(p);
-  FIT_HIP(hipGetLastError());
-  return FIT_OK;
-}
-
-template <typename R, class LD, class ST>
-int launch_row_fwd(fit_plan* p, LD ld, ST st, int nfields, int nout) {
-  const int L = p->G[0];
-  const size_t csz = sizeof(cpx<R>);
-  const int P = tile_for(L, csz, 8);
-  const int nrows = p->G[1] * p->G[2];
-  const int npairs = (nrows + 1) / 2;
-  dim3 grid((npairs + P - 1) / P, 1, nfields);
-  const size_t lds = (size_t)2 * L * P * csz;
-  prof_begin(p, FIT_K_FFT_ROW_FORWARD, ((double)nout * csz + (double)L * sizeof(R)) * nrows * nfields);
-  hipLaunchKernelGGL((k_row_fwd<R, LD, ST>), grid, dim3(kThreads), lds, p->stream, ld, st, p->fd[0],
-                     (const cpx<R>*)p->tw[0], p->G[1], nrows, P, nout);
-  prof_end(p);
-  FIT_HIP(hipGetLastError());
-  return FIT_OK;
-}
-
-template <typename R> LdHalf<R> ld_work(fit_plan* p) {
-  return LdHalf<R>{(const cpx<R>*)p->work, (int64_t)p->h1, (int64_t)p->h1 * p->G[1], p->Hp};
-}
-template <typename R> StHalf<R> st_work(fit_plan* p) {
-  return StHalf<R>{(cpx<R>*)p->work, (int64_t)p->h1, (int64_t)p->h1 * p->G[1], p->Hp};
-}
-
-// Inverse transform of one spectral-source field into a physical row storer RS.  `bi` is the
-// folded support of the source (pruning); the plain pipeline passes full bands.
-template <typename R, class RS>
-int inverse_field(fit_plan* p, const LdSpectral<R>& ls, RS rs, double extra_row_bytes, const int* bi) {
-  int rc;
-  const int xa = std::min(bi[0], p->G[0] / 2) + 1;
-  if (p->nd == 3) {
-    if ((rc = launch_col<R, 1>(p, ls, st_work<R>(p), 2, 1, xa, Band{bi[1], p->G[1]}, Band{bi[2], p->G[2]},
-                               full_band(p->G[2]))))
-      return rc;
-    if ((rc = launch_col<R, 1>(p, ld_work<R>(p), st_work<R>(p), 1, 1, xa, full_band(p->G[2]),
-                               Band{bi[1], p->G[1]}, full_band(p->G[1]))))
-      return rc;
-    return launch_row_inv<R>(p, ld_work<R>(p), rs, 1, extra_row_bytes, xa - 1);
-  }
-  if (p->nd == 2) {
-    if ((rc = launch_col<R, 1>(p, ls, st_work<R>(p), 1, 1, xa, full_band(1), Band{bi[1], p->G[1]},
-                               full_band(p->G[1]))))
-      return rc;
-    return launch_row_inv<R>(p, ld_work<R>(p), rs, 1, extra_row_bytes, xa - 1);
-  }
-  return launch_row_inv<R>(p, ls, rs, 1, extra_row_bytes, xa - 1);
-}
-
-// Forward transform of the physical product (nphys) into full original-grid N̂ component c.
-template <typename R>
-int forward_field(fit_plan* p, int c, double scale) {
-  Grid g = grid_of<R>(p);
-  LdReal<R> lr{(const R*)p->nphys, (int64_t)p->G[0], (int64_t)p->G[0] * p->G[1], 0};
-  StNhat<R> sn{g, (cpx<R>*)(p->nhat_dst ? p->nhat_dst : p->nhat) + (int64_t)c * p->Np, 0, p->dealias == FIT_DEALIAS_TWOTHIRDS,
-               (R)scale};
-  int rc;
-  if (p->nd == 1) return launch_row_fwd<R>(p, lr, sn, 1, p->h1);
-  if ((rc = launch_row_fwd<R>(p, lr, st_work<R>(p), 1, p->h1))) return rc;
-  if (p->nd == 2) return launch_col_full<R, -1>(p, ld_work<R>(p), sn, 1, 1);
-  if ((rc = launch_col_full<R, -1>(p, ld_work<R>(p), st_work<R>(p), 1, 1))) return rc;
-  return launch_col_full<R, -1>(p, ld_work<R>(p), sn, 2, 1);
-}
-
-template <typename R> LdSpectral<R> spectral_loader(fit_plan* p, const void* src, int gradj, int giver) {
-  LdSpectral<R> ls;
-  ls.g = grid_of<R>(p);
-  ls.src = (const cpx<R>*)src;
-  ls.sf = p->Np;
-  ls.trunc = p->dealias == FIT_DEALIAS_TWOTHIRDS;
-  ls.gradj = gradj;
-  ls.gradj_per_field = 0;
-  ls.src_per_field_div = 1;
-  ls.ks0 = (const R*)p->ks[0];
-  ls.ks1 = (const R*)p->ks[1];
-  ls.ks2 = (const R*)p->ks[2];
-  ls.shell = p->shell;
-  ls.giver = giver;
-  ls.srcs = nullptr;
-  ls.cps = 1;
-  ls.gstride = 0;
-  return ls;
-}
-
-const int* full_bi(fit_plan* p) {
-  static thread_local int b[3];
-  b[0] = p->G[0]; b[1] = p->G[1]; b[2] = p->G[2];
-  return b;
-}
-
-// Advecting velocity u_j on the transform grid (FFTWExt.jl:80-86 / :171): u_j = Re ifft(keep â_j)
-template <typename R>
-int compute_u(fit_plan* p, const void* adv) {
-  const double inv_mtot = 1.0 / (double)p->Gp;
-  const int* bi = p->plain ? full_bi(p) : p->band_in;
-  for (int j = 0; j < p->nd; ++j) {
-    LdSpectral<R> ls = spectral_loader<R>(p, (const cpx<R>*)adv + (int64_t)j * p->Np, -1, 0);
-    StReal<R> sr{(R*)p->uphys + (int64_t)j * p->Gp, (int64_t)p->G[0], (int64_t)p->G[0] * p->G[1], 0,
-                 (R)inv_mtot};
-    int rc = inverse_field<R>(p, ls, sr, 0.0, bi);
-    if (rc) return rc;
-  }
-  return FIT_OK;
-}
-
-// Plain pipeline: N̂ for the primary field f (optionally restricted to giver shell m), u computed.
-template <typename R>
-int nonlinear_from_u(fit_plan* p, const void* f, int giver) {
-  const double inv_mtot = 1.0 / (double)p->Gp;
-  const double fscale = (double)p->Gp / ((double)p->Np * (double)p->Np);
-  const int64_t sy = p->G[0], sz = (int64_t)p->G[0] * p->G[1];
-  for (int c = 0; c < p->M; ++c) {
-    for (int j = 0; j < p->nd; ++j) {
-      LdSpectral<R> ls = spectral_loader<R>(p, (const cpx<R>*)f + (int64_t)c * p->Np, j, giver);
-      StProduct<R> sp{(R*)p->nphys, (const R*)p->uphys + (int64_t)j * p->Gp, sy, sz, (R)inv_mtot, j == 0};
-      int rc = inverse_field<R>(p, ls, sp, (double)p->G[0] * sizeof(R) * (j == 0 ? 2.0 : 3.0), full_bi(p));
-      if (rc) return rc;
-    }
-    int rc = forward_field<R>(p, c, fscale);
-    if (rc) return rc;
-  }
-  return FIT_OK;
-}
-
-// ------------------------------ fused / pruned pipeline ---------------------------------------
 
 int grow(fit_plan* p, void** buf, size_t* have, size_t need) {
   if (*have >= need) return FIT_OK;
@@ -458,374 +112,8 @@ int grow(fit_plan* p, void** buf, size_t* have, size_t need) {
   return FIT_OK;
 }
 
-// x-extent of compact storage for a folded band on the half grid
-int xext(fit_plan* p, int a) { return std::min(a, p->G[0] / 2) + 1; }
-// row pitch of compact storage: the extent rounded up to 16 elements, so that every row starts
-// on a 128-byte line and column tiles never straddle two lines
-int xpitch(int xe) { return (xe + 15) / 16 * 16; }
-
-// shared-transform gradient passes apply: exact for this plan, register lengths on every axis
-bool commute_ok(const fit_plan* p) {
-  if (!p->commute || p->plain || p->nd < 2) return false;
-  for (int d = 0; d < p->nd; ++d)
-    if (!reg_supported(p->G[d])) return false;
-  return true;
-}
-
-// Columns per workgroup of the two-line gradient passes (512 threads at most): 16 within 64 KB
-// of LDS, or the plan's column tile within 160 KB.
-int grad_tile(const fit_plan* p, int L, size_t csz) {
-  const int T = two_T(L);
-  int TX = std::min(p->col_tx > 0 ? p->col_tx : 16, 512 / T);
-  const size_t cap = p->col_tx > 0 ? (size_t)160 * 1024 : g_max_lds;
-  while (TX > 1 && (size_t)L * TX * csz > cap) TX /= 2;
-  return TX;
-}
-
-// First and (3D) y passes of the shared-transform gradients of ng*M fields into work2.
-template <typename R>
-int grads_shared(fit_plan* p, const void* f, int giver0, int ng, const int* bi, const int4* fband, int xa,
-                 StHalf<R> st, const void* const* srcs) {
-  const size_t csz = sizeof(cpx<R>);
-  const int nd = p->nd, ngc = ng * p->M;
-  const int ax = nd - 1;
-  LdSpectral<R> ls = spectral_loader<R>(p, f, 0, 0);
-  if (srcs) {
-    ls.srcs = (const cpx<R>* const*)srcs;
-    ls.cps = p->M;
-    ls.gstride = p->M;
-  }
-  {
-    const int L = p->G[ax];
-    const int TX = grad_tile(p, L, csz);
-    const Band ob = nd == 3 ? Band{bi[1], p->G[1]} : full_band(1);
-    const Band ib = Band{bi[ax], L};
-    const Band wb = full_band(L);
-    const int nob = ob.count();
-    dim3 grid((xa + TX - 1) / TX, nob, ngc);
-    prof_begin(p, FIT_K_FFT_COLUMN, (double)ngc * xa * nob * (2.0 * ib.count() + 2.0 * wb.count()) * csz);
-    switch (L) {
-#define FIT_CASE(LL)                                                                                         \
-  case LL:                                                                                                   \
-    hipLaunchKernelGGL((k_grad1_r<R, LL>), grid, dim3(two_T(LL) * TX), (size_t)L * TX * csz, p->stream, ls,  \
-                       p->M, giver0, st, (const cpx<R>*)p->tw[ax], xa, TX, ax, ob, ib, wb, fband, p->G[1]);  \
-    break;
-      FIT_REG_LENGTHS(FIT_CASE)
-#undef FIT_CASE
-      default: break;
-    }
-    prof_end(p);
-    FIT_HIP(hipGetLastError());
-  }
-  if (nd == 3) {
-    const int L = p->G[1];
-    const int TX = grad_tile(p, L, csz);
-    const Band ob = full_band(p->G[2]);
-    const Band ib = Band{bi[1], L};
-    const Band wb = full_band(L);
-    const bool split = p->grad_split < 0 ? L < 512 : p->grad_split != 0;
-    dim3 grid((xa + TX - 1) / TX, ob.count(), ngc);
-    prof_begin(p, FIT_K_FFT_COLUMN,
-               (double)ngc * xa * ob.count() * ((split ? 1.0 : 2.0) * ib.count() + (split ? 2.0 : 3.0) * wb.count()) * csz);
-    switch (L) {
-#define FIT_CASE(LL)                                                                                         \
-  case LL:                                                                                                   \
-    hipLaunchKernelGGL((k_grad2_r<R, LL>), grid, dim3(two_T(LL) * TX), (size_t)L * TX * csz, p->stream,      \
-                       st.p, st.sy, st.sz, st.sf, (const R*)p->kt[1], (const cpx<R>*)p->tw[1], xa, TX, ob,   \
-                       ib, wb, fband, p->M, split ? 0 : 1);                                                  \
-    break;
-      FIT_REG_LENGTHS(FIT_CASE)
-#undef FIT_CASE
-      default: break;
-    }
-    prof_end(p);
-    FIT_HIP(hipGetLastError());
-    if (split) {
-      // slot 2 (the last-axis derivative) needs only its own y transform: the one-line column
-      // pass, which holds twice the waves of the two-line kernel
-      LdHalf<R> l2{st.p + 2 * st.sf, st.sy, st.sz, 3 * st.sf};
-      StHalf<R> s2{st.p + 2 * st.sf, st.sy, st.sz, 3 * st.sf};
-      int rc = launch_col<R, 1>(p, l2, s2, 1, ngc, xa, ob, ib, wb, fband, p->M);
-      if (rc) return rc;
-    }
-  }
-  return FIT_OK;
-}
-
-// Inverse passes (all but x) of ng*M*nd gradient fields (giver g, component c, direction j) of
-// f̂ into work2 (compact, pitch xpitch(xa)).  giver0 = 0: no shell filter.  fband: per-giver bounds.
-// srcs (ensembles): device table of ng snapshot arrays (M components each); group g reads srcs[g].
-template <typename R>
-int v2_inverse_grads(fit_plan* p, const void* f, int giver0, int ng, const int* bi, const int4* fband,
-                     const void* const* srcs = nullptr) {
-  const int nf = ng * p->M * p->nd;
-  const int xa = xext(p, bi[0]), pa = xpitch(xa);
-  const size_t csz = sizeof(cpx<R>);
-  int rc = grow(p, &p->work2, &p->work2_bytes, (size_t)nf * pa * p->G[1] * p->G[2] * csz);
-  if (rc) return rc;
-  const int64_t sy = pa, sz = (int64_t)pa * p->G[1], sf = sz * p->G[2];
-  StHalf<R> st{(cpx<R>*)p->work2, sy, sz, sf};
-  p->grads_commuted = false;
-  if (commute_ok(p)) {
-    if ((rc = grads_shared<R>(p, f, giver0, ng, bi, fband, xa, st, srcs))) return rc;
-    p->grads_commuted = true;
-    return FIT_OK;
-  }
-  LdSpectralB<R> ls{spectral_loader<R>(p, f, 0, 0), p->M, giver0};
-  if (srcs) {
-    ls.base.srcs = (const cpx<R>* const*)srcs;
-    ls.base.cps = p->M;
-    ls.base.gstride = p->M;
-  }
-  LdHalf<R> ld{(const cpx<R>*)p->work2, sy, sz, sf};
-  const int fdiv = p->M * p->nd;
-  if (p->nd == 3) {
-    if ((rc = launch_col<R, 1>(p, ls, st, 2, nf, xa, Band{bi[1], p->G[1]}, Band{bi[2], p->G[2]},
-                               full_band(p->G[2]), fband, fdiv)))
-      return rc;
-    return launch_col<R, 1>(p, ld, st, 1, nf, xa, full_band(p->G[2]), Band{bi[1], p->G[1]},
-                            full_band(p->G[1]), fband, fdiv);
-  }
-  return launch_col<R, 1>(p, ls, st, 1, nf, xa, full_band(1), Band{bi[1], p->G[1]}, full_band(p->G[1]),
-                          fband, fdiv);
-}
-
-// Fused x pass: ng*M field groups -> forward-x half rows kx <= bo[0] in nxbuf (pitch xpitch(xb)).
-// u: physical advecting velocity (nd components, pitch Gp) of group 0; ugs: its stride per group
-// (0: one velocity for every group, as for the giver shells; ensembles: nd*Gp).
-template <typename R>
-int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* bo, const void* u = nullptr,
-              int64_t ugs = 0) {
-  if (!u) u = p->uphys;
-  const int nfg = ng * p->M;
-  const int xa = xext(p, bi[0]), pa = xpitch(xa);
-  const int xb = xext(p, bo[0]), pb = xpitch(xb);
-  const size_t csz = sizeof(cpx<R>);
-  int rc = grow(p, &p->nxbuf, &p->nxbuf_bytes, (size_t)nfg * pb * p->G[1] * p->G[2] * csz);
-  if (rc) return rc;
-  const int L = p->G[0];
-  const int nrows = p->G[1] * p->G[2];
-  const int npairs = (nrows + 1) / 2;
-  const int64_t gsy = pa, gsz = (int64_t)pa * p->G[1], gsf = gsz * p->G[2];
-  const int64_t osy = pb, osz = (int64_t)pb * p->G[1], osf = osz * p->G[2];
-  const int64_t usy = L, usz = (int64_t)L * p->G[1];
-  const double bytes = (double)nfg * nrows * (p->nd * (xa * csz) + xb * csz) + (double)p->nd * nrows * L * sizeof(R);
-  const R* kxt = p->grads_commuted ? (const R*)p->kt[0] : nullptr;
-  if (p->grads_commuted && !reg_supported(L)) return fail(FIT_E_UNSUPPORTED, "shared-transform gradients need a register x pass");
-  prof_begin(p, FIT_K_FUSED_X, bytes);
-  // wave form for short register footprints (E <= 16); the workgroup form otherwise
-  const bool wave_ok = reg_supported(L) && !p->xfused_block && wE(L) <= 16 &&
-                       (size_t)4 * (64 / (L / wE(L))) * ((size_t)L + L / 16) * csz <= g_max_lds;
-  if (wave_ok) {
-    // wave-per-line kernel: 4 waves per workgroup, 64/T row pairs per wave
-    const int T = L / wE(L);
-    const int pairs_per_block = 4 * (64 / T);
-    const size_t LL = (size_t)L + L / 16;
-    const size_t lds = (size_t)pairs_per_block * LL * csz;
-    const int nrg = (npairs + pairs_per_block - 1) / pairs_per_block;
-    const int ngiv = nfg / p->M;
-    const int GC = ugs ? 1 : std::max(1, std::min(ngiv, p->xfused_gc > 0 ? p->xfused_gc : 4));
-    const int nchunk = (ngiv + GC - 1) / GC;
-    // only lengths with at most 16 points per lane are instantiated (the others never qualify)
-    switch (L) {
-#define FIT_CASE(LLEN)                                                                                            \
-  case LLEN:                                                                                                      \
-    if constexpr (wE(LLEN) <= 16) {                                                                               \
-      hipLaunchKernelGGL((k_xfused_w<R, LLEN>), dim3((unsigned)((int64_t)nrg * nchunk)), dim3(256), lds,          \
-                         p->stream, (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf, (const R*)u, \
-                         usy, usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M, p->G[1], nrows,          \
-                         (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, ngiv, GC, kxt, ugs);                     \
-    } else {                                                                                                      \
-      prof_end(p);                                                                                                \
-      return fail(FIT_E_UNSUPPORTED, "wave-form x pass needs at most 16 points per lane");                        \
-    }                                                                                                             \
-    break;
-      FIT_REG_LENGTHS(FIT_CASE)
-#undef FIT_CASE
-      default: break;
-    }
-  } else if (reg_supported(L)) {
-    // workgroup form: B row pairs per workgroup, single LDS exchange buffer
-    const int T = xblk_T(L);
-    const size_t LL = (size_t)L + L / 16;
-    int B = std::max(1, 256 / T);
-    while (B > 1 && B * LL * csz > g_max_lds) --B;
-    const int nrg = (npairs + B - 1) / B;
-    const size_t lds = B * LL * csz;
-    // field groups per workgroup: every component of up to xfused_gc givers (or one snapshot)
-    const int ngiv = nfg / p->M;
-    const int GCM = p->M * (ugs ? 1 : std::max(1, std::min(ngiv, p->xfused_gc > 0 ? p->xfused_gc : 4)));
-    const int nchunk = (nfg + GCM - 1) / GCM;
-    switch (L) {
-#define FIT_CASE(LLEN)                                                                                         \
-  case LLEN:                                                                                                   \
-    hipLaunchKernelGGL((k_xfused_r<R, LLEN>), dim3((unsigned)((int64_t)nrg * nchunk)), dim3(B * xblk_T(LLEN)), \
-                       lds, p->stream, (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf,        \
-                       (const R*)u, usy, usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M,            \
-                       p->G[1], nrows, B, (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, nfg, kxt, ugs, GCM); \
-    break;
-      FIT_REG_LENGTHS(FIT_CASE)
-#undef FIT_CASE
-      default: break;
-    }
-  } else {
-    int P = 8;
-    while (P > 1 && (size_t)3 * L * P * csz > 65536) --P;
-    const int nrg = (npairs + P - 1) / P;
-    hipLaunchKernelGGL(k_xfused<R>, dim3((unsigned)((int64_t)nrg * nfg)), dim3(kThreads), (size_t)3 * L * P * csz,
-                       p->stream, p->fd[0], (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf,
-                       (const R*)u, usy, usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M, p->G[1],
-                       nrows, P, (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, nfg, ugs);
-  }
-  prof_end(p);
-  FIT_HIP(hipGetLastError());
-  return FIT_OK;
-}
-
-// Fused x pass of a self-advected field whose gradients went through the shared-transform
-// passes: u_j is formed in registers from slot 0 (no u in HBM), then N_c for every component.
-template <typename R>
-int v2_xfused_self(fit_plan* p, const int* bi, const int* bo) {
-  const int nfg = p->M;
-  const int xa = xext(p, bi[0]), pa = xpitch(xa);
-  const int xb = xext(p, bo[0]), pb = xpitch(xb);
-  const size_t csz = sizeof(cpx<R>);
-  int rc = grow(p, &p->nxbuf, &p->nxbuf_bytes, (size_t)nfg * pb * p->G[1] * p->G[2] * csz);
-  if (rc) return rc;
-  const int L = p->G[0];
-  const int nrows = p->G[1] * p->G[2];
-  const int npairs = (nrows + 1) / 2;
-  const int64_t gsy = pa, gsz = (int64_t)pa * p->G[1], gsf = gsz * p->G[2];
-  const int64_t osy = pb, osz = (int64_t)pb * p->G[1], osf = osz * p->G[2];
-  const double bytes = (double)nrows * (((double)p->nd + (double)nfg * p->nd) * xa * csz + (double)nfg * xb * csz);
-  prof_begin(p, FIT_K_FUSED_X, bytes);
-  if constexpr (std::is_same<R, float>::value) {
-    // one line pair per wave (T = 64): the LDS-staged form, 4 pairs per workgroup
-    if (p->xstage && (L == 512 || L == 768)) {
-      const int nrg = (npairs + 3) / 4;
-#define FIT_STG(LLEN)                                                                                         \
-  hipLaunchKernelGGL((k_xfused_self_stg<LLEN, sE(LLEN)>), dim3((unsigned)nrg), dim3(256), 0, p->stream,        \
-                     (const cpx<float>*)p->tw[0], (const cpx<float>*)p->work2, gsy, gsz, gsf,                 \
-                     (cpx<float>*)p->nxbuf, osy, osz, osf, p->nd, p->M, p->G[1], nrows,                       \
-                     (float)(1.0 / (double)p->Gp), xa - 1, xb - 1, (const float*)p->kt[0])
-      if (L == 512) FIT_STG(512);
-      else FIT_STG(768);
-#undef FIT_STG
-      prof_end(p);
-      FIT_HIP(hipGetLastError());
-      return FIT_OK;
-    }
-  }
-  switch (L) {
-#define FIT_CASE(LLEN)                                                                                          \
-  case LLEN: {                                                                                                  \
-    constexpr int T = LLEN / sE(LLEN);                                                                          \
-    const int B = std::max(1, kThreads / T);                                                                    \
-    const size_t lds = (size_t)B * ldsline(LLEN) * csz;                                                         \
-    const int nrg = (npairs + B - 1) / B;                                                                       \
-    hipLaunchKernelGGL((k_xfused_self<R, LLEN, sE(LLEN)>), dim3((unsigned)nrg), dim3(B * T), lds, p->stream,   \
-                       (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf, (cpx<R>*)p->nxbuf, osy, \
-                       osz, osf, p->nd, p->M, p->G[1], nrows, B, (R)(1.0 / (double)p->Gp), xa - 1, xb - 1,     \
-                       (const R*)p->kt[0]);                                                                     \
-    break;                                                                                                      \
-  }
-    FIT_REG_LENGTHS(FIT_CASE)
-#undef FIT_CASE
-    default:
-      return fail(FIT_E_UNSUPPORTED, "self-advection x pass needs a register FFT length");
-  }
-  prof_end(p);
-  FIT_HIP(hipGetLastError());
-  return FIT_OK;
-}
-
-// Remaining forward passes of nxbuf (M fields, one field group per component) into full N̂.
-template <typename R>
-int v2_forward_nhat(fit_plan* p, const int* bo) {
-  const int xb = xext(p, bo[0]), pb = xpitch(xb);
-  const int64_t sy = pb, sz = (int64_t)pb * p->G[1], sf = sz * p->G[2];
-  LdHalf<R> ld{(const cpx<R>*)p->nxbuf, sy, sz, sf};
-  StHalf<R> st{(cpx<R>*)p->nxbuf, sy, sz, sf};
-  Grid g = grid_of<R>(p);
-  const bool trunc = p->dealias == FIT_DEALIAS_TWOTHIRDS;
-  const double scale = (double)p->Gp / ((double)p->Np * (double)p->Np);
-  void* dst = p->nhat_dst ? p->nhat_dst : p->nhat;
-  StNhat<R> sn{g, (cpx<R>*)dst, p->Np, trunc, (R)scale};
-  int rc;
-  if (trunc) FIT_HIP(hipMemsetAsync(dst, 0, (size_t)p->M * p->Np * sizeof(cpx<R>), p->stream));
-  if (p->nd == 3) {
-    if ((rc = launch_col<R, -1>(p, ld, st, 1, p->M, xb, full_band(p->G[2]), full_band(p->G[1]),
-                                Band{bo[1], p->G[1]})))
-      return rc;
-    return launch_col<R, -1>(p, ld, sn, 2, p->M, xb, Band{bo[1], p->G[1]}, full_band(p->G[2]),
-                             Band{bo[2], p->G[2]});
-  }
-  return launch_col<R, -1>(p, ld, sn, 1, p->M, xb, full_band(1), full_band(p->G[1]), Band{bo[1], p->G[1]});
-}
-
-// N̂ of f advected by u (already in uphys) through the fused pipeline.
-template <typename R>
-int v2_nonlinear(fit_plan* p, const void* f) {
-  const int* bo = p->band_in;  // every mode the input band feeds (2/3: kept box; padded: block)
-  int rc;
-  if ((rc = v2_inverse_grads<R>(p, f, 0, 1, p->band_in, nullptr))) return rc;
-  if ((rc = v2_xfused<R>(p, 1, p->band_in, nullptr, bo))) return rc;
-  return v2_forward_nhat<R>(p, bo);
-}
-
-// density (+ shell partial sums, reduced into p->dcol) from carrier and N̂ (both full grid)
-template <typename R>
-int density_launch(fit_plan* p, int inv, const void* car, const void* nh, void* t_out, bool shells) {
-  Grid g = grid_of<R>(p);
-  const int nsh = shells ? p->nsh : 0;
-  const int nblk = p->nblk_density;
-  const size_t lds = (size_t)std::max(nsh, 1) * sizeof(double);
-  const size_t csz = sizeof(cpx<R>);
-  prof_begin(p, FIT_K_DENSITY_SHELL,
-             (double)p->Np * (2.0 * p->M * csz + (shells ? 4.0 : 0.0) + (t_out ? sizeof(R) : 0.0)));
-  hipLaunchKernelGGL(k_density<R>, dim3(nblk), dim3(kThreads), lds, p->stream, g, int4{inv, 0, 0, 0}, 1, p->M,
-                     (const cpx<R>*)car, p->Np, (const cpx<R>*)nh, p->Np, (const R*)p->ks[0],
-                     (const R*)p->ks[1], (const R*)p->ks[2], shells ? p->shell : nullptr, nsh,
-                     p->partials, (R*)t_out);
-  prof_end(p);
-  FIT_HIP(hipGetLastError());
-  if (shells) {
-    prof_begin(p, FIT_K_REDUCE, (double)nblk * nsh * 8.0 + nsh * 8.0);
-    launch_reduce(p, p->partials, nblk, nsh, 1, p->dcol);
-    prof_end(p);
-    FIT_HIP(hipGetLastError());
-  }
-  return FIT_OK;
-}
-
-// shell sums of ninv (<= 4) invariants in one density pass over carrier and N̂: rows of nsh
-// doubles, invariant q at dcolG + q*nsh
-template <typename R>
-int density_multi(fit_plan* p, int ninv, const int* invs, const void* car, const void* nh) {
-  Grid g = grid_of<R>(p);
-  const int nsh = p->nsh, nb = ninv * nsh;
-  const int nblk = p->nblk_density;
-  const size_t csz = sizeof(cpx<R>);
-  int rc;
-  if ((rc = grow(p, (void**)&p->partials2, &p->partials2_bytes, (size_t)nblk * nb * sizeof(double)))) return rc;
-  if ((rc = grow(p, (void**)&p->dcolG, &p->dcolG_bytes, (size_t)nb * sizeof(double)))) return rc;
-  const int4 iv{invs[0], ninv > 1 ? invs[1] : 0, ninv > 2 ? invs[2] : 0, ninv > 3 ? invs[3] : 0};
-  prof_begin(p, FIT_K_DENSITY_SHELL, (double)p->Np * (2.0 * p->M * csz + 4.0));
-  hipLaunchKernelGGL(k_density<R>, dim3(nblk), dim3(kThreads), (size_t)nb * sizeof(double), p->stream, g, iv, ninv,
-                     p->M, (const cpx<R>*)car, p->Np, (const cpx<R>*)nh, p->Np, (const R*)p->ks[0],
-                     (const R*)p->ks[1], (const R*)p->ks[2], p->shell, nsh, p->partials2, (R*)nullptr);
-  prof_end(p);
-  FIT_HIP(hipGetLastError());
-  prof_begin(p, FIT_K_REDUCE, (double)nblk * nb * 8.0 + nb * 8.0);
-  launch_reduce(p, p->partials2, nblk, nb, 1, p->dcolG);
-  prof_end(p);
-  FIT_HIP(hipGetLastError());
-  return FIT_OK;
-}
-
 // ------------------------------ input / output staging ----------------------------------------
 
-size_t csize(int dtype) { return dtype == FIT_F64 ? 16 : 8; }
 
 // Make a device array of `ncomp` full-grid complex components in compute precision available.
 // Returns the device pointer to use (user pointer if usable as is, else the staging buffer).
@@ -936,31 +224,7 @@ int stage_fa(fit_plan* p, const void* f_hat, const void* adv_hat, int io_dtype, 
   return FIT_OK;
 }
 
-template <typename R>
-int nonlinear_term_impl(fit_plan* p, const void* f, const void* a) {
-  int rc;
-  if (a == f && commute_ok(p)) {
-    // self-advection: u_j = Re ifft(keep f̂_j) is the slot-0 transform of component j, so only
-    // its x pass remains (the y and z passes are shared with the gradients); the fused x pass
-    // forms it in registers
-    if ((rc = v2_inverse_grads<R>(p, f, 0, 1, p->band_in, nullptr))) return rc;
-    if (p->xself) {
-      if ((rc = v2_xfused_self<R>(p, p->band_in, p->band_in))) return rc;
-      return v2_forward_nhat<R>(p, p->band_in);
-    }
-    const int xa = xext(p, p->band_in[0]), pa = xpitch(xa);
-    const int64_t gsf = (int64_t)pa * p->G[1] * p->G[2];
-    LdHalf<R> ld{(const cpx<R>*)p->work2, (int64_t)pa, (int64_t)pa * p->G[1], (int64_t)p->nd * gsf};
-    StReal<R> sr{(R*)p->uphys, (int64_t)p->G[0], (int64_t)p->G[0] * p->G[1], p->Gp, (R)(1.0 / (double)p->Gp)};
-    if ((rc = launch_row_inv<R>(p, ld, sr, p->nd, 0.0, xa - 1))) return rc;
-    if ((rc = v2_xfused<R>(p, 1, p->band_in, nullptr, p->band_in))) return rc;
-    return v2_forward_nhat<R>(p, p->band_in);
-  }
-  rc = compute_u<R>(p, a);
-  if (rc) return rc;
-  if (p->plain || p->nd == 1) return nonlinear_from_u<R>(p, f, 0);
-  return v2_nonlinear<R>(p, f);
-}
+
 
 int finish(fit_plan* p) {
   FIT_HIP(hipStreamSynchronize(p->stream));
@@ -982,304 +246,7 @@ int spectral_flux_tail(fit_plan* p, double* T, double* Pi) {
   return FIT_OK;
 }
 
-// Plain giver loop: one giver per pipeline pass, full transforms, N̂ through HBM.
-template <typename R>
-int s2s_plain(fit_plan* p, int inv, const void* f, std::vector<double>& Tm) {
-  int rc;
-  const int nsh = p->nsh;
-  for (int m = 1; m <= nsh; ++m) {
-    if ((rc = nonlinear_from_u<R>(p, f, m))) return rc;
-    if ((rc = density_launch<R>(p, inv, f, p->nhat, nullptr, true))) return rc;
-    FIT_HIP(hipMemcpyAsync(Tm.data() + (size_t)(m - 1) * nsh, p->dcol, nsh * sizeof(double), hipMemcpyDeviceToHost,
-                           p->stream));
-    // dcol is reused by the next giver: order the copy before the next reduction
-    FIT_HIP(hipStreamSynchronize(p->stream));
-  }
-  return FIT_OK;
-}
 
-// Tail of a batched pass for gb field groups already through the fused x pass (nxbuf, pitch
-// xpitch(xext(band_rx))): the remaining forward pass(es), the pair density against eh (+ g*ehgs for group
-// g) and FP64 shell bins, reduced in fixed order into out (gb rows of nsh doubles, device).
-template <typename R>
-int batched_lastdens(fit_plan* p, int gb, const void* eh, int64_t ehgs, double* out) {
-  const int nsh = p->nsh;
-  const int* bo = p->band_rx;
-  const size_t csz = sizeof(cpx<R>);
-  const int n1h = p->n[0] / 2 + 1;
-  const int64_t Hn = (int64_t)n1h * p->n[1] * p->n[2];
-  const int lax = p->nd == 3 ? 2 : 1;
-  const int L = p->G[lax];
-  // columns per workgroup: up to 16 (128-byte rows) within the 160 KB a workgroup may hold; long
-  // columns (2048-point y lines of 2D ensembles) keep at least two lines per workgroup this way
-  int TX = 16;
-  // (TX*L <= 16*256 keeps the per-thread accumulators at 16: about 160 VGPRs, three waves per SIMD)
-  while (TX > 1 && ((size_t)2 * L * TX * csz + (size_t)nsh * 8 > (size_t)160 * 1024 || TX * L > 16 * kThreads))
-    TX /= 2;
-  const int xb = std::max(1, xext(p, bo[0]));
-  const Band ob = p->nd == 3 ? Band{bo[1], p->G[1]} : full_band(1);
-  const Band wb = p->nd == 3 ? Band{bo[2], p->G[2]} : Band{bo[1], p->G[1]};
-  const int gx = (xb + TX - 1) / TX, gy = ob.count();
-  const int nblk = gx * gy;
-  int rc;
-  if ((rc = grow(p, (void**)&p->partials2, &p->partials2_bytes, (size_t)gb * nblk * nsh * sizeof(double)))) return rc;
-  const double scale = 1.0 / (double)p->Np;
-  const int pb = xpitch(xb);
-  const int64_t sy = pb, sz = (int64_t)pb * p->G[1], sf = sz * p->G[2];
-  if (p->nd == 3) {
-    LdHalf<R> ld{(const cpx<R>*)p->nxbuf, sy, sz, sf};
-    StHalf<R> st{(cpx<R>*)p->nxbuf, sy, sz, sf};
-    if ((rc = launch_col<R, -1>(p, ld, st, 1, gb * p->M, xb, full_band(p->G[2]), full_band(p->G[1]),
-                                Band{bo[1], p->G[1]})))
-      return rc;
-  }
-  prof_begin(p, FIT_K_DENSITY_SHELL,
-             (double)gb * p->M * xb * gy * L * csz + (double)gb * p->M * xb * gy * wb.count() * csz);
-  // per-thread accumulators: the smallest register array that covers TX*L points per workgroup
-  const int qneed = (L * TX + kThreads - 1) / kThreads;
-#define FIT_LASTDENS(QQ)                                                                                         \
-  hipLaunchKernelGGL((k_lastdens<R, QQ>), dim3(gx, gy, gb), dim3(kThreads), (size_t)2 * L * TX * csz + (size_t)nsh * 8, \
-                     p->stream, p->fd[lax], (const cpx<R>*)p->tw[lax], (const cpx<R>*)p->nxbuf, sy, sz, sf, xb, TX, \
-                     lax, ob, wb, p->M, (R)scale, (const cpx<R>*)eh, (int64_t)n1h, (int64_t)n1h * p->n[1], Hn,     \
-                     p->shh, nsh, p->n[0], p->partials2, ehgs)
-  if (qneed <= 4) FIT_LASTDENS(4);
-  else if (qneed <= 8) FIT_LASTDENS(8);
-  else if (qneed <= 16) FIT_LASTDENS(16);
-  else FIT_LASTDENS(32);
-#undef FIT_LASTDENS
-  prof_end(p);
-  FIT_HIP(hipGetLastError());
-  prof_begin(p, FIT_K_REDUCE, (double)gb * nblk * nsh * 8.0);
-  launch_reduce(p, p->partials2, nblk, nsh, gb, out);
-  prof_end(p);
-  FIT_HIP(hipGetLastError());
-  return FIT_OK;
-}
-
-// Hermitian part of the masked effective carrier on the receiving band, for nb snapshots
-// (srcs: device table of their arrays, or nullptr for the single array f) at eh + b*M*Hn.
-template <typename R>
-int carrier_half(fit_plan* p, int inv, const void* f, const void* const* srcs, int nb, void* eh) {
-  const int* bo = p->band_rx;
-  const size_t csz = sizeof(cpx<R>);
-  const int n1h = p->n[0] / 2 + 1;
-  const int64_t Hn = (int64_t)n1h * p->n[1] * p->n[2];
-  Grid g = grid_of<R>(p);
-  prof_begin(p, FIT_K_OTHER, (double)nb * Hn * p->M * csz * 3.0);
-  hipLaunchKernelGGL(k_carrier_half<R>, dim3(srcs ? 256 : 2048, nb), dim3(256), 0, p->stream, g, inv, p->M,
-                     p->dealias == FIT_DEALIAS_TWOTHIRDS ? 1 : 0, (const cpx<R>*)f, p->Np, (const R*)p->ks[0],
-                     (const R*)p->ks[1], (const R*)p->ks[2], Band{bo[1], p->n[1]}, Band{bo[2], p->n[2]}, bo[0],
-                     (cpx<R>*)eh, Hn, (const cpx<R>* const*)srcs, (int64_t)p->M * Hn);
-  prof_end(p);
-  FIT_HIP(hipGetLastError());
-  return FIT_OK;
-}
-
-// Bytes of batched workspace one field group (giver or snapshot) needs in the pruned pipeline.
-template <typename R>
-double batch_bytes_per_group(fit_plan* p, int xa_max) {
-  const int xb = std::max(1, xext(p, p->band_rx[0]));
-  return ((double)p->M * p->nd * xpitch(xa_max) + (double)p->M * xpitch(xb)) * p->G[1] * p->G[2] * sizeof(cpx<R>);
-}
-
-// Batched giver loop: every giver of a batch goes through each pruned pass in one launch; the
-// last forward pass computes the pair density and the shell sums in place (no N̂ in HBM).
-template <typename R>
-int s2s_batched(fit_plan* p, int inv, const void* f, std::vector<double>& Tm) {
-  const int nsh = p->nsh;
-  const int* bo = p->band_rx;
-  const size_t csz = sizeof(cpx<R>);
-  const int n1h = p->n[0] / 2 + 1;
-  const int64_t Hn = (int64_t)n1h * p->n[1] * p->n[2];
-  int rc;
-  // carrier Hermitian part on the receiving band
-  if ((rc = grow(p, &p->eh, &p->eh_bytes, (size_t)p->M * Hn * csz))) return rc;
-  if ((rc = carrier_half<R>(p, inv, f, nullptr, 1, p->eh))) return rc;
-  // batch size from the device memory budget
-  int bmax[3] = {-1, -1, -1};
-  for (int s = 0; s < nsh; ++s) {
-    bmax[0] = std::max(bmax[0], p->shell_band[s].x);
-    bmax[1] = std::max(bmax[1], p->shell_band[s].y);
-    bmax[2] = std::max(bmax[2], p->shell_band[s].z);
-  }
-  const int xa_max = std::max(1, xext(p, bmax[0]));
-  const double per_giver = batch_bytes_per_group<R>(p, xa_max);
-  size_t freeb = 0, totalb = 0;
-  FIT_HIP(hipMemGetInfo(&freeb, &totalb));
-  const double budget = 0.6 * ((double)freeb + (double)p->work2_bytes + (double)p->nxbuf_bytes);
-  int ng = (int)std::max(1.0, std::min((double)nsh, budget / per_giver));
-  if (p->giver_batch > 0) ng = std::min(ng, p->giver_batch);
-  if ((rc = grow(p, (void**)&p->dcolG, &p->dcolG_bytes, (size_t)ng * nsh * sizeof(double)))) return rc;
-  for (int m0 = 1; m0 <= nsh; m0 += ng) {
-    const int gb = std::min(ng, nsh - m0 + 1);
-    int bi[3] = {-1, -1, -1};
-    for (int s = m0 - 1; s < m0 - 1 + gb; ++s) {
-      bi[0] = std::max(bi[0], p->shell_band[s].x);
-      bi[1] = std::max(bi[1], p->shell_band[s].y);
-      bi[2] = std::max(bi[2], p->shell_band[s].z);
-    }
-    if (bi[0] < 0) continue;  // empty givers: zero columns
-    const int4* fb = p->d_shell_band + (m0 - 1);
-    if ((rc = v2_inverse_grads<R>(p, f, m0, gb, bi, fb))) return rc;
-    if ((rc = v2_xfused<R>(p, gb, bi, fb, bo))) return rc;
-    if ((rc = batched_lastdens<R>(p, gb, p->eh, 0, p->dcolG))) return rc;
-    FIT_HIP(hipMemcpyAsync(Tm.data() + (size_t)(m0 - 1) * nsh, p->dcolG, (size_t)gb * nsh * sizeof(double),
-                           hipMemcpyDeviceToHost, p->stream));
-    FIT_HIP(hipStreamSynchronize(p->stream));
-  }
-  return FIT_OK;
-}
-
-template <typename R>
-int s2s_impl(fit_plan* p, int inv, const void* f, const void* a, std::vector<double>& Tm) {
-  int rc = compute_u<R>(p, a);
-  if (rc) return rc;
-  if (p->plain || p->nd == 1 || p->dealias == FIT_DEALIAS_PADDED || !p->shell_symmetric)
-    return s2s_plain<R>(p, inv, f, Tm);
-  return s2s_batched<R>(p, inv, f, Tm);
-}
-
-// Giver loop on the band-limited sub-grid.  With receivers and givers in |k_d| <= R_d, every
-// triad k = p + q that T(n,m) sums has |q_d| <= 2 R_d, and on a grid of L_d > |q|+|p|+|k| points
-// the product of those modes is alias-free at the receivers, so the child grid yields the same
-// convolution the full grid does (FFTWExt.jl:234-274).  N̂ = conv/Np² on either grid, hence the
-// (Np_sub/Np)² rescale of the shell sums.
-template <typename R>
-int s2s_sub(fit_plan* p, int inv, const void* f, const void* a, std::vector<double>& Tm) {
-  fit_plan* c = p->sub;
-  const size_t csz = sizeof(cpx<R>);
-  auto gather = [&](const void* src, int ncomp, void** dst, size_t* have) -> int {
-    int r = grow(p, dst, have, (size_t)ncomp * c->Np * csz);
-    if (r) return r;
-    prof_begin(p, FIT_K_OTHER, (double)ncomp * c->Np * csz * 2.0);
-    hipLaunchKernelGGL(k_subgrid_gather<R>, dim3(2048), dim3(256), 0, p->stream, (const cpx<R>*)src, p->n[0],
-                       p->n[1], p->n[2], p->Np, (cpx<R>*)*dst, c->n[0], c->n[1], c->n[2], c->Np, p->sub_q[0],
-                       p->sub_q[1], p->sub_q[2], ncomp);
-    prof_end(p);
-    FIT_HIP(hipGetLastError());
-    return FIT_OK;
-  };
-  int rc = gather(f, p->M, &p->sub_f, &p->sub_f_bytes);
-  if (rc) return rc;
-  const void* ca = p->sub_f;
-  if (a != f) {
-    if ((rc = gather(a, p->D, &p->sub_a, &p->sub_a_bytes))) return rc;
-    ca = p->sub_a;
-  }
-  c->prof = p->prof;
-  if ((rc = s2s_impl<R>(c, inv, p->sub_f, ca, Tm))) return rc;
-  FIT_HIP(hipStreamSynchronize(p->stream));
-  for (double& v : Tm) v *= p->sub_tscale;
-  return FIT_OK;
-}
-
-template <typename R>
-int s2s_entry(fit_plan* p, int inv, const void* f, const void* a, std::vector<double>& Tm) {
-  if (p->sub && p->use_sub && !p->plain) return s2s_sub<R>(p, inv, f, a, Tm);
-  return s2s_impl<R>(p, inv, f, a, Tm);
-}
-
-// Smooth band-to-band transfer (BandTransfer.jl:66-113): for every band m, the nonlinear term of
-// w_m ⊙ f̂ advected by the full velocity (fused pipeline, u computed once), then the band sums
-// Σ_k w_n(k) t(k) of its density for every n in one pass.  kc: per-mode band coordinate.
-template <typename R>
-int b2b_impl(fit_plan* p, int inv, const void* f, const void* a, const R* kc, const double* cen, int nb,
-             double inv2s2, std::vector<double>& Tm) {
-  int rc = compute_u<R>(p, a);
-  if (rc) return rc;
-  const size_t cs = sizeof(cpx<R>);
-  if (!p->in_c && (rc = dalloc(p, &p->in_c, (size_t)std::max(p->M, p->D) * p->Np * cs))) return rc;
-  const int nblk = p->nblk_density;
-  if ((rc = grow(p, (void**)&p->partials2, &p->partials2_bytes, (size_t)nblk * nb * sizeof(double)))) return rc;
-  if ((rc = grow(p, (void**)&p->dcolG, &p->dcolG_bytes, (size_t)nb * sizeof(double)))) return rc;
-  Grid g = grid_of<R>(p);
-  p->nhat_dst = nullptr;
-  for (int m = 0; m < nb; ++m) {
-    prof_begin(p, FIT_K_OTHER, (double)p->Np * (2.0 * p->M * cs + sizeof(R)));
-    hipLaunchKernelGGL(k_band_field<R>, dim3(1024), dim3(256), 0, p->stream, p->Np, p->M, (const cpx<R>*)f, p->Np,
-                       kc, cen, nb, inv2s2, m, (cpx<R>*)p->in_c);
-    prof_end(p);
-    FIT_HIP(hipGetLastError());
-    rc = (p->plain || p->nd == 1) ? nonlinear_from_u<R>(p, p->in_c, 0) : v2_nonlinear<R>(p, p->in_c);
-    if (rc) return rc;
-    prof_begin(p, FIT_K_DENSITY_SHELL, (double)p->Np * (2.0 * p->M * cs + sizeof(R)));
-    hipLaunchKernelGGL(k_band_density<R>, dim3(nblk), dim3(kThreads), (size_t)nb * sizeof(double), p->stream, g, inv,
-                       p->M, (const cpx<R>*)f, p->Np, (const cpx<R>*)p->nhat, p->Np, (const R*)p->ks[0],
-                       (const R*)p->ks[1], (const R*)p->ks[2], kc, cen, nb, inv2s2, p->partials2);
-    prof_end(p);
-    FIT_HIP(hipGetLastError());
-    prof_begin(p, FIT_K_REDUCE, (double)nblk * nb * 8.0);
-    launch_reduce(p, p->partials2, nblk, nb, 1, p->dcolG);
-    prof_end(p);
-    FIT_HIP(hipGetLastError());
-    FIT_HIP(hipMemcpyAsync(Tm.data() + (size_t)m * nb, p->dcolG, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost,
-                           p->stream));
-    FIT_HIP(hipStreamSynchronize(p->stream));
-  }
-  return FIT_OK;
-}
-
-template <typename R>
-int m2m_impl(fit_plan* p, int inv, const void* f, const void* a, void* S, void* net, int io_dtype,
-                    int ptr_kind) {
-  Grid g = grid_of<R>(p);
-  const size_t cs = 2 * sizeof(R);
-  void *A = nullptr, *E = nullptr, *Sd = nullptr, *netpart = nullptr, *netd = nullptr;
-  int rc = FIT_OK;
-  const int64_t Np = p->Np;
-  const int64_t pchunk = 256;
-  const int nchunk = (int)((Np + pchunk - 1) / pchunk);
-  const size_t os = io_dtype == FIT_F64 ? 8 : 4;
-  auto cleanup = [&]() { if (A) (void)hipFree(A); if (E) (void)hipFree(E); if (Sd && Sd != S) (void)hipFree(Sd); if (netpart) (void)hipFree(netpart); if (netd) (void)hipFree(netd); };
-  if (hipMalloc(&A, (size_t)p->nd * Np * cs) != hipSuccess || hipMalloc(&E, (size_t)p->M * Np * cs) != hipSuccess ||
-      hipMalloc(&netpart, (size_t)nchunk * Np * sizeof(double)) != hipSuccess ||
-      hipMalloc(&netd, (size_t)Np * sizeof(double)) != hipSuccess) {
-    cleanup();
-    return fail(FIT_E_OUT_OF_MEMORY, "mode-to-mode scratch allocation failed");
-  }
-  if (ptr_kind == FIT_PTR_DEVICE) Sd = S;
-  else if (hipMalloc(&Sd, (size_t)Np * Np * os) != hipSuccess) { cleanup(); return fail(FIT_E_OUT_OF_MEMORY, "S staging"); }
-  const int trunc = p->dealias == FIT_DEALIAS_TWOTHIRDS;
-  hipLaunchKernelGGL(k_m2m_prep<R>, dim3(512), dim3(256), 0, p->stream, g, trunc, inv, p->M, (const cpx<R>*)f, Np,
-                     (const cpx<R>*)a, Np, (const R*)p->ks[0], (const R*)p->ks[1], (const R*)p->ks[2], (cpx<R>*)A,
-                     (cpx<R>*)E);
-  dim3 grid((unsigned)((Np + 255) / 256), nchunk);
-  prof_begin(p, FIT_K_OTHER, (double)Np * Np * os);
-  if (io_dtype == FIT_F64)
-    hipLaunchKernelGGL((k_m2m<R, double>), grid, dim3(256), 0, p->stream, g, trunc, p->M, (const cpx<R>*)f, Np,
-                       (const cpx<R>*)A, (const cpx<R>*)E, (const R*)p->ks[0], (const R*)p->ks[1], (const R*)p->ks[2],
-                       pchunk, (double*)Sd, (double*)netpart);
-  else
-    hipLaunchKernelGGL((k_m2m<R, float>), grid, dim3(256), 0, p->stream, g, trunc, p->M, (const cpx<R>*)f, Np,
-                       (const cpx<R>*)A, (const cpx<R>*)E, (const R*)p->ks[0], (const R*)p->ks[1], (const R*)p->ks[2],
-                       pchunk, (float*)Sd, (double*)netpart);
-  prof_end(p);
-  launch_reduce(p, (const double*)netpart, nchunk, (int)Np, 1, (double*)netd);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) { cleanup(); return fail(FIT_E_DEVICE, hipGetErrorString(e)); }
-  // net in io precision
-  if (io_dtype == FIT_F64) {
-    e = hipMemcpyAsync(net, netd, Np * 8, ptr_kind == FIT_PTR_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
-                       p->stream);
-  } else {
-    void* nf = nullptr;
-    if (hipMalloc(&nf, Np * 4) != hipSuccess) { cleanup(); return fail(FIT_E_OUT_OF_MEMORY, "net staging"); }
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(256), dim3(256), 0, p->stream, (const double*)netd, (float*)nf, Np);
-    e = hipMemcpyAsync(net, nf, Np * 4, ptr_kind == FIT_PTR_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
-                       p->stream);
-    (void)hipStreamSynchronize(p->stream);
-    (void)hipFree(nf);
-  }
-  if (e == hipSuccess && ptr_kind == FIT_PTR_HOST)
-    e = hipMemcpyAsync(S, Sd, (size_t)Np * Np * os, hipMemcpyDeviceToHost, p->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
-  cleanup();
-  if (e != hipSuccess) return fail(FIT_E_DEVICE, hipGetErrorString(e));
-  (void)rc;
-  return FIT_OK;
-}
-
-}  // namespace
 
 // =============================================================================================
 // C ABI
@@ -1550,143 +517,6 @@ static int create_plan(fit_plan** out, int nd, const int64_t* ns, int M, int D, 
 // a HIP graph and replayed by later calls, which only rewrite the pointer tables.
 // =============================================================================================
 namespace {
-
-struct EnsGroup {
-  fit_plan* p = nullptr;          // the group's grid, shells and workspaces (M = D = nd)
-  std::vector<int> snaps;         // snapshot indices, input order
-  void** d_tab = nullptr;         // 2B device pointers: velocities, then scalars
-  std::vector<void*> h_tab;       // host copy of the table (outlives the async upload)
-  void* ub = nullptr; size_t ub_bytes = 0;    // physical velocities of one batch (ng, nd, Gp)
-  void* ehk = nullptr; size_t ehk_bytes = 0;  // kinetic-energy carrier, Hermitian part (ng, nd, Hn)
-  void* ehs = nullptr; size_t ehs_bytes = 0;  // scalar carrier (ng, Hn)
-  double* rows = nullptr;         // (2, B, nsh): kinetic-energy rows, then scalar rows
-  std::vector<double> h_rows;
-  void* stage = nullptr; size_t stage_bytes = 0;  // inputs staged from the host / other dtype
-  int ng = 0;                     // snapshots per batch (device memory budget)
-};
-
-}  // namespace
-
-struct fit_ensemble {
-  int n = 0, nd = 0, dtype = 0, dealias = 0, device = 0, with_scalar = 0;
-  hipStream_t stream = nullptr;
-  std::vector<EnsGroup> groups;
-  std::vector<int> nsh;           // per snapshot
-  std::vector<int64_t> off;       // per snapshot: first row of its shells in the outputs
-  int use_graph = 1;
-  int batch_cap = 0;
-  bool prof = false;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  int graph_io = -1;              // (ptr_kind, io_dtype) the graph was captured for
-};
-
-namespace {
-
-// restores a plan's component count when a batch runs the scalar part with M = 1
-struct MGuard {
-  fit_plan* p; int m;
-  MGuard(fit_plan* q, int mm) : p(q), m(q->M) { q->M = mm; }
-  ~MGuard() { p->M = m; }
-};
-
-// u_j = Re ifft(keep û_j) for the gb snapshots tabled at tu: fields (b, j) -> ub (b*nd + j)*Gp
-template <typename R>
-int ens_velocity(fit_plan* p, const void* const* tu, int gb, void* ub) {
-  const int nd = p->nd, nf = gb * nd;
-  const int* bi = p->band_in;
-  const int xa = xext(p, bi[0]), pa = xpitch(xa);
-  const size_t csz = sizeof(cpx<R>);
-  int rc = grow(p, &p->work2, &p->work2_bytes, (size_t)nf * pa * p->G[1] * p->G[2] * csz);
-  if (rc) return rc;
-  const int64_t sy = pa, sz = (int64_t)pa * p->G[1], sf = sz * p->G[2];
-  StHalf<R> st{(cpx<R>*)p->work2, sy, sz, sf};
-  LdHalf<R> ld{(const cpx<R>*)p->work2, sy, sz, sf};
-  LdSpectral<R> ls = spectral_loader<R>(p, nullptr, -1, 0);
-  ls.srcs = (const cpx<R>* const*)tu;
-  ls.cps = nd;
-  if (nd == 3) {
-    if ((rc = launch_col<R, 1>(p, ls, st, 2, nf, xa, Band{bi[1], p->G[1]}, Band{bi[2], p->G[2]}, full_band(p->G[2]))))
-      return rc;
-    if ((rc = launch_col<R, 1>(p, ld, st, 1, nf, xa, full_band(p->G[2]), Band{bi[1], p->G[1]}, full_band(p->G[1]))))
-      return rc;
-  } else {
-    if ((rc = launch_col<R, 1>(p, ls, st, 1, nf, xa, full_band(1), Band{bi[1], p->G[1]}, full_band(p->G[1]))))
-      return rc;
-  }
-  StReal<R> sr{(R*)ub, (int64_t)p->G[0], (int64_t)p->G[0] * p->G[1], p->Gp, (R)(1.0 / (double)p->Gp)};
-  return launch_row_inv<R>(p, ld, sr, nf, 0.0, xa - 1);
-}
-
-// one batch [b0, b0 + gb) of a group: velocities, kinetic-energy rows, scalar rows
-template <typename R>
-int ens_batch(fit_ensemble* e, EnsGroup& G, int b0, int gb) {
-  fit_plan* p = G.p;
-  const int nd = p->nd, nsh = p->nsh;
-  const int B = (int)G.snaps.size();
-  const int64_t Hn = (int64_t)(p->n[0] / 2 + 1) * p->n[1] * p->n[2];
-  const void* const* tu = (const void* const*)(G.d_tab + b0);
-  const void* const* tt = (const void* const*)(G.d_tab + B + b0);
-  int rc;
-  if ((rc = ens_velocity<R>(p, tu, gb, G.ub))) return rc;
-  const int64_t ugs = (int64_t)nd * p->Gp;
-  {
-    MGuard mg(p, nd);
-    if ((rc = carrier_half<R>(p, FIT_KINETIC_ENERGY, nullptr, tu, gb, G.ehk))) return rc;
-    if ((rc = v2_inverse_grads<R>(p, nullptr, 0, gb, p->band_in, nullptr, tu))) return rc;
-    if ((rc = v2_xfused<R>(p, gb, p->band_in, nullptr, p->band_rx, G.ub, ugs))) return rc;
-    if ((rc = batched_lastdens<R>(p, gb, G.ehk, (int64_t)nd * Hn, G.rows + (size_t)b0 * nsh))) return rc;
-  }
-  if (e->with_scalar) {
-    MGuard mg(p, 1);
-    if ((rc = carrier_half<R>(p, FIT_PASSIVE_SCALAR, nullptr, tt, gb, G.ehs))) return rc;
-    if ((rc = v2_inverse_grads<R>(p, nullptr, 0, gb, p->band_in, nullptr, tt))) return rc;
-    if ((rc = v2_xfused<R>(p, gb, p->band_in, nullptr, p->band_rx, G.ub, ugs))) return rc;
-    if ((rc = batched_lastdens<R>(p, gb, G.ehs, Hn, G.rows + (size_t)(B + b0) * nsh))) return rc;
-  }
-  return FIT_OK;
-}
-
-// batch size of a group from the device memory budget (first run only: the graph reuses it)
-template <typename R>
-int ens_batch_size(fit_ensemble* e, EnsGroup& G) {
-  if (G.ng > 0) return FIT_OK;
-  fit_plan* p = G.p;
-  const int nd = p->nd;
-  const int B = (int)G.snaps.size();
-  const size_t csz = sizeof(cpx<R>);
-  const int64_t Hn = (int64_t)(p->n[0] / 2 + 1) * p->n[1] * p->n[2];
-  const int xa = xext(p, p->band_in[0]);
-  double per = 0.0;
-  {
-    MGuard mg(p, nd);
-    per = batch_bytes_per_group<R>(p, xa);
-  }
-  per += (double)nd * p->Gp * sizeof(R) + (double)(nd + 1) * Hn * csz;
-  size_t freeb = 0, totalb = 0;
-  FIT_HIP(hipMemGetInfo(&freeb, &totalb));
-  int ng = (int)std::max(1.0, std::min((double)B, 0.5 * (double)freeb / per));
-  if (e->batch_cap > 0) ng = std::min(ng, e->batch_cap);
-  G.ng = ng;
-  int rc;
-  if ((rc = grow(p, &G.ub, &G.ub_bytes, (size_t)ng * nd * p->Gp * sizeof(R)))) return rc;
-  if ((rc = grow(p, &G.ehk, &G.ehk_bytes, (size_t)ng * nd * Hn * csz))) return rc;
-  if (e->with_scalar && (rc = grow(p, &G.ehs, &G.ehs_bytes, (size_t)ng * Hn * csz))) return rc;
-  return FIT_OK;
-}
-
-template <typename R>
-int ens_run(fit_ensemble* e) {
-  for (EnsGroup& G : e->groups) {
-    int rc = ens_batch_size<R>(e, G);
-    if (rc) return rc;
-    const int B = (int)G.snaps.size();
-    for (int b0 = 0; b0 < B; b0 += G.ng)
-      if ((rc = ens_batch<R>(e, G, b0, std::min(G.ng, B - b0)))) return rc;
-  }
-  return FIT_OK;
-}
-
 int ens_run_any(fit_ensemble* e) { return e->dtype == FIT_F64 ? ens_run<double>(e) : ens_run<float>(e); }
 
 void ens_drop_graph(fit_ensemble* e) {
@@ -1698,6 +528,10 @@ void ens_drop_graph(fit_ensemble* e) {
 }
 
 }  // namespace
+
+}  // namespace fitx
+
+using namespace fitx;
 
 extern "C" {
 

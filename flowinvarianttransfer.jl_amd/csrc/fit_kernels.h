@@ -1240,7 +1240,7 @@ __global__ void k_carrier_half(Grid g, int inv, int M, int trunc, const cpx<R>* 
 // is fixed by the indices alone, so the result does not depend on scheduling; 16 row groups keep
 // 16 times more loads in flight than one thread per bin.
 constexpr int kRedBins = 64, kRedRows = 16;
-__global__ void __launch_bounds__(kRedBins * kRedRows) k_reduce_rows(const double* __restrict__ partials, int nblk,
+static __global__ void __launch_bounds__(kRedBins * kRedRows) k_reduce_rows(const double* __restrict__ partials, int nblk,
                                                                      int nsh, double* __restrict__ out) {
   __shared__ double part[kRedRows][kRedBins];
   const int tx = threadIdx.x % kRedBins, y = threadIdx.x / kRedBins;
