@@ -47,6 +47,8 @@ SIGNATURES = {
     "fit_profile_enable": (_I, [_P, _I]),
     "fit_profile_query": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int64), _D, _D]),
     "fit_plan_stream": (_P, [_P]),
+    "fit_plan_wait_stream": (_I, [_P, _P]),
+    "fit_ensemble_wait_stream": (_I, [_P, _P]),
     "fit_ensemble_create": (_I, [ctypes.POINTER(_P), _I, _I, ctypes.POINTER(ctypes.c_int64), _I, _I,
                                  ctypes.POINTER(ctypes.POINTER(ctypes.c_double)),
                                  ctypes.POINTER(ctypes.POINTER(ctypes.c_int64)), ctypes.POINTER(_I), _I,

@@ -9,6 +9,7 @@ the component axis last, column-major in memory.  Host inputs are numpy arrays; 
 torch tensors on a HIP device (results then stay on the device).
 """
 import ctypes
+import threading
 from collections import OrderedDict
 
 import numpy as np
@@ -54,6 +55,11 @@ def _marshal(a, io):
     want = np.complex64 if io == _lib.F32 else np.complex128
     arr = np.asfortranarray(np.asarray(a, dtype=want))
     return arr.ctypes.data, arr, _lib.PTR_HOST
+
+
+def _caller_stream(t):
+    """hipStream_t of torch's current stream on the device of tensor t."""
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
 
 
 def _alloc(like_device, shape, io, complex_=True):
@@ -192,6 +198,13 @@ class Plan:
         return out
 
     # -- entry points --------------------------------------------------------------------------
+    def _order(self, kind, t):
+        """Device arrays: the plan's stream waits for torch's current stream, where the inputs
+        (and the copies _marshal made of them) were produced and where recycled output blocks
+        may still be in use."""
+        if kind == _lib.PTR_DEVICE:
+            _lib.check(self.lib.fit_plan_wait_stream(self.handle, _caller_stream(t)))
+
     def _fa(self, f, adv):
         fp, fk, kind = _marshal(f, self.io)
         if adv is None:
@@ -204,6 +217,7 @@ class Plan:
     def nonlinear_term(self, f, adv=None):
         fp, ap, keep, kind = self._fa(f, adv)
         out, op = _alloc(keep[0] if kind == _lib.PTR_DEVICE else None, self.ns + (self.M,), self.io)
+        self._order(kind, keep[0])
         _lib.check(self.lib.fit_nonlinear_term(self.handle, fp, ap, op, self.io, kind))
         return out
 
@@ -211,6 +225,7 @@ class Plan:
         cp, ck, kind = _marshal(c_hat, self.io)
         npp, nk, _ = _marshal(N_hat, self.io)
         out, op = _alloc(ck if kind == _lib.PTR_DEVICE else None, self.ns, self.io, complex_=False)
+        self._order(kind, ck)
         _lib.check(self.lib.fit_transfer_density(self.handle, invariant.code, cp, npp, op, self.io,
                                                  kind))
         return out
@@ -219,6 +234,7 @@ class Plan:
         fp, ap, keep, kind = self._fa(f, adv)
         T = np.zeros(self.n_shells)
         Pi = np.zeros(self.n_shells)
+        self._order(kind, keep[0])
         _lib.check(self.lib.fit_spectral_flux(self.handle, invariant.code, fp, ap, _dptr(T),
                                               _dptr(Pi), self.io, kind))
         return T, Pi
@@ -228,6 +244,7 @@ class Plan:
         npp, nk, _ = _marshal(N_hat, self.io)
         T = np.zeros(self.n_shells)
         Pi = np.zeros(self.n_shells)
+        self._order(kind, ck)
         _lib.check(self.lib.fit_spectral_flux_with_nhat(self.handle, invariant.code, cp, npp,
                                                         _dptr(T), _dptr(Pi), self.io, kind))
         return T, Pi
@@ -240,6 +257,7 @@ class Plan:
         T = np.zeros(q * self.n_shells)
         Pi = np.zeros(q * self.n_shells)
         codes = (ctypes.c_int * q)(*[inv.code for inv in invariants])
+        self._order(kind, ck)
         _lib.check(self.lib.fit_spectral_fluxes_with_nhat(self.handle, q, codes, cp, npp, _dptr(T),
                                                           _dptr(Pi), self.io, kind))
         n = self.n_shells
@@ -251,6 +269,7 @@ class Plan:
         T = np.zeros((n, n), order="F")
         net = np.zeros(n)
         asym = np.zeros(1)
+        self._order(kind, keep[0])
         _lib.check(self.lib.fit_shell_to_shell(self.handle, invariant.code, fp, ap, _dptr(T),
                                                _dptr(net), _dptr(asym), int(bool(verify)), self.io,
                                                kind))
@@ -261,6 +280,7 @@ class Plan:
         like = keep[0] if kind == _lib.PTR_DEVICE else None
         S, sp = _alloc(like, self.ns + self.ns, self.io, complex_=False)
         net, npp = _alloc(like, self.ns, self.io, complex_=False)
+        self._order(kind, keep[0])
         _lib.check(self.lib.fit_mode_to_mode(self.handle, invariant.code, fp, ap, sp, npp, self.io,
                                              kind))
         return net, S
@@ -270,22 +290,29 @@ class Plan:
         like = uk if kind == _lib.PTR_DEVICE else None
         a, ap = _alloc(like, self.ns + (self.M,), self.io)
         b, bp = _alloc(like, self.ns + (self.M,), self.io)
+        self._order(kind, uk)
         _lib.check(self.lib.fit_helical_decompose(self.handle, up, ap, bp, self.io, kind))
         return a, b
 
 
+# A plan is not safe for concurrent calls (its workspaces are shared by its launches), so cached
+# plans are per host thread, as the reference keeps one workspace per task
+# (OhMyThreadsExt.jl:46).  The caches themselves are guarded by one lock.
 _PLANS = OrderedDict()
 _MAX_PLANS = 12
+_CACHE_LOCK = threading.Lock()
 
 
 def _plan_for(ns, M, D, ks, dealiasing, precision, device, binning=None, geometry=None,
               shells=False):
-    key = (tuple(ns), M, D, tuple(np.asarray(k, dtype=np.float64).tobytes() for k in ks),
+    key = (threading.get_ident(), tuple(ns), M, D,
+           tuple(np.asarray(k, dtype=np.float64).tobytes() for k in ks),
            type(dealiasing).__name__, np.dtype(precision).name, device,
            repr(binning) if shells else None, repr(geometry) if shells else None)
-    if key in _PLANS:
-        _PLANS.move_to_end(key)
-        return _PLANS[key]
+    with _CACHE_LOCK:
+        if key in _PLANS:
+            _PLANS.move_to_end(key)
+            return _PLANS[key]
     centers = edges = None
     sidx = None
     nsh = 0
@@ -294,17 +321,23 @@ def _plan_for(ns, M, D, ks, dealiasing, precision, device, binning=None, geometr
         nsh = len(centers)
     plan = Plan(ns, M, D, ks, dealiasing, precision, sidx, nsh, device)
     plan.centers, plan.edges = centers, edges
-    _PLANS[key] = plan
-    while len(_PLANS) > _MAX_PLANS:
-        _PLANS.popitem(last=False)[1].close()
+    with _CACHE_LOCK:
+        _PLANS[key] = plan
+        while len(_PLANS) > _MAX_PLANS:
+            # not closed here: another thread may be inside a call on it; the plan is destroyed
+            # when its last reference goes away (Plan.__del__)
+            _PLANS.popitem(last=False)
     return plan
 
 
 def clear_plan_cache():
-    while _PLANS:
-        _PLANS.popitem()[1].close()
-    while _ENSEMBLES:
-        _ENSEMBLES.popitem()[1].close()
+    with _CACHE_LOCK:
+        plans = list(_PLANS.values())
+        ens = list(_ENSEMBLES.values())
+        _PLANS.clear()
+        _ENSEMBLES.clear()
+    for p in plans + ens:
+        p.close()
 
 
 def _shape_info(f_hat, ks, advecting_hat):
@@ -463,6 +496,7 @@ def calculate_band_to_band_transfer(velocity_hat, ks, *, bands, dealiasing=Orsza
     T = np.zeros((nb, nb), order="F")
     net = np.zeros(nb)
     asym = np.zeros(1)
+    plan._order(kind, keep[0])
     _lib.check(plan.lib.fit_band_to_band(plan.handle, invariant.code, _dptr(cen), nb,
                                          float(bands.logwidth), _dptr(kc), fp, ap, _dptr(T),
                                          _dptr(net), _dptr(asym), plan.io, kind))
@@ -664,6 +698,8 @@ class Ensemble:
         tot = int(self.offsets[-1])
         T, Pi = np.zeros(tot), np.zeros(tot)
         Ts, Ps = np.zeros(tot), np.zeros(tot)
+        if kind == _lib.PTR_DEVICE:
+            _lib.check(self.lib.fit_ensemble_wait_stream(self.handle, _caller_stream(keep[0])))
         _lib.check(self.lib.fit_ensemble_flux(self.handle, up, tp if self.with_scalar else None,
                                               _dptr(T), _dptr(Pi), _dptr(Ts), _dptr(Ps), self.io,
                                               kind))
@@ -717,13 +753,17 @@ def calculate_ensemble_fluxes(velocities, ks_list, scalars=None, *, binning=None
                                     for ks in ks_list),
                type(dealiasing).__name__, np.dtype(spectral.compute_dtype(ft)).name, spectral.device,
                repr(binning), repr(geometry), scalars is not None)
-        ens = _ENSEMBLES.get(key)
+        key = (threading.get_ident(),) + key
+        with _CACHE_LOCK:
+            ens = _ENSEMBLES.get(key)
         if ens is None:
             ens = Ensemble(shapes, ks_list, with_scalar=scalars is not None, binning=binning,
                            dealiasing=dealiasing, precision=spectral.compute_dtype(ft),
                            geometry=geometry, device=spectral.device)
-            _ENSEMBLES.clear()
-            _ENSEMBLES[key] = ens
+            with _CACHE_LOCK:
+                for k in [k for k in _ENSEMBLES if k[0] == key[0]]:
+                    del _ENSEMBLES[k]   # one cached ensemble per thread
+                _ENSEMBLES[key] = ens
         sc = None
         if scalars is not None:
             sc = [as_component_field(s, nd) for s in scalars]

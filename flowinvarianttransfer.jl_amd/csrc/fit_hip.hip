@@ -565,6 +565,7 @@ int fit_plan_destroy(fit_plan* p) {
   { void* q = p->dcolG; dfree(q); }
   void* dc = p->dcol; dfree(dc);
   for (auto& r : p->recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+  if (p->order_ev) hipEventDestroy(p->order_ev);
   if (p->stream && p->own_stream) hipStreamDestroy(p->stream);
   delete p;
   return FIT_OK;
@@ -613,6 +614,22 @@ int fit_plan_set_option(fit_plan* p, int option, int value) {
 }
 
 void* fit_plan_stream(const fit_plan* p) { return p ? (void*)p->stream : nullptr; }
+
+// The plan's stream waits for an event recorded on the caller's stream: everything the caller
+// queued there (the kernels that produced the inputs, pending users of recycled output memory)
+// completes before the plan's next launch.
+static int order_after(int device, hipEvent_t* ev, hipStream_t own, void* stream) {
+  FIT_HIP(hipSetDevice(device));
+  if (!*ev) FIT_HIP(hipEventCreateWithFlags(ev, hipEventDisableTiming));
+  FIT_HIP(hipEventRecord(*ev, (hipStream_t)stream));
+  FIT_HIP(hipStreamWaitEvent(own, *ev, 0));
+  return FIT_OK;
+}
+
+int fit_plan_wait_stream(fit_plan* p, void* stream) {
+  if (!p) return fail(FIT_E_ARGUMENT, "null plan");
+  return order_after(p->device, &p->order_ev, p->stream, stream);
+}
 
 int fit_profile_enable(fit_plan* p, int on) {
   if (!p) return fail(FIT_E_ARGUMENT, "null plan");
@@ -781,7 +798,6 @@ int fit_mode_to_mode(fit_plan* p, int inv, const void* f_hat, const void* adv_ha
   if (p->dealias == FIT_DEALIAS_PADDED)
     return fail(FIT_E_UNSUPPORTED, "mode-to-mode supports NoDealiasing and OrszagTwoThirds");
   if ((rc = check_invariant(p, inv, p->M))) return rc;
-  if (p->M > 3 && (inv == FIT_HELICITY || inv == FIT_ENSTROPHY)) return fail(FIT_E_DIMENSION, "M > 3");
   const void *f, *a;
   if ((rc = stage_fa(p, f_hat, adv_hat, io_dtype, ptr_kind, &f, &a))) return rc;
   rc = p->dtype == FIT_F64 ? m2m_impl<double>(p, inv, f, a, S, net, io_dtype, ptr_kind)
@@ -976,6 +992,7 @@ int fit_ensemble_destroy(fit_ensemble* e) {
     { void* q = G.rows; dfree(q); }
     fit_plan_destroy(G.p);
   }
+  if (e->order_ev) hipEventDestroy(e->order_ev);
   if (e->stream) hipStreamDestroy(e->stream);
   delete e;
   return FIT_OK;
@@ -1130,5 +1147,10 @@ int fit_ensemble_profile_query(const fit_ensemble* e, int cls, int64_t* launches
 }
 
 void* fit_ensemble_stream(const fit_ensemble* e) { return e ? (void*)e->stream : nullptr; }
+
+int fit_ensemble_wait_stream(fit_ensemble* e, void* stream) {
+  if (!e) return fail(FIT_E_ARGUMENT, "null ensemble");
+  return order_after(e->device, &e->order_ev, e->stream, stream);
+}
 
 }  // extern "C"

@@ -1197,16 +1197,25 @@ __global__ void k_carrier_half(Grid g, int inv, int M, int trunc, const cpx<R>* 
     if (ix > xb || !by.has(iy) || !bz.has(iz)) continue;
     const int mx = ix == 0 ? 0 : g.n1 - ix, my = iy == 0 ? 0 : g.n2 - iy, mz = iz == 0 ? 0 : g.n3 - iz;
     const bool keep = !(trunc && g.dealiased(ix, iy, iz));
-    cpx<R> ep[3], em[3];
     const int64_t op = g.oidx(ix, iy, iz), om = g.oidx(mx, my, mz);
+    const cpx<R> z0 = mk<R>((R)0, (R)0);
+    if (inv == 0 || inv == 3) {  // kinetic energy / scalar: every component is its own carrier
+      for (int c = 0; c < M; ++c) {
+        const cpx<R> a = f[op + c * fsf], b = f[om + c * fsf];
+        eh[hidx + c * ehsf] = keep ? mk<R>((R)0.5 * (a.x + b.x), (R)0.5 * (a.y - b.y)) : z0;
+      }
+      continue;
+    }
+    // vorticity invariants: the carrier has 3 components (2 for 2D enstrophy); the density
+    // ignores the others (Invariants.jl:61-104), so their carrier is zero
+    const int ne = (inv == 1 || g.nd == 3) ? 3 : 2;
+    cpx<R> ep[3], em[3];
     for (int side = 0; side < 2; ++side) {
       const int64_t o = side == 0 ? op : om;
       const int jx = side == 0 ? ix : mx, jy = side == 0 ? iy : my, jz = side == 0 ? iz : mz;
       cpx<R>* e = side == 0 ? ep : em;
       const R kx = ks0[jx], ky = g.nd >= 2 ? ks1[jy] : (R)0, kz = g.nd >= 3 ? ks2[jz] : (R)0;
-      if (inv == 0 || inv == 3) {
-        for (int c = 0; c < M && c < 3; ++c) e[c] = f[o + c * fsf];
-      } else if (inv == 1 || g.nd == 3) {
+      if (ne == 3) {
         const cpx<R> ux = f[o], uy = f[o + fsf], uz = f[o + 2 * fsf];
         cpx<R> wx = cmuli<1>(csub(cscale(uz, ky), cscale(uy, kz)));
         cpx<R> wy = cmuli<1>(csub(cscale(ux, kz), cscale(uz, kx)));
@@ -1223,11 +1232,12 @@ __global__ void k_carrier_half(Grid g, int inv, int M, int trunc, const cpx<R>* 
         const cpx<R> w = cmuli<1>(csub(cscale(uy, kx), cscale(ux, ky)));
         e[0] = cmuli<1>(cscale(w, ky));
         e[1] = cmuli<-1>(cscale(w, kx));
+        e[2] = z0;
       }
     }
-    for (int c = 0; c < M && c < 3; ++c) {
-      cpx<R> v = mk<R>((R)0.5 * (ep[c].x + em[c].x), (R)0.5 * (ep[c].y - em[c].y));
-      if (!keep) v = mk<R>((R)0, (R)0);
+    for (int c = 0; c < M; ++c) {
+      cpx<R> v = z0;
+      if (keep && c < ne) v = mk<R>((R)0.5 * (ep[c].x + em[c].x), (R)0.5 * (ep[c].y - em[c].y));
       eh[hidx + c * ehsf] = v;
     }
   }
@@ -1493,6 +1503,11 @@ __global__ void k_m2m_prep(Grid g, int trunc, int inv, int M, const cpx<R>* __re
       const cpx<R> w = cmuli<1>(csub(cscale(uy, kx), cscale(ux, ky)));        // ω̂ = i(kx uy - ky ux)
       eout[o] = cmuli<1>(cscale(w, ky));                                        // e_x =  i ky ω̂
       eout[o + g.Np] = cmuli<-1>(cscale(w, kx));                                // e_y = -i kx ω̂
+    }
+    // vorticity invariants use 3 carrier components (2 for 2D enstrophy): zero the rest
+    if (inv == 1 || inv == 2) {
+      const int ne = (inv == 1 || g.nd == 3) ? 3 : 2;
+      for (int c = ne; c < M; ++c) eout[o + c * g.Np] = mk<R>((R)0, (R)0);
     }
   }
 }

@@ -254,6 +254,15 @@ int fit_profile_query(const fit_plan* plan, int kernel_class, int64_t* launches,
 /* The HIP stream the plan launches on (as void*: hipStream_t). */
 void* fit_plan_stream(const fit_plan* plan);
 
+/* Stream ordering for device arrays (no reference counterpart: the reference's CPU and
+ * KernelAbstractions paths run on the caller's own task or queue).  A plan launches on its own
+ * stream; before a call that reads device arrays produced on another stream (the caller's
+ * ROCArray / torch stream), call this with that stream (a hipStream_t, NULL = the null stream):
+ * everything already queued on it finishes before the plan's next launch.  Calls return after
+ * the plan's stream has finished, so results may be used on any stream afterwards. */
+int fit_plan_wait_stream(fit_plan* plan, void* stream);
+int fit_ensemble_wait_stream(fit_ensemble* ens, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -108,3 +108,61 @@ def test_oracle_reproduces_fixture(oracle, name):
     if "Tnm" in z:
         Tm = oracle.calculate_shell_to_shell_transfer(f, ks, b, dea, invariant=inv, advecting_hat=adv)[2]
         assert rel(Tm, z["Tnm"]) < 1e-12
+
+
+def _padded_brute_force(u, ks, nyq_freq_sign):
+    """Exact 3/2-padded nonlinear term of a 2D field by integer-wavevector convolution, with no
+    FFT: every original index i is a frequency f(i) in [-n/2, n/2) (the fftshift embedding,
+    FFTWExt.jl:134-141, puts index n/2 at -n/2; nyq_freq_sign=+1 plants the mistake of putting it
+    at +n/2).  The derivative factor is i·ks[j][i] at the original index (Utils.jl:59: +n/2 at
+    the Nyquist index).  real() of the padded inverse transform is the Hermitian part
+    H(q) = (E(q) + conj E(-q))/2 of the embedded spectrum E; the product's transform, truncated
+    back to [-n/2, n/2) and scaled by Mtot/Ntot² (FFTWExt.jl:144-149, :167), is
+    N̂(k) = Ntot^-2 Σ_{p+q≡k (mod Mp)} Σ_j Hu_j(q) Hg_jc(p) on the padded length Mp = 12: with
+    Nyquist content |p+q| reaches n, which wraps onto the kept bin -n/2."""
+    n = u.shape[0]
+    M = u.shape[2]
+    h = n // 2
+    Mp = 12 if n == 8 else None      # FFTWExt.jl:128-131 for n = 8 (3n/2, even excess)
+    assert Mp is not None
+
+    def freq(i):
+        f = i if i < h else i - n
+        return -f if (i == h and nyq_freq_sign > 0) else f
+
+    def herm(E):            # dict freq -> value  ->  Hermitian part on [-n/2, n/2]
+        H = {}
+        for q, v in E.items():
+            H[q] = H.get(q, 0) + 0.5 * v
+            mq = (-q[0], -q[1])
+            H[mq] = H.get(mq, 0) + 0.5 * np.conj(v)
+        return H
+
+    Hu = [herm({(freq(a), freq(b)): u[a, b, j] for a in range(n) for b in range(n)}) for j in range(2)]
+    out = np.zeros_like(u)
+    for c in range(M):
+        for j in range(2):
+            kj = ks[j]
+            Hg = herm({(freq(a), freq(b)): 1j * (kj[a] if j == 0 else kj[b]) * u[a, b, c]
+                       for a in range(n) for b in range(n)})
+            for q, hu in Hu[j].items():
+                for p, hg in Hg.items():
+                    k = tuple((p[d] + q[d] + Mp // 2) % Mp - Mp // 2 for d in range(2))
+                    if -h <= k[0] < h and -h <= k[1] < h:
+                        out[k[0] % n, k[1] % n, c] += hu * hg
+    return out / float(n * n) ** 2
+
+
+def test_padded_nyquist_fixture_by_brute_force(oracle):
+    """ke2d_8_padded_nyquist (written by the oracle's FFT restatement) equals the FFT-free
+    integer convolution with the reference's Nyquist placement, and the planted mistake (Nyquist
+    bin at +n/2) is told apart."""
+    g = load("ke2d_8_padded_nyquist")
+    u = g["u_hat"]
+    ks = ks_of(oracle, u.shape[:2])
+    assert np.max(np.abs(u[4, :, :])) > 1e-3 and np.max(np.abs(u[:, 4, :])) > 1e-3  # Nyquist populated
+    bf = _padded_brute_force(u, ks, -1)
+    assert rel(g["N_hat"], bf) < 1e-12
+    assert rel(oracle.compute_nonlinear_term(u, ks, "pad"), bf) < 1e-12
+    wrong = _padded_brute_force(u, ks, +1)
+    assert rel(wrong, bf) > 1e-3

@@ -6,8 +6,12 @@ an FP64 device run of the same kernels (itself pinned to the oracle on small gri
       T(n,m), and columns against the oracle's closed form
   C5' 256³ helical 8-channel partial fluxes: the channels sum to the total flux
   C4  an uneven 2D ensemble (scalar + kinetic-energy fluxes, results in input order)
+  C1  64² FP64 Π(K) against the oracle
+  C2  256³ FP32 Π(K) and T(n,m) columns against the FP64 oracle
+  C3  512³ KE + helicity Π(K) with 3/2 padding and the 48-shell T(n,m)
 """
 import math
+import os
 import time
 
 import numpy as np
@@ -219,3 +223,112 @@ def test_c4_uneven_ensemble(fit, oracle):
         assert nerr(out[j][0].flux, Pk) < 1e-5 and nerr(out[j][1].flux, Ps) < 1e-5
         checked += 1
     assert checked == 3
+
+
+def test_c1_64_2d_fp64_flux(fit, oracle):
+    """C1: 2D 64² FP64 kinetic-energy Π(K), LinearBinning(1) (46 shells), OrszagTwoThirds, through
+    the HIP path against the oracle."""
+    n = 64
+    say("case C1", n)
+    u = oracle.synthetic_field((n, n), 2, seed=1)
+    ks = fit.wavenumber_grid((n, n), (2 * math.pi,) * 2)
+    c, T, Pi = oracle.calculate_spectral_flux(u, ks, oracle.LinearBinning(1.0), "23")
+    r = fit.calculate_spectral_flux(u, ks, binning=fit.LinearBinning(1.0))
+    assert len(r.flux) == 46 and np.allclose(r.k_shells, c)
+    eT, eP = nerr(r.transfer_spectrum, T), nerr(r.flux, Pi)
+    say("  T err", eT, "Π err", eP)
+    assert eT < 1e-11 and eP < 1e-11
+    assert abs(T.sum()) < 1e-10 * np.abs(T).sum()
+
+
+def test_c2_256_against_oracle(fit, oracle):
+    """C2 against the FP64 oracle (not only an FP64 run of the same kernels): FP32 Π(K), T(k) on
+    the device vs the oracle's c2c restatement at 256³, and FP32 / FP64 T(n,m) giver columns
+    1, 2, 16 and 32 of the 32-shell matrix vs the oracle's giver loop for those columns (the full
+    32-column oracle loop is ~400 transforms of 256³ on the host)."""
+    import torch
+    n = 256
+    say("case C2 vs oracle", n)
+    ud, u = device_field(oracle, (n,) * 3, 3, seed=2)
+    ks = fit.wavenumber_grid((n,) * 3, (2 * math.pi,) * 3)
+    u32 = ud.to(torch.complex64)
+    r32 = fit.calculate_spectral_flux(u32, ks, binning=fit.LinearBinning(1.0))
+    s32 = fit.calculate_shell_to_shell_transfer(u32, ks, binning=fit.CustomBinning(range(0, 33)))
+    s64 = fit.calculate_shell_to_shell_transfer(ud, ks, binning=fit.CustomBinning(range(0, 33)))
+    t0 = time.perf_counter()
+    workers = min(16, os.cpu_count() or 1)
+    _, T, Pi = oracle.calculate_spectral_flux(u, ks, oracle.LinearBinning(1.0), "23", workers=workers)
+    say("  oracle Π s:", time.perf_counter() - t0)
+    eT, eP = nerr(r32.transfer_spectrum, T), nerr(r32.flux, Pi)
+    say("  FP32 vs oracle: T", eT, "Π", eP)
+    assert eT < 1e-5 and eP < 1e-6
+    t0 = time.perf_counter()
+    kc = oracle.shell_coordinate(None, ks)
+    sidx = oracle.assign_shells_fast(kc, np.arange(0.0, 33.0))
+    cols = [1, 2, 16, 32]
+    Tc = oracle.shell_to_shell_columns(u, ks, sidx, 32, cols, "23", "ke", workers=workers)
+    say("  oracle columns s:", time.perf_counter() - t0)
+    got32 = s32.transfer_matrix[:, [m - 1 for m in cols]]
+    got64 = s64.transfer_matrix[:, [m - 1 for m in cols]]
+    e32, e64 = nerr(got32, Tc), nerr(got64, Tc)
+    say("  T(n,m) columns vs oracle: FP32", e32, "FP64", e64)
+    assert e32 < 1e-5 and e64 < 1e-11
+
+
+def test_c3_512_full_size(fit, oracle):
+    """C3 at full size: 512³ seeded solenoidal k⁴exp(-2(k/8)²) / k^-5/3 field (zero at |k| >= n/3).
+    Π(K) of kinetic energy and helicity with PaddedThreeHalves (768³ transforms, 444 shells, one
+    shared N̂): FP32 vs FP64 ≤ 1e-6, Σ_k T ≈ 0 for both invariants (FP64).  48-shell T(n,m)
+    (OrszagTwoThirds): FP64 antisymmetry < 1e-12 ‖T‖, the band-limited sub-grid (FP64) against the
+    plain full-grid giver loop (FP64) ≤ 1e-12, FP32 vs FP64 ≤ 1e-5."""
+    import torch
+    import bench
+    from fit_amd.grid import shell_setup
+    n, ng = 512, 48
+    say("case C3", n)
+    # the generator projects in FP32 (divergence ~1e-7 relative, which breaks the antisymmetry of
+    # T(n,m) at ~1e-8): project again in FP64, so the FP64 checks see a solenoidal field
+    u64 = bench.synthetic_velocity_gpu(n, 3, torch).to(torch.complex128)
+    i = torch.arange(n, device="cuda")
+    k1 = torch.where(i <= n // 2, i, i - n).to(torch.float64)
+    kx, ky, kz = k1.view(n, 1, 1), k1.view(1, n, 1), k1.view(1, 1, n)    # (x, y, z) view axes
+    k2 = kx * kx + ky * ky + kz * kz
+    k2 = torch.where(k2 > 0, k2, torch.ones_like(k2))
+    div = kx * u64[..., 0] + ky * u64[..., 1] + kz * u64[..., 2]
+    for j, kj in enumerate((kx, ky, kz)):
+        u64[..., j] -= kj * div / k2
+    del div, k2
+    u = u64.to(torch.complex64)
+    ks = fit.wavenumber_grid((n,) * 3, (2 * math.pi,) * 3)
+    c_pi, _, s_pi = shell_setup(ks, fit.LinearBinning(1.0), None)
+    c_tm, _, s_tm = shell_setup(ks, fit.CustomBinning(range(0, ng + 1)), None)
+    assert len(c_pi) == 444 and len(c_tm) == ng
+    KE, HEL = fit.KineticEnergy(), fit.Helicity()
+    res = {}
+    for prec, uu in ((np.float32, u), (np.float64, u64)):
+        with fit.Plan((n,) * 3, 3, 3, ks, fit.PaddedThreeHalves(), prec, s_pi, len(c_pi)) as pp:
+            N = pp.nonlinear_term(uu)
+            (Tk, Pk), (Th, Ph) = pp.spectral_fluxes_with_nhat((KE, HEL), uu, N)
+            del N
+        with fit.Plan((n,) * 3, 3, 3, ks, fit.OrszagTwoThirds(), prec, s_tm, len(c_tm)) as tp:
+            assert tp.subgrid is not None
+            Tm, net, asym = tp.shell_to_shell(KE, uu)
+            if prec == np.float64:
+                tp.set_subgrid(False)
+                tp.set_plain(True)
+                Tm_plain, _, _ = tp.shell_to_shell(KE, uu)
+                res["plain"] = Tm_plain
+        res[prec] = (Tk, Pk, Th, Ph, Tm, asym)
+        say("  ", prec.__name__, "done")
+    Tk, Pk, Th, Ph, Tm, asym = res[np.float64]
+    Tk32, Pk32, Th32, Ph32, Tm32, _ = res[np.float32]
+    e = {"pi_ke": nerr(Pk32, Pk), "pi_hel": nerr(Ph32, Ph), "t_ke": nerr(Tk32, Tk), "t_nm": nerr(Tm32, Tm),
+         "sub_vs_plain": nerr(Tm, res["plain"])}
+    say("  errors", e)
+    assert e["pi_ke"] <= 1e-6 and e["pi_hel"] <= 1e-6 and e["t_nm"] < 1e-5
+    assert e["sub_vs_plain"] <= 1e-12
+    for T in (Tk, Th):
+        assert abs(T.sum()) < 1e-9 * np.abs(T).sum()
+    nrm = math.sqrt(np.sum(Tm ** 2))
+    say("  antisymmetry", asym / nrm)
+    assert asym < 1e-12 * nrm
