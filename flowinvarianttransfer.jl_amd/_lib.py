@@ -40,6 +40,7 @@ SIGNATURES = {
     "fit_spectral_flux": (_I, [_P, _I, _P, _P, _D, _D, _I, _I]),
     "fit_spectral_flux_with_nhat": (_I, [_P, _I, _P, _P, _D, _D, _I, _I]),
     "fit_spectral_fluxes_with_nhat": (_I, [_P, _I, ctypes.POINTER(_I), _P, _P, _D, _D, _I, _I]),
+    "fit_spectral_flux_with_nhat_noalloc": (_I, [_P, _I, _P, _P, _P, _P, _I, _I]),
     "fit_shell_to_shell": (_I, [_P, _I, _P, _P, _D, _D, _D, _I, _I, _I]),
     "fit_mode_to_mode": (_I, [_P, _I, _P, _P, _P, _P, _I, _I]),
     "fit_helical_decompose": (_I, [_P, _P, _P, _P, _I, _I]),

@@ -6,6 +6,13 @@
 // and their kernels) is compiled per precision in fit_f32.hip / fit_f64.hip.
 #include "fit_internal.h"
 
+// Device scratch of fit_spectral_flux_with_nhat_noalloc: the shell partial rows and their fixed-
+// order sums.  One copy per GPU, held until the process ends; every call on a GPU shares it.  In
+// host code the array's name is only a stand-in: kernels get the per-device address found by
+// hipGetSymbolAddress (scratch_for below).
+constexpr size_t kScratchDoubles = FIT_NOALLOC_SCRATCH_BYTES / sizeof(double);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace fitx {
 FIT_PIPELINE_EXTERN(float)
 FIT_PIPELINE_EXTERN(double)
@@ -529,6 +536,39 @@ void ens_drop_graph(fit_ensemble* e) {
 
 }  // namespace
 
+// ---- allocation-free spectral flux tail --------------------------------------------------------
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_dev[kMaxDevices] = {nullptr};
+
+// address of `scratch` on the current device (looked up once per device, with it current)
+int scratch_for(int device, double** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (device < 0 || device >= kMaxDevices) return fail(FIT_E_ARGUMENT, "device index out of range");
+  if (!g_scratch_dev[device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(FIT_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[device] = ptr;
+  }
+  *out = (double*)g_scratch_dev[device];
+  return FIT_OK;
+}
+
+// T (copy) and Π = +cumsum(T) (SpectralFlux.jl:172-177) into device arrays, in shell order
+static __global__ void k_cumsum_shells(const double* __restrict__ sums, int nsh, double* __restrict__ T,
+                                       double* __restrict__ Pi) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double acc = 0.0;
+  for (int s = 0; s < nsh; ++s) {
+    const double v = sums[s];
+    acc += v;
+    T[s] = v;
+    Pi[s] = acc;
+  }
+}
+
 }  // namespace fitx
 
 using namespace fitx;
@@ -753,6 +793,36 @@ int fit_spectral_fluxes_with_nhat(fit_plan* p, int n_inv, const int* invariants,
       if (Pi) Pi[(size_t)q * p->nsh + s] = acc;
     }
   }
+  return FIT_OK;
+}
+
+int fit_spectral_flux_with_nhat_noalloc(fit_plan* p, int inv, const void* c_hat, const void* N_hat, double* T,
+                                        double* Pi, int io_dtype, int ptr_kind) {
+  int rc = check_plan(p);  // makes the plan's device current
+  if (rc) return rc;
+  if (p->nsh <= 0) return fail(FIT_E_ARGUMENT, "plan has no shells");
+  if (p->nsh > FIT_NOALLOC_MAX_SHELLS)
+    return fail(FIT_E_UNSUPPORTED, "fit_spectral_flux_with_nhat_noalloc: " + std::to_string(p->nsh) +
+                                       " shells exceed its maximum of " + std::to_string(FIT_NOALLOC_MAX_SHELLS));
+  if ((rc = check_invariant(p, inv, p->M))) return rc;
+  if (!c_hat || !N_hat || !T || !Pi) return fail(FIT_E_ARGUMENT, "NULL array");
+  if (ptr_kind != FIT_PTR_DEVICE || io_dtype != p->dtype)
+    return fail(FIT_E_ARGUMENT, "fit_spectral_flux_with_nhat_noalloc takes device arrays in the plan's compute "
+                                "precision (staging them would allocate)");
+  double* scr = nullptr;
+  if ((rc = scratch_for(p->device, &scr))) return rc;
+  const int nsh = p->nsh;
+  // partial rows fill the scratch up to the space the sums need; every row is written in full
+  // by each call, so nothing has to start at zero
+  const int nblk = (int)std::min<size_t>(1024, (kScratchDoubles - (size_t)nsh) / (size_t)nsh);
+  double* partials = scr;
+  double* sums = scr + (size_t)nblk * nsh;
+  rc = p->dtype == FIT_F64 ? density_shells_into<double>(p, inv, c_hat, N_hat, partials, nblk, sums)
+                           : density_shells_into<float>(p, inv, c_hat, N_hat, partials, nblk, sums);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_cumsum_shells, dim3(1), dim3(64), 0, p->stream, (const double*)sums, nsh, T, Pi);
+  FIT_HIP(hipGetLastError());
+  FIT_HIP(hipStreamSynchronize(p->stream));
   return FIT_OK;
 }
 

@@ -249,6 +249,8 @@ template <typename R> int nonlinear_term_impl(fit_plan* p, const void* f, const 
 template <typename R> int density_launch(fit_plan* p, int inv, const void* car, const void* nh, void* t_out,
                                          bool shells);
 template <typename R> int density_multi(fit_plan* p, int ninv, const int* invs, const void* car, const void* nh);
+template <typename R> int density_shells_into(fit_plan* p, int inv, const void* car, const void* nh, double* partials,
+                                              int nblk, double* out);
 template <typename R> int s2s_entry(fit_plan* p, int inv, const void* f, const void* a, std::vector<double>& Tm);
 template <typename R> int b2b_impl(fit_plan* p, int inv, const void* f, const void* a, const R* kc,
                                    const double* cen, int nb, double inv2s2, std::vector<double>& Tm);
@@ -260,6 +262,7 @@ template <typename R> int ens_run(fit_ensemble* e);
   extern template int nonlinear_term_impl<R>(fit_plan*, const void*, const void*);                              \
   extern template int density_launch<R>(fit_plan*, int, const void*, const void*, void*, bool);                 \
   extern template int density_multi<R>(fit_plan*, int, const int*, const void*, const void*);                   \
+  extern template int density_shells_into<R>(fit_plan*, int, const void*, const void*, double*, int, double*);   \
   extern template int s2s_entry<R>(fit_plan*, int, const void*, const void*, std::vector<double>&);            \
   extern template int b2b_impl<R>(fit_plan*, int, const void*, const void*, const R*, const double*, int, double, \
                                   std::vector<double>&);                                                        \

@@ -534,6 +534,23 @@ int density_launch(fit_plan* p, int inv, const void* car, const void* nh, void* 
   return FIT_OK;
 }
 
+// density + shell partial sums into buffers the caller provides (nothing grown or allocated):
+// nblk partial rows of nsh doubles at partials, reduced in fixed order into out (nsh doubles)
+template <typename R>
+int density_shells_into(fit_plan* p, int inv, const void* car, const void* nh, double* partials, int nblk,
+                        double* out) {
+  Grid g = grid_of<R>(p);
+  const int nsh = p->nsh;
+  hipLaunchKernelGGL(k_density<R>, dim3(nblk), dim3(kThreads), (size_t)nsh * sizeof(double), p->stream, g,
+                     int4{inv, 0, 0, 0}, 1, p->M, (const cpx<R>*)car, p->Np, (const cpx<R>*)nh, p->Np,
+                     (const R*)p->ks[0], (const R*)p->ks[1], (const R*)p->ks[2], p->shell, nsh, partials,
+                     (R*)nullptr);
+  FIT_HIP(hipGetLastError());
+  launch_reduce(p, partials, nblk, nsh, 1, out);
+  FIT_HIP(hipGetLastError());
+  return FIT_OK;
+}
+
 // shell sums of ninv (<= 4) invariants in one density pass over carrier and N̂: rows of nsh
 // doubles, invariant q at dcolG + q*nsh
 template <typename R>

@@ -142,6 +142,23 @@ int fit_spectral_fluxes_with_nhat(fit_plan* plan, int n_inv, const int* invarian
                                   const void* N_hat, double* T, double* Pi, int io_dtype, int ptr_kind);
 
 /*
+ * Allocation-free variant of fit_spectral_flux_with_nhat.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It calls no allocating routine: c_hat and N_hat must
+ * be DEVICE arrays in the plan's compute precision (io_dtype == compute dtype, ptr_kind ==
+ * FIT_PTR_DEVICE; anything else is rejected, since staging would allocate), and T and Pi are
+ * DEVICE arrays of n_shells doubles.  The shell partial sums live in a static device buffer of
+ * FIT_NOALLOC_SCRATCH_BYTES that the library holds on each GPU until the process ends; plans
+ * with more than FIT_NOALLOC_MAX_SHELLS shells are rejected (FIT_E_UNSUPPORTED).  Every call on
+ * a GPU shares that buffer: calls that overlap on one GPU (several host threads, several plans)
+ * race.  Returns after the plan's stream has finished.
+ */
+#define FIT_NOALLOC_SCRATCH_BYTES 4194304
+#define FIT_NOALLOC_MAX_SHELLS 4096
+int fit_spectral_flux_with_nhat_noalloc(fit_plan* plan, int invariant, const void* c_hat, const void* N_hat,
+                                        double* T, double* Pi, int io_dtype, int ptr_kind);
+
+/*
  * Shell-to-shell matrix T(n,m) over the plan's shells (every shell is a giver).  Replaces
  * ShellToShellTransfer._calculate_shell_to_shell! (ShellToShellTransfer.jl:152-159), i.e. the
  * FFT giver loop FFTWExt._shell_to_shell_fft! (FFTWExt.jl:209-295) and the KernelAbstractions
