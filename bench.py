@@ -141,7 +141,11 @@ def ensemble_bench(fit, torch, steps):
     ens.profile(True)
     ens.raw_fluxes(us, ths)
     stats = ens.profile_stats()
+    kst = ens.kernel_stats()
     ens.profile(False)
+    ktab = sorted(({"kernel": k, "ms": v["ms"], "launches": v["launches"],
+                    "alg_GBs": v["alg_bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0}
+                   for k, v in kst.items()), key=lambda r: -r["ms"])
     # dense algorithmic bytes (each logical transform reads its input and writes its output once:
     # 4·Np + 8·H per transform, 11 transforms per snapshot, plus the inputs and the densities)
     alg = 0.0
@@ -156,15 +160,15 @@ def ensemble_bench(fit, torch, steps):
             "value": steps / elapsed, "unit": "ensembles/s", "ms_per_ensemble": elapsed * 1e3 / steps,
             "groups": 4, "setup_s": t_setup,
             "dense_alg_bytes": alg, "dense_alg_GBs": alg / (elapsed / steps) / 1e9,
-            "eager_device_ms": dev_ms, "kernel_classes": stats}
+            "eager_device_ms": dev_ms, "kernel_classes": stats, "kernels": ktab[:12]}
 
 
-def measured_traffic(plan_key, kernel_class, n, ng):
-    """HBM bytes per launch of the dominant kernel class, from the committed rocprofv3 PMC summary
-    of this workload (FETCH_SIZE + WRITE_SIZE of the class's kernels, calibrated on a 1 GiB copy),
-    or None when no summary covers it."""
+def measured_traffic(plan_key, kernel, n, ng):
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 PMC summary of
+    this workload (FETCH_SIZE + WRITE_SIZE of that kernel, calibrated on a 1 GiB copy), or None
+    when no summary covers it."""
     path = os.path.join(ROOT, "profiles", f"snapshot{n}_tnm{ng}_pmc_traffic.md")
-    key = f"traffic_bytes_per_launch[{plan_key}:{kernel_class}]"
+    key = f"traffic_bytes_per_launch[{plan_key}:{kernel}]"
     try:
         for line in open(path):
             if line.startswith(key):
@@ -280,14 +284,25 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     stats = {"pi": pi_plan.profile_stats(), "tnm": tm_plan.profile_stats()}
+    kst = {"pi": pi_plan.kernel_stats(), "tnm": tm_plan.kernel_stats()}
     pi_plan.profile(False)
     tm_plan.profile(False)
-    # dominant kernel = the (plan, kernel class) with the most HIP-event time over the timed steps
-    plan_key, dom = max(((pk, k) for pk in stats for k in stats[pk]), key=lambda pk: stats[pk[0]][pk[1]]["ms"])
-    d = stats[plan_key][dom]
+    # dominant kernel = the single kernel (label) with the most HIP-event time over the timed steps
+    plan_key, kname = max(((pk, k) for pk in kst for k in kst[pk]), key=lambda t: kst[t[0]][t[1]]["ms"])
+    d = kst[plan_key][kname]
     achieved = d["alg_bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
-    merged = stats
-    log("[bench] kernel classes: " + json.dumps(merged))
+    log("[bench] kernel classes: " + json.dumps(stats))
+    table = []
+    for pk in kst:
+        for k, v in kst[pk].items():
+            if v["launches"] == 0 or v["ms"] <= 0:
+                continue
+            gbs = v["alg_bytes"] / (v["ms"] * 1e-3) / 1e9
+            table.append({"kernel": f"{pk}:{k}", "ms_per_step": v["ms"] / args.steps,
+                          "avg_launch_ms": v["ms"] / v["launches"],
+                          "alg_GB_per_launch": v["alg_bytes"] / v["launches"] / 1e9,
+                          "alg_GBs": gbs, "frac": gbs / HBM_PEAK_GBS})
+    table.sort(key=lambda r: -r["ms_per_step"])
     ms_step = elapsed * 1e3 / args.steps
     fx = stats["tnm"]["fused_x"]
     tgrid = tm_plan.subgrid or (n, n, n)
@@ -315,11 +330,12 @@ def main():
         "config": {"workload": f"{n}^3 D=3 KE+helicity Pi(K) PaddedThreeHalves ({int(n * 1.5)}^3) "
                                f"+ {ng}-shell T(n,m) OrszagTwoThirds, one snapshot per step",
                    "grid": n, "givers": ng, "pi_shells": len(c_pi), "tnm_grid": list(tgrid)},
-        "roofline": {"bound": "hbm", "kernel": f"{plan_key}:{dom}", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": f"{plan_key}:{kname}", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(plan_key, dom, n, ng),
+                     "traffic": measured_traffic(plan_key, kname, n, ng),
                      "launches": d["launches"], "avg_launch_ms": d["ms"] / max(d["launches"], 1),
                      "alg_bytes_per_launch": d["alg_bytes"] / max(d["launches"], 1)},
+        "kernels": table[:16],
         "cpu_baseline": None,
         "fused_x_fft_tflops": fx_tflops,
     }

@@ -48,6 +48,12 @@ SIGNATURES = {
     "fit_profile_enable": (_I, [_P, _I]),
     "fit_profile_query": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int64), _D, _D]),
     "fit_plan_stream": (_P, [_P]),
+    "fit_profile_kernel_count": (_I, [_P]),
+    "fit_profile_kernel": (_I, [_P, _I, ctypes.c_char_p, _I, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_int64),
+                                _D, _D]),
+    "fit_ensemble_profile_kernel_count": (_I, [_P]),
+    "fit_ensemble_profile_kernel": (_I, [_P, _I, ctypes.c_char_p, _I, ctypes.POINTER(_I),
+                                         ctypes.POINTER(ctypes.c_int64), _D, _D]),
     "fit_plan_wait_stream": (_I, [_P, _P]),
     "fit_ensemble_wait_stream": (_I, [_P, _P]),
     "fit_ensemble_create": (_I, [ctypes.POINTER(_P), _I, _I, ctypes.POINTER(ctypes.c_int64), _I, _I,
@@ -92,6 +98,21 @@ def load_library(path=None):
     if path is None:
         _lib = lib
     return lib
+
+
+def kernel_stats(count_fn, kernel_fn, handle):
+    """{label: {"class", "launches", "ms", "alg_bytes"}} from the per-kernel profiler."""
+    out = {}
+    for i in range(count_fn(handle)):
+        name = ctypes.create_string_buffer(256)
+        cls = ctypes.c_int()
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        by = ctypes.c_double()
+        check(kernel_fn(handle, i, name, 256, ctypes.byref(cls), ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)))
+        out[name.value.decode()] = {"class": K_CLASSES[cls.value], "launches": n.value, "ms": ms.value,
+                                    "alg_bytes": by.value}
+    return out
 
 
 def check(rc):

@@ -197,6 +197,10 @@ class Plan:
             out[name] = {"launches": n.value, "ms": ms.value, "alg_bytes": by.value}
         return out
 
+    def kernel_stats(self):
+        """Per-kernel-label totals of the profiled launches (launches, ms, algorithmic bytes)."""
+        return _lib.kernel_stats(self.lib.fit_profile_kernel_count, self.lib.fit_profile_kernel, self.handle)
+
     # -- entry points --------------------------------------------------------------------------
     def _order(self, kind, t):
         """Device arrays: the plan's stream waits for torch's current stream, where the inputs
@@ -664,6 +668,10 @@ class Ensemble:
                                                            ctypes.byref(ms), ctypes.byref(by)))
             out[name] = {"launches": n.value, "ms": ms.value, "alg_bytes": by.value}
         return out
+
+    def kernel_stats(self):
+        return _lib.kernel_stats(self.lib.fit_ensemble_profile_kernel_count, self.lib.fit_ensemble_profile_kernel,
+                                 self.handle)
 
     def raw_fluxes(self, velocities, scalars=None):
         """-> (T_ke, Pi_ke, T_sc, Pi_sc) concatenated over snapshots (offsets in self.offsets)."""

@@ -62,7 +62,7 @@ void dfree(void*& ptr) {
 }
 
 // profiling brackets
-void prof_begin(fit_plan* p, int cls, double bytes) {
+void prof_begin(fit_plan* p, int cls, double bytes, const char* kname, int L, const char* extra) {
   if (!p->prof) return;
   if (p->nrec >= p->recs.size()) {
     ProfRec r;
@@ -73,6 +73,8 @@ void prof_begin(fit_plan* p, int cls, double bytes) {
   ProfRec& r = p->recs[p->nrec];
   r.cls = cls;
   r.bytes = bytes;
+  r.name = std::string(kname) + "<" + (p->rsz == 8 ? "f64" : "f32") + (L > 0 ? "," + std::to_string(L) : "") +
+           extra + ">";
   hipEventRecord(r.a, p->stream);
 }
 
@@ -93,6 +95,16 @@ void prof_collect(fit_plan* p) {
       p->launches[s->recs[i].cls] += 1;
       p->ms[s->recs[i].cls] += t;
       p->abytes[s->recs[i].cls] += s->recs[i].bytes;
+      KernAgg* k = nullptr;
+      for (KernAgg& a : p->kagg)
+        if (a.name == s->recs[i].name) { k = &a; break; }
+      if (!k) {
+        p->kagg.push_back(KernAgg{s->recs[i].name, s->recs[i].cls});
+        k = &p->kagg.back();
+      }
+      k->launches += 1;
+      k->ms += t;
+      k->bytes += s->recs[i].bytes;
     }
     s->nrec = 0;
   }
@@ -653,6 +665,29 @@ int fit_plan_set_option(fit_plan* p, int option, int value) {
   }
 }
 
+static int kagg_out(const std::vector<KernAgg>& v, int i, char* name, int name_len, int* kernel_class,
+                    int64_t* launches, double* total_ms, double* alg_bytes) {
+  if (i < 0 || i >= (int)v.size()) return fail(FIT_E_ARGUMENT, "kernel index out of range");
+  const KernAgg& a = v[(size_t)i];
+  if (name && name_len > 0) {
+    std::strncpy(name, a.name.c_str(), (size_t)name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (kernel_class) *kernel_class = a.cls;
+  if (launches) *launches = a.launches;
+  if (total_ms) *total_ms = a.ms;
+  if (alg_bytes) *alg_bytes = a.bytes;
+  return FIT_OK;
+}
+
+int fit_profile_kernel_count(const fit_plan* p) { return p ? (int)p->kagg.size() : -1; }
+
+int fit_profile_kernel(const fit_plan* p, int i, char* name, int name_len, int* kernel_class, int64_t* launches,
+                       double* total_ms, double* alg_bytes) {
+  if (!p) return fail(FIT_E_ARGUMENT, "null plan");
+  return kagg_out(p->kagg, i, name, name_len, kernel_class, launches, total_ms, alg_bytes);
+}
+
 void* fit_plan_stream(const fit_plan* p) { return p ? (void*)p->stream : nullptr; }
 
 // The plan's stream waits for an event recorded on the caller's stream: everything the caller
@@ -676,6 +711,7 @@ int fit_profile_enable(fit_plan* p, int on) {
   p->prof = on != 0;
   p->nrec = 0;
   for (int i = 0; i < FIT_K_COUNT; ++i) { p->launches[i] = 0; p->ms[i] = 0; p->abytes[i] = 0; }
+  p->kagg.clear();
   if (p->sub) return fit_profile_enable(p->sub, on);
   return FIT_OK;
 }
@@ -1214,6 +1250,30 @@ int fit_ensemble_profile_query(const fit_ensemble* e, int cls, int64_t* launches
   if (total_ms) *total_ms = ms;
   if (alg_bytes) *alg_bytes = by;
   return FIT_OK;
+}
+
+// per-kernel totals of an ensemble: its groups' plans merged by label
+static std::vector<KernAgg> ens_kagg(const fit_ensemble* e) {
+  std::vector<KernAgg> v;
+  for (const EnsGroup& G : e->groups)
+    for (const KernAgg& a : G.p->kagg) {
+      KernAgg* k = nullptr;
+      for (KernAgg& b : v)
+        if (b.name == a.name) { k = &b; break; }
+      if (!k) { v.push_back(KernAgg{a.name, a.cls}); k = &v.back(); }
+      k->launches += a.launches;
+      k->ms += a.ms;
+      k->bytes += a.bytes;
+    }
+  return v;
+}
+
+int fit_ensemble_profile_kernel_count(const fit_ensemble* e) { return e ? (int)ens_kagg(e).size() : -1; }
+
+int fit_ensemble_profile_kernel(const fit_ensemble* e, int i, char* name, int name_len, int* kernel_class,
+                                int64_t* launches, double* total_ms, double* alg_bytes) {
+  if (!e) return fail(FIT_E_ARGUMENT, "null ensemble");
+  return kagg_out(ens_kagg(e), i, name, name_len, kernel_class, launches, total_ms, alg_bytes);
 }
 
 void* fit_ensemble_stream(const fit_ensemble* e) { return e ? (void*)e->stream : nullptr; }

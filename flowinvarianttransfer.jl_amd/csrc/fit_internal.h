@@ -42,6 +42,15 @@ struct ProfRec {
   hipEvent_t a = nullptr, b = nullptr;
   int cls = 0;
   double bytes = 0;
+  std::string name;  // kernel label, e.g. "k_grad2_r<f32,768>"
+};
+
+// per-kernel-label totals of the profiled launches
+struct KernAgg {
+  std::string name;
+  int cls = 0;
+  int64_t launches = 0;
+  double ms = 0, bytes = 0;
 };
 
 }  // namespace fitx
@@ -117,6 +126,7 @@ struct fit_plan {
   int64_t launches[FIT_K_COUNT] = {0};
   double ms[FIT_K_COUNT] = {0};
   double abytes[FIT_K_COUNT] = {0};
+  std::vector<fitx::KernAgg> kagg;
 };
 
 namespace fitx {
@@ -157,7 +167,8 @@ namespace fitx {
 // ---- host helpers (fit_hip.hip) -------------------------------------------------------------
 int dalloc(fit_plan* p, void** ptr, size_t bytes);
 void dfree(void*& ptr);
-void prof_begin(fit_plan* p, int cls, double bytes);
+// kname<prec,L extra>: the label a launch is aggregated under (fit_profile_kernel)
+void prof_begin(fit_plan* p, int cls, double bytes, const char* kname, int L = 0, const char* extra = "");
 void prof_end(fit_plan* p);
 void prof_collect(fit_plan* p);
 int grow(fit_plan* p, void** buf, size_t* have, size_t need);
@@ -234,6 +245,17 @@ template <typename R> Grid grid_of(const fit_plan* p) {
   g.Np = p->Np;
   return g;
 }
+
+// loader / storer tags in kernel labels
+template <class T> struct KTag { static constexpr const char* v = ""; };
+template <typename R> struct KTag<LdSpectral<R>> { static constexpr const char* v = ",LdSpectral"; };
+template <typename R> struct KTag<LdSpectralB<R>> { static constexpr const char* v = ",LdSpectralB"; };
+template <typename R> struct KTag<LdHalf<R>> { static constexpr const char* v = ",LdHalf"; };
+template <typename R> struct KTag<LdReal<R>> { static constexpr const char* v = ",LdReal"; };
+template <typename R> struct KTag<StHalf<R>> { static constexpr const char* v = ",StHalf"; };
+template <typename R> struct KTag<StReal<R>> { static constexpr const char* v = ",StReal"; };
+template <typename R> struct KTag<StProduct<R>> { static constexpr const char* v = ",StProduct"; };
+template <typename R> struct KTag<StNhat<R>> { static constexpr const char* v = ",StNhat"; };
 
 inline size_t csize(int dtype) { return dtype == FIT_F64 ? 16 : 8; }
 

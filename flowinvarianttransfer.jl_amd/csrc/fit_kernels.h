@@ -50,6 +50,15 @@ __device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz) {
   bz = (int)(l / (gx * gy));
 }
 
+// One-dimensional form of xcd_tile: the logical workgroup index such that each XCD runs one
+// contiguous range of logical indices, in order.
+__device__ __forceinline__ int xcd_linear() {
+  constexpr unsigned X = 8;
+  const unsigned n = gridDim.x, b = blockIdx.x;
+  const unsigned q = n / X, r = n % X, xcd = b % X, loc = b / X;
+  return (int)(xcd < r ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc);
+}
+
 // Folded-index band {i : min(i, G - i) <= a} of one axis (all of it when 2a+1 >= G); used to
 // prune FFT passes to the spectral support of their input and the modes their output feeds.
 struct Band {
@@ -658,8 +667,10 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
   cpx<R>* lds0 = reinterpret_cast<cpx<R>*>(smem_raw);
   const int line = threadIdx.x / T, t = threadIdx.x % T;
   const int nchunk = (nfg + GCM - 1) / GCM;
-  const int chunk = blockIdx.x % nchunk;
-  const int pair = (blockIdx.x / nchunk) * B + line;
+  // the chunks of one row block read the same velocity rows: run them on one XCD (one L2)
+  const int blk = xcd_linear();
+  const int chunk = blk % nchunk;
+  const int pair = (blk / nchunk) * B + line;
   const int r0 = 2 * pair, r1 = r0 + 1;
   const bool ok0 = r0 < nrows, ok1 = r1 < nrows;
   const int64_t ro0 = ok0 ? (int64_t)(r0 % G2) * gsy + (int64_t)(r0 / G2) * gsz : 0;
@@ -985,8 +996,9 @@ __global__ void __launch_bounds__(256) k_xfused_w(const cpx<R>* __restrict__ tw,
   const int t = lane % T;
   const int line = wave * LPW + lane / T;
   const int nchunk = (ngiv + GC - 1) / GC;
-  const int chunk = blockIdx.x % nchunk;
-  const int pair = (blockIdx.x / nchunk) * (4 * LPW) + line;
+  const int blk = xcd_linear();  // chunks of one row block share velocity rows: one XCD
+  const int chunk = blk % nchunk;
+  const int pair = (blk / nchunk) * (4 * LPW) + line;
   // ensembles (ugs != 0, launched with GC = 1): the chunk's batch index has its own velocity
   u += (int64_t)(chunk * GC) * ugs;
   const int r0 = 2 * pair, r1 = r0 + 1;

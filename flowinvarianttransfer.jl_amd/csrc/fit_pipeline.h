@@ -16,7 +16,9 @@ int launch_col(fit_plan* p, LD ld, ST st, int axis, int nfields, int nx, Band ob
   if (nx <= 0 || nob <= 0 || nfields <= 0) return FIT_OK;
   const double rows_in = ib.count(), rows_out = wb.count();
   const int gother = axis == 2 ? p->G[1] : p->G[2];
-  prof_begin(p, FIT_K_FFT_COLUMN, (double)nx * nob * (rows_in + rows_out) * csz * nfields);
+  prof_begin(p, FIT_K_FFT_COLUMN, (double)nx * nob * (rows_in + rows_out) * csz * nfields,
+             (!p->plain && reg_supported(L)) ? "k_col_r" : "k_col", L,
+             (std::string(SIGN > 0 ? ",inv" : ",fwd") + KTag<LD>::v + KTag<ST>::v).c_str());
   if (!p->plain && reg_supported(L)) {
     const int T = reg_T(L);
     // columns per workgroup: 16 (128-byte rows in FP32) within 64 KB of LDS, or the plan's cap
@@ -66,7 +68,8 @@ int launch_row_inv(fit_plan* p, LD ld, ST st, int nfields, double extra_bytes_pe
   const size_t lds = (size_t)2 * L * P * csz;
   const int nin = std::min(kxa + 1, p->h1);
   prof_begin(p, FIT_K_FFT_ROW_INVERSE,
-             ((double)nin * csz + (double)L * sizeof(R) + extra_bytes_per_row) * nrows * nfields);
+             ((double)nin * csz + (double)L * sizeof(R) + extra_bytes_per_row) * nrows * nfields,
+             (!p->plain && reg_supported(L)) ? "k_row_inv_r" : "k_row_inv", L, KTag<ST>::v);
   if (!p->plain && reg_supported(L)) {
     const int T = reg_T(L);
     const size_t LL = (size_t)L + L / 16;
@@ -101,7 +104,8 @@ int launch_row_fwd(fit_plan* p, LD ld, ST st, int nfields, int nout) {
   const int npairs = (nrows + 1) / 2;
   dim3 grid((npairs + P - 1) / P, 1, nfields);
   const size_t lds = (size_t)2 * L * P * csz;
-  prof_begin(p, FIT_K_FFT_ROW_FORWARD, ((double)nout * csz + (double)L * sizeof(R)) * nrows * nfields);
+  prof_begin(p, FIT_K_FFT_ROW_FORWARD, ((double)nout * csz + (double)L * sizeof(R)) * nrows * nfields, "k_row_fwd",
+             L, KTag<ST>::v);
   hipLaunchKernelGGL((k_row_fwd<R, LD, ST>), grid, dim3(kThreads), lds, p->stream, ld, st, p->fd[0],
                      (const cpx<R>*)p->tw[0], p->G[1], nrows, P, nout);
   prof_end(p);
@@ -239,7 +243,8 @@ int grads_shared(fit_plan* p, const void* f, int giver0, int ng, const int* bi, 
     const Band wb = full_band(L);
     const int nob = ob.count();
     dim3 grid((xa + TX - 1) / TX, nob, ngc);
-    prof_begin(p, FIT_K_FFT_COLUMN, (double)ngc * xa * nob * (2.0 * ib.count() + 2.0 * wb.count()) * csz);
+    prof_begin(p, FIT_K_FFT_COLUMN, (double)ngc * xa * nob * (2.0 * ib.count() + 2.0 * wb.count()) * csz,
+               "k_grad1_r", L);
     switch (L) {
 #define FIT_CASE(LL)                                                                                         \
   case LL:                                                                                                   \
@@ -262,7 +267,8 @@ int grads_shared(fit_plan* p, const void* f, int giver0, int ng, const int* bi, 
     const bool split = p->grad_split < 0 ? L < 512 : p->grad_split != 0;
     dim3 grid((xa + TX - 1) / TX, ob.count(), ngc);
     prof_begin(p, FIT_K_FFT_COLUMN,
-               (double)ngc * xa * ob.count() * ((split ? 1.0 : 2.0) * ib.count() + (split ? 2.0 : 3.0) * wb.count()) * csz);
+               (double)ngc * xa * ob.count() * ((split ? 1.0 : 2.0) * ib.count() + (split ? 2.0 : 3.0) * wb.count()) * csz,
+               "k_grad2_r", L);
     switch (L) {
 #define FIT_CASE(LL)                                                                                         \
   case LL:                                                                                                   \
@@ -348,10 +354,10 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
   const double bytes = (double)nfg * nrows * (p->nd * (xa * csz) + xb * csz) + (double)p->nd * nrows * L * sizeof(R);
   const R* kxt = p->grads_commuted ? (const R*)p->kt[0] : nullptr;
   if (p->grads_commuted && !reg_supported(L)) return fail(FIT_E_UNSUPPORTED, "shared-transform gradients need a register x pass");
-  prof_begin(p, FIT_K_FUSED_X, bytes);
   // wave form for short register footprints (E <= 16); the workgroup form otherwise
   const bool wave_ok = reg_supported(L) && !p->xfused_block && wE(L) <= 16 &&
                        (size_t)4 * (64 / (L / wE(L))) * ((size_t)L + L / 16) * csz <= g_max_lds;
+  prof_begin(p, FIT_K_FUSED_X, bytes, wave_ok ? "k_xfused_w" : reg_supported(L) ? "k_xfused_r" : "k_xfused", L);
   if (wave_ok) {
     // wave-per-line kernel: 4 waves per workgroup, 64/T row pairs per wave
     const int T = L / wE(L);
@@ -434,7 +440,10 @@ int v2_xfused_self(fit_plan* p, const int* bi, const int* bo) {
   const int64_t gsy = pa, gsz = (int64_t)pa * p->G[1], gsf = gsz * p->G[2];
   const int64_t osy = pb, osz = (int64_t)pb * p->G[1], osf = osz * p->G[2];
   const double bytes = (double)nrows * (((double)p->nd + (double)nfg * p->nd) * xa * csz + (double)nfg * xb * csz);
-  prof_begin(p, FIT_K_FUSED_X, bytes);
+  prof_begin(p, FIT_K_FUSED_X, bytes,
+             (std::is_same<R, float>::value && p->xstage && (L == 512 || L == 768)) ? "k_xfused_self_stg"
+                                                                                     : "k_xfused_self",
+             L);
   if constexpr (std::is_same<R, float>::value) {
     // one line pair per wave (T = 64): the LDS-staged form, 4 pairs per workgroup
     if (p->xstage && (L == 512 || L == 768)) {
@@ -518,7 +527,7 @@ int density_launch(fit_plan* p, int inv, const void* car, const void* nh, void* 
   const size_t lds = (size_t)std::max(nsh, 1) * sizeof(double);
   const size_t csz = sizeof(cpx<R>);
   prof_begin(p, FIT_K_DENSITY_SHELL,
-             (double)p->Np * (2.0 * p->M * csz + (shells ? 4.0 : 0.0) + (t_out ? sizeof(R) : 0.0)));
+             (double)p->Np * (2.0 * p->M * csz + (shells ? 4.0 : 0.0) + (t_out ? sizeof(R) : 0.0)), "k_density");
   hipLaunchKernelGGL(k_density<R>, dim3(nblk), dim3(kThreads), lds, p->stream, g, int4{inv, 0, 0, 0}, 1, p->M,
                      (const cpx<R>*)car, p->Np, (const cpx<R>*)nh, p->Np, (const R*)p->ks[0],
                      (const R*)p->ks[1], (const R*)p->ks[2], shells ? p->shell : nullptr, nsh,
@@ -526,7 +535,7 @@ int density_launch(fit_plan* p, int inv, const void* car, const void* nh, void* 
   prof_end(p);
   FIT_HIP(hipGetLastError());
   if (shells) {
-    prof_begin(p, FIT_K_REDUCE, (double)nblk * nsh * 8.0 + nsh * 8.0);
+    prof_begin(p, FIT_K_REDUCE, (double)nblk * nsh * 8.0 + nsh * 8.0, "k_reduce_rows");
     launch_reduce(p, p->partials, nblk, nsh, 1, p->dcol);
     prof_end(p);
     FIT_HIP(hipGetLastError());
@@ -563,13 +572,13 @@ int density_multi(fit_plan* p, int ninv, const int* invs, const void* car, const
   if ((rc = grow(p, (void**)&p->partials2, &p->partials2_bytes, (size_t)nblk * nb * sizeof(double)))) return rc;
   if ((rc = grow(p, (void**)&p->dcolG, &p->dcolG_bytes, (size_t)nb * sizeof(double)))) return rc;
   const int4 iv{invs[0], ninv > 1 ? invs[1] : 0, ninv > 2 ? invs[2] : 0, ninv > 3 ? invs[3] : 0};
-  prof_begin(p, FIT_K_DENSITY_SHELL, (double)p->Np * (2.0 * p->M * csz + 4.0));
+  prof_begin(p, FIT_K_DENSITY_SHELL, (double)p->Np * (2.0 * p->M * csz + 4.0), "k_density");
   hipLaunchKernelGGL(k_density<R>, dim3(nblk), dim3(kThreads), (size_t)nb * sizeof(double), p->stream, g, iv, ninv,
                      p->M, (const cpx<R>*)car, p->Np, (const cpx<R>*)nh, p->Np, (const R*)p->ks[0],
                      (const R*)p->ks[1], (const R*)p->ks[2], p->shell, nsh, p->partials2, (R*)nullptr);
   prof_end(p);
   FIT_HIP(hipGetLastError());
-  prof_begin(p, FIT_K_REDUCE, (double)nblk * nb * 8.0 + nb * 8.0);
+  prof_begin(p, FIT_K_REDUCE, (double)nblk * nb * 8.0 + nb * 8.0, "k_reduce_rows");
   launch_reduce(p, p->partials2, nblk, nb, 1, p->dcolG);
   prof_end(p);
   FIT_HIP(hipGetLastError());
@@ -655,7 +664,7 @@ int batched_lastdens(fit_plan* p, int gb, const void* eh, int64_t ehgs, double* 
       return rc;
   }
   prof_begin(p, FIT_K_DENSITY_SHELL,
-             (double)gb * p->M * xb * gy * L * csz + (double)gb * p->M * xb * gy * wb.count() * csz);
+             (double)gb * p->M * xb * gy * L * csz + (double)gb * p->M * xb * gy * wb.count() * csz, "k_lastdens", L);
   // per-thread accumulators: the smallest register array that covers TX*L points per workgroup
   const int qneed = (L * TX + kThreads - 1) / kThreads;
 #define FIT_LASTDENS(QQ)                                                                                         \
@@ -670,7 +679,7 @@ int batched_lastdens(fit_plan* p, int gb, const void* eh, int64_t ehgs, double* 
 #undef FIT_LASTDENS
   prof_end(p);
   FIT_HIP(hipGetLastError());
-  prof_begin(p, FIT_K_REDUCE, (double)gb * nblk * nsh * 8.0);
+  prof_begin(p, FIT_K_REDUCE, (double)gb * nblk * nsh * 8.0, "k_reduce_rows");
   launch_reduce(p, p->partials2, nblk, nsh, gb, out);
   prof_end(p);
   FIT_HIP(hipGetLastError());
@@ -686,7 +695,7 @@ int carrier_half(fit_plan* p, int inv, const void* f, const void* const* srcs, i
   const int n1h = p->n[0] / 2 + 1;
   const int64_t Hn = (int64_t)n1h * p->n[1] * p->n[2];
   Grid g = grid_of<R>(p);
-  prof_begin(p, FIT_K_OTHER, (double)nb * Hn * p->M * csz * 3.0);
+  prof_begin(p, FIT_K_OTHER, (double)nb * Hn * p->M * csz * 3.0, "k_carrier_half");
   hipLaunchKernelGGL(k_carrier_half<R>, dim3(srcs ? 256 : 2048, nb), dim3(256), 0, p->stream, g, inv, p->M,
                      p->dealias == FIT_DEALIAS_TWOTHIRDS ? 1 : 0, (const cpx<R>*)f, p->Np, (const R*)p->ks[0],
                      (const R*)p->ks[1], (const R*)p->ks[2], Band{bo[1], p->n[1]}, Band{bo[2], p->n[2]}, bo[0],
@@ -772,7 +781,7 @@ int s2s_sub(fit_plan* p, int inv, const void* f, const void* a, std::vector<doub
   auto gather = [&](const void* src, int ncomp, void** dst, size_t* have) -> int {
     int r = grow(p, dst, have, (size_t)ncomp * c->Np * csz);
     if (r) return r;
-    prof_begin(p, FIT_K_OTHER, (double)ncomp * c->Np * csz * 2.0);
+    prof_begin(p, FIT_K_OTHER, (double)ncomp * c->Np * csz * 2.0, "k_subgrid_gather");
     hipLaunchKernelGGL(k_subgrid_gather<R>, dim3(2048), dim3(256), 0, p->stream, (const cpx<R>*)src, p->n[0],
                        p->n[1], p->n[2], p->Np, (cpx<R>*)*dst, c->n[0], c->n[1], c->n[2], c->Np, p->sub_q[0],
                        p->sub_q[1], p->sub_q[2], ncomp);
@@ -816,20 +825,20 @@ int b2b_impl(fit_plan* p, int inv, const void* f, const void* a, const R* kc, co
   Grid g = grid_of<R>(p);
   p->nhat_dst = nullptr;
   for (int m = 0; m < nb; ++m) {
-    prof_begin(p, FIT_K_OTHER, (double)p->Np * (2.0 * p->M * cs + sizeof(R)));
+    prof_begin(p, FIT_K_OTHER, (double)p->Np * (2.0 * p->M * cs + sizeof(R)), "k_band_field");
     hipLaunchKernelGGL(k_band_field<R>, dim3(1024), dim3(256), 0, p->stream, p->Np, p->M, (const cpx<R>*)f, p->Np,
                        kc, cen, nb, inv2s2, m, (cpx<R>*)p->in_c);
     prof_end(p);
     FIT_HIP(hipGetLastError());
     rc = (p->plain || p->nd == 1) ? nonlinear_from_u<R>(p, p->in_c, 0) : v2_nonlinear<R>(p, p->in_c);
     if (rc) return rc;
-    prof_begin(p, FIT_K_DENSITY_SHELL, (double)p->Np * (2.0 * p->M * cs + sizeof(R)));
+    prof_begin(p, FIT_K_DENSITY_SHELL, (double)p->Np * (2.0 * p->M * cs + sizeof(R)), "k_band_density");
     hipLaunchKernelGGL(k_band_density<R>, dim3(nblk), dim3(kThreads), (size_t)nb * sizeof(double), p->stream, g, inv,
                        p->M, (const cpx<R>*)f, p->Np, (const cpx<R>*)p->nhat, p->Np, (const R*)p->ks[0],
                        (const R*)p->ks[1], (const R*)p->ks[2], kc, cen, nb, inv2s2, p->partials2);
     prof_end(p);
     FIT_HIP(hipGetLastError());
-    prof_begin(p, FIT_K_REDUCE, (double)nblk * nb * 8.0);
+    prof_begin(p, FIT_K_REDUCE, (double)nblk * nb * 8.0, "k_reduce_rows");
     launch_reduce(p, p->partials2, nblk, nb, 1, p->dcolG);
     prof_end(p);
     FIT_HIP(hipGetLastError());
@@ -865,7 +874,7 @@ int m2m_impl(fit_plan* p, int inv, const void* f, const void* a, void* S, void* 
                      (const cpx<R>*)a, Np, (const R*)p->ks[0], (const R*)p->ks[1], (const R*)p->ks[2], (cpx<R>*)A,
                      (cpx<R>*)E);
   dim3 grid((unsigned)((Np + 255) / 256), nchunk);
-  prof_begin(p, FIT_K_OTHER, (double)Np * Np * os);
+  prof_begin(p, FIT_K_OTHER, (double)Np * Np * os, "k_m2m");
   if (io_dtype == FIT_F64)
     hipLaunchKernelGGL((k_m2m<R, double>), grid, dim3(256), 0, p->stream, g, trunc, p->M, (const cpx<R>*)f, Np,
                        (const cpx<R>*)A, (const cpx<R>*)E, (const R*)p->ks[0], (const R*)p->ks[1], (const R*)p->ks[2],

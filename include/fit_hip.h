@@ -268,6 +268,17 @@ enum {
 int fit_profile_query(const fit_plan* plan, int kernel_class, int64_t* launches, double* total_ms,
                       double* alg_bytes);
 
+/* Per-kernel totals of the profiled launches (labels such as "k_grad2_r<f32,768>": kernel,
+ * compute precision, transform length, loader / storer): the number of distinct labels, and for
+ * label `index` its name (NUL-terminated, truncated to name_len), kernel class, launch count,
+ * summed time (ms) and summed algorithmic HBM bytes. */
+int fit_profile_kernel_count(const fit_plan* plan);
+int fit_profile_kernel(const fit_plan* plan, int index, char* name, int name_len, int* kernel_class,
+                       int64_t* launches, double* total_ms, double* alg_bytes);
+int fit_ensemble_profile_kernel_count(const fit_ensemble* ens);
+int fit_ensemble_profile_kernel(const fit_ensemble* ens, int index, char* name, int name_len, int* kernel_class,
+                                int64_t* launches, double* total_ms, double* alg_bytes);
+
 /* The HIP stream the plan launches on (as void*: hipStream_t). */
 void* fit_plan_stream(const fit_plan* plan);
 
