@@ -5,7 +5,7 @@ Workload (one step = one snapshot):
   * 512³ grid, D=3, seeded solenoidal field with per-mode |û|² ∝ k⁴exp(-2(k/8)²) (k<=8) joined to
     k^(-5/3), zero for |k| >= 512/3, Hermitian, Σ|û|² = 1 (synthetic, generated on the device)
   * Π(K), T(k) for kinetic energy and helicity with PaddedThreeHalves (768³ transforms), 444
-    LinearBinning(1) shells, one shared N̂
+    LinearBinning(1) shells, one shared N̂ whose two densities the last forward pass computes
   * T(n,m) over the 48 unit shells (CustomBinning(0:48)) with OrszagTwoThirds at 512³, every shell
     a giver
 Prints ONE JSON line (stdout); progress goes to stderr.
@@ -264,8 +264,7 @@ def main():
     KE, HEL = fit.KineticEnergy(), fit.Helicity()
 
     def step():
-        N = pi_plan.nonlinear_term(u)
-        (T_ke, Pi_ke), (T_h, Pi_h) = pi_plan.spectral_fluxes_with_nhat((KE, HEL), u, N)
+        (T_ke, Pi_ke), (T_h, Pi_h) = pi_plan.spectral_fluxes((KE, HEL), u)
         Tm, net, asym = tm_plan.shell_to_shell(KE, u)
         return Pi_ke, Pi_h, Tm, asym
 
@@ -356,10 +355,7 @@ def main():
             t64.set_plain(plain)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            N64 = p64.nonlinear_term(u64)
-            _, pk = p64.spectral_flux_with_nhat(KE, u64, N64)
-            _, ph = p64.spectral_flux_with_nhat(HEL, u64, N64)
-            del N64
+            (_, pk), (_, ph) = p64.spectral_fluxes((KE, HEL), u64)
             tm, _, _ = t64.shell_to_shell(KE, u64)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0

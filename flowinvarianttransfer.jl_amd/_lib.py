@@ -17,6 +17,7 @@ OPT_PLAIN_PIPELINE, OPT_GIVER_BATCH, OPT_XPASS_GIVERS, OPT_XPASS_BLOCK, OPT_SUBG
 OPT_XPASS_SELF = 7
 OPT_XPASS_STAGE = 8
 OPT_GRAD_SPLIT = 9
+OPT_FUSED_FLUX = 10
 ENS_OPT_GRAPH, ENS_OPT_BATCH = 1, 2
 K_CLASSES = ("fft_column", "fft_row_inverse", "fft_row_forward", "density_shell", "reduce", "other",
              "fused_x")
@@ -39,6 +40,7 @@ SIGNATURES = {
     "fit_transfer_density": (_I, [_P, _I, _P, _P, _P, _I, _I]),
     "fit_spectral_flux": (_I, [_P, _I, _P, _P, _D, _D, _I, _I]),
     "fit_spectral_flux_with_nhat": (_I, [_P, _I, _P, _P, _D, _D, _I, _I]),
+    "fit_spectral_fluxes": (_I, [_P, _I, ctypes.POINTER(_I), _P, _P, _D, _D, _I, _I]),
     "fit_spectral_fluxes_with_nhat": (_I, [_P, _I, ctypes.POINTER(_I), _P, _P, _D, _D, _I, _I]),
     "fit_spectral_flux_with_nhat_noalloc": (_I, [_P, _I, _P, _P, _P, _P, _I, _I]),
     "fit_shell_to_shell": (_I, [_P, _I, _P, _P, _D, _D, _D, _I, _I, _I]),

@@ -243,6 +243,19 @@ class Plan:
                                               _dptr(Pi), self.io, kind))
         return T, Pi
 
+    def spectral_fluxes(self, invariants, f, adv=None):
+        """[(T, Π)] for several invariants of f advected by adv (or f) with one shared N̂, whose
+        densities the last forward pass computes (fit_spectral_fluxes)."""
+        fp, ap, keep, kind = self._fa(f, adv)
+        q = len(invariants)
+        T = np.zeros(q * self.n_shells)
+        Pi = np.zeros(q * self.n_shells)
+        codes = (ctypes.c_int * q)(*[inv.code for inv in invariants])
+        self._order(kind, keep[0])
+        _lib.check(self.lib.fit_spectral_fluxes(self.handle, q, codes, fp, ap, _dptr(T), _dptr(Pi), self.io, kind))
+        n = self.n_shells
+        return [(T[i * n:(i + 1) * n], Pi[i * n:(i + 1) * n]) for i in range(q)]
+
     def spectral_flux_with_nhat(self, invariant, c_hat, N_hat):
         cp, ck, kind = _marshal(c_hat, self.io)
         npp, nk, _ = _marshal(N_hat, self.io)

@@ -251,6 +251,26 @@ int finish(fit_plan* p) {
   return FIT_OK;
 }
 
+// T and Π = +cumsum(T) (SpectralFlux.jl:172-177) of n_inv invariants from the shell-sum rows in
+// dcolG (invariant q at q*nsh), into host arrays at the same offsets
+int rows_tail(fit_plan* p, int n_inv, double* T, double* Pi) {
+  const int nb = n_inv * p->nsh;
+  std::vector<double> h((size_t)nb);
+  FIT_HIP(hipMemcpyAsync(h.data(), p->dcolG, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  int rc = finish(p);
+  if (rc) return rc;
+  for (int q = 0; q < n_inv; ++q) {
+    double acc = 0.0;
+    for (int s = 0; s < p->nsh; ++s) {
+      const double v = h[(size_t)q * p->nsh + s];
+      acc += v;
+      if (T) T[(size_t)q * p->nsh + s] = v;
+      if (Pi) Pi[(size_t)q * p->nsh + s] = acc;
+    }
+  }
+  return FIT_OK;
+}
+
 int spectral_flux_tail(fit_plan* p, double* T, double* Pi) {
   std::vector<double> h(p->nsh);
   FIT_HIP(hipMemcpyAsync(h.data(), p->dcol, p->nsh * sizeof(double), hipMemcpyDeviceToHost, p->stream));
@@ -661,6 +681,7 @@ int fit_plan_set_option(fit_plan* p, int option, int value) {
     case FIT_OPT_XPASS_SELF: p->xself = value != 0; return FIT_OK;
     case FIT_OPT_XPASS_STAGE: p->xstage = value != 0; return FIT_OK;
     case FIT_OPT_GRAD_SPLIT: p->grad_split = value < 0 ? -1 : (value != 0); return FIT_OK;
+    case FIT_OPT_FUSED_FLUX: p->fused_flux = value != 0; return FIT_OK;
     default: return fail(FIT_E_ARGUMENT, "unknown plan option");
   }
 }
@@ -775,14 +796,27 @@ int fit_spectral_flux(fit_plan* p, int inv, const void* f_hat, const void* adv_h
   if ((rc = check_invariant(p, inv, p->M))) return rc;
   const void *f, *a;
   if ((rc = stage_fa(p, f_hat, adv_hat, io_dtype, ptr_kind, &f, &a))) return rc;
-  if (p->dtype == FIT_F64) {
-    if ((rc = nonlinear_term_impl<double>(p, f, a))) return rc;
-    if ((rc = density_launch<double>(p, inv, f, p->nhat, nullptr, true))) return rc;
-  } else {
-    if ((rc = nonlinear_term_impl<float>(p, f, a))) return rc;
-    if ((rc = density_launch<float>(p, inv, f, p->nhat, nullptr, true))) return rc;
-  }
-  return spectral_flux_tail(p, T, Pi);
+  rc = p->dtype == FIT_F64 ? flux_multi<double>(p, 1, &inv, f, a) : flux_multi<float>(p, 1, &inv, f, a);
+  if (rc) return rc;
+  return rows_tail(p, 1, T, Pi);
+}
+
+int fit_spectral_fluxes(fit_plan* p, int n_inv, const int* invariants, const void* f_hat, const void* adv_hat,
+                        double* T, double* Pi, int io_dtype, int ptr_kind) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (p->nsh <= 0) return fail(FIT_E_ARGUMENT, "plan has no shells");
+  if (!invariants || n_inv < 1 || n_inv > 4) return fail(FIT_E_ARGUMENT, "1 to 4 invariants per call");
+  for (int q = 0; q < n_inv; ++q)
+    if ((rc = check_invariant(p, invariants[q], p->M))) return rc;
+  if ((size_t)n_inv * p->nsh * sizeof(double) > (size_t)160 * 1024)
+    return fail(FIT_E_UNSUPPORTED, "n_inv * n_shells exceeds the shell bins one workgroup holds");
+  const void *f, *a;
+  if ((rc = stage_fa(p, f_hat, adv_hat, io_dtype, ptr_kind, &f, &a))) return rc;
+  rc = p->dtype == FIT_F64 ? flux_multi<double>(p, n_inv, invariants, f, a)
+                           : flux_multi<float>(p, n_inv, invariants, f, a);
+  if (rc) return rc;
+  return rows_tail(p, n_inv, T, Pi);
 }
 
 int fit_spectral_flux_with_nhat(fit_plan* p, int inv, const void* c_hat, const void* N_hat, double* T,
@@ -816,20 +850,7 @@ int fit_spectral_fluxes_with_nhat(fit_plan* p, int n_inv, const int* invariants,
   rc = p->dtype == FIT_F64 ? density_multi<double>(p, n_inv, invariants, c, n)
                            : density_multi<float>(p, n_inv, invariants, c, n);
   if (rc) return rc;
-  const int nb = n_inv * p->nsh;
-  std::vector<double> h((size_t)nb);
-  FIT_HIP(hipMemcpyAsync(h.data(), p->dcolG, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, p->stream));
-  if ((rc = finish(p))) return rc;
-  for (int q = 0; q < n_inv; ++q) {
-    double acc = 0.0;
-    for (int s = 0; s < p->nsh; ++s) {
-      const double v = h[(size_t)q * p->nsh + s];
-      acc += v;  // Π = +cumsum(T) (SpectralFlux.jl:172-177)
-      if (T) T[(size_t)q * p->nsh + s] = v;
-      if (Pi) Pi[(size_t)q * p->nsh + s] = acc;
-    }
-  }
-  return FIT_OK;
+  return rows_tail(p, n_inv, T, Pi);
 }
 
 int fit_spectral_flux_with_nhat_noalloc(fit_plan* p, int inv, const void* c_hat, const void* N_hat, double* T,

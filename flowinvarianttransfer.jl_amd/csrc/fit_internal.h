@@ -117,6 +117,13 @@ struct fit_plan {
   void* sub_a = nullptr; size_t sub_a_bytes = 0;
   bool own_stream = true;
   int use_sub = 1;                // FIT_OPT_SUBGRID
+  // fused last pass + density of the flux (flux_multi): requested invariants and whether the
+  // pipeline took the fused form (densities in dcolG) or stored N̂
+  int fused_flux = 0;            // FIT_OPT_FUSED_FLUX: 1 lets the flux compute its densities in the last pass
+  int fused_ninv = 0;
+  int fused_inv[2] = {0, 0};
+  const void* fused_car = nullptr;
+  bool fused_done = false;
   // ordering after a caller's stream (fit_plan_wait_stream)
   hipEvent_t order_ev = nullptr;
   // profiling
@@ -183,6 +190,7 @@ int stage_fa(fit_plan* p, const void* f_hat, const void* adv_hat, int io_dtype, 
              const void** a);
 int finish(fit_plan* p);
 int spectral_flux_tail(fit_plan* p, double* T, double* Pi);
+int rows_tail(fit_plan* p, int n_inv, double* T, double* Pi);
 
 extern size_t g_max_lds;  // per-workgroup LDS the launches may use
 constexpr int kThreads = 256;
@@ -271,6 +279,7 @@ template <typename R> int nonlinear_term_impl(fit_plan* p, const void* f, const 
 template <typename R> int density_launch(fit_plan* p, int inv, const void* car, const void* nh, void* t_out,
                                          bool shells);
 template <typename R> int density_multi(fit_plan* p, int ninv, const int* invs, const void* car, const void* nh);
+template <typename R> int flux_multi(fit_plan* p, int ninv, const int* invs, const void* f, const void* a);
 template <typename R> int density_shells_into(fit_plan* p, int inv, const void* car, const void* nh, double* partials,
                                               int nblk, double* out);
 template <typename R> int s2s_entry(fit_plan* p, int inv, const void* f, const void* a, std::vector<double>& Tm);
@@ -284,6 +293,7 @@ template <typename R> int ens_run(fit_ensemble* e);
   extern template int nonlinear_term_impl<R>(fit_plan*, const void*, const void*);                              \
   extern template int density_launch<R>(fit_plan*, int, const void*, const void*, void*, bool);                 \
   extern template int density_multi<R>(fit_plan*, int, const int*, const void*, const void*);                   \
+  extern template int flux_multi<R>(fit_plan*, int, const int*, const void*, const void*);                      \
   extern template int density_shells_into<R>(fit_plan*, int, const void*, const void*, double*, int, double*);   \
   extern template int s2s_entry<R>(fit_plan*, int, const void*, const void*, std::vector<double>&);            \
   extern template int b2b_impl<R>(fit_plan*, int, const void*, const void*, const R*, const double*, int, double, \

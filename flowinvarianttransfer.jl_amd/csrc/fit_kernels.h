@@ -1190,6 +1190,213 @@ __global__ void __launch_bounds__(256) k_lastdens(FftDesc d, const cpx<R>* __res
   for (int s = tid; s < nsh; s += nth) partials[blk * nsh + s] = bins[s];
 }
 
+// ---------------------------------------------------------------------------------------------
+// Fused last forward pass + transfer density + shell partial sums, register FFT along the last
+// axis (z in 3D, y in 2D).  TX adjacent columns per workgroup (x fastest), T = L/E lanes per
+// column; the workgroup walks the rows `row = blockIdx.y, + gridDim.y, ...` of the other-index
+// band and bins every density into FP64 LDS bins: one partial row per workgroup.  N̂ never
+// touches HBM.  Two density forms:
+//   kDensFull : the flux of one field (any dealiasing).  A transform-grid half bin gives
+//               N̂ = scale·v at its original index o and scale·conj(v) at the original index of its
+//               mirrored bin (the StNhat rule, which keeps the reference's 3/2-padding Nyquist
+//               placement), 0 where the 2/3 mask drops o; the density of up to 2 invariants
+//               (Invariants.jl:40-104) at each such mode from the full carrier and the full
+//               shell map.
+//   kDensPair : batched giver / snapshot loops on plans without padding (N̂ Hermitian): the pair
+//               density t(k) + t(-k) = w(kx)·2 Re Σ_c conj(eh_c) N̂_c, eh the Hermitian part of the
+//               masked carrier (w = 1/2 on the kx = 0, n1/2 planes), from a precomputed half-grid
+//               eh, or formed here from the snapshot arrays (ensembles: kinetic energy / scalar).
+// ---------------------------------------------------------------------------------------------
+constexpr int kDensFull = 0, kDensPair = 1;
+
+template <typename R> struct DensArgs {
+  Grid g;
+  // kDensFull: carrier (M components at car + c*csf, full original grid), invariants, masks
+  const cpx<R>* car; int64_t csf; int trunc; int ninv; int inv0, inv1;
+  const R* ks0; const R* ks1; const R* ks2;
+  const int* shell;                       // full grid (kDensFull) or half grid (kDensPair)
+  // kDensPair: precomputed half-grid eh (batch stride ehgs), or srcs != nullptr: snapshot b's
+  // array srcs[b] (component stride ssf) with the 2/3 mask when trunc
+  const cpx<R>* eh; int64_t ehsy, ehsz, ehsf, ehgs;
+  const cpx<R>* const* srcs; int64_t ssf;
+};
+
+// component c of the effective carrier e of invariant inv at mode o (t = Re Σ_c conj(e_c) N̂_c):
+// ĉ_c for kinetic energy / scalar, (i k × û)_c for helicity, (i k × (i k × û))_c for 3D enstrophy,
+// (i k_y ω̂, -i k_x ω̂) with ω̂ = i(k_x û_y - k_y û_x) for 2D enstrophy; 0 for components the
+// invariant does not use
+template <typename R>
+__device__ __forceinline__ cpx<R> carrier_comp(int inv, int nd, int c, const cpx<R>* __restrict__ car, int64_t csf,
+                                               int64_t o, R kx, R ky, R kz) {
+  if (inv == 0 || inv == 3) return car[o + c * csf];
+  if (inv == 2 && nd == 2) {
+    if (c > 1) return mk<R>((R)0, (R)0);
+    const cpx<R> ux = car[o], uy = car[o + csf];
+    const cpx<R> w = cmuli<1>(csub(cscale(uy, kx), cscale(ux, ky)));
+    return c == 0 ? cmuli<1>(cscale(w, ky)) : cmuli<-1>(cscale(w, kx));
+  }
+  if (c > 2) return mk<R>((R)0, (R)0);
+  const cpx<R> ux = car[o], uy = car[o + csf], uz = car[o + 2 * csf];
+  if (inv == 1) {
+    if (c == 0) return cmuli<1>(csub(cscale(uz, ky), cscale(uy, kz)));
+    if (c == 1) return cmuli<1>(csub(cscale(ux, kz), cscale(uz, kx)));
+    return cmuli<1>(csub(cscale(uy, kx), cscale(ux, ky)));
+  }
+  const cpx<R> wx = cmuli<1>(csub(cscale(uz, ky), cscale(uy, kz)));
+  const cpx<R> wy = cmuli<1>(csub(cscale(ux, kz), cscale(uz, kx)));
+  const cpx<R> wz = cmuli<1>(csub(cscale(uy, kx), cscale(ux, ky)));
+  if (c == 0) return cmuli<1>(csub(cscale(wz, ky), cscale(wy, kz)));
+  if (c == 1) return cmuli<1>(csub(cscale(wx, kz), cscale(wz, kx)));
+  return cmuli<1>(csub(cscale(wy, kx), cscale(wx, ky)));
+}
+
+template <typename R> constexpr int lastdens_threads() { return sizeof(R) == 8 ? 256 : 512; }
+
+template <typename R, int L, int MODE>
+__global__ void __launch_bounds__(lastdens_threads<R>()) k_lastdens_r(
+    const cpx<R>* __restrict__ src, int64_t sy, int64_t sz, int64_t sf, const cpx<R>* __restrict__ tw, int nx,
+    int TX, int lax, Band ob, Band wb, int M, R scale, DensArgs<R> da, int nbins, int nsh,
+    double* __restrict__ partials) {
+  constexpr int E = gE(L), T = L / E;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* bins = reinterpret_cast<double*>(smem_raw);
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw + (((size_t)nbins * 8 + 15) / 16) * 16);
+  int bx, by, bz;
+  xcd_tile(bx, by, bz);
+  const int xl = threadIdx.x % TX, t = threadIdx.x / TX;
+  const int x = bx * TX + xl;
+  const int gi = bz;  // batch index (kDensPair); 0 for kDensFull
+  for (int s = threadIdx.x; s < nbins; s += blockDim.x) bins[s] = 0.0;
+  const Grid& g = da.g;
+  const cpx<R>* ehb = da.eh + (int64_t)gi * da.ehgs;
+  const cpx<R>* sb = da.srcs ? da.srcs[gi] : nullptr;
+  const int n1h = g.n1 / 2 + 1;
+  __syncthreads();
+  const int nrow = ob.count();
+  for (int row = by; row < nrow; row += gridDim.y) {
+    const int o = ob.at(row);
+    R ta[MODE == kDensFull ? E : 1][4];
+    R tq[MODE == kDensPair ? E : 1];
+    // line of component c at (x, o), transformed (scaled by the caller)
+    auto column = [&](int c, cpx<R> (&v)[E]) {
+      const cpx<R>* fs = src + (int64_t)(gi * M + c) * sf;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int i = t + T * e;
+        v[e] = mk<R>((R)0, (R)0);
+        if (x < nx) v[e] = lax == 1 ? fs[x + i * sy + o * sz] : fs[x + o * sy + i * sz];
+      }
+      __syncthreads();  // the previous transform is done with the exchange buffer
+      reg_fft_m<L, E, -1, true, false, R>(v, lds, IdxCol{xl, TX}, t, tw);
+    };
+    if constexpr (MODE == kDensFull) {
+      // densities of the direct (d) and mirrored (m) mode of each point, both invariants,
+      // accumulated component by component (t is linear in N̂)
+#pragma unroll
+      for (int e = 0; e < E; ++e) { ta[e][0] = ta[e][1] = ta[e][2] = ta[e][3] = (R)0; }
+      for (int c = 0; c < M; ++c) {
+        cpx<R> v[E];
+        column(c, v);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int i = t + T * e;
+          if (x >= nx || !wb.has(i)) continue;
+          const int yy = lax == 1 ? i : o, zz = lax == 1 ? o : i;
+          const cpx<R> n = cscale(v[e], scale);
+          int ox, oy, oz;
+          if (g.pre(x, yy, zz, ox, oy, oz) && !(da.trunc && g.dealiased(ox, oy, oz))) {
+            const int64_t om = g.oidx(ox, oy, oz);
+            const R kx = da.ks0[ox], ky = g.nd >= 2 ? da.ks1[oy] : (R)0, kz = g.nd >= 3 ? da.ks2[oz] : (R)0;
+            const cpx<R> e0 = carrier_comp<R>(da.inv0, g.nd, c, da.car, da.csf, om, kx, ky, kz);
+            ta[e][0] += e0.x * n.x + e0.y * n.y;
+            if (da.ninv > 1) {
+              const cpx<R> e1 = carrier_comp<R>(da.inv1, g.nd, c, da.car, da.csf, om, kx, ky, kz);
+              ta[e][1] += e1.x * n.x + e1.y * n.y;
+            }
+          }
+          const int mx = x == 0 ? 0 : g.G1 - x;
+          if (mx > g.G1 / 2) {
+            const int my = yy == 0 ? 0 : g.G2 - yy, mz = zz == 0 ? 0 : g.G3 - zz;
+            if (g.pre(mx, my, mz, ox, oy, oz) && !(da.trunc && g.dealiased(ox, oy, oz))) {
+              const int64_t om = g.oidx(ox, oy, oz);
+              const R kx = da.ks0[ox], ky = g.nd >= 2 ? da.ks1[oy] : (R)0, kz = g.nd >= 3 ? da.ks2[oz] : (R)0;
+              const cpx<R> e0 = carrier_comp<R>(da.inv0, g.nd, c, da.car, da.csf, om, kx, ky, kz);
+              ta[e][2] += e0.x * n.x - e0.y * n.y;   // Re{conj(e) conj(n)}
+              if (da.ninv > 1) {
+                const cpx<R> e1 = carrier_comp<R>(da.inv1, g.nd, c, da.car, da.csf, om, kx, ky, kz);
+                ta[e][3] += e1.x * n.x - e1.y * n.y;
+              }
+            }
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) tq[e] = (R)0;
+      for (int c = 0; c < M; ++c) {
+        cpx<R> v[E];
+        column(c, v);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int i = t + T * e;
+          if (x < nx && wb.has(i)) {
+            const int yy = lax == 1 ? i : o, zz = lax == 1 ? o : i;
+            cpx<R> ev;
+            if (sb) {
+              // eh from the snapshot array: Hermitian part of the masked field at (x, yy, zz)
+              const int my = yy == 0 ? 0 : g.n2 - yy, mz = zz == 0 ? 0 : g.n3 - zz, mx = x == 0 ? 0 : g.n1 - x;
+              const bool keep = !(da.trunc && g.dealiased(x, yy, zz));
+              const cpx<R> a = sb[g.oidx(x, yy, zz) + c * da.ssf], b = sb[g.oidx(mx, my, mz) + c * da.ssf];
+              ev = keep ? mk<R>((R)0.5 * (a.x + b.x), (R)0.5 * (a.y - b.y)) : mk<R>((R)0, (R)0);
+            } else {
+              ev = ehb[x + yy * da.ehsy + zz * da.ehsz + c * da.ehsf];
+            }
+            const cpx<R> w = cscale(v[e], scale);
+            tq[e] += ev.x * w.x + ev.y * w.y;
+          }
+        }
+      }
+    }
+    // shell bins
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = t + T * e;
+      if (x >= nx || !wb.has(i)) continue;
+      const int yy = lax == 1 ? i : o, zz = lax == 1 ? o : i;
+      if constexpr (MODE == kDensPair) {
+        const int shv = da.shell[x + (int64_t)n1h * (yy + (int64_t)g.n2 * zz)];
+        if (shv > 0) {
+          const double w = (x == 0 || 2 * x == g.n1) ? 1.0 : 2.0;
+          atomicAdd(&bins[shv - 1], w * (double)tq[e]);
+        }
+      } else {
+        int ox, oy, oz;
+        if (g.pre(x, yy, zz, ox, oy, oz) && !(da.trunc && g.dealiased(ox, oy, oz))) {
+          const int shv = da.shell[g.oidx(ox, oy, oz)];
+          if (shv > 0) {
+            atomicAdd(&bins[shv - 1], (double)ta[e][0]);
+            if (da.ninv > 1) atomicAdd(&bins[nsh + shv - 1], (double)ta[e][1]);
+          }
+        }
+        const int mx = x == 0 ? 0 : g.G1 - x;
+        if (mx > g.G1 / 2) {
+          const int my = yy == 0 ? 0 : g.G2 - yy, mz = zz == 0 ? 0 : g.G3 - zz;
+          if (g.pre(mx, my, mz, ox, oy, oz) && !(da.trunc && g.dealiased(ox, oy, oz))) {
+            const int shv = da.shell[g.oidx(ox, oy, oz)];
+            if (shv > 0) {
+              atomicAdd(&bins[shv - 1], (double)ta[e][2]);
+              if (da.ninv > 1) atomicAdd(&bins[nsh + shv - 1], (double)ta[e][3]);
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int64_t blk = ((int64_t)gi * gridDim.y + by) * gridDim.x + bx;
+  for (int s = threadIdx.x; s < nbins; s += blockDim.x) partials[blk * nbins + s] = bins[s];
+}
+
 // Hermitian part of the masked effective carrier on the original half grid (x <= n1/2), only on
 // the output band: eh_c(k) = keep(k)·(e_c(k) + conj e_c(-k))/2, e from Invariants.jl:40-104 as
 // t = Re Σ_c conj(e_c) N̂_c (KE/scalar: ĉ; helicity: i k×û; enstrophy: i k×(i k×û), 2D form).

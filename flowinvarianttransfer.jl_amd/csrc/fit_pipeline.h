@@ -485,9 +485,42 @@ int v2_xfused_self(fit_plan* p, const int* bi, const int* bo) {
 }
 
 // Remaining forward passes of nxbuf (M fields, one field group per component) into full N̂.
+template <typename R> int lastdens_launch(fit_plan* p, int gb, const int* bo, int mode, const DensArgs<R>& da,
+                                          int nbins, double* out);
+template <typename R> int carrier_half(fit_plan* p, int inv, const void* f, const void* const* srcs, int nb, void* eh);
+
 template <typename R>
 int v2_forward_nhat(fit_plan* p, const int* bo) {
   const int xb = xext(p, bo[0]), pb = xpitch(xb);
+  if (p->fused_ninv > 0) {
+    // flux: the last forward pass computes the densities and shell sums (N̂ never stored)
+    if (p->nd == 3) {
+      const int64_t sy = pb, sz = (int64_t)pb * p->G[1], sf = sz * p->G[2];
+      LdHalf<R> ld{(const cpx<R>*)p->nxbuf, sy, sz, sf};
+      StHalf<R> st{(cpx<R>*)p->nxbuf, sy, sz, sf};
+      int rc = launch_col<R, -1>(p, ld, st, 1, p->M, xb, full_band(p->G[2]), full_band(p->G[1]),
+                                 Band{bo[1], p->G[1]});
+      if (rc) return rc;
+    }
+    DensArgs<R> da{};
+    da.g = grid_of<R>(p);
+    da.car = (const cpx<R>*)p->fused_car;
+    da.csf = p->Np;
+    da.trunc = p->dealias == FIT_DEALIAS_TWOTHIRDS;
+    da.ninv = p->fused_ninv;
+    da.inv0 = p->fused_inv[0];
+    da.inv1 = p->fused_inv[1];
+    da.ks0 = (const R*)p->ks[0];
+    da.ks1 = (const R*)p->ks[1];
+    da.ks2 = (const R*)p->ks[2];
+    da.shell = p->shell;
+    const int nb = p->fused_ninv * p->nsh;
+    int rc = grow(p, (void**)&p->dcolG, &p->dcolG_bytes, (size_t)nb * sizeof(double));
+    if (rc) return rc;
+    if ((rc = lastdens_launch<R>(p, 1, bo, kDensFull, da, nb, p->dcolG))) return rc;
+    p->fused_done = true;
+    return FIT_OK;
+  }
   const int64_t sy = pb, sz = (int64_t)pb * p->G[1], sf = sz * p->G[2];
   LdHalf<R> ld{(const cpx<R>*)p->nxbuf, sy, sz, sf};
   StHalf<R> st{(cpx<R>*)p->nxbuf, sy, sz, sf};
@@ -612,6 +645,36 @@ int nonlinear_term_impl(fit_plan* p, const void* f, const void* a) {
   return v2_nonlinear<R>(p, f);
 }
 
+// Shell sums of ninv (<= 4) invariants of the flux of f advected by a, into dcolG (rows of nsh
+// doubles): N̂ is stored and one density pass reads it for all invariants, or (FIT_OPT_FUSED_FLUX)
+// the last forward pass computes the densities itself and N̂ is never stored.  Measured on the
+// 512³ flux with 3/2 padding: stored 3.5 ms for the last pass + density, fused 5.1 ms (the fused
+// kernel holds N̂ of all components per point and gathers the carrier at both modes of a bin; one
+// workgroup per CU at 178 VGPRs), so the stored form is the default.
+template <typename R>
+int flux_multi(fit_plan* p, int ninv, const int* invs, const void* f, const void* a) {
+  const int lax = p->nd == 3 ? 2 : 1;
+  const bool fusable = p->fused_flux == 1 && !p->plain && p->nd >= 2 && p->M <= 3 && ninv <= 2 && p->shell &&
+                       reg_supported(p->G[lax]) && (size_t)ninv * p->nsh * 8 <= (size_t)96 * 1024;
+  int rc;
+  if (fusable) {
+    p->fused_ninv = ninv;
+    p->fused_inv[0] = invs[0];
+    p->fused_inv[1] = ninv > 1 ? invs[1] : invs[0];
+    p->fused_car = f;
+    p->fused_done = false;
+    rc = nonlinear_term_impl<R>(p, f, a);
+    p->fused_ninv = 0;
+    p->fused_car = nullptr;
+    if (rc) return rc;
+    if (p->fused_done) return FIT_OK;
+    // the pipeline took an unfused route and stored N̂
+    return density_multi<R>(p, ninv, invs, f, p->nhat);
+  }
+  if ((rc = nonlinear_term_impl<R>(p, f, a))) return rc;
+  return density_multi<R>(p, ninv, invs, f, p->nhat);
+}
+
 // Plain giver loop: one giver per pipeline pass, full transforms, N̂ through HBM.
 template <typename R>
 int s2s_plain(fit_plan* p, int inv, const void* f, std::vector<double>& Tm) {
@@ -628,11 +691,71 @@ int s2s_plain(fit_plan* p, int inv, const void* f, std::vector<double>& Tm) {
   return FIT_OK;
 }
 
-// Tail of a batched pass for gb field groups already through the fused x pass (nxbuf, pitch
+// Fused last forward pass + density + shell partial sums (k_lastdens_r) over gb field groups of M
+// components in nxbuf (pitch xpitch(xext(bo[0]))), already through every other forward pass;
+// shell sums reduced in fixed order into out (gb rows of nbins doubles, device).
+template <typename R>
+int lastdens_launch(fit_plan* p, int gb, const int* bo, int mode, const DensArgs<R>& da, int nbins, double* out) {
+  const size_t csz = sizeof(cpx<R>);
+  const int lax = p->nd == 3 ? 2 : 1;
+  const int L = p->G[lax];
+  const int xb = std::max(1, xext(p, bo[0])), pb = xpitch(xb);
+  const int64_t sy = pb, sz = (int64_t)pb * p->G[1], sf = sz * p->G[2];
+  const Band ob = p->nd == 3 ? Band{bo[1], p->G[1]} : full_band(1);
+  const Band wb = p->nd == 3 ? Band{bo[2], p->G[2]} : Band{bo[1], p->G[1]};
+  const int T = L / gE(L);
+  const int nthr = std::is_same<R, double>::value ? lastdens_threads<double>() : lastdens_threads<float>();
+  // columns per workgroup: 16 (128-byte rows in FP32) within the thread bound and 64 KB of LDS
+  int TX = std::max(1, std::min(16, nthr / T));
+  while (TX > 1 && (size_t)L * TX * csz > 65536) TX /= 2;
+  const int gx = (xb + TX - 1) / TX;
+  const int nrow = ob.count();
+  // rows per workgroup: enough workgroups to fill the chip, few partial rows to reduce
+  const int gy = std::max(1, std::min(nrow, (2048 + gx * gb - 1) / (gx * gb)));
+  const int nblk = gx * gy;
+  int rc;
+  if ((rc = grow(p, (void**)&p->partials2, &p->partials2_bytes, (size_t)gb * nblk * nbins * sizeof(double))))
+    return rc;
+  const size_t lds = ((size_t)nbins * 8 + 15) / 16 * 16 + (size_t)L * TX * csz;
+  const double bytes = (double)gb * p->M * xb * nrow * L * csz +
+                       (mode == kDensFull ? (double)p->M * p->Np * csz + 4.0 * p->Np
+                                          : (double)gb * p->M * xb * nrow * wb.count() * csz * (da.srcs ? 2.0 : 1.0));
+  prof_begin(p, FIT_K_DENSITY_SHELL, bytes, mode == kDensFull ? "k_lastdens_r_full" : "k_lastdens_r", L);
+  const R scale = mode == kDensFull ? (R)((double)p->Gp / ((double)p->Np * (double)p->Np)) : (R)(1.0 / (double)p->Np);
+  dim3 grid((unsigned)gx, (unsigned)gy, (unsigned)gb), block((unsigned)(T * TX));
+  switch (L) {
+#define FIT_CASE(LL)                                                                                             \
+  case LL:                                                                                                       \
+    if (mode == kDensFull)                                                                                       \
+      hipLaunchKernelGGL((k_lastdens_r<R, LL, kDensFull>), grid, block, lds, p->stream, (const cpx<R>*)p->nxbuf, sy, \
+                         sz, sf, (const cpx<R>*)p->tw[lax], xb, TX, lax, ob, wb, p->M, scale, da, nbins, p->nsh,  \
+                         p->partials2);                                                                          \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_lastdens_r<R, LL, kDensPair>), grid, block, lds, p->stream, (const cpx<R>*)p->nxbuf, sy, \
+                         sz, sf, (const cpx<R>*)p->tw[lax], xb, TX, lax, ob, wb, p->M, scale, da, nbins, p->nsh,  \
+                         p->partials2);                                                                          \
+    break;
+    FIT_REG_LENGTHS(FIT_CASE)
+#undef FIT_CASE
+    default:
+      prof_end(p);
+      return fail(FIT_E_UNSUPPORTED, "fused last pass needs a register FFT length");
+  }
+  prof_end(p);
+  FIT_HIP(hipGetLastError());
+  prof_begin(p, FIT_K_REDUCE, (double)gb * nblk * nbins * 8.0, "k_reduce_rows");
+  launch_reduce(p, p->partials2, nblk, nbins, gb, out);
+  prof_end(p);
+  FIT_HIP(hipGetLastError());
+  return FIT_OK;
+}
+
+// LDS-engine form of batched_lastdens for lengths without a register FFT: tail of a batched pass
+// for gb field groups already through the fused x pass (nxbuf, pitch
 // xpitch(xext(band_rx))): the remaining forward pass(es), the pair density against eh (+ g*ehgs for group
 // g) and FP64 shell bins, reduced in fixed order into out (gb rows of nsh doubles, device).
 template <typename R>
-int batched_lastdens(fit_plan* p, int gb, const void* eh, int64_t ehgs, double* out) {
+int batched_lastdens_lds(fit_plan* p, int gb, const void* eh, int64_t ehgs, double* out) {
   const int nsh = p->nsh;
   const int* bo = p->band_rx;
   const size_t csz = sizeof(cpx<R>);
@@ -684,6 +807,50 @@ int batched_lastdens(fit_plan* p, int gb, const void* eh, int64_t ehgs, double* 
   prof_end(p);
   FIT_HIP(hipGetLastError());
   return FIT_OK;
+}
+
+// Tail of a batched pass for gb field groups already through the fused x pass (nxbuf, pitch
+// xpitch(xext(band_rx))): the remaining forward pass(es), the pair density against eh (+ g*ehgs for
+// group g; or, with srcs, the Hermitian part of snapshot g's own masked array, component stride
+// ssf) and FP64 shell bins, reduced in fixed order into out (gb rows of nsh doubles, device).
+template <typename R>
+int batched_lastdens(fit_plan* p, int gb, const void* eh, int64_t ehgs, double* out,
+                     const void* const* srcs = nullptr) {
+  const int nsh = p->nsh;
+  const int* bo = p->band_rx;
+  const int n1h = p->n[0] / 2 + 1;
+  const int xb = std::max(1, xext(p, bo[0])), pb = xpitch(xb);
+  const int64_t sy = pb, sz = (int64_t)pb * p->G[1], sf = sz * p->G[2];
+  int rc;
+  if (!reg_supported(p->G[p->nd == 3 ? 2 : 1])) {
+    if (srcs) {  // the LDS form reads a precomputed carrier
+      const int64_t Hn = (int64_t)n1h * p->n[1] * p->n[2];
+      if ((rc = grow(p, &p->eh, &p->eh_bytes, (size_t)gb * p->M * Hn * sizeof(cpx<R>)))) return rc;
+      if ((rc = carrier_half<R>(p, FIT_KINETIC_ENERGY, nullptr, srcs, gb, p->eh))) return rc;
+      eh = p->eh;
+      ehgs = (int64_t)p->M * Hn;
+    }
+    return batched_lastdens_lds<R>(p, gb, eh, ehgs, out);
+  }
+  if (p->nd == 3) {
+    LdHalf<R> ld{(const cpx<R>*)p->nxbuf, sy, sz, sf};
+    StHalf<R> st{(cpx<R>*)p->nxbuf, sy, sz, sf};
+    if ((rc = launch_col<R, -1>(p, ld, st, 1, gb * p->M, xb, full_band(p->G[2]), full_band(p->G[1]),
+                                Band{bo[1], p->G[1]})))
+      return rc;
+  }
+  DensArgs<R> da{};
+  da.g = grid_of<R>(p);
+  da.trunc = p->dealias == FIT_DEALIAS_TWOTHIRDS;
+  da.shell = p->shh;
+  da.eh = (const cpx<R>*)eh;
+  da.ehsy = n1h;
+  da.ehsz = (int64_t)n1h * p->n[1];
+  da.ehsf = (int64_t)n1h * p->n[1] * p->n[2];
+  da.ehgs = ehgs;
+  da.srcs = (const cpx<R>* const*)srcs;
+  da.ssf = p->Np;
+  return lastdens_launch<R>(p, gb, bo, kDensPair, da, nsh, out);
 }
 
 // Hermitian part of the masked effective carrier on the receiving band, for nb snapshots
@@ -952,17 +1119,15 @@ int ens_batch(fit_ensemble* e, EnsGroup& G, int b0, int gb) {
   const int64_t ugs = (int64_t)nd * p->Gp;
   {
     MGuard mg(p, nd);
-    if ((rc = carrier_half<R>(p, FIT_KINETIC_ENERGY, nullptr, tu, gb, G.ehk))) return rc;
     if ((rc = v2_inverse_grads<R>(p, nullptr, 0, gb, p->band_in, nullptr, tu))) return rc;
     if ((rc = v2_xfused<R>(p, gb, p->band_in, nullptr, p->band_rx, G.ub, ugs))) return rc;
-    if ((rc = batched_lastdens<R>(p, gb, G.ehk, (int64_t)nd * Hn, G.rows + (size_t)b0 * nsh))) return rc;
+    if ((rc = batched_lastdens<R>(p, gb, nullptr, 0, G.rows + (size_t)b0 * nsh, tu))) return rc;
   }
   if (e->with_scalar) {
     MGuard mg(p, 1);
-    if ((rc = carrier_half<R>(p, FIT_PASSIVE_SCALAR, nullptr, tt, gb, G.ehs))) return rc;
     if ((rc = v2_inverse_grads<R>(p, nullptr, 0, gb, p->band_in, nullptr, tt))) return rc;
     if ((rc = v2_xfused<R>(p, gb, p->band_in, nullptr, p->band_rx, G.ub, ugs))) return rc;
-    if ((rc = batched_lastdens<R>(p, gb, G.ehs, Hn, G.rows + (size_t)(B + b0) * nsh))) return rc;
+    if ((rc = batched_lastdens<R>(p, gb, nullptr, 0, G.rows + (size_t)(B + b0) * nsh, tt))) return rc;
   }
   return FIT_OK;
 }
@@ -982,7 +1147,7 @@ int ens_batch_size(fit_ensemble* e, EnsGroup& G) {
     MGuard mg(p, nd);
     per = batch_bytes_per_group<R>(p, xa);
   }
-  per += (double)nd * p->Gp * sizeof(R) + (double)(nd + 1) * Hn * csz;
+  per += (double)nd * p->Gp * sizeof(R);
   size_t freeb = 0, totalb = 0;
   FIT_HIP(hipMemGetInfo(&freeb, &totalb));
   int ng = (int)std::max(1.0, std::min((double)B, 0.5 * (double)freeb / per));
@@ -990,8 +1155,6 @@ int ens_batch_size(fit_ensemble* e, EnsGroup& G) {
   G.ng = ng;
   int rc;
   if ((rc = grow(p, &G.ub, &G.ub_bytes, (size_t)ng * nd * p->Gp * sizeof(R)))) return rc;
-  if ((rc = grow(p, &G.ehk, &G.ehk_bytes, (size_t)ng * nd * Hn * csz))) return rc;
-  if (e->with_scalar && (rc = grow(p, &G.ehs, &G.ehs_bytes, (size_t)ng * Hn * csz))) return rc;
   return FIT_OK;
 }
 

@@ -86,10 +86,14 @@ int64_t fit_plan_device_bytes(const fit_plan* plan);
  *   FIT_OPT_GRAD_SPLIT = 9: 1 runs the y transform of the last-axis derivative (3D shared-
  *     transform gradients) as its own one-line column pass; 0 keeps it in the two-line gradient
  *     kernel after the other two; -1 (default) splits for y lines shorter than 512 points
- *     (measured: 0.17 ms faster at 192, 0.11 ms slower at 768 on the 512³ snapshot). */
+ *     (measured: 0.17 ms faster at 192, 0.11 ms slower at 768 on the 512³ snapshot).
+ *   FIT_OPT_FUSED_FLUX = 10: 1 makes fit_spectral_flux / fit_spectral_fluxes compute the densities
+ *     in the last forward pass of the nonlinear term (N̂ never stored; M <= 3, up to 2 invariants,
+ *     register FFT length on the last axis); 0 (default) stores N̂ and reads it in one density
+ *     pass (measured faster on the 512³ flux with 3/2 padding). */
 enum { FIT_OPT_PLAIN_PIPELINE = 1, FIT_OPT_GIVER_BATCH = 2, FIT_OPT_XPASS_GIVERS = 3,
        FIT_OPT_XPASS_BLOCK = 4, FIT_OPT_SUBGRID = 5, FIT_OPT_COL_TILE = 6, FIT_OPT_XPASS_SELF = 7,
-       FIT_OPT_XPASS_STAGE = 8, FIT_OPT_GRAD_SPLIT = 9 };
+       FIT_OPT_XPASS_STAGE = 8, FIT_OPT_GRAD_SPLIT = 9, FIT_OPT_FUSED_FLUX = 10 };
 int fit_plan_set_option(fit_plan* plan, int option, int value);
 
 /* Band-limited sub-grid of the giver loop (no reference counterpart).  When every shell lies in
@@ -122,6 +126,16 @@ int fit_transfer_density(fit_plan* plan, int invariant, const void* c_hat, const
  */
 int fit_spectral_flux(fit_plan* plan, int invariant, const void* f_hat, const void* adv_hat,
                       double* T, double* Pi, int io_dtype, int ptr_kind);
+
+/*
+ * T and Pi of n_inv (1..4) invariants of one field: calculate_spectral_flux (SpectralFlux.jl:49-94)
+ * for each invariant with the single N̂ they share (as a decomposition shares it, :96-122).  For up
+ * to 2 invariants and M <= 3 the last forward pass of the nonlinear term computes the densities
+ * and shell sums itself (N̂ is never stored).  T, Pi: host arrays of n_inv*n_shells doubles,
+ * invariant q's shells at offset q*n_shells.
+ */
+int fit_spectral_fluxes(fit_plan* plan, int n_inv, const int* invariants, const void* f_hat, const void* adv_hat,
+                        double* T, double* Pi, int io_dtype, int ptr_kind);
 
 /*
  * T and Pi from a given carrier and N̂ (both (ns..., M)): the density + shell sum + cumsum tail

@@ -127,3 +127,20 @@ def test_ensemble_api_uses_batched_path_and_rejects_padding(fit, oracle):
     check(fit, oracle, res, us, ths, kss, 1e-5)
     with pytest.raises(NotImplementedError):
         fit.Ensemble(SHAPES_2D[:3], kss, dealiasing=fit.PaddedThreeHalves())
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_ensemble_register_lengths_carrier_in_last_pass(fit, oracle, prec):
+    """Snapshots whose last axis has a register FFT (128, 192, 256): the last pass forms each
+    snapshot's Hermitian carrier from its own array (no carrier pass) -- against the oracle."""
+    import torch
+    print("case ensemble register lengths", prec, flush=True)
+    dt = torch.complex128 if prec == "f64" else torch.complex64
+    tol = 1e-11 if prec == "f64" else 1e-5
+    shapes = [(128, 128), (256, 256), (128, 128), (192, 192), (256, 256)]
+    us, ths, kss = make_ensemble(oracle, shapes, 700)
+    res = fit.calculate_ensemble_fluxes([to_dev(u, dt) for u in us], kss, [to_dev(t, dt) for t in ths],
+                                        binning=fit.LinearBinning(1.0))
+    res = fit.calculate_ensemble_fluxes([to_dev(u, dt) for u in us], kss, [to_dev(t, dt) for t in ths],
+                                        binning=fit.LinearBinning(1.0))          # graph replay
+    check(fit, oracle, res, us, ths, kss, tol)

@@ -346,3 +346,66 @@ def test_several_invariants_one_density_pass(fit, oracle, ns, dea, invs):
                 say("  ", prec.__name__, inv, e1, e2)
                 assert e1 < tol and e2 < tol
                 assert rel(T, Ts) < 1e-12 and rel(Pi, Ps) < 1e-12
+
+
+# last axis at a register FFT length: the fused last pass (k_lastdens_r) computes the densities
+FUSED_CASES = [((12, 10, 128), "23", ("ke", "helicity")), ((10, 8, 192), "pad", ("ke", "helicity")),
+               ((12, 12, 128), "none", ("enstrophy",)), ((10, 128), "23", ("ke", "enstrophy")),
+               ((12, 192), "pad", ("ke",)), ((16, 256), "none", ("ke",))]
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("ns,dea,invs", FUSED_CASES)
+def test_fused_flux_last_pass_matches_oracle(fit, oracle, ns, dea, invs, prec):
+    """fit_spectral_fluxes / fit_spectral_flux with the density in the last forward pass
+    (FIT_OPT_FUSED_FLUX: N̂ never stored; 3/2-padded Nyquist placement from the StNhat rule) against
+    the oracle and the stored-N̂ form, for several invariants at once, non-Hermitian inputs."""
+    say("case fused flux", ns, dea, invs, prec)
+    nd = len(ns)
+    ks = ks_of(oracle, ns)
+    rng = np.random.default_rng(len(ns) * 100 + ns[-1])
+    u = rand_c(rng, ns + (nd,))
+    deal = {"none": fit.NoDealiasing(), "23": fit.OrszagTwoThirds(), "pad": fit.PaddedThreeHalves()}[dea]
+    I = {"ke": fit.KineticEnergy(), "helicity": fit.Helicity(), "enstrophy": fit.Enstrophy()}
+    dt = np.complex128 if prec == "f64" else np.complex64
+    tol = TOL64 if prec == "f64" else 1e-5
+    from fit_amd.grid import shell_setup
+    c, _, sidx = shell_setup(ks, fit.LinearBinning(1.0), None)
+    from fit_amd import _lib
+    with fit.Plan(ns, nd, nd, ks, deal, np.float64 if prec == "f64" else np.float32, sidx, len(c)) as plan:
+        plan.set_option(_lib.OPT_FUSED_FLUX, 1)
+        multi = plan.spectral_fluxes([I[i] for i in invs], u.astype(dt))
+        single = plan.spectral_flux(I[invs[0]], u.astype(dt))
+        plan.set_option(_lib.OPT_FUSED_FLUX, 0)
+        stored = plan.spectral_fluxes([I[i] for i in invs], u.astype(dt))
+    for (T, Pi), (Ts, Ps) in zip(multi, stored):
+        assert rel(T, Ts) < tol and rel(Pi, Ps) < tol
+    for inv, (T, Pi) in zip(invs, multi):
+        _, To, Po = oracle.calculate_spectral_flux(u, ks, oracle.LinearBinning(1.0), dea, inv)
+        e1, e2 = rel(T, To), rel(Pi, Po)
+        say("  ", inv, e1, e2)
+        assert e1 < tol and e2 < tol
+    _, To, _ = oracle.calculate_spectral_flux(u, ks, oracle.LinearBinning(1.0), dea, invs[0])
+    assert rel(single[0], To) < tol
+
+
+@pytest.mark.parametrize("ns,dea,inv", [((12, 10, 128), "23", "ke"), ((10, 12, 192), "none", "helicity"),
+                                        ((12, 192), "23", "enstrophy"), ((10, 256), "none", "ke")])
+def test_batched_giver_loop_register_last_pass(fit, oracle, ns, dea, inv):
+    """The batched giver loop with its last pass on the register kernel (pair density against the
+    precomputed Hermitian carrier) against the oracle."""
+    say("case batched register last pass", ns, dea, inv)
+    nd = len(ns)
+    ks = ks_of(oracle, ns)
+    u = oracle.synthetic_field(ns, nd, seed=17)
+    u = u / np.max(np.abs(u))
+    deal = {"none": fit.NoDealiasing(), "23": fit.OrszagTwoThirds()}[dea]
+    I = {"ke": fit.KineticEnergy(), "helicity": fit.Helicity(), "enstrophy": fit.Enstrophy()}[inv]
+    r = fit.calculate_shell_to_shell_transfer(u, ks, binning=fit.CustomBinning(range(0, 6)), dealiasing=deal,
+                                              invariant=I)
+    _, _, To, _, _ = oracle.calculate_shell_to_shell_transfer(u, ks, oracle.CustomBinning(tuple(range(0, 6))), dea,
+                                                               invariant=inv)
+    assert np.max(np.abs(To)) > 1e-12
+    e = rel(r.transfer_matrix, To)
+    say("  T(n,m) err", e)
+    assert e < TOL64
