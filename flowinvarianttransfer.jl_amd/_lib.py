@@ -36,6 +36,7 @@ SIGNATURES = {
     "fit_plan_device_bytes": (ctypes.c_int64, [_P]),
     "fit_plan_set_option": (_I, [_P, _I, _I]),
     "fit_plan_subgrid": (_I, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    "fit_plan_subgrid_classes": (_I, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_I), _I]),
     "fit_nonlinear_term": (_I, [_P, _P, _P, _P, _I, _I]),
     "fit_transfer_density": (_I, [_P, _I, _P, _P, _P, _I, _I]),
     "fit_spectral_flux": (_I, [_P, _I, _P, _P, _D, _D, _I, _I]),

@@ -175,6 +175,17 @@ class Plan:
         return tuple(int(d) for d in dims[:self.nd]) if rc == 1 else None
 
     @property
+    def subgrid_classes(self):
+        """[(grid dims, (first giver, last giver))] of the band-limited giver loop's classes."""
+        dims = (ctypes.c_int64 * 96)()
+        giv = (ctypes.c_int * 64)()
+        n = self.lib.fit_plan_subgrid_classes(self.handle, dims, giv, 32)
+        if n < 0:
+            _lib.check(1)
+        return [(tuple(int(dims[3 * i + d]) for d in range(self.nd)), (int(giv[2 * i]), int(giv[2 * i + 1])))
+                for i in range(min(n, 32))]
+
+    @property
     def device_bytes(self):
         return int(self.lib.fit_plan_device_bytes(self.handle))
 

@@ -87,7 +87,9 @@ void prof_end(fit_plan* p) {
 // launches recorded by the plan and by its sub-grid child are reported together
 void prof_collect(fit_plan* p) {
   if (!p->prof) return;
-  for (fit_plan* s : {p, p->sub}) {
+  std::vector<fit_plan*> all{p};
+  for (const SubGrid& sg : p->subs) all.push_back(sg.c);
+  for (fit_plan* s : all) {
     if (!s || !s->prof) continue;
     for (size_t i = 0; i < s->nrec; ++i) {
       float t = 0.f;
@@ -344,24 +346,11 @@ static int subgrid_len(int need) {
   return best;
 }
 
-// Attach the band-limited sub-grid child when every shell lies in |k_d| <= R_d with 4 R_d < n_d
-// and the child grid is at most half the transform grid.  Child: no dealiasing, the API grid's
-// wavenumbers at the same folded indices, the shell map restricted to the box |k_d| <= R_d.
-static int setup_subgrid(fit_plan* p, const double* const* ks, const std::vector<int>& sh) {
-  if (p->nd < 2 || p->nsh <= 0 || !p->shell_symmetric) return FIT_OK;
-  int R[3] = {0, 0, 0}, Q[3] = {0, 0, 0}, Ls[3] = {1, 1, 1};
-  double ratio = 1.0;
-  for (int d = 0; d < p->nd; ++d) {
-    R[d] = std::max(p->band_rx[d], 0);
-    if (4 * R[d] >= p->n[d]) return FIT_OK;
-    Q[d] = 2 * R[d];
-    if (p->dealias == FIT_DEALIAS_TWOTHIRDS) Q[d] = std::min(Q[d], p->band_in[d]);
-    const int need = Q[d] + 2 * R[d] + 1;  // <= 4 R_d + 1 <= n_d
-    Ls[d] = subgrid_len(need);
-    if (Ls[d] > p->n[d]) Ls[d] = p->n[d];
-    ratio *= (double)Ls[d] / (double)p->G[d];
-  }
-  if (ratio > 0.5) return FIT_OK;
+// Child plan of the band-limited giver loop on a grid of Ls points per axis: no dealiasing, the
+// API grid's wavenumbers at the same folded indices, the shell map restricted to the box
+// |k_d| <= R_d, input band pruned to the copied box |q_d| <= Q_d.
+static int make_subgrid_child(fit_plan* p, const double* const* ks, const std::vector<int>& sh, const int* R,
+                              const int* Q, const int* Ls, fit_plan** out) {
   std::vector<double> kc[3];
   const double* kp[3] = {nullptr, nullptr, nullptr};
   for (int d = 0; d < p->nd; ++d) {
@@ -395,10 +384,91 @@ static int setup_subgrid(fit_plan* p, const double* const* ks, const std::vector
   (void)hipStreamDestroy(c->stream);
   c->stream = p->stream;
   c->own_stream = false;
-  p->sub = c;
-  for (int d = 0; d < 3; ++d) p->sub_q[d] = Q[d];
-  const double r = (double)c->Np / (double)p->Np;
-  p->sub_tscale = r * r;
+  *out = c;
+  return FIT_OK;
+}
+
+// Band-limited sub-grids of the giver loop.  With receivers in |k_d| <= R_d and a giver's modes
+// in |p_d| <= P_d, every triad k = p + q it feeds has |q_d| <= R_d + P_d; copying the advecting
+// field's box |q_d| <= Q_d = R_d + P_d onto a grid of L_d >= Q_d + P_d + R_d + 1 points keeps the
+// products alias-free at the receivers, so the giver's column of T(n,m) equals the full-grid one.
+// Consecutive givers with the same grid form a class (adjacent classes merge when that costs less);
+// a sub-grid is used when the largest class's grid is at most half the transform grid (4 R_d < n_d).
+static int setup_subgrid(fit_plan* p, const double* const* ks, const std::vector<int>& sh) {
+  if (p->nd < 2 || p->nsh <= 0 || !p->shell_symmetric) return FIT_OK;
+  int R[3] = {0, 0, 0};
+  for (int d = 0; d < p->nd; ++d) {
+    R[d] = std::max(p->band_rx[d], 0);
+    if (4 * R[d] >= p->n[d]) return FIT_OK;
+  }
+  const bool trunc = p->dealias == FIT_DEALIAS_TWOTHIRDS;
+  auto grid_for = [&](const int* P, int* Q, int* Ls) {
+    for (int d = 0; d < 3; ++d) {
+      if (d >= p->nd) { Q[d] = 0; Ls[d] = 1; continue; }
+      Q[d] = R[d] + std::max(P[d], 0);
+      if (trunc) Q[d] = std::min(Q[d], p->band_in[d]);
+      Ls[d] = std::min(subgrid_len(Q[d] + std::max(P[d], 0) + R[d] + 1), p->n[d]);
+    }
+  };
+  {
+    int Q[3], Ls[3];
+    grid_for(R, Q, Ls);  // the largest giver support is the receivers' R
+    double ratio = 1.0;
+    for (int d = 0; d < p->nd; ++d) ratio *= (double)Ls[d] / (double)p->G[d];
+    if (ratio > 0.5) return FIT_OK;
+  }
+  struct Cls { int m0, m1, Q[3], L[3]; };
+  std::vector<Cls> cls;
+  for (int m = 1; m <= p->nsh; ++m) {
+    const int4 b = p->shell_band[m - 1];
+    const int P[3] = {std::max(b.x, 0), std::max(b.y, 0), std::max(b.z, 0)};
+    Cls c{m, m, {0, 0, 0}, {1, 1, 1}};
+    grid_for(P, c.Q, c.L);
+    if (!cls.empty() && cls.back().L[0] == c.L[0] && cls.back().L[1] == c.L[1] && cls.back().L[2] == c.L[2]) {
+      Cls& k = cls.back();
+      k.m1 = m;
+      for (int d = 0; d < 3; ++d) k.Q[d] = std::max(k.Q[d], c.Q[d]);
+    } else {
+      cls.push_back(c);
+    }
+  }
+  auto vol = [](const Cls& c) { return (double)c.L[0] * c.L[1] * c.L[2]; };
+  // cost of a class: its givers plus about half a giver for the class's own velocity transforms
+  // and input gathers, each at the class's grid volume; adjacent classes merge while that lowers
+  // the total
+  auto cost = [&](const Cls& c) { return ((double)(c.m1 - c.m0 + 1) + 0.5) * vol(c); };
+  for (;;) {
+    double best = 0.0;
+    size_t bi = cls.size();
+    for (size_t i = 0; i + 1 < cls.size(); ++i) {
+      Cls m = cls[i + 1];
+      m.m0 = cls[i].m0;
+      for (int d = 0; d < 3; ++d) { m.L[d] = std::max(cls[i].L[d], m.L[d]); m.Q[d] = std::max(cls[i].Q[d], m.Q[d]); }
+      const double saving = cost(cls[i]) + cost(cls[i + 1]) - cost(m);
+      if (saving > best) { best = saving; bi = i; }
+    }
+    if (bi == cls.size()) break;
+    Cls& b = cls[bi + 1];
+    b.m0 = cls[bi].m0;
+    for (int d = 0; d < 3; ++d) { b.L[d] = std::max(cls[bi].L[d], b.L[d]); b.Q[d] = std::max(cls[bi].Q[d], b.Q[d]); }
+    cls.erase(cls.begin() + (long)bi);
+  }
+  size_t big = 0;
+  for (const Cls& c : cls) {
+    SubGrid sg;
+    sg.m0 = c.m0;
+    sg.m1 = c.m1;
+    for (int d = 0; d < 3; ++d) sg.q[d] = c.Q[d];
+    int rc = make_subgrid_child(p, ks, sh, R, c.Q, c.L, &sg.c);
+    if (rc) return rc;
+    sg.c->giver_lo = c.m0;
+    sg.c->giver_hi = c.m1;
+    const double r = (double)sg.c->Np / (double)p->Np;
+    sg.tscale = r * r;
+    p->subs.push_back(sg);
+    if (vol(c) >= vol(cls[big])) big = p->subs.size() - 1;
+  }
+  p->sub = p->subs[big].c;
   return FIT_OK;
 }
 
@@ -622,7 +692,9 @@ int fit_plan_destroy(fit_plan* p) {
   if (!p) return FIT_OK;
   hipSetDevice(p->device);
   if (p->stream) hipStreamSynchronize(p->stream);
-  if (p->sub) { fit_plan_destroy(p->sub); p->sub = nullptr; }
+  for (SubGrid& sg : p->subs) fit_plan_destroy(sg.c);
+  p->subs.clear();
+  p->sub = nullptr;
   dfree(p->sub_f); dfree(p->sub_a);
   for (int d = 0; d < 3; ++d) { dfree(p->tw[d]); dfree(p->ks[d]); }
   dfree(p->kt[0]); dfree(p->kt[1]);
@@ -652,15 +724,35 @@ int fit_plan_subgrid(const fit_plan* p, int64_t* dims) {
   return 1;
 }
 
-int64_t fit_plan_device_bytes(const fit_plan* p) { return p ? p->bytes + (p->sub ? p->sub->bytes : 0) : 0; }
+int fit_plan_subgrid_classes(const fit_plan* p, int64_t* dims, int* givers, int max_classes) {
+  if (!p) { fail(FIT_E_ARGUMENT, "null plan"); return -1; }
+  const int n = (int)p->subs.size();
+  for (int i = 0; i < n && i < max_classes; ++i) {
+    if (dims)
+      for (int d = 0; d < 3; ++d) dims[3 * i + d] = p->subs[(size_t)i].c->n[d];
+    if (givers) {
+      givers[2 * i] = p->subs[(size_t)i].m0;
+      givers[2 * i + 1] = p->subs[(size_t)i].m1;
+    }
+  }
+  return n;
+}
+
+int64_t fit_plan_device_bytes(const fit_plan* p) {
+  if (!p) return 0;
+  int64_t b = p->bytes;
+  for (const SubGrid& sg : p->subs) b += sg.c->bytes;
+  return b;
+}
 
 int fit_plan_set_option(fit_plan* p, int option, int value) {
   if (!p) return fail(FIT_E_ARGUMENT, "null plan");
   // pipeline-shape options apply to the sub-grid child as well
-  if (p->sub && option != FIT_OPT_PLAIN_PIPELINE && option != FIT_OPT_SUBGRID) {
-    int rc = fit_plan_set_option(p->sub, option, value);
-    if (rc) return rc;
-  }
+  if (option != FIT_OPT_PLAIN_PIPELINE && option != FIT_OPT_SUBGRID)
+    for (SubGrid& sg : p->subs) {
+      int rc = fit_plan_set_option(sg.c, option, value);
+      if (rc) return rc;
+    }
   switch (option) {
     case FIT_OPT_PLAIN_PIPELINE: p->plain = value != 0; return FIT_OK;
     case FIT_OPT_GIVER_BATCH:
@@ -733,7 +825,10 @@ int fit_profile_enable(fit_plan* p, int on) {
   p->nrec = 0;
   for (int i = 0; i < FIT_K_COUNT; ++i) { p->launches[i] = 0; p->ms[i] = 0; p->abytes[i] = 0; }
   p->kagg.clear();
-  if (p->sub) return fit_profile_enable(p->sub, on);
+  for (SubGrid& sg : p->subs) {
+    int rc = fit_profile_enable(sg.c, on);
+    if (rc) return rc;
+  }
   return FIT_OK;
 }
 

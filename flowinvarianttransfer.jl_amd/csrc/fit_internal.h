@@ -38,6 +38,14 @@ int fail(int code, const std::string& msg);
 bool make_desc(int L, FftDesc& d);
 int padded_len(int n);
 
+// one class of givers [m0, m1] of the band-limited sub-grid giver loop
+struct SubGrid {
+  fit_plan* c = nullptr;         // child plan on the class's grid (no dealiasing, same shells)
+  int q[3] = {0, 0, 0};          // folded box of input modes copied into the child
+  double tscale = 1.0;           // (Np_sub / Np)^2: N̂ carries 1/Np^2 of the grid it is taken on
+  int m0 = 1, m1 = 0;
+};
+
 struct ProfRec {
   hipEvent_t a = nullptr, b = nullptr;
   int cls = 0;
@@ -110,9 +118,11 @@ struct fit_plan {
   // band-limited sub-grid for the giver loop: when every shell lies in |k_d| <= R_d with
   // 4 R_d < n_d, T(n,m) only involves advecting modes |q_d| <= 2 R_d and the exact convolution
   // is evaluated alias-free on a grid of at least 4 R_d + 1 points per axis
-  fit_plan* sub = nullptr;       // child plan on that grid (no dealiasing, same shells)
-  int sub_q[3] = {0, 0, 0};      // folded box of input modes copied into the child
-  double sub_tscale = 1.0;       // (Np_sub / Np)^2: N̂ carries 1/Np^2 of the grid it is taken on
+  // (givers of smaller support need fewer advecting modes, so the givers are split into classes
+  // of consecutive shells, each on the smallest grid alias-free for it)
+  std::vector<fitx::SubGrid> subs;  // giver classes, each with its child plan
+  fit_plan* sub = nullptr;       // the child with the largest grid (reported by fit_plan_subgrid)
+  int giver_lo = 1, giver_hi = 0;  // giver shells this plan's loop runs (giver_hi = 0: all)
   void* sub_f = nullptr; size_t sub_f_bytes = 0;
   void* sub_a = nullptr; size_t sub_a_bytes = 0;
   bool own_stream = true;

@@ -680,7 +680,7 @@ template <typename R>
 int s2s_plain(fit_plan* p, int inv, const void* f, std::vector<double>& Tm) {
   int rc;
   const int nsh = p->nsh;
-  for (int m = 1; m <= nsh; ++m) {
+  for (int m = p->giver_lo; m <= (p->giver_hi > 0 ? p->giver_hi : nsh); ++m) {
     if ((rc = nonlinear_from_u<R>(p, f, m))) return rc;
     if ((rc = density_launch<R>(p, inv, f, p->nhat, nullptr, true))) return rc;
     FIT_HIP(hipMemcpyAsync(Tm.data() + (size_t)(m - 1) * nsh, p->dcol, nsh * sizeof(double), hipMemcpyDeviceToHost,
@@ -893,8 +893,9 @@ int s2s_batched(fit_plan* p, int inv, const void* f, std::vector<double>& Tm) {
   if ((rc = grow(p, &p->eh, &p->eh_bytes, (size_t)p->M * Hn * csz))) return rc;
   if ((rc = carrier_half<R>(p, inv, f, nullptr, 1, p->eh))) return rc;
   // batch size from the device memory budget
+  const int glo = p->giver_lo, ghi = p->giver_hi > 0 ? p->giver_hi : nsh;
   int bmax[3] = {-1, -1, -1};
-  for (int s = 0; s < nsh; ++s) {
+  for (int s = glo - 1; s < ghi; ++s) {
     bmax[0] = std::max(bmax[0], p->shell_band[s].x);
     bmax[1] = std::max(bmax[1], p->shell_band[s].y);
     bmax[2] = std::max(bmax[2], p->shell_band[s].z);
@@ -904,11 +905,11 @@ int s2s_batched(fit_plan* p, int inv, const void* f, std::vector<double>& Tm) {
   size_t freeb = 0, totalb = 0;
   FIT_HIP(hipMemGetInfo(&freeb, &totalb));
   const double budget = 0.6 * ((double)freeb + (double)p->work2_bytes + (double)p->nxbuf_bytes);
-  int ng = (int)std::max(1.0, std::min((double)nsh, budget / per_giver));
+  int ng = (int)std::max(1.0, std::min((double)(ghi - glo + 1), budget / per_giver));
   if (p->giver_batch > 0) ng = std::min(ng, p->giver_batch);
   if ((rc = grow(p, (void**)&p->dcolG, &p->dcolG_bytes, (size_t)ng * nsh * sizeof(double)))) return rc;
-  for (int m0 = 1; m0 <= nsh; m0 += ng) {
-    const int gb = std::min(ng, nsh - m0 + 1);
+  for (int m0 = glo; m0 <= ghi; m0 += ng) {
+    const int gb = std::min(ng, ghi - m0 + 1);
     int bi[3] = {-1, -1, -1};
     for (int s = m0 - 1; s < m0 - 1 + gb; ++s) {
       bi[0] = std::max(bi[0], p->shell_band[s].x);
@@ -943,30 +944,33 @@ int s2s_impl(fit_plan* p, int inv, const void* f, const void* a, std::vector<dou
 // (Np_sub/Np)² rescale of the shell sums.
 template <typename R>
 int s2s_sub(fit_plan* p, int inv, const void* f, const void* a, std::vector<double>& Tm) {
-  fit_plan* c = p->sub;
   const size_t csz = sizeof(cpx<R>);
-  auto gather = [&](const void* src, int ncomp, void** dst, size_t* have) -> int {
-    int r = grow(p, dst, have, (size_t)ncomp * c->Np * csz);
-    if (r) return r;
-    prof_begin(p, FIT_K_OTHER, (double)ncomp * c->Np * csz * 2.0, "k_subgrid_gather");
-    hipLaunchKernelGGL(k_subgrid_gather<R>, dim3(2048), dim3(256), 0, p->stream, (const cpx<R>*)src, p->n[0],
-                       p->n[1], p->n[2], p->Np, (cpx<R>*)*dst, c->n[0], c->n[1], c->n[2], c->Np, p->sub_q[0],
-                       p->sub_q[1], p->sub_q[2], ncomp);
-    prof_end(p);
-    FIT_HIP(hipGetLastError());
-    return FIT_OK;
-  };
-  int rc = gather(f, p->M, &p->sub_f, &p->sub_f_bytes);
-  if (rc) return rc;
-  const void* ca = p->sub_f;
-  if (a != f) {
-    if ((rc = gather(a, p->D, &p->sub_a, &p->sub_a_bytes))) return rc;
-    ca = p->sub_a;
+  const int nsh = p->nsh;
+  for (const SubGrid& sg : p->subs) {
+    fit_plan* c = sg.c;
+    auto gather = [&](const void* src, int ncomp, void** dst, size_t* have) -> int {
+      int r = grow(p, dst, have, (size_t)ncomp * c->Np * csz);
+      if (r) return r;
+      prof_begin(p, FIT_K_OTHER, (double)ncomp * c->Np * csz * 2.0, "k_subgrid_gather");
+      hipLaunchKernelGGL(k_subgrid_gather<R>, dim3(2048), dim3(256), 0, p->stream, (const cpx<R>*)src, p->n[0],
+                         p->n[1], p->n[2], p->Np, (cpx<R>*)*dst, c->n[0], c->n[1], c->n[2], c->Np, sg.q[0], sg.q[1],
+                         sg.q[2], ncomp);
+      prof_end(p);
+      FIT_HIP(hipGetLastError());
+      return FIT_OK;
+    };
+    int rc = gather(f, p->M, &p->sub_f, &p->sub_f_bytes);
+    if (rc) return rc;
+    const void* ca = p->sub_f;
+    if (a != f) {
+      if ((rc = gather(a, p->D, &p->sub_a, &p->sub_a_bytes))) return rc;
+      ca = p->sub_a;
+    }
+    c->prof = p->prof;
+    if ((rc = s2s_impl<R>(c, inv, p->sub_f, ca, Tm))) return rc;
+    FIT_HIP(hipStreamSynchronize(p->stream));
+    for (size_t i = (size_t)(sg.m0 - 1) * nsh; i < (size_t)sg.m1 * nsh; ++i) Tm[i] *= sg.tscale;
   }
-  c->prof = p->prof;
-  if ((rc = s2s_impl<R>(c, inv, p->sub_f, ca, Tm))) return rc;
-  FIT_HIP(hipStreamSynchronize(p->stream));
-  for (double& v : Tm) v *= p->sub_tscale;
   return FIT_OK;
 }
 

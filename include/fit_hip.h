@@ -103,6 +103,14 @@ int fit_plan_set_option(fit_plan* plan, int option, int value);
  * when such a grid is attached, 0 otherwise; negative on a NULL plan. */
 int fit_plan_subgrid(const fit_plan* plan, int64_t* dims);
 
+/* Giver classes of the band-limited giver loop: givers of smaller spectral support need fewer
+ * advecting modes (|q_d| <= R_d + P_d for support P_d), so consecutive giver shells are grouped by
+ * the smallest alias-free grid (L_d >= 2 R_d + 2 P_d + 1) and each class runs on its own grid.
+ * Writes up to max_classes entries: dims[3i..3i+2] = L_d of class i, givers[2i], givers[2i+1] =
+ * its first and last giver shell (1-based).  Returns the number of classes (0: no sub-grid),
+ * negative on a NULL plan. */
+int fit_plan_subgrid_classes(const fit_plan* plan, int64_t* dims, int* givers, int max_classes);
+
 /*
  * N̂ = keep·FFT[(u_adv·∇) f]/Np into N_hat (ns..., M).  Replaces
  * NonlinearTerm._compute_nonlinear_term! for the FFT path (NonlinearTerm.jl:102-116) i.e.

@@ -106,3 +106,30 @@ def test_no_subgrid_when_shells_reach_quarter_grid(fit, oracle):
     c, e, sidx = fit.grid.shell_setup(ks, fit.LinearBinning(1.0), fit.IsotropicShells())
     with fit.Plan(ns, 3, 3, ks, fit.OrszagTwoThirds(), np.float64, sidx, len(c)) as plan:
         assert plan.subgrid is None
+
+
+def test_giver_classes_on_smaller_grids(fit, oracle):
+    """Givers of small support run on smaller grids (L_d >= 2 R_d + 2 P_d + 1): the 64³ plan with
+    shells 0:9 (R = 8) splits its givers into classes on 24³, 32³ and 48³; every class's columns
+    equal the full-grid oracle and the plan's own grid."""
+    from fit_amd.grid import shell_setup
+    ns = (64, 64, 64)
+    say("case giver classes", ns)
+    ks = oracle.wavenumber_grid(ns, (2 * math.pi,) * 3)
+    u = oracle.synthetic_field(ns, 3, seed=23)
+    c, _, sidx = shell_setup(ks, fit.CustomBinning(range(0, 10)), None)
+    for prec, tol in ((np.float64, TOL64), (np.float32, TOL32)):
+        with fit.Plan(ns, 3, 3, ks, fit.OrszagTwoThirds(), prec, sidx, len(c)) as plan:
+            cls = plan.subgrid_classes
+            say("  classes", cls)
+            assert [k[0] for k in cls] == [(24,) * 3, (32,) * 3, (48,) * 3]
+            assert cls[0][1][0] == 1 and cls[-1][1][1] == len(c)
+            assert all(cls[i][1][1] + 1 == cls[i + 1][1][0] for i in range(len(cls) - 1))
+            assert cls[0][0][0] < cls[-1][0][0] and plan.subgrid == cls[-1][0]
+            Tm, _, _ = plan.shell_to_shell(fit.KineticEnergy(), u.astype(np.complex128 if prec == np.float64 else np.complex64))
+            plan.set_subgrid(False)
+            To, _, _ = plan.shell_to_shell(fit.KineticEnergy(), u.astype(np.complex128 if prec == np.float64 else np.complex64))
+        _, _, Tr, _, _ = oracle.calculate_shell_to_shell_transfer(u, ks, oracle.CustomBinning(tuple(range(0, 10))), "23")
+        e1, e2 = rel(Tm, Tr), rel(Tm, To)
+        say("  ", prec.__name__, "vs oracle", e1, "vs own grid", e2)
+        assert e1 < tol and e2 < tol
