@@ -8,7 +8,8 @@ from .api import (Ensemble, Plan, batch_map, calculate_band_to_band_transfer, ca
                   calculate_partial_fluxes, calculate_scalar_flux,
                   calculate_scalar_mode_to_mode_transfer, calculate_scalar_shell_to_shell_transfer,
                   calculate_shell_to_shell_transfer, calculate_spectral_flux, clear_plan_cache,
-                  compute_nonlinear_term, decompose_field, transfer_density)
+                  compute_nonlinear_term, decompose_field, pencil_spectral_flux, physical_to_spectral,
+                  transfer_density)
 from .grid import (as_component_field, assign_shells, default_binning, n_shells, shell_centers,
                    shell_coordinate, shell_edges, wavenumber_grid, wavenumber_magnitude_grid)
 from .types import (BandToBandResult, CustomBinning, DyadicBinning, Enstrophy, Helicity, HelicalDecomposition,
@@ -16,6 +17,6 @@ from .types import (BandToBandResult, CustomBinning, DyadicBinning, Enstrophy, H
                     LogarithmicBinning, ModeToModeTriadResult, NoDealiasing, NoDecomposition,
                     OrszagTwoThirds, PaddedThreeHalves, ParallelShells, PassiveScalar,
                     PerpendicularShells, ShellMagnitude, ShellToShellResult, SmoothBands,
-                    SpectralFluxResult)
+                    SpectralFluxResult, ToroidalPoloidalDecomposition)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -19,6 +19,7 @@ OPT_XPASS_STAGE = 8
 OPT_GRAD_SPLIT = 9
 OPT_FUSED_FLUX = 10
 ENS_OPT_GRAPH, ENS_OPT_BATCH = 1, 2
+DECOMP_HELICAL, DECOMP_TOROIDAL_POLOIDAL = 1, 2
 K_CLASSES = ("fft_column", "fft_row_inverse", "fft_row_forward", "density_shell", "reduce", "other",
              "fused_x")
 
@@ -47,6 +48,8 @@ SIGNATURES = {
     "fit_shell_to_shell": (_I, [_P, _I, _P, _P, _D, _D, _D, _I, _I, _I]),
     "fit_mode_to_mode": (_I, [_P, _I, _P, _P, _P, _P, _I, _I]),
     "fit_helical_decompose": (_I, [_P, _P, _P, _P, _I, _I]),
+    "fit_decompose": (_I, [_P, _I, _P, _P, _P, _I, _I]),
+    "fit_physical_to_spectral": (_I, [_P, _P, _I, _P, _I, _I]),
     "fit_band_to_band": (_I, [_P, _I, _D, _I, ctypes.c_double, _D, _P, _P, _D, _D, _D, _I, _I]),
     "fit_profile_enable": (_I, [_P, _I]),
     "fit_profile_query": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int64), _D, _D]),

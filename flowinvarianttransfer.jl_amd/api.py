@@ -17,9 +17,12 @@ import numpy as np
 from . import _lib
 from .grid import as_component_field, shell_coordinate, shell_setup
 from .types import (AbstractDealiasing, AbstractInvariant, BandToBandResult, HelicalDecomposition,
-                    HIPExecution, HIPSpectral, KineticEnergy, ModeToModeTriadResult,
+                    HIPExecution, HIPSpectral, KineticEnergy, ModeToModeTriadResult, NoDealiasing,
                     NoDecomposition, OrszagTwoThirds, PaddedThreeHalves, PassiveScalar,
-                    ShellMagnitude, ShellToShellResult, SmoothBands, SpectralFluxResult)
+                    ShellMagnitude, ShellToShellResult, SmoothBands, SpectralFluxResult,
+                    ToroidalPoloidalDecomposition)
+
+_SPLITS = (HelicalDecomposition, ToroidalPoloidalDecomposition)
 
 try:  # torch is only plumbing for device arrays
     import torch
@@ -60,6 +63,22 @@ def _marshal(a, io):
 def _caller_stream(t):
     """hipStream_t of torch's current stream on the device of tensor t."""
     return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def _marshal_real(a, io):
+    """-> (pointer, keep-alive object, ptr_kind, io) for a real array, column-major; io follows the
+    array's own precision (F32 / F64)."""
+    if _is_device(a):
+        t = a if a.dtype in (torch.float32, torch.float64) else a.to(torch.float64)
+        rev = tuple(reversed(range(t.dim())))
+        if not t.permute(rev).is_contiguous():
+            t = t.permute(rev).contiguous().permute(rev)
+        return t.data_ptr(), t, _lib.PTR_DEVICE, (_lib.F64 if t.dtype == torch.float64 else _lib.F32)
+    arr = np.asarray(a)
+    if arr.dtype not in (np.float32, np.float64):
+        arr = arr.astype(np.float64)
+    arr = np.asfortranarray(arr)
+    return arr.ctypes.data, arr, _lib.PTR_HOST, (_lib.F64 if arr.dtype == np.float64 else _lib.F32)
 
 
 def _alloc(like_device, shape, io, complex_=True):
@@ -313,14 +332,29 @@ class Plan:
                                              kind))
         return net, S
 
-    def helical_decompose(self, u):
+    def decompose(self, decomposition, u):
+        """The two named parts of u (fit_decompose): helical (positive, negative) or
+        toroidal/poloidal (toroidal, poloidal)."""
         up, uk, kind = _marshal(u, self.io)
         like = uk if kind == _lib.PTR_DEVICE else None
         a, ap = _alloc(like, self.ns + (self.M,), self.io)
         b, bp = _alloc(like, self.ns + (self.M,), self.io)
         self._order(kind, uk)
-        _lib.check(self.lib.fit_helical_decompose(self.handle, up, ap, bp, self.io, kind))
+        _lib.check(self.lib.fit_decompose(self.handle, decomposition.code, up, ap, bp, self.io, kind))
         return a, b
+
+    def physical_to_spectral(self, u_phys):
+        """û = fft(u)/Np per component of a real physical field (ns..., C) (fit_physical_to_spectral),
+        in the field's own precision; device in, device out."""
+        up, uk, kind, io = _marshal_real(u_phys, self.io)
+        C = int(u_phys.shape[self.nd]) if len(u_phys.shape) == self.nd + 1 else 1
+        out, op = _alloc(uk if kind == _lib.PTR_DEVICE else None, self.ns + (C,), io)
+        self._order(kind, uk)
+        _lib.check(self.lib.fit_physical_to_spectral(self.handle, up, C, op, io, kind))
+        return out
+
+    def helical_decompose(self, u):
+        return self.decompose(HelicalDecomposition(), u)
 
 
 # A plan is not safe for concurrent calls (its workspaces are shared by its launches), so cached
@@ -428,11 +462,12 @@ def calculate_spectral_flux(velocity_hat, ks, *, binning=None, dealiasing=Orszag
     plan = _plan_for(ns, M, D, ks, dealiasing, spectral.compute_dtype(ft), spectral.device,
                      binning, geometry, shells=True)
     centers = _ft_cast(plan.centers, ft)
-    if isinstance(decomposition, HelicalDecomposition):
+    if isinstance(decomposition, _SPLITS):
+        # SpectralFlux.jl:96-122: one N̂ of the full field, the density of each named part
         N = plan.nonlinear_term(velocity_hat, advecting_hat)
-        up, um = plan.helical_decompose(velocity_hat)
+        parts = plan.decompose(decomposition, velocity_hat)
         res = {}
-        for name, comp in (("positive", up), ("negative", um)):
+        for name, comp in zip(decomposition.names, parts):
             T, Pi = plan.spectral_flux_with_nhat(invariant, comp, N)
             res[name] = SpectralFluxResult(centers, _ft_cast(T, ft), _ft_cast(Pi, ft))
         return res
@@ -540,21 +575,82 @@ def calculate_scalar_mode_to_mode_transfer(velocity_hat, scalar_hat, ks, **kw):
 
 
 def decompose_field(decomposition, velocity_hat, ks, *, spectral=HIPSpectral()):
-    """Decomposition.jl:42-98 (NoDecomposition -> identity; helical on the device)."""
+    """Decomposition.jl:42-141 (NoDecomposition -> identity; the helical and toroidal/poloidal
+    splits on the device, returned as named parts in the reference's order)."""
     if isinstance(decomposition, NoDecomposition):
         return velocity_hat
-    if not isinstance(decomposition, HelicalDecomposition):
+    if not isinstance(decomposition, _SPLITS):
         raise ValueError(f"decomposition {decomposition!r} is not on the HIP path")
     nd = len(ks)
+    name = type(decomposition).__name__
     if nd != 3:
-        raise ValueError(f"HelicalDecomposition is defined in 3D only (got nd={nd}).")
+        raise ValueError(f"{name} is defined in 3D only (got nd={nd}).")
     ns = tuple(int(s) for s in velocity_hat.shape[:nd])
     M = int(velocity_hat.shape[nd])
+    if M < 3:
+        raise ValueError(f"{name} needs ≥3 velocity components (got D={M}).")
     ft = _real_dtype(velocity_hat)
     plan = _plan_for(ns, M, max(M, nd), ks, OrszagTwoThirds(), spectral.compute_dtype(ft),
                      spectral.device)
-    up, um = plan.helical_decompose(velocity_hat)
-    return {"positive": up, "negative": um}
+    a, b = plan.decompose(decomposition, velocity_hat)
+    return dict(zip(decomposition.names, (a, b)))
+
+
+def physical_to_spectral(u_phys, ks, *, spectral=HIPSpectral()):
+    """Spectral coefficients of a physical field in the package's convention, û = fft(u)/Np per
+    component (the front-end of PencilFFTsExt.jl:83-88), on the device.  u_phys: (ns...,) or
+    (ns..., C); a complex field is transformed as its real and imaginary parts,
+    fft(a + i b) = fft(a) + i fft(b)."""
+    _check_spectral(spectral)
+    nd = len(ks)
+    shape = tuple(int(n) for n in u_phys.shape)
+    ns = shape[:nd]
+    single = len(shape) == nd
+    if single:
+        u_phys = u_phys[..., None]
+    elif len(shape) != nd + 1:
+        raise _lib.DimensionMismatch(f"expected an (ns...) or (ns..., C) array with nd={nd}")
+    C = int(u_phys.shape[nd])
+    is_complex = (u_phys.is_complex() if _is_device(u_phys) else np.iscomplexobj(u_phys))
+    if is_complex:
+        re = physical_to_spectral(u_phys.real, ks, spectral=spectral)
+        im = physical_to_spectral(u_phys.imag, ks, spectral=spectral)
+        out = re + 1j * im
+        return out[..., 0] if single else out
+    if _is_device(u_phys):
+        ft = np.float64 if u_phys.dtype == torch.float64 else np.float32
+    else:
+        u_phys = np.asarray(u_phys)
+        ft = np.float32 if u_phys.dtype == np.float32 else np.float64
+    plan = _plan_for(ns, max(C, 1), max(C, nd), ks, NoDealiasing(), spectral.compute_dtype(ft), spectral.device)
+    out = plan.physical_to_spectral(u_phys)
+    return out[..., 0] if single else out
+
+
+def pencil_spectral_flux(u_phys, ks, *, binning, dealiasing=OrszagTwoThirds(), invariant=KineticEnergy(),
+                         geometry=None, spectral=HIPSpectral()):
+    """PencilFFTsExt.jl:40-137 on one GPU: the kinetic-energy flux of a PHYSICAL velocity (a tuple of
+    nd component arrays, or one (ns..., nd) array), with û = fft(u)/Np formed on the device.  Same
+    argument checks and result fields (k_shells, transfer_spectrum, flux) as the reference."""
+    if not isinstance(invariant, KineticEnergy):
+        raise ValueError("pencil_spectral_flux currently supports KineticEnergy only "
+                         f"(got {type(invariant).__name__}).")
+    if geometry is not None and not (isinstance(geometry, ShellMagnitude) and geometry.dims is None):
+        raise ValueError("pencil_spectral_flux currently supports isotropic |k| shells (IsotropicShells()) only.")
+    nd = len(ks)
+    if isinstance(u_phys, (tuple, list)):
+        D = len(u_phys)
+        if _is_device(u_phys[0]):
+            u_phys = torch.stack(list(u_phys), dim=-1)
+        else:
+            u_phys = np.stack([np.asarray(c) for c in u_phys], axis=-1)
+    else:
+        D = int(u_phys.shape[nd]) if len(u_phys.shape) == nd + 1 else 1
+    if D != nd:
+        raise ValueError(f"got {D} velocity components for an {nd}-D grid; pencil flux needs D == nd.")
+    uh = physical_to_spectral(u_phys, ks, spectral=spectral)
+    return calculate_spectral_flux(uh, ks, binning=binning, dealiasing=dealiasing, invariant=invariant,
+                                   spectral=spectral)
 
 
 def calculate_partial_fluxes(velocity_hat, ks, *, decomposition=HelicalDecomposition(),
@@ -755,16 +851,22 @@ class Ensemble:
 
 def calculate_ensemble_fluxes(velocities, ks_list, scalars=None, *, binning=None,
                               dealiasing=OrszagTwoThirds(), spectral=HIPSpectral(),
-                              invariant=KineticEnergy(), geometry=None):
+                              invariant=KineticEnergy(), geometry=None, physical=False):
     """Spectral fluxes of an ensemble of snapshots of possibly different sizes, in input order
     (the collation contract of mpi_batch_map, MPIExt.jl:14-23, :39-68).
 
     velocities[j]: (ns_j..., D) array; ks_list[j]: its wavenumber vectors; scalars[j] (optional):
-    a scalar advected by velocities[j].  Returns a list of (flux of `invariant`, scalar-variance
+    a scalar advected by velocities[j]; physical=True: the arrays are physical fields, transformed
+    to û = fft(u)/Np on the device first.  Returns a list of (flux of `invariant`, scalar-variance
     flux or None).  Kinetic-energy ensembles of 2D / 3D snapshots (D = nd, no padding) run as one
     batched, graph-captured fit_ensemble; anything else runs snapshot by snapshot on cached
     plans."""
     _check_spectral(spectral)
+    if physical:
+        # physical snapshots: û = fft(u)/Np on the device first (PencilFFTsExt.jl:83-88)
+        velocities = [physical_to_spectral(v, k, spectral=spectral) for v, k in zip(velocities, ks_list)]
+        if scalars is not None:
+            scalars = [physical_to_spectral(t, k, spectral=spectral) for t, k in zip(scalars, ks_list)]
     if scalars is not None and len(scalars) != len(velocities):
         raise ValueError("scalars must match velocities one to one")
     if len(ks_list) != len(velocities):

@@ -6,6 +6,7 @@ template int nonlinear_term_impl<double>(fit_plan*, const void*, const void*);
 template int density_launch<double>(fit_plan*, int, const void*, const void*, void*, bool);
 template int density_multi<double>(fit_plan*, int, const int*, const void*, const void*);
 template int flux_multi<double>(fit_plan*, int, const int*, const void*, const void*);
+template int phys_to_spec<double>(fit_plan*, const void*, int, int, void*);
 template int density_shells_into<double>(fit_plan*, int, const void*, const void*, double*, int, double*);
 template int s2s_entry<double>(fit_plan*, int, const void*, const void*, std::vector<double>&);
 template int b2b_impl<double>(fit_plan*, int, const void*, const void*, const double*, const double*, int, double,

@@ -1028,12 +1028,18 @@ int fit_mode_to_mode(fit_plan* p, int inv, const void* f_hat, const void* adv_ha
   return finish(p);
 }
 
-int fit_helical_decompose(fit_plan* p, const void* u_hat, void* u_plus, void* u_minus, int io_dtype, int ptr_kind) {
+int fit_decompose(fit_plan* p, int decomposition, const void* u_hat, void* out_a, void* out_b, int io_dtype,
+                  int ptr_kind) {
   int rc = check_plan(p);
   if (rc) return rc;
-  if (p->nd != 3) return fail(FIT_E_ARGUMENT, "HelicalDecomposition is defined in 3D only (got nd=" + std::to_string(p->nd) + ").");
-  if (p->M < 3) return fail(FIT_E_ARGUMENT, "HelicalDecomposition needs >=3 velocity components");
-  if (!u_hat || !u_plus || !u_minus) return fail(FIT_E_ARGUMENT, "NULL array");
+  const char* name = decomposition == FIT_DECOMP_HELICAL ? "HelicalDecomposition"
+                     : decomposition == FIT_DECOMP_TOROIDAL_POLOIDAL ? "ToroidalPoloidalDecomposition"
+                                                                     : nullptr;
+  if (!name) return fail(FIT_E_ARGUMENT, "unknown decomposition code");
+  if (p->nd != 3)
+    return fail(FIT_E_ARGUMENT, std::string(name) + " is defined in 3D only (got nd=" + std::to_string(p->nd) + ").");
+  if (p->M < 3) return fail(FIT_E_ARGUMENT, std::string(name) + " needs >=3 velocity components");
+  if (!u_hat || !out_a || !out_b) return fail(FIT_E_ARGUMENT, "NULL array");
   const void* u;
   if ((rc = stage_in(p, u_hat, p->M, io_dtype, ptr_kind, &p->in_f, &u))) return rc;
   const int64_t Np = p->Np;
@@ -1042,14 +1048,24 @@ int fit_helical_decompose(fit_plan* p, const void* u_hat, void* u_plus, void* u_
   if (!p->in_c) { if ((rc = dalloc(p, &p->in_c, (size_t)std::max(p->M, p->D) * Np * cs))) return rc; }
   if (p->dtype == FIT_F64) {
     Grid g = grid_of<double>(p);
-    hipLaunchKernelGGL(k_helical<double>, dim3(1024), dim3(256), 0, p->stream, g, (const double2*)u, Np,
-                       (const double*)p->ks[0], (const double*)p->ks[1], (const double*)p->ks[2], (double2*)p->nhat,
-                       (double2*)p->in_c, Np);
+    if (decomposition == FIT_DECOMP_HELICAL)
+      hipLaunchKernelGGL(k_helical<double>, dim3(1024), dim3(256), 0, p->stream, g, (const double2*)u, Np,
+                         (const double*)p->ks[0], (const double*)p->ks[1], (const double*)p->ks[2], (double2*)p->nhat,
+                         (double2*)p->in_c, Np);
+    else
+      hipLaunchKernelGGL(k_torpol<double>, dim3(1024), dim3(256), 0, p->stream, g, (const double2*)u, Np,
+                         (const double*)p->ks[0], (const double*)p->ks[1], (const double*)p->ks[2], (double2*)p->nhat,
+                         (double2*)p->in_c, Np);
   } else {
     Grid g = grid_of<float>(p);
-    hipLaunchKernelGGL(k_helical<float>, dim3(1024), dim3(256), 0, p->stream, g, (const float2*)u, Np,
-                       (const float*)p->ks[0], (const float*)p->ks[1], (const float*)p->ks[2], (float2*)p->nhat,
-                       (float2*)p->in_c, Np);
+    if (decomposition == FIT_DECOMP_HELICAL)
+      hipLaunchKernelGGL(k_helical<float>, dim3(1024), dim3(256), 0, p->stream, g, (const float2*)u, Np,
+                         (const float*)p->ks[0], (const float*)p->ks[1], (const float*)p->ks[2], (float2*)p->nhat,
+                         (float2*)p->in_c, Np);
+    else
+      hipLaunchKernelGGL(k_torpol<float>, dim3(1024), dim3(256), 0, p->stream, g, (const float2*)u, Np,
+                         (const float*)p->ks[0], (const float*)p->ks[1], (const float*)p->ks[2], (float2*)p->nhat,
+                         (float2*)p->in_c, Np);
   }
   FIT_HIP(hipGetLastError());
   // components beyond 3 are zero in both outputs
@@ -1057,13 +1073,45 @@ int fit_helical_decompose(fit_plan* p, const void* u_hat, void* u_plus, void* u_
     FIT_HIP(hipMemsetAsync((char*)p->nhat + 3 * Np * cs, 0, (size_t)(p->M - 3) * Np * cs, p->stream));
     FIT_HIP(hipMemsetAsync((char*)p->in_c + 3 * Np * cs, 0, (size_t)(p->M - 3) * Np * cs, p->stream));
   }
-  if ((rc = stage_out_real(p, p->nhat, u_plus, 2 * (int64_t)p->M * Np, io_dtype, ptr_kind))) return rc;
+  if ((rc = stage_out_real(p, p->nhat, out_a, 2 * (int64_t)p->M * Np, io_dtype, ptr_kind))) return rc;
   if ((rc = finish(p))) return rc;
-  if ((rc = stage_out_real(p, p->in_c, u_minus, 2 * (int64_t)p->M * Np, io_dtype, ptr_kind))) return rc;
+  if ((rc = stage_out_real(p, p->in_c, out_b, 2 * (int64_t)p->M * Np, io_dtype, ptr_kind))) return rc;
   return finish(p);
 }
 
+int fit_physical_to_spectral(fit_plan* p, const void* u_phys, int ncomp, void* u_hat, int io_dtype, int ptr_kind) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!u_phys || !u_hat) return fail(FIT_E_ARGUMENT, "NULL array");
+  if (io_dtype != FIT_F32 && io_dtype != FIT_F64) return fail(FIT_E_ARGUMENT, "io_dtype must be FIT_F32 or FIT_F64");
+  if (ptr_kind != FIT_PTR_HOST && ptr_kind != FIT_PTR_DEVICE) return fail(FIT_E_ARGUMENT, "bad ptr_kind");
+  if (ncomp < 1 || ncomp > std::max(p->M, 3))
+    return fail(FIT_E_DIMENSION, "ncomp must be between 1 and max(M, 3) of the plan");
+  if (p->dealias == FIT_DEALIAS_PADDED)
+    return fail(FIT_E_UNSUPPORTED, "the physical-to-spectral transform runs on a plan without 3/2 padding");
+  const size_t ib = io_dtype == FIT_F64 ? 8 : 4;
+  const void* src = u_phys;
+  void* tmp = nullptr;
+  if (ptr_kind == FIT_PTR_HOST) {
+    FIT_HIP(hipMalloc(&tmp, (size_t)ncomp * p->Np * ib));
+    hipError_t e = hipMemcpyAsync(tmp, u_phys, (size_t)ncomp * p->Np * ib, hipMemcpyHostToDevice, p->stream);
+    if (e != hipSuccess) { hipFree(tmp); return fail(FIT_E_DEVICE, hipGetErrorString(e)); }
+    src = tmp;
+  }
+  // a device output of the compute precision is written directly; otherwise through nhat
+  const bool direct = ptr_kind == FIT_PTR_DEVICE && io_dtype == p->dtype;
+  void* dst = direct ? u_hat : p->nhat;
+  rc = p->dtype == FIT_F64 ? phys_to_spec<double>(p, src, io_dtype, ncomp, dst)
+                           : phys_to_spec<float>(p, src, io_dtype, ncomp, dst);
+  if (!rc && !direct) rc = stage_out_real(p, p->nhat, u_hat, 2 * (int64_t)ncomp * p->Np, io_dtype, ptr_kind);
+  if (!rc) rc = finish(p);
+  if (tmp) hipFree(tmp);
+  return rc;
+}
 
+int fit_helical_decompose(fit_plan* p, const void* u_hat, void* u_plus, void* u_minus, int io_dtype, int ptr_kind) {
+  return fit_decompose(p, FIT_DECOMP_HELICAL, u_hat, u_plus, u_minus, io_dtype, ptr_kind);
+}
 
 int fit_band_to_band(fit_plan* p, int inv, const double* centers, int n_bands, double logwidth,
                      const double* k_coord, const void* f_hat, const void* adv_hat, double* Tmat, double* net,

@@ -290,6 +290,7 @@ template <typename R> int density_launch(fit_plan* p, int inv, const void* car, 
                                          bool shells);
 template <typename R> int density_multi(fit_plan* p, int ninv, const int* invs, const void* car, const void* nh);
 template <typename R> int flux_multi(fit_plan* p, int ninv, const int* invs, const void* f, const void* a);
+template <typename R> int phys_to_spec(fit_plan* p, const void* u, int phys_dtype, int ncomp, void* out);
 template <typename R> int density_shells_into(fit_plan* p, int inv, const void* car, const void* nh, double* partials,
                                               int nblk, double* out);
 template <typename R> int s2s_entry(fit_plan* p, int inv, const void* f, const void* a, std::vector<double>& Tm);
@@ -304,6 +305,7 @@ template <typename R> int ens_run(fit_ensemble* e);
   extern template int density_launch<R>(fit_plan*, int, const void*, const void*, void*, bool);                 \
   extern template int density_multi<R>(fit_plan*, int, const int*, const void*, const void*);                   \
   extern template int flux_multi<R>(fit_plan*, int, const int*, const void*, const void*);                      \
+  extern template int phys_to_spec<R>(fit_plan*, const void*, int, int, void*);                                \
   extern template int density_shells_into<R>(fit_plan*, int, const void*, const void*, double*, int, double*);   \
   extern template int s2s_entry<R>(fit_plan*, int, const void*, const void*, std::vector<double>&);            \
   extern template int b2b_impl<R>(fit_plan*, int, const void*, const void*, const R*, const double*, int, double, \

@@ -87,6 +87,14 @@ template <typename R> struct LdReal {
   }
 };
 
+// real physical field stored in precision T, read in compute precision R
+template <typename R, typename T> struct LdRealAs {
+  const T* p; int64_t sy, sz, sf;
+  __device__ __forceinline__ R operator()(int x, int y, int z, int f) const {
+    return (R)p[x + y * sy + z * sz + f * sf];
+  }
+};
+
 // Hermitian part, on the transform grid, of  mult(o) * src(o)  where o is the original-grid
 // preimage of the bin (zero outside the embedded block) and
 //   mult = keep(o) [2/3] * (i ks_j[o_j] if gradj >= 0) * 1[shell(o) == giver] (if giver > 0).
@@ -1816,6 +1824,45 @@ __global__ void k_helical(Grid g, const cpx<R>* __restrict__ u, int64_t usf, con
       const cpx<R> hm = mk<R>(e1[c] * is2, -e2[c] * is2);
       up[o + c * osf] = cmul(upc, hp);
       um[o + c * osf] = cmul(umc, hm);
+    }
+  }
+}
+
+
+// ---------------------------------------------------------------------------------------------
+// Toroidal / poloidal split (Decomposition.jl:107-141): e1 = k̂ × ẑ / |k̂ × ẑ| (horizontal), e2 =
+// k̂ × e1; (x̂, ŷ) when k_⊥ = 0; tor = (û·e1) e1, pol = (û·e2) e2; zero at k = 0.
+// ---------------------------------------------------------------------------------------------
+template <typename R>
+__global__ void k_torpol(Grid g, const cpx<R>* __restrict__ u, int64_t usf, const R* ks0, const R* ks1,
+                         const R* ks2, cpx<R>* __restrict__ tor, cpx<R>* __restrict__ pol, int64_t osf) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < g.Np; o += stride) {
+    const int ix = (int)(o % g.n1), iy = (int)((o / g.n1) % g.n2), iz = (int)(o / ((int64_t)g.n1 * g.n2));
+    const R kx = ks0[ix], ky = ks1[iy], kz = ks2[iz];
+    const R kk = sqrt(kx * kx + ky * ky + kz * kz);
+    const cpx<R> zero = mk<R>((R)0, (R)0);
+    if (kk == (R)0) {
+      for (int c = 0; c < 3; ++c) { tor[o + c * osf] = zero; pol[o + c * osf] = zero; }
+      continue;
+    }
+    const R hx = kx / kk, hy = ky / kk, hz = kz / kk;
+    const R kperp = sqrt(kx * kx + ky * ky);
+    R e1[3], e2[3];
+    if (kperp > (R)0) {
+      const R n1 = sqrt(hy * hy + hx * hx);
+      e1[0] = hy / n1; e1[1] = -hx / n1; e1[2] = (R)0;
+      e2[0] = hy * e1[2] - hz * e1[1]; e2[1] = hz * e1[0] - hx * e1[2]; e2[2] = hx * e1[1] - hy * e1[0];
+    } else {
+      e1[0] = (R)1; e1[1] = (R)0; e1[2] = (R)0;
+      e2[0] = (R)0; e2[1] = (R)1; e2[2] = (R)0;
+    }
+    const cpx<R> u1 = u[o], u2 = u[o + usf], u3 = u[o + 2 * usf];
+    const cpx<R> c1 = cadd(cadd(cscale(u1, e1[0]), cscale(u2, e1[1])), cscale(u3, e1[2]));
+    const cpx<R> c2 = cadd(cadd(cscale(u1, e2[0]), cscale(u2, e2[1])), cscale(u3, e2[2]));
+    for (int c = 0; c < 3; ++c) {
+      tor[o + c * osf] = cscale(c1, e1[c]);
+      pol[o + c * osf] = cscale(c2, e2[c]);
     }
   }
 }

@@ -675,6 +675,38 @@ int flux_multi(fit_plan* p, int ninv, const int* invs, const void* f, const void
   return density_multi<R>(p, ninv, invs, f, p->nhat);
 }
 
+// Physical -> spectral front-end (PencilFFTsExt.jl:83-88): û_c = fft(u_c)/Np for ncomp real
+// physical components (precision T, (ns..., ncomp) column-major) into the full complex spectral
+// array out (compute precision, (ns..., ncomp)).  r2c row pass, column passes, then the full-grid
+// storer (each original index from its own half-grid bin or the conjugate of its mirror).
+template <typename R, typename T>
+int phys_to_spec_t(fit_plan* p, const T* u, int ncomp, void* out) {
+  Grid g = grid_of<R>(p);
+  for (int c = 0; c < ncomp; ++c) {
+    LdRealAs<R, T> lr{u + (int64_t)c * p->Np, (int64_t)p->n[0], (int64_t)p->n[0] * p->n[1], 0};
+    StNhat<R> sn{g, (cpx<R>*)out + (int64_t)c * p->Np, 0, 0, (R)(1.0 / (double)p->Np)};
+    int rc;
+    if (p->nd == 1) {
+      if ((rc = launch_row_fwd<R>(p, lr, sn, 1, p->h1))) return rc;
+      continue;
+    }
+    if ((rc = launch_row_fwd<R>(p, lr, st_work<R>(p), 1, p->h1))) return rc;
+    if (p->nd == 2) {
+      if ((rc = launch_col_full<R, -1>(p, ld_work<R>(p), sn, 1, 1))) return rc;
+      continue;
+    }
+    if ((rc = launch_col_full<R, -1>(p, ld_work<R>(p), st_work<R>(p), 1, 1))) return rc;
+    if ((rc = launch_col_full<R, -1>(p, ld_work<R>(p), sn, 2, 1))) return rc;
+  }
+  return FIT_OK;
+}
+
+template <typename R>
+int phys_to_spec(fit_plan* p, const void* u, int phys_dtype, int ncomp, void* out) {
+  return phys_dtype == FIT_F64 ? phys_to_spec_t<R, double>(p, (const double*)u, ncomp, out)
+                               : phys_to_spec_t<R, float>(p, (const float*)u, ncomp, out);
+}
+
 // Plain giver loop: one giver per pipeline pass, full transforms, N̂ through HBM.
 template <typename R>
 int s2s_plain(fit_plan* p, int inv, const void* f, std::vector<double>& Tm) {

@@ -68,7 +68,15 @@ class NoDecomposition(AbstractFieldDecomposition):
 
 
 class HelicalDecomposition(AbstractFieldDecomposition):
-    pass
+    """Types.jl: positive / negative helical parts (Decomposition.jl:63-98)."""
+    names = ("positive", "negative")
+    code = 1
+
+
+class ToroidalPoloidalDecomposition(AbstractFieldDecomposition):
+    """Types.jl:195-209: toroidal / poloidal Craya-Herring parts (Decomposition.jl:107-141)."""
+    names = ("toroidal", "poloidal")
+    code = 2
 
 
 # --- binning (Types.jl:370-421) ---------------------------------------------------------------

@@ -214,11 +214,35 @@ int fit_band_to_band(fit_plan* plan, int invariant, const double* centers, int n
                      double* max_asym, int io_dtype, int ptr_kind);
 
 /*
- * Helical (Craya-Herring) split of a 3D velocity into u+ and u- (each (ns..., 3)).
- * Replaces Decomposition._decompose_field_spectral(::HelicalDecomposition) (Decomposition.jl:63-98).
+ * Helical (Craya-Herring) split of a 3D velocity into u+ and u- (each (ns..., M), components past
+ * the third zero).  Replaces Decomposition._decompose_field_spectral(::HelicalDecomposition)
+ * (Decomposition.jl:63-98).  Same as fit_decompose(plan, FIT_DECOMP_HELICAL, ...).
  */
 int fit_helical_decompose(fit_plan* plan, const void* u_hat, void* u_plus, void* u_minus,
                           int io_dtype, int ptr_kind);
+
+/*
+ * Physical -> spectral front-end: u_hat[..., c] = fft(u_phys[..., c]) / Np for ncomp REAL physical
+ * components (ns..., ncomp) in io_dtype, into the full complex spectral array (ns..., ncomp) in
+ * io_dtype -- the package's coefficient convention, as the reference's distributed front-end
+ * forms it (ext/FlowInvariantTransferPencilFFTsExt.jl:83-88).  The plan must not be 3/2-padded;
+ * 1 <= ncomp <= max(M, 3).
+ */
+int fit_physical_to_spectral(fit_plan* plan, const void* u_phys, int ncomp, void* u_hat, int io_dtype,
+                             int ptr_kind);
+
+/* spectral decompositions of a 3D velocity into two named components */
+enum { FIT_DECOMP_HELICAL = 1, FIT_DECOMP_TOROIDAL_POLOIDAL = 2 };
+
+/*
+ * Split a 3D velocity (ns..., M >= 3) into two components (each (ns..., M), components past the
+ * third zero): FIT_DECOMP_HELICAL -> (positive, negative) helical parts (Decomposition.jl:63-98);
+ * FIT_DECOMP_TOROIDAL_POLOIDAL -> (toroidal, poloidal) Craya-Herring parts (Decomposition.jl:107-141,
+ * e1 = k × ẑ / |k × ẑ|, e2 = k̂ × e1, (x̂, ŷ) for k ∥ ẑ).  Replaces
+ * Decomposition._decompose_field_spectral for both.
+ */
+int fit_decompose(fit_plan* plan, int decomposition, const void* u_hat, void* out_a, void* out_b, int io_dtype,
+                  int ptr_kind);
 
 /* ---- ensembles of snapshots ---------------------------------------------------------------- */
 

@@ -483,8 +483,8 @@ def calculate_spectral_flux(u_hat, ks, binning=None, dealiasing="23", invariant=
     centers, edges, sidx = shell_setup(ks, binning, geometry)
     nsh = len(centers)
     N_hat = compute_nonlinear_term(u_hat, ks, dealiasing, spectral, advecting_hat, workers)
-    if decomposition is not None and _kind(decomposition) == "HelicalDecomposition":
-        comps = helical_decompose(u_hat, ks)
+    if decomposition is not None and _kind(decomposition) != "NoDecomposition":
+        comps = decompose(decomposition, u_hat, ks)
         res = {}
         for name, comp in comps.items():
             T = shell_sum(transfer_density(invariant, comp, N_hat, ks), sidx, nsh)
@@ -744,10 +744,57 @@ def helical_decompose(u_hat, ks):
     return {"positive": up, "negative": um}
 
 
+def toroidal_poloidal_decompose(u_hat, ks):
+    """Decomposition.jl:107-141 -- Craya–Herring toroidal / poloidal split: e1 = k̂ × ẑ / |k̂ × ẑ|
+    (horizontal, toroidal direction), e2 = k̂ × e1 (poloidal); for k ∥ ẑ (k_⊥ = 0) the pair
+    (x̂, ŷ); tor = (û·e1) e1, pol = (û·e2) e2, zero at k = 0 and in components past the third."""
+    nd = len(ks)
+    if nd != 3:
+        raise ValueError(f"ToroidalPoloidalDecomposition is defined in 3D only (got nd={nd}).")
+    if u_hat.shape[nd] < 3:
+        raise ValueError("ToroidalPoloidalDecomposition needs ≥3 velocity components")
+    kx, ky, kz = _mesh(ks)
+    kk = np.sqrt(kx * kx + ky * ky + kz * kz)
+    zero = kk == 0
+    kks = np.where(zero, 1.0, kk)
+    kh = (kx / kks, ky / kks, kz / kks)
+    kperp = np.sqrt(kx * kx + ky * ky)
+    deg = kperp == 0
+    e1 = (kh[1], -kh[0], np.zeros_like(kk))                       # k̂ × ẑ
+    n1 = np.sqrt(e1[0] ** 2 + e1[1] ** 2)
+    n1 = np.where(deg, 1.0, n1)
+    e1 = (np.where(deg, 1.0, e1[0] / n1), np.where(deg, 0.0, e1[1] / n1), np.zeros_like(kk))
+    e2 = (kh[1] * e1[2] - kh[2] * e1[1], kh[2] * e1[0] - kh[0] * e1[2], kh[0] * e1[1] - kh[1] * e1[0])
+    e2 = (np.where(deg, 0.0, e2[0]), np.where(deg, 1.0, e2[1]), np.where(deg, 0.0, e2[2]))
+    u = [u_hat[..., c] for c in range(3)]
+    c1 = u[0] * e1[0] + u[1] * e1[1] + u[2] * e1[2]
+    c2 = u[0] * e2[0] + u[1] * e2[1] + u[2] * e2[2]
+    tor = np.zeros(u_hat.shape, dtype=np.complex128)
+    pol = np.zeros(u_hat.shape, dtype=np.complex128)
+    for c in range(3):
+        tor[..., c] = np.where(zero, 0, c1 * e1[c])
+        pol[..., c] = np.where(zero, 0, c2 * e2[c])
+    return {"toroidal": tor, "poloidal": pol}
+
+
+def decompose(decomposition, u_hat, ks):
+    """Decomposition.jl:42-141 -- the spectral decompositions on the path: NoDecomposition returns
+    û itself; the helical and toroidal/poloidal splits return named components in order."""
+    k = _kind(decomposition)
+    if k == "NoDecomposition":
+        return u_hat
+    if k == "HelicalDecomposition":
+        return helical_decompose(u_hat, ks)
+    if k == "ToroidalPoloidalDecomposition":
+        return toroidal_poloidal_decompose(u_hat, ks)
+    raise ValueError(f"decomposition {decomposition!r} is not a spectral decomposition of this path")
+
+
 def calculate_partial_fluxes(u_hat, ks, binning=None, dealiasing="23", spectral="FFTBackend",
-                             geometry=None, workers=1):
-    """SpectralFlux.jl:240-296 with the helical decomposition: 8 channels (s_k, s_p, s_q)."""
-    comps = helical_decompose(u_hat, ks)
+                             geometry=None, workers=1, decomposition="HelicalDecomposition"):
+    """SpectralFlux.jl:240-285: channels (s_k, s_p, s_q) of Re{û_{s_k}* · [(u_{s_p}·∇) u_{s_q}]} over
+    the named components of the decomposition (8 for the helical and toroidal/poloidal splits)."""
+    comps = decompose(decomposition, u_hat, ks)
     centers, edges, sidx = shell_setup(ks, binning, geometry)
     nsh = len(centers)
     ch = {}

@@ -409,3 +409,35 @@ def test_batched_giver_loop_register_last_pass(fit, oracle, ns, dea, inv):
     e = rel(r.transfer_matrix, To)
     say("  T(n,m) err", e)
     assert e < TOL64
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_toroidal_poloidal_decomposition_and_partial_fluxes(fit, oracle, prec):
+    """ToroidalPoloidalDecomposition (Decomposition.jl:107-141) on the device against the oracle:
+    the two parts, the 8 partial-flux channels (SpectralFlux.jl:240-285) and the per-part fluxes
+    of calculate_spectral_flux (SpectralFlux.jl:96-122)."""
+    say("case toroidal/poloidal", prec)
+    u, ks = oracle.curl_velocity_3d(16, 95)
+    u = np.concatenate([u, 0.3 * u[..., :1]], axis=3)        # a fourth component: zero in both parts
+    dt = np.complex128 if prec == "f64" else np.complex64
+    tol = TOL64 if prec == "f64" else TOL32
+    dec_o = oracle.toroidal_poloidal_decompose(u, ks)
+    dec = fit.decompose_field(fit.ToroidalPoloidalDecomposition(), u.astype(dt), ks)
+    assert list(dec) == ["toroidal", "poloidal"]
+    for name in dec:
+        assert rel(dec[name], dec_o[name]) < tol
+    u3 = u[..., :3]
+    b = fit.LinearBinning(1.0)
+    ch_o, tot_o, _ = oracle.calculate_partial_fluxes(u3, ks, oracle.LinearBinning(1.0), "23",
+                                                     decomposition="ToroidalPoloidalDecomposition")
+    pf = fit.calculate_partial_fluxes(u3.astype(dt), ks, decomposition=fit.ToroidalPoloidalDecomposition(), binning=b)
+    assert len(pf["channels"]) == 8
+    scale = max(math.sqrt(np.sum(v[1] ** 2)) for v in ch_o.values())
+    for key, (c, T, P) in ch_o.items():
+        e = np.max(np.abs(pf["channels"][key].transfer_spectrum - T)) / scale
+        assert e < (1e-10 if prec == "f64" else 2e-5), (key, e)
+    per_o = oracle.calculate_spectral_flux(u3, ks, oracle.LinearBinning(1.0), "23",
+                                           decomposition="ToroidalPoloidalDecomposition")
+    per = fit.calculate_spectral_flux(u3.astype(dt), ks, binning=b, decomposition=fit.ToroidalPoloidalDecomposition())
+    for name in ("toroidal", "poloidal"):
+        assert rel(per[name].flux, per_o[name][2]) < (TOL64 if prec == "f64" else 1e-5)

@@ -386,6 +386,68 @@ def test_helical_fluxes_sum(oracle):
     assert np.max(np.abs(tot[1] - T)) < 1e-10 * scale
 
 
+# ----------------------------------------------------------------------------- toroidal / poloidal
+
+def test_toroidal_poloidal_properties(oracle):
+    # runtests.jl:634-660: û = i k × Â (solenoidal); tor + pol = û, orthogonal, tor has no vertical
+    # velocity, both divergence-free
+    M = 8
+    ks = oracle.wavenumber_grid((M, M, M), (2 * math.pi,) * 3)
+    kx, ky, kz = np.meshgrid(*ks, indexing="ij")
+    rng = np.random.default_rng(52)
+    A = rng.standard_normal((M, M, M, 3)) + 1j * rng.standard_normal((M, M, M, 3))
+    u = np.stack([1j * (ky * A[..., 2] - kz * A[..., 1]), 1j * (kz * A[..., 0] - kx * A[..., 2]),
+                  1j * (kx * A[..., 1] - ky * A[..., 0])], axis=-1)
+    dec = oracle.decompose("ToroidalPoloidalDecomposition", u, ks)
+    tor, pol = dec["toroidal"], dec["poloidal"]
+    assert list(dec) == ["toroidal", "poloidal"]
+    um = np.max(np.abs(u))
+    assert np.max(np.abs(tor + pol - u)) < 1e-12 * um
+    assert abs(np.sum(np.abs(tor) ** 2) + np.sum(np.abs(pol) ** 2) - np.sum(np.abs(u) ** 2)) < 1e-12 * np.sum(np.abs(u) ** 2)
+    assert np.max(np.abs(tor[..., 2])) < 1e-12 * um
+    for w in (tor, pol):
+        assert np.max(np.abs(kx * w[..., 0] + ky * w[..., 1] + kz * w[..., 2])) < 1e-10 * um
+    # the toroidal part carries all of the vertical vorticity ω_z = i(k_x u_y - k_y u_x)
+    wz = lambda w: 1j * (kx * w[..., 1] - ky * w[..., 0])
+    assert np.max(np.abs(wz(pol))) < 1e-10 * np.max(np.abs(wz(u)))
+
+
+def test_toroidal_poloidal_known_answers(oracle):
+    # k = (1, 0, 0): e1 = k̂ × ẑ = -ŷ, e2 = k̂ × e1 = -ẑ -> tor = (0, u_y, 0), pol = (0, 0, u_z);
+    # k = (0, 0, 1) (k_⊥ = 0): (x̂, ŷ) -> tor = (u_x, 0, 0), pol = (0, u_y, 0); k = 0 -> both zero;
+    # k = (1, 1, 0): e1 = (1, -1, 0)/√2, e2 = -ẑ
+    M = 4
+    ks = oracle.wavenumber_grid((M, M, M), (2 * math.pi,) * 3)
+    u = np.zeros((M, M, M, 4), dtype=np.complex128)
+    vals = {(1, 0, 0): (0.3, 1 + 2j, -0.5j), (0, 0, 1): (2 - 1j, 0.25, 7.0), (0, 0, 0): (1, 1, 1),
+            (1, 1, 0): (1.5, 0.5j, 2.0)}
+    for idx, v in vals.items():
+        u[idx + (slice(0, 3),)] = v
+        u[idx + (3,)] = 9.0                     # a fourth component: zero in both parts
+    d = oracle.toroidal_poloidal_decompose(u, ks)
+    t, p = d["toroidal"], d["poloidal"]
+    assert np.allclose(t[1, 0, 0], [0, 1 + 2j, 0, 0]) and np.allclose(p[1, 0, 0], [0, 0, -0.5j, 0])
+    assert np.allclose(t[0, 0, 1], [2 - 1j, 0, 0, 0]) and np.allclose(p[0, 0, 1], [0, 0.25, 0, 0])
+    assert np.allclose(t[0, 0, 0], 0) and np.allclose(p[0, 0, 0], 0)
+    c1 = (1.5 - 0.5j) / math.sqrt(2)
+    assert np.allclose(t[1, 1, 0], [c1 / math.sqrt(2), -c1 / math.sqrt(2), 0, 0])
+    assert np.allclose(p[1, 1, 0], [0, 0, 2.0, 0])
+
+
+def test_toroidal_poloidal_partial_fluxes_sum(oracle):
+    # SpectralFlux.jl:240-285 with the toroidal/poloidal split: 8 channels summing to the total
+    # kinetic-energy flux of the (solenoidal) field; per-component fluxes sum to it as well
+    u, ks = oracle.curl_velocity_3d(16, 93)
+    b = oracle.LinearBinning(1.0)
+    _, T, _ = oracle.calculate_spectral_flux(u, ks, b, "23")
+    ch, tot, _ = oracle.calculate_partial_fluxes(u, ks, b, "23", decomposition="ToroidalPoloidalDecomposition")
+    assert len(ch) == 8 and set(k[0] for k in ch) == {"toroidal", "poloidal"}
+    scale = max(math.sqrt(np.sum(v[1] ** 2)) for v in ch.values())
+    assert np.max(np.abs(tot[1] - T)) < 1e-10 * scale
+    per = oracle.calculate_spectral_flux(u, ks, b, "23", decomposition="ToroidalPoloidalDecomposition")
+    assert np.max(np.abs(per["toroidal"][1] + per["poloidal"][1] - T)) < 1e-9 * math.sqrt(np.sum(T ** 2))
+
+
 def test_synthetic_field_properties(oracle):
     u = oracle.synthetic_field((16, 12, 8), 3, seed=1)
     ks = oracle.wavenumber_grid((16, 12, 8), (2 * math.pi,) * 3)
