@@ -225,6 +225,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the uneven 2D ensemble (for profiling it on its own)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the plans' default execution knobs (no fit_plan_autotune before timing)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device copy first (reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -262,6 +264,12 @@ def main():
     tm_plan = fit.Plan((n,) * 3, 3, 3, ks, fit.OrszagTwoThirds(), np.float32, s_tm, len(c_tm))
     del s_pi, s_tm
     KE, HEL = fit.KineticEnergy(), fit.Helicity()
+    tuning = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuning = {"pi": pi_plan.autotune("flux", KE, u), "tnm": tm_plan.autotune("shell_to_shell", KE, u)}
+        tuning["seconds"] = time.perf_counter() - t0
+        log("[bench] autotune: " + json.dumps(tuning))
 
     def step():
         (T_ke, Pi_ke), (T_h, Pi_h) = pi_plan.spectral_fluxes((KE, HEL), u)
@@ -337,6 +345,7 @@ def main():
         "kernels": table[:16],
         "cpu_baseline": None,
         "fused_x_fft_tflops": fx_tflops,
+        "autotune": tuning,
     }
     # FP32 result of the last timed step, compared with one FP64 device run of the same snapshot
     Pi_ke32, Pi_h32, Tm32, _ = out

@@ -18,6 +18,11 @@ OPT_XPASS_SELF = 7
 OPT_XPASS_STAGE = 8
 OPT_GRAD_SPLIT = 9
 OPT_FUSED_FLUX = 10
+OPT_RADIX_ORDER = 11
+TUNE_FLUX, TUNE_SHELL_TO_SHELL = 1, 2
+OPT_NAMES = {1: "plain_pipeline", 2: "giver_batch", 3: "xpass_givers", 4: "xpass_block", 5: "subgrid",
+             6: "col_tile", 7: "xpass_self", 8: "xpass_stage", 9: "grad_split", 10: "fused_flux",
+             11: "radix_order"}
 ENS_OPT_GRAPH, ENS_OPT_BATCH = 1, 2
 DECOMP_HELICAL, DECOMP_TOROIDAL_POLOIDAL = 1, 2
 K_CLASSES = ("fft_column", "fft_row_inverse", "fft_row_forward", "density_shell", "reduce", "other",
@@ -36,6 +41,9 @@ SIGNATURES = {
     "fit_plan_destroy": (_I, [_P]),
     "fit_plan_device_bytes": (ctypes.c_int64, [_P]),
     "fit_plan_set_option": (_I, [_P, _I, _I]),
+    "fit_plan_get_option": (_I, [_P, _I, ctypes.POINTER(_I)]),
+    "fit_plan_autotune": (_I, [_P, _I, _I, _P, _P, _I, _I, _I]),
+    "fit_plan_tuning": (_I, [_P, ctypes.c_char_p, _I]),
     "fit_plan_subgrid": (_I, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "fit_plan_subgrid_classes": (_I, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_I), _I]),
     "fit_nonlinear_term": (_I, [_P, _P, _P, _P, _I, _I]),

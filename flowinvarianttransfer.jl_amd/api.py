@@ -9,6 +9,7 @@ the component axis last, column-major in memory.  Host inputs are numpy arrays; 
 torch tensors on a HIP device (results then stay on the device).
 """
 import ctypes
+import os
 import threading
 from collections import OrderedDict
 
@@ -173,6 +174,27 @@ class Plan:
         """FIT_OPT_* knobs: plain=True runs the unfused reference dataflow (comparison baseline);
         giver_batch caps the givers per batched launch."""
         _lib.check(self.lib.fit_plan_set_option(self.handle, option, int(value)))
+
+    def get_option(self, option):
+        v = ctypes.c_int()
+        _lib.check(self.lib.fit_plan_get_option(self.handle, option, ctypes.byref(v)))
+        return int(v.value)
+
+    def options(self):
+        """{name: value} of every execution knob."""
+        return {name: self.get_option(code) for code, name in _lib.OPT_NAMES.items()}
+
+    def autotune(self, workload, invariant, f, adv=None, reps=2):
+        """fit_plan_autotune on this plan with the given input (workload "flux" or
+        "shell_to_shell"); returns {"report": ..., "options": {...}}."""
+        code = {"flux": _lib.TUNE_FLUX, "shell_to_shell": _lib.TUNE_SHELL_TO_SHELL}[workload]
+        fp, ap, keep, kind = self._fa(f, adv)
+        self._order(kind, keep[0])
+        _lib.check(self.lib.fit_plan_autotune(self.handle, code, invariant.code, fp, ap, self.io, kind, int(reps)))
+        buf = ctypes.create_string_buffer(2048)
+        self.lib.fit_plan_tuning(self.handle, buf, 2048)
+        self.tuned = getattr(self, "tuned", set()) | {workload}
+        return {"report": buf.value.decode(), "options": self.options()}
 
     def set_plain(self, on=True):
         self.set_option(_lib.OPT_PLAIN_PIPELINE, int(bool(on)))
@@ -392,6 +414,14 @@ def _plan_for(ns, M, D, ks, dealiasing, precision, device, binning=None, geometr
     return plan
 
 
+def _maybe_tune(plan, workload, invariant, f, adv):
+    """With FIT_AUTOTUNE=1 in the environment, a cached plan is autotuned on its first call of a
+    workload, with that call's input (the choices then hold for the plan and its key)."""
+    if os.environ.get("FIT_AUTOTUNE") != "1" or workload in getattr(plan, "tuned", set()):
+        return
+    plan.autotune(workload, invariant, f, adv, reps=1)
+
+
 def clear_plan_cache():
     with _CACHE_LOCK:
         plans = list(_PLANS.values())
@@ -473,6 +503,7 @@ def calculate_spectral_flux(velocity_hat, ks, *, binning=None, dealiasing=Orszag
         return res
     if not isinstance(decomposition, NoDecomposition):
         raise ValueError(f"decomposition {decomposition!r} is not on the HIP path")
+    _maybe_tune(plan, "flux", invariant, velocity_hat, advecting_hat)
     T, Pi = plan.spectral_flux(invariant, velocity_hat, advecting_hat)
     return SpectralFluxResult(centers, _ft_cast(T, ft), _ft_cast(Pi, ft))
 
@@ -500,6 +531,7 @@ def calculate_shell_to_shell_transfer(velocity_hat, ks, *, binning=None,
     geometry = geometry if geometry is not None else ShellMagnitude(None)
     plan = _plan_for(ns, M, D, ks, dealiasing, spectral.compute_dtype(ft), spectral.device,
                      binning, geometry, shells=True)
+    _maybe_tune(plan, "shell_to_shell", invariant, velocity_hat, advecting_hat)
     T, net, asym = plan.shell_to_shell(invariant, velocity_hat, advecting_hat,
                                        verify_antisymmetry)
     return ShellToShellResult(_ft_cast(plan.centers, ft), _ft_cast(plan.edges, ft),

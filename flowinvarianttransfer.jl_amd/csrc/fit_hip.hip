@@ -6,6 +6,9 @@
 // and their kernels) is compiled per precision in fit_f32.hip / fit_f64.hip.
 #include "fit_internal.h"
 
+#include <chrono>
+#include <cstdlib>
+
 // Device scratch of fit_spectral_flux_with_nhat_noalloc: the shell partial rows and their fixed-
 // order sums.  One copy per GPU, held until the process ends; every call on a GPU shares it.  In
 // host code the array's name is only a stand-in: kernels get the per-device address found by
@@ -25,16 +28,16 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-// radix plan: as many 8s as possible, then 4 / 2, then 3s and 5s (any order is valid Stockham)
-bool make_desc(int L, FftDesc& d) {
+// Radix plan of the LDS-engine FFT (any order is valid Stockham).  order 0: as many 8s as
+// possible, then 4s, 2s, 3s, 5s; 1: 4s first, then 2s, 8s, 3s, 5s; 2: the odd radices first.
+bool make_desc(int L, FftDesc& d, int order) {
   d.L = L;
   d.nst = 0;
   int r = L;
-  while (r % 8 == 0 && d.nst < FIT_MAX_STAGES) { d.rad[d.nst++] = 8; r /= 8; }
-  while (r % 4 == 0 && d.nst < FIT_MAX_STAGES) { d.rad[d.nst++] = 4; r /= 4; }
-  while (r % 2 == 0 && d.nst < FIT_MAX_STAGES) { d.rad[d.nst++] = 2; r /= 2; }
-  while (r % 3 == 0 && d.nst < FIT_MAX_STAGES) { d.rad[d.nst++] = 3; r /= 3; }
-  while (r % 5 == 0 && d.nst < FIT_MAX_STAGES) { d.rad[d.nst++] = 5; r /= 5; }
+  static const int orders[3][5] = {{8, 4, 2, 3, 5}, {4, 2, 8, 3, 5}, {3, 5, 8, 4, 2}};
+  const int* o = orders[(order >= 0 && order < 3) ? order : 0];
+  for (int i = 0; i < 5; ++i)
+    while (r % o[i] == 0 && d.nst < FIT_MAX_STAGES) { d.rad[d.nst++] = o[i]; r /= o[i]; }
   return r == 1;
 }
 
@@ -494,7 +497,7 @@ static int create_plan(fit_plan** out, int nd, const int64_t* ns, int M, int D, 
     p->G[d] = dealias == FIT_DEALIAS_PADDED ? padded_len((int)ns[d]) : (int)ns[d];
   }
   for (int d = 0; d < 3; ++d) {
-    if (!make_desc(p->G[d], p->fd[d])) {
+    if (!make_desc(p->G[d], p->fd[d], 0)) {
       int bad = p->G[d];
       delete p;
       return fail(FIT_E_UNSUPPORTED, "transform length " + std::to_string(bad) + " is not of the form 2^a 3^b 5^c");
@@ -675,6 +678,81 @@ static __global__ void k_cumsum_shells(const double* __restrict__ sums, int nsh,
 
 using namespace fitx;
 
+// ---- autotuning ----------------------------------------------------------------------------
+// Choices of the plan knobs found by fit_plan_autotune, per (grid, components, precision,
+// dealiasing, shells, workload); applied to every plan created afterwards with the same key
+// (within the process, and across processes through the file FIT_TUNE_CACHE names).
+namespace fitx {
+
+struct TuneEntry {
+  std::string key;
+  std::vector<std::pair<int, int>> opts;
+};
+std::vector<TuneEntry> g_tune;
+bool g_tune_file_read = false;
+
+std::string tune_key(const fit_plan* p) {
+  return std::to_string(p->nd) + ":" + std::to_string(p->n[0]) + "x" + std::to_string(p->n[1]) + "x" +
+         std::to_string(p->n[2]) + ":M" + std::to_string(p->M) + ":D" + std::to_string(p->D) + ":" +
+         (p->dtype == FIT_F64 ? "f64" : "f32") + ":dealias" + std::to_string(p->dealias) + ":sh" +
+         std::to_string(p->nsh);
+}
+
+void tune_store(const std::string& key, const std::vector<std::pair<int, int>>& opts) {
+  for (TuneEntry& e : g_tune)
+    if (e.key == key) { e.opts = opts; return; }
+  g_tune.push_back(TuneEntry{key, opts});
+}
+
+// lines "key|opt=value,opt=value" (the caller holds g_create_mutex)
+void tune_read_file() {
+  if (g_tune_file_read) return;
+  g_tune_file_read = true;
+  const char* path = std::getenv("FIT_TUNE_CACHE");
+  if (!path || !*path) return;
+  FILE* fp = std::fopen(path, "r");
+  if (!fp) return;
+  char line[1024];
+  while (std::fgets(line, sizeof line, fp)) {
+    std::string l(line);
+    const size_t bar = l.find('|');
+    if (bar == std::string::npos) continue;
+    std::vector<std::pair<int, int>> opts;
+    size_t i = bar + 1;
+    while (i < l.size()) {
+      int o = 0, v = 0, used = 0;
+      if (std::sscanf(l.c_str() + i, "%d=%d%n", &o, &v, &used) != 2) break;
+      opts.emplace_back(o, v);
+      i += (size_t)used;
+      if (i < l.size() && l[i] == ',') ++i;
+    }
+    tune_store(l.substr(0, bar), opts);
+  }
+  std::fclose(fp);
+}
+
+void tune_append_file(const std::string& key, const std::vector<std::pair<int, int>>& opts) {
+  const char* path = std::getenv("FIT_TUNE_CACHE");
+  if (!path || !*path) return;
+  FILE* fp = std::fopen(path, "a");
+  if (!fp) return;
+  std::fprintf(fp, "%s|", key.c_str());
+  for (size_t i = 0; i < opts.size(); ++i) std::fprintf(fp, "%s%d=%d", i ? "," : "", opts[i].first, opts[i].second);
+  std::fprintf(fp, "\n");
+  std::fclose(fp);
+}
+
+// apply the cached choice of this plan's key, if any (the caller holds g_create_mutex)
+void tune_apply(fit_plan* p) {
+  tune_read_file();
+  const std::string key = tune_key(p);
+  for (const TuneEntry& e : g_tune) {
+    if (e.key.compare(0, key.size(), key) != 0) continue;
+    for (const auto& ov : e.opts) (void)fit_plan_set_option(p, ov.first, ov.second);
+  }
+}
+}  // namespace fitx
+
 extern "C" {
 
 const char* fit_last_error(void) { return g_last_error.c_str(); }
@@ -685,7 +763,9 @@ int fit_plan_create(fit_plan** out, int nd, const int64_t* ns, int M, int D, int
                     const double* const* ks, const int64_t* shell_idx, int n_shells, int device) {
   std::lock_guard<std::mutex> lock(g_create_mutex);
   g_last_error.clear();
-  return create_plan(out, nd, ns, M, D, compute_dtype, dealias, ks, shell_idx, n_shells, device, true);
+  int rc = create_plan(out, nd, ns, M, D, compute_dtype, dealias, ks, shell_idx, n_shells, device, true);
+  if (rc == FIT_OK) tune_apply(*out);
+  return rc;
 }
 
 int fit_plan_destroy(fit_plan* p) {
@@ -774,8 +854,121 @@ int fit_plan_set_option(fit_plan* p, int option, int value) {
     case FIT_OPT_XPASS_STAGE: p->xstage = value != 0; return FIT_OK;
     case FIT_OPT_GRAD_SPLIT: p->grad_split = value < 0 ? -1 : (value != 0); return FIT_OK;
     case FIT_OPT_FUSED_FLUX: p->fused_flux = value != 0; return FIT_OK;
+    case FIT_OPT_RADIX_ORDER:
+      if (value < 0 || value > 2) return fail(FIT_E_ARGUMENT, "radix order must be 0, 1 or 2");
+      for (int d = 0; d < 3; ++d) make_desc(p->G[d], p->fd[d], value);
+      p->radix_order = value;
+      return FIT_OK;
     default: return fail(FIT_E_ARGUMENT, "unknown plan option");
   }
+}
+
+int fit_plan_get_option(const fit_plan* p, int option, int* value) {
+  if (!p || !value) return fail(FIT_E_ARGUMENT, "null plan / value");
+  switch (option) {
+    case FIT_OPT_PLAIN_PIPELINE: *value = p->plain; return FIT_OK;
+    case FIT_OPT_GIVER_BATCH: *value = p->giver_batch; return FIT_OK;
+    case FIT_OPT_XPASS_GIVERS: *value = p->xfused_gc; return FIT_OK;
+    case FIT_OPT_XPASS_BLOCK: *value = p->xfused_block; return FIT_OK;
+    case FIT_OPT_SUBGRID: *value = p->use_sub; return FIT_OK;
+    case FIT_OPT_COL_TILE: *value = p->col_tx; return FIT_OK;
+    case FIT_OPT_XPASS_SELF: *value = p->xself; return FIT_OK;
+    case FIT_OPT_XPASS_STAGE: *value = p->xstage; return FIT_OK;
+    case FIT_OPT_GRAD_SPLIT: *value = p->grad_split; return FIT_OK;
+    case FIT_OPT_FUSED_FLUX: *value = p->fused_flux; return FIT_OK;
+    case FIT_OPT_RADIX_ORDER: *value = p->radix_order; return FIT_OK;
+    default: return fail(FIT_E_ARGUMENT, "unknown plan option");
+  }
+}
+
+int fit_plan_autotune(fit_plan* p, int workload, int invariant, const void* f_hat, const void* adv_hat,
+                      int io_dtype, int ptr_kind, int reps) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (workload != FIT_TUNE_FLUX && workload != FIT_TUNE_SHELL_TO_SHELL)
+    return fail(FIT_E_ARGUMENT, "workload must be FIT_TUNE_FLUX or FIT_TUNE_SHELL_TO_SHELL");
+  if (p->nsh <= 0) return fail(FIT_E_ARGUMENT, "plan has no shells");
+  if (reps < 1) reps = 1;
+  const int nsh = p->nsh;
+  std::vector<double> T((size_t)nsh), Pi((size_t)nsh), Tm, net((size_t)nsh);
+  double asym = 0.0;
+  if (workload == FIT_TUNE_SHELL_TO_SHELL) Tm.resize((size_t)nsh * nsh);
+  auto run = [&]() -> int {
+    return workload == FIT_TUNE_FLUX
+               ? fit_spectral_flux(p, invariant, f_hat, adv_hat, T.data(), Pi.data(), io_dtype, ptr_kind)
+               : fit_shell_to_shell(p, invariant, f_hat, adv_hat, Tm.data(), net.data(), &asym, 0, io_dtype,
+                                    ptr_kind);
+  };
+  // best of reps timed calls after one untimed call (every call synchronises its stream)
+  auto timed = [&](double* ms) -> int {
+    int r = run();
+    if (r) return r;
+    double best = 1e300;
+    for (int i = 0; i < reps; ++i) {
+      const auto t0 = std::chrono::steady_clock::now();
+      if ((r = run())) return r;
+      best = std::min(best, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+    *ms = best;
+    return FIT_OK;
+  };
+  struct Knob { int opt; std::vector<int> vals; };
+  std::vector<Knob> knobs;
+  if (workload == FIT_TUNE_FLUX) {
+    knobs = {{FIT_OPT_XPASS_SELF, {1, 0}}, {FIT_OPT_XPASS_STAGE, {1, 0}}, {FIT_OPT_GRAD_SPLIT, {-1, 0, 1}},
+             {FIT_OPT_COL_TILE, {0, 8, 32}}, {FIT_OPT_FUSED_FLUX, {0, 1}}, {FIT_OPT_XPASS_BLOCK, {0, 1}},
+             {FIT_OPT_RADIX_ORDER, {0, 1, 2}}};
+  } else {
+    knobs = {{FIT_OPT_XPASS_GIVERS, {0, 2, 8}}, {FIT_OPT_XPASS_BLOCK, {0, 1}}, {FIT_OPT_GRAD_SPLIT, {-1, 0, 1}},
+             {FIT_OPT_COL_TILE, {0, 8, 32}}, {FIT_OPT_GIVER_BATCH, {0, std::max(1, nsh / 2), std::max(1, nsh / 4)}},
+             {FIT_OPT_SUBGRID, {1, 0}}, {FIT_OPT_RADIX_ORDER, {0, 1, 2}}};
+  }
+  double cur = 0.0;
+  if ((rc = timed(&cur))) return rc;
+  const double t_start = cur;
+  std::string log = std::string("workload=") + (workload == FIT_TUNE_FLUX ? "flux" : "shell_to_shell") +
+                    " start_ms=" + std::to_string(t_start);
+  std::vector<std::pair<int, int>> chosen;
+  for (const Knob& k : knobs) {
+    int best_v = 0;
+    if ((rc = fit_plan_get_option(p, k.opt, &best_v))) return rc;
+    for (int v : k.vals) {
+      if (v == best_v) continue;
+      if (fit_plan_set_option(p, k.opt, v) != FIT_OK) continue;
+      double ms = 0.0;
+      const int r = timed(&ms);
+      if (r == FIT_E_DEVICE || r == FIT_E_OUT_OF_MEMORY) {
+        (void)fit_plan_set_option(p, k.opt, best_v);
+        return r;
+      }
+      if (r == FIT_OK && ms < cur * 0.98) {  // a choice must win by more than timing noise
+        cur = ms;
+        best_v = v;
+      }
+    }
+    if ((rc = fit_plan_set_option(p, k.opt, best_v))) return rc;
+    chosen.emplace_back(k.opt, best_v);
+    log += " opt" + std::to_string(k.opt) + "=" + std::to_string(best_v);
+  }
+  log += " tuned_ms=" + std::to_string(cur);
+  p->tune_report = log;
+  {
+    std::lock_guard<std::mutex> lock(g_create_mutex);
+    tune_read_file();
+    const std::string key = tune_key(p) + (workload == FIT_TUNE_FLUX ? ":flux" : ":s2s");
+    tune_store(key, chosen);
+    tune_append_file(key, chosen);
+  }
+  return FIT_OK;
+}
+
+int fit_plan_tuning(const fit_plan* p, char* buf, int len) {
+  if (!p) return fail(FIT_E_ARGUMENT, "null plan");
+  if (buf && len > 0) {
+    std::strncpy(buf, p->tune_report.c_str(), (size_t)len - 1);
+    buf[len - 1] = 0;
+  }
+  return (int)p->tune_report.size();
 }
 
 static int kagg_out(const std::vector<KernAgg>& v, int i, char* name, int name_len, int* kernel_class,

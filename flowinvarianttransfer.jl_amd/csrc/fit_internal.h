@@ -35,7 +35,7 @@ int fail(int code, const std::string& msg);
     }                                                                                          \
   } while (0)
 
-bool make_desc(int L, FftDesc& d);
+bool make_desc(int L, FftDesc& d, int order);
 int padded_len(int n);
 
 // one class of givers [m0, m1] of the band-limited sub-grid giver loop
@@ -136,6 +136,8 @@ struct fit_plan {
   bool fused_done = false;
   // ordering after a caller's stream (fit_plan_wait_stream)
   hipEvent_t order_ev = nullptr;
+  int radix_order = 0;            // FIT_OPT_RADIX_ORDER of the LDS-engine passes
+  std::string tune_report;        // the last fit_plan_autotune: workload, times, choices
   // profiling
   bool prof = false;
   std::vector<fitx::ProfRec> recs;

@@ -90,11 +90,31 @@ int64_t fit_plan_device_bytes(const fit_plan* plan);
  *   FIT_OPT_FUSED_FLUX = 10: 1 makes fit_spectral_flux / fit_spectral_fluxes compute the densities
  *     in the last forward pass of the nonlinear term (N̂ never stored; M <= 3, up to 2 invariants,
  *     register FFT length on the last axis); 0 (default) stores N̂ and reads it in one density
- *     pass (measured faster on the 512³ flux with 3/2 padding). */
+ *     pass (measured faster on the 512³ flux with 3/2 padding).
+ *   FIT_OPT_RADIX_ORDER = 11: radix order of the LDS-engine FFT passes (lengths without a
+ *     register FFT): 0 (default) radix 8 first, then 4, 2, 3, 5; 1 radix 4 first; 2 the odd
+ *     radices first. */
 enum { FIT_OPT_PLAIN_PIPELINE = 1, FIT_OPT_GIVER_BATCH = 2, FIT_OPT_XPASS_GIVERS = 3,
        FIT_OPT_XPASS_BLOCK = 4, FIT_OPT_SUBGRID = 5, FIT_OPT_COL_TILE = 6, FIT_OPT_XPASS_SELF = 7,
-       FIT_OPT_XPASS_STAGE = 8, FIT_OPT_GRAD_SPLIT = 9, FIT_OPT_FUSED_FLUX = 10 };
+       FIT_OPT_XPASS_STAGE = 8, FIT_OPT_GRAD_SPLIT = 9, FIT_OPT_FUSED_FLUX = 10, FIT_OPT_RADIX_ORDER = 11 };
 int fit_plan_set_option(fit_plan* plan, int option, int value);
+int fit_plan_get_option(const fit_plan* plan, int option, int* value);
+
+/* Autotuner (no reference counterpart; the reference's FFT planning, FFTWExt.jl:42-45, is the
+ * nearest equivalent): times the plan's execution knobs on the caller's own input by coordinate
+ * descent and keeps the fastest -- column tile, x-pass form and givers per workgroup, gradient
+ * split, LDS staging, self-advection pass, fused density, giver batch depth (against the device
+ * memory the batches may use), band-limited sub-grid, radix order.  workload FIT_TUNE_FLUX times
+ * fit_spectral_flux of `invariant`, FIT_TUNE_SHELL_TO_SHELL times fit_shell_to_shell; each timing
+ * is the best of `reps` calls after one untimed call, and a change must win by 2%.  The choice is
+ * cached per (grid, components, precision, dealiasing, shells, workload): plans created later with
+ * the same key start from it, also in other processes when the environment variable
+ * FIT_TUNE_CACHE names a file (one line per choice, appended).  fit_plan_tuning writes the last
+ * report ("workload=... start_ms=... optN=V ... tuned_ms=...") and returns its length. */
+enum { FIT_TUNE_FLUX = 1, FIT_TUNE_SHELL_TO_SHELL = 2 };
+int fit_plan_autotune(fit_plan* plan, int workload, int invariant, const void* f_hat, const void* adv_hat,
+                      int io_dtype, int ptr_kind, int reps);
+int fit_plan_tuning(const fit_plan* plan, char* buf, int len);
 
 /* Band-limited sub-grid of the giver loop (no reference counterpart).  When every shell lies in
  * |k_d| <= R_d with 4 R_d < n_d, fit_plan_create attaches a grid of L_d >= 4 R_d + 1 points per
