@@ -178,33 +178,167 @@ def measured_traffic(plan_key, kernel, n, ng):
     return None
 
 
-def cpu_baseline(u_dev, n, ngivers, torch, fit, workers):
-    """FP64 CPU restatement of the reference's FFT path (c2c, its dataflow) on a bounded sample:
-    one giver column of T(n,m) at 512³ (the advecting field's 3 inverse transforms are shared
-    by all givers and timed separately)."""
+def host_cores():
+    """(threads this process may use, nproc, CPU model) of the host."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        aff = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return aff, os.cpu_count() or 1, model
+
+
+def c2_field():
+    """The C2 input on the host: 256³ seeded (seed 2) solenoidal k^-5/3 field, FP64."""
+    sys.path.insert(0, ROOT)
+    from oracle import fit_oracle as oracle
+    n = 256
+    return oracle.synthetic_field((n,) * 3, 3, seed=2, workers=min(16, host_cores()[0])), n
+
+
+def c2_bench(fit, torch, u_host, n, steps):
+    """C2 on the GPU: 256³ kinetic-energy T(k), Π(K) (LinearBinning(1), 222 shells) + 32-shell
+    T(n,m) (OrszagTwoThirds), FP32, one snapshot per step, inputs resident."""
+    from fit_amd.grid import shell_setup
+    import numpy as np
+    rev = (3, 2, 1, 0)
+    u = torch.from_numpy(np.ascontiguousarray(u_host.transpose(rev))).to("cuda").permute(rev).to(torch.complex64)
+    ks = fit.wavenumber_grid((n,) * 3, (2 * math.pi,) * 3)
+    c_pi, _, s_pi = shell_setup(ks, fit.LinearBinning(1.0), fit.IsotropicShells())
+    c_tm, _, s_tm = shell_setup(ks, fit.CustomBinning(range(0, 33)), fit.IsotropicShells())
+    pp = fit.Plan((n,) * 3, 3, 3, ks, fit.OrszagTwoThirds(), np.float32, s_pi, len(c_pi))
+    tp = fit.Plan((n,) * 3, 3, 3, ks, fit.OrszagTwoThirds(), np.float32, s_tm, len(c_tm))
+    KE = fit.KineticEnergy()
+
+    def step():
+        pp.spectral_flux(KE, u)
+        tp.shell_to_shell(KE, u)
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    pp.close()
+    tp.close()
+    # dense model (each logical transform reads its input and writes its output once, 4·Np + 8·H
+    # bytes): Π 15 transforms + 32 givers × 12 -> 67.6 GB
+    Np, H = n ** 3, (n // 2 + 1) * n * n
+    alg = (15 + 32 * 12) * (4.0 * Np + 8.0 * H)
+    return {"workload": "256^3 D=3 KE T(k), Pi(K) (222 shells) + 32-shell T(n,m), OrszagTwoThirds, FP32, "
+                        "one snapshot per step", "value": 1.0 / dt, "unit": "snapshots/s", "ms_per_step": dt * 1e3,
+            "dense_alg_bytes": alg, "dense_alg_GBs": alg / dt / 1e9}
+
+
+def c5_bench(fit, torch, steps):
+    """C5 on the GPU: mode-resolved S(k|p) at 32³ (kinetic energy, OrszagTwoThirds) over all
+    32768² pairs in FP32 into a resident device array (4.29 GB), with the FP64 per-k net sum."""
+    import ctypes
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from fit_amd import _lib
+    n = 32
+    ks = fit.wavenumber_grid((n,) * 3, (2 * math.pi,) * 3)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    uf = torch.complex(torch.randn((3, n, n, n), generator=g, device="cuda"),
+                       torch.randn((3, n, n, n), generator=g, device="cuda"))
+    u = (uf / uf.abs().max()).permute(3, 2, 1, 0)            # column-major (n, n, n, 3) view
+    Np = n ** 3
+    S = torch.empty((Np * Np,), dtype=torch.float32, device="cuda")
+    net = torch.empty((Np,), dtype=torch.float32, device="cuda")
+    plan = fit.Plan((n,) * 3, 3, 3, ks, fit.OrszagTwoThirds(), np.float32)
+    lib = plan.lib
+    uc = u.contiguous() if not u.permute(3, 2, 1, 0).is_contiguous() else u
+
+    def call():
+        _lib.check(lib.fit_mode_to_mode(plan.handle, 0, ctypes.c_void_p(uc.data_ptr()), None,
+                                        ctypes.c_void_p(S.data_ptr()), ctypes.c_void_p(net.data_ptr()),
+                                        _lib.F32, _lib.PTR_DEVICE))
+
+    call()
+    torch.cuda.synchronize()
+    plan.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        call()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ks_ = plan.kernel_stats()
+    plan.profile(False)
+    k = ks_.get("k_m2m<f32>", {"ms": float("nan"), "launches": 1})
+    kms = k["ms"] / max(k["launches"], 1)
+    pairs = float(Np) * Np
+    flops = 72.0 * pairs                                     # per pair: 2 triad sums + 3 carrier dots
+    wbytes = 4.0 * pairs
+    plan.close()
+    del S
+    torch.cuda.empty_cache()
+    return {"workload": "32^3 D=3 kinetic-energy S(k|p) over all 32768^2 pairs, OrszagTwoThirds, FP32 S, FP64 net",
+            "ms_per_call": dt * 1e3, "kernel": "k_m2m<f32>", "kernel_ms": kms,
+            "write_GBs": wbytes / (kms * 1e-3) / 1e9, "write_frac_hbm": wbytes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "model_flop_per_pair": 72, "fp32_TFLOPs": flops / (kms * 1e-3) / 1e12,
+            "fp32_frac_valu": flops / (kms * 1e-3) / 1e12 / 157.3}
+
+
+def cpu_baseline(u_c2, n2, n, ngivers):
+    """FP64 CPU restatement of the reference's FFT path (c2c, its dataflow), timed on this host.
+    Sample: the C2 workload at 256³ -- Π(K) with multithreaded transforms (all cores) and the
+    giver loop in the reference's threaded schedule (OhMyThreadsExt.jl:24-97: one task per giver
+    column, each computing its own nonlinear term, single-threaded transforms) on all cores, one
+    round of W givers timed and the 32-giver loop extrapolated linearly in giver count; a 1-core
+    figure from Π and one giver single-threaded.  The 512³ snapshot (Π at 768³ with 3/2 padding,
+    48 givers at 512³) is scaled from it by N log N per transform."""
     sys.path.insert(0, ROOT)
     from oracle import fit_oracle as oracle
     import numpy as np
-    uh = u_dev.permute(3, 2, 1, 0).cpu().numpy().astype(np.complex128)   # (c, z, y, x)
-    uh = np.ascontiguousarray(uh).transpose(3, 2, 1, 0)                  # (x, y, z, c) view
-    ks = oracle.wavenumber_grid((n, n, n), (2 * math.pi,) * 3)
+    W, nproc, model = host_cores()
+    W = max(1, min(16, W))
+    ks = oracle.wavenumber_grid((n2,) * 3, (2 * math.pi,) * 3)
     kc = oracle.shell_coordinate(None, ks)
-    sidx = oracle.assign_shells_fast(kc, np.arange(0.0, ngivers + 1.0))
+    sidx = oracle.assign_shells_fast(kc, np.arange(0.0, 33.0))
     t0 = time.perf_counter()
-    oracle.shell_to_shell_columns(uh, ks, sidx, ngivers, [ngivers // 2], "23", "ke",
-                                  workers=workers)
-    t_sample = time.perf_counter() - t0
-    # per snapshot: 48 givers (12 transforms each, the sample's own cost minus the shared
-    # advecting transforms ~ 3/15 of it) + the shared 3 + Π at 768³ with 15 transforms
-    t_shared = t_sample * 3.0 / 15.0
-    t_giver = t_sample - t_shared
-    pad_ratio = (15.0 / 12.0) * (768.0 / 512.0) ** 3 * (math.log(768.0 ** 3) / math.log(512.0 ** 3))
-    t_snap = ngivers * t_giver + t_shared + pad_ratio * t_giver
-    return {"value": 1.0 / t_snap, "unit": "snapshots/s", "cores": workers, "kind": "port",
-            "sample": f"FP64 c2c restatement (scipy.fft, workers={workers}): one T(n,m) giver "
-                      f"column at {n}³ timed ({t_sample:.1f} s incl. the 3 shared advecting "
-                      f"transforms); snapshot = {ngivers} givers + shared + Π at 768³ "
-                      f"extrapolated as {pad_ratio:.2f} giver-equivalents"}
+    oracle.calculate_spectral_flux(u_c2, ks, oracle.LinearBinning(1.0), "23", workers=W)
+    t_pi = time.perf_counter() - t0
+    givers = [int(round(1 + i * 31 / max(W - 1, 1))) for i in range(min(W, 32))]
+    t0 = time.perf_counter()
+    oracle.shell_to_shell_threaded(u_c2, ks, sidx, 32, givers, "23", "ke", workers=W)
+    t_round = time.perf_counter() - t0
+    t_c2 = t_pi + t_round * math.ceil(32 / len(givers))
+    t0 = time.perf_counter()
+    oracle.calculate_spectral_flux(u_c2, ks, oracle.LinearBinning(1.0), "23", workers=1)
+    t_pi1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.shell_to_shell_columns(u_c2, ks, sidx, 32, [16], "23", "ke", workers=1)
+    t_g1 = time.perf_counter() - t0
+    t_c2_1 = t_pi1 + 32 * t_g1
+    # scale to the 512³ snapshot: a giver's transforms at 512³ cost (512/256)³·log(512³)/log(256³)
+    # of those at 256³; Π runs its 15 transforms at 768³
+    rg = (n / n2) ** 3 * math.log(n ** 3) / math.log(n2 ** 3)
+    rp = (1.5 * n / n2) ** 3 * math.log((1.5 * n) ** 3) / math.log(n2 ** 3)
+    t_s2s = t_c2 - t_pi
+    t_snap = t_pi * rp + t_s2s * (ngivers / 32) * rg
+    t_snap1 = t_pi1 * rp + 32 * t_g1 * (ngivers / 32) * rg
+    return {"value": 1.0 / t_snap, "unit": "snapshots/s", "cores": W, "kind": "port",
+            "nproc": nproc, "cpu_model": model, "one_core_value": 1.0 / t_snap1,
+            "c2_measured": {"all_cores_s_per_snapshot": t_c2, "one_core_s_per_snapshot": t_c2_1,
+                            "pi_s": t_pi, "giver_round_s": t_round, "givers_in_round": len(givers),
+                            "pi_one_core_s": t_pi1, "giver_one_core_s": t_g1},
+            "sample": (f"FP64 c2c restatement of the reference path (scipy.fft) on {W} of {nproc} host threads "
+                       f"({model}): the 256³ C2 snapshot -- Π(K) with {W}-thread transforms ({t_pi:.1f} s) and "
+                       f"one round of {len(givers)} giver columns in the reference's threaded schedule "
+                       f"({len(givers)} single-threaded tasks, {t_round:.1f} s), the 32-giver loop extrapolated "
+                       f"linearly in giver count; 1 core: Π ({t_pi1:.1f} s) + 32 x one giver ({t_g1:.1f} s); the "
+                       f"512³ snapshot scaled by N log N per transform (giver x{rg:.2f}, Π at 768³ x{rp:.1f}) and "
+                       f"{ngivers}/32 givers")}
 
 
 def main():
@@ -215,7 +349,6 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--givers", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-workers", type=int, default=16)
     ap.add_argument("--no-fp64-check", action="store_true",
                     help="skip the one-off FP64 device run the FP32 errors are measured against")
     ap.add_argument("--no-fp64-plain", action="store_true",
@@ -225,6 +358,9 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the uneven 2D ensemble (for profiling it on its own)")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 256³ C2 line (and the CPU baseline)")
+    ap.add_argument("--no-c5", action="store_true", help="skip the 32³ S(k|p) line")
+    ap.add_argument("--c5-only", action="store_true", help="run only the 32³ S(k|p) line (for profiling it)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the plans' default execution knobs (no fit_plan_autotune before timing)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -238,6 +374,10 @@ def main():
     import numpy as np
     import torch
     fit = load_pkg()
+    if args.c5_only:
+        c5 = c5_bench(fit, torch, args.steps)
+        print(json.dumps({"metric": "S(k|p) at 32^3", "c5": c5}), flush=True)
+        return
     if args.ensemble_only:
         ens = ensemble_bench(fit, torch, args.ensemble_steps)
         print(json.dumps({"metric": "uneven 2D ensemble fluxes", "value": ens["value"], "unit": ens["unit"],
@@ -402,10 +542,22 @@ def main():
         torch.cuda.empty_cache()
         result["ensemble"] = ensemble_bench(fit, torch, args.ensemble_steps)
         log(f"[bench] ensemble: {result['ensemble']['ms_per_ensemble']:.2f} ms per ensemble")
+    if not args.no_c5:
+        log("[bench] C5: S(k|p) at 32³")
+        result["c5"] = c5_bench(fit, torch, 3)
+    if args.no_c2:
+        print(json.dumps(result), flush=True)
+        return
+    log("[bench] C2: 256³ Π + 32-shell T(n,m)")
+    u_c2, n2 = c2_field()
+    result["c2"] = c2_bench(fit, torch, u_c2, n2, 5)
     if not args.no_cpu_baseline:
-        log("[bench] CPU baseline sample (FP64 oracle, one giver column)")
-        workers = max(1, min(args.cpu_workers, os.cpu_count() or 1))
-        result["cpu_baseline"] = cpu_baseline(u, n, ng, torch, fit, workers)
+        log("[bench] CPU baseline: the C2 sample on the host (FP64 restatement, threaded giver loop)")
+        result["cpu_baseline"] = cpu_baseline(u_c2, n2, n, ng)
+        cm = result["cpu_baseline"]["c2_measured"]
+        result["c2"]["cpu_all_cores_ms_per_snapshot"] = cm["all_cores_s_per_snapshot"] * 1e3
+        result["c2"]["cpu_one_core_ms_per_snapshot"] = cm["one_core_s_per_snapshot"] * 1e3
+        result["c2"]["gpu_speedup_vs_cpu_all_cores"] = cm["all_cores_s_per_snapshot"] * 1e3 / result["c2"]["ms_per_step"]
     print(json.dumps(result), flush=True)
 
 

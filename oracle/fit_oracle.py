@@ -552,6 +552,44 @@ def shell_to_shell_columns(u_hat, ks, sidx, nsh, givers, dealiasing="23", invari
     return T
 
 
+def _giver_column_task(args):
+    """One task of the threaded giver loop: column m of T(n,m) from memory-mapped inputs, with its
+    own workspace and single-threaded transforms (OhMyThreadsExt.jl:44-73)."""
+    u_path, s_path, ks, nsh, m, dealiasing, invariant = args
+    u_hat = np.load(u_path, mmap_mode="r")
+    sidx = np.load(s_path, mmap_mode="r")
+    return m, shell_to_shell_columns(np.asarray(u_hat), ks, np.asarray(sidx), nsh, [m], dealiasing, invariant,
+                                     workers=1)[:, 0]
+
+
+def shell_to_shell_threaded(u_hat, ks, sidx, nsh, givers, dealiasing="23", invariant="ke", workers=1,
+                            tmpdir=None):
+    """OhMyThreadsExt.jl:24-97 -- the reference's threaded giver loop: one task per giver column m
+    (each computing the full nonlinear term of the band-m field with its own workspace), run on
+    `workers` spawned processes with single-threaded transforms.  û and the shell map reach the
+    workers as memory-mapped .npy files (written under tmpdir).  Returns T[:, givers]."""
+    import multiprocessing as mp
+    import os
+    import tempfile
+    d = tempfile.mkdtemp(dir=tmpdir)
+    u_path, s_path = os.path.join(d, "u.npy"), os.path.join(d, "s.npy")
+    try:
+        np.save(u_path, np.asarray(u_hat))
+        np.save(s_path, np.asarray(sidx))
+        tasks = [(u_path, s_path, [np.asarray(k) for k in ks], nsh, int(m), dealiasing, invariant) for m in givers]
+        T = np.zeros((nsh, len(givers)))
+        col = {int(m): i for i, m in enumerate(givers)}
+        with mp.get_context("spawn").Pool(processes=max(1, int(workers))) as pool:
+            for m, c in pool.imap_unordered(_giver_column_task, tasks):
+                T[:, col[m]] = c
+        return T
+    finally:
+        for p in (u_path, s_path):
+            if os.path.exists(p):
+                os.remove(p)
+        os.rmdir(d)
+
+
 def calculate_shell_to_shell_transfer(u_hat, ks, binning=None, dealiasing="23",
                                       verify_antisymmetry=True, invariant="ke",
                                       spectral="FFTBackend", advecting_hat=None, geometry=None,
