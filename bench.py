@@ -163,11 +163,11 @@ def ensemble_bench(fit, torch, steps):
             "eager_device_ms": dev_ms, "kernel_classes": stats, "kernels": ktab[:12]}
 
 
-def measured_traffic(plan_key, kernel, n, ng):
-    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 PMC summary of
-    this workload (FETCH_SIZE + WRITE_SIZE of that kernel, calibrated on a 1 GiB copy), or None
-    when no summary covers it."""
-    path = os.path.join(ROOT, "profiles", f"snapshot{n}_tnm{ng}_pmc_traffic.md")
+def measured_traffic(plan_key, kernel, n, ng, summary=None):
+    """HBM bytes per launch of a kernel, from the committed rocprofv3 PMC summary of this
+    workload (FETCH_SIZE + WRITE_SIZE of that kernel, calibrated on a 1 GiB copy), or None when
+    no summary covers it."""
+    path = os.path.join(ROOT, "profiles", summary or f"snapshot{n}_tnm{ng}_pmc_traffic.md")
     key = f"traffic_bytes_per_launch[{plan_key}:{kernel}]"
     try:
         for line in open(path):
@@ -277,15 +277,22 @@ def c5_bench(fit, torch, steps):
     k = ks_.get("k_m2m<f32>", {"ms": float("nan"), "launches": 1})
     kms = k["ms"] / max(k["launches"], 1)
     pairs = float(Np) * Np
-    flops = 72.0 * pairs                                     # per pair: 2 triad sums + 3 carrier dots
+    # pairs that carry arithmetic: receiver and giver both inside the 2/3 cube (|k_d| < n/3
+    # along every axis); every other S(k|p) entry is a written zero
+    kept_1d = sum(1 for i in range(n) if min(i, n - i) < n // 3)
+    live = float(kept_1d ** 3) ** 2
+    flops = 72.0 * live                                      # per live pair: 2 triad sums + 3 carrier dots
     wbytes = 4.0 * pairs
     plan.close()
     del S
     torch.cuda.empty_cache()
     return {"workload": "32^3 D=3 kinetic-energy S(k|p) over all 32768^2 pairs, OrszagTwoThirds, FP32 S, FP64 net",
             "ms_per_call": dt * 1e3, "kernel": "k_m2m<f32>", "kernel_ms": kms,
-            "write_GBs": wbytes / (kms * 1e-3) / 1e9, "write_frac_hbm": wbytes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "model_flop_per_pair": 72, "fp32_TFLOPs": flops / (kms * 1e-3) / 1e12,
+            "roofline": {"bound": "hbm", "achieved": wbytes / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": wbytes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "traffic": measured_traffic("m2m", "k_m2m<f32>", n, 0, "modetomode32_k_m2m.md")},
+            "live_pairs": live, "model_flop_per_live_pair": 72,
+            "fp32_TFLOPs_live": flops / (kms * 1e-3) / 1e12,
             "fp32_frac_valu": flops / (kms * 1e-3) / 1e12 / 157.3}
 
 

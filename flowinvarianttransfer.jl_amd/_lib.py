@@ -40,6 +40,7 @@ SIGNATURES = {
                              ctypes.POINTER(ctypes.c_int64), _I, _I]),
     "fit_plan_destroy": (_I, [_P]),
     "fit_plan_device_bytes": (ctypes.c_int64, [_P]),
+    "fit_plan_live_count": (ctypes.c_int64, []),
     "fit_plan_set_option": (_I, [_P, _I, _I]),
     "fit_plan_get_option": (_I, [_P, _I, ctypes.POINTER(_I)]),
     "fit_plan_autotune": (_I, [_P, _I, _I, _P, _P, _I, _I, _I]),

@@ -22,6 +22,7 @@ FIT_PIPELINE_EXTERN(double)
 
 thread_local std::string g_last_error;
 std::mutex g_create_mutex;
+int64_t g_live_plans = 0;   // plans made by fit_plan_create and not yet destroyed (under g_create_mutex)
 
 int fail(int code, const std::string& msg) {
   g_last_error = msg;
@@ -764,12 +765,25 @@ int fit_plan_create(fit_plan** out, int nd, const int64_t* ns, int M, int D, int
   std::lock_guard<std::mutex> lock(g_create_mutex);
   g_last_error.clear();
   int rc = create_plan(out, nd, ns, M, D, compute_dtype, dealias, ks, shell_idx, n_shells, device, true);
-  if (rc == FIT_OK) tune_apply(*out);
+  if (rc == FIT_OK) {
+    tune_apply(*out);
+    (*out)->user_plan = true;
+    ++g_live_plans;
+  }
   return rc;
+}
+
+int64_t fit_plan_live_count(void) {
+  std::lock_guard<std::mutex> lock(g_create_mutex);
+  return g_live_plans;
 }
 
 int fit_plan_destroy(fit_plan* p) {
   if (!p) return FIT_OK;
+  if (p->user_plan) {
+    std::lock_guard<std::mutex> lock(g_create_mutex);
+    --g_live_plans;
+  }
   hipSetDevice(p->device);
   if (p->stream) hipStreamSynchronize(p->stream);
   for (SubGrid& sg : p->subs) fit_plan_destroy(sg.c);

@@ -126,6 +126,7 @@ struct fit_plan {
   void* sub_f = nullptr; size_t sub_f_bytes = 0;
   void* sub_a = nullptr; size_t sub_a_bytes = 0;
   bool own_stream = true;
+  bool user_plan = false;     // made by fit_plan_create (counted by fit_plan_live_count)
   int use_sub = 1;                // FIT_OPT_SUBGRID
   // fused last pass + density of the flux (flux_multi): requested invariants and whether the
   // pipeline took the fused form (densities in dcolG) or stored N̂

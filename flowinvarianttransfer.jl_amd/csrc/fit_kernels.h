@@ -1740,51 +1740,89 @@ __global__ void k_m2m_prep(Grid g, int trunc, int inv, int M, const cpx<R>* __re
 }
 
 // One thread per receiver k, looping over a chunk of givers p.  S written column-major
-// S[k + Np p] in OUT precision; FP64 partial net sums per chunk.
-template <typename R, typename OUT>
+// S[k + Np p] in OUT precision; FP64 partial net sums per chunk.  The giver's grid indices advance
+// incrementally (no integer division in the loop) and the dimension count is a template argument
+// so the wavenumber and carrier arrays stay in registers.
+__device__ __forceinline__ bool m2m_keep1(int i, int n, int h, int t) { return (i <= h ? i : n - i) < t; }
+
+template <typename R, typename OUT, int ND>
 __global__ void __launch_bounds__(256) k_m2m(Grid g, int trunc, int M, const cpx<R>* __restrict__ f, int64_t fsf,
                                              const cpx<R>* __restrict__ A, const cpx<R>* __restrict__ e,
                                              const R* __restrict__ ks0, const R* __restrict__ ks1,
                                              const R* __restrict__ ks2, int64_t pchunk, OUT* __restrict__ S,
                                              double* __restrict__ netpart) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.Np) return;
   const int64_t p0 = (int64_t)blockIdx.y * pchunk;
   const int64_t p1 = p0 + pchunk < g.Np ? p0 + pchunk : g.Np;
-  if (k >= g.Np) return;
-  const int kx = (int)(k % g.n1), ky = (int)((k / g.n1) % g.n2), kz = (int)(k / ((int64_t)g.n1 * g.n2));
-  const bool keepk = !(trunc && g.dealiased(kx, ky, kz));
-  cpx<R> ek[3];
-  for (int c = 0; c < M && c < 3; ++c) ek[c] = e[k + c * g.Np];
+  const int n1 = g.n1, n2 = ND >= 2 ? g.n2 : 1, n3 = ND >= 3 ? g.n3 : 1;
+  const int h1 = n1 / 2, h2 = n2 / 2, h3 = n3 / 2, t1 = n1 / 3, t2 = n2 / 3, t3 = n3 / 3;
+  const int np = (int)g.Np;
+  const int kx = (int)(k % n1), ky = (int)((k / n1) % n2), kz = (int)(k / ((int64_t)n1 * n2));
+  const bool keepk = !trunc || (m2m_keep1(kx, n1, h1, t1) && (ND < 2 || m2m_keep1(ky, n2, h2, t2)) &&
+                                (ND < 3 || m2m_keep1(kz, n3, h3, t3)));
+  cpx<R> ek[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) ek[c] = c < M ? e[k + (int64_t)c * np] : mk<R>((R)0, (R)0);
   const R inv2 = (R)(0.5 / ((double)g.Np * (double)g.Np));
+  int px = (int)(p0 % n1), py = (int)((p0 / n1) % n2), pz = (int)(p0 / ((int64_t)n1 * n2));
+  OUT* __restrict__ sp = S + k + g.Np * p0;
   double net = 0.0;
+  // keep(p) along y and z changes only when px wraps
+  bool keep_yz = (ND < 2 || m2m_keep1(py, n2, h2, t2)) && (ND < 3 || m2m_keep1(pz, n3, h3, t3));
   for (int64_t p = p0; p < p1; ++p) {
-    const int px = (int)(p % g.n1), py = (int)((p / g.n1) % g.n2), pz = (int)(p / ((int64_t)g.n1 * g.n2));
     R s = (R)0;
-    if (keepk && !(trunc && g.dealiased(px, py, pz))) {
-      const int ax = kx - px < 0 ? kx - px + g.n1 : kx - px;
-      const int ay = ky - py < 0 ? ky - py + g.n2 : ky - py;
-      const int az = kz - pz < 0 ? kz - pz + g.n3 : kz - pz;
-      const int bx = kx + px >= g.n1 ? kx + px - g.n1 : kx + px;
-      const int by = ky + py >= g.n2 ? ky + py - g.n2 : ky + py;
-      const int bz = kz + pz >= g.n3 ? kz + pz - g.n3 : kz + pz;
-      const int64_t om = g.oidx(ax, ay, az), op = g.oidx(bx, by, bz);
-      const R pj[3] = {ks0[px], g.nd >= 2 ? ks1[py] : (R)0, g.nd >= 3 ? ks2[pz] : (R)0};
-      cpx<R> dm = mk<R>((R)0, (R)0), dp = dm;
-      for (int j = 0; j < g.nd; ++j) {
-        dm = cadd(dm, cscale(A[om + j * g.Np], pj[j]));
-        dp = cadd(dp, cscale(A[op + j * g.Np], pj[j]));
+    const bool keepp = !trunc || (keep_yz && m2m_keep1(px, n1, h1, t1));
+    if (keepk && keepp) {
+      const int ax = kx - px < 0 ? kx - px + n1 : kx - px;
+      const int bx = kx + px >= n1 ? kx + px - n1 : kx + px;
+      int om = ax, op = bx;
+      if (ND >= 2) {
+        const int ay = ky - py < 0 ? ky - py + n2 : ky - py;
+        const int by = ky + py >= n2 ? ky + py - n2 : ky + py;
+        om += n1 * ay; op += n1 * by;
       }
-      for (int c = 0; c < M; ++c) {
-        const cpx<R> fp = f[p + c * fsf];
-        // i dm fp - i dp conj(fp)
+      if (ND >= 3) {
+        const int az = kz - pz < 0 ? kz - pz + n3 : kz - pz;
+        const int bz = kz + pz >= n3 ? kz + pz - n3 : kz + pz;
+        om += n1 * n2 * az; op += n1 * n2 * bz;
+      }
+      R pj[3];
+      pj[0] = ks0[px];
+      if (ND >= 2) pj[1] = ks1[py];
+      if (ND >= 3) pj[2] = ks2[pz];
+      cpx<R> dm = mk<R>((R)0, (R)0), dp = dm;
+#pragma unroll
+      for (int j = 0; j < ND; ++j) {
+        dm = cadd(dm, cscale(A[om + j * np], pj[j]));
+        dp = cadd(dp, cscale(A[op + j * np], pj[j]));
+      }
+      const int pi = (int)p;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (c < M) {
+          const cpx<R> fp = f[pi + c * fsf];
+          // i dm fp - i dp conj(fp)
+          const cpx<R> n = csub(cmuli<1>(cmul(dm, fp)), cmuli<1>(cmul(dp, cconj(fp))));
+          s += ek[c].x * n.x + ek[c].y * n.y;
+        }
+      }
+      for (int c = 4; c < M; ++c) {
+        const cpx<R> fp = f[pi + c * fsf];
         const cpx<R> n = csub(cmuli<1>(cmul(dm, fp)), cmuli<1>(cmul(dp, cconj(fp))));
-        const cpx<R> ec = M <= 3 ? ek[c] : e[k + c * g.Np];
+        const cpx<R> ec = e[k + (int64_t)c * np];
         s += ec.x * n.x + ec.y * n.y;
       }
       s *= inv2;
     }
-    S[k + g.Np * p] = (OUT)s;
+    __builtin_nontemporal_store((OUT)s, sp);      // S is written once and not re-read here
+    sp += g.Np;
     net += (double)s;
+    if (++px == n1) {
+      px = 0;
+      if (ND >= 2 && ++py == n2) { py = 0; ++pz; }
+      keep_yz = (ND < 2 || m2m_keep1(py, n2, h2, t2)) && (ND < 3 || m2m_keep1(pz, n3, h3, t3));
+    }
   }
   netpart[(int64_t)blockIdx.y * g.Np + k] = net;
 }

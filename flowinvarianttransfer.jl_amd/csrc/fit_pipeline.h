@@ -1078,14 +1078,20 @@ int m2m_impl(fit_plan* p, int inv, const void* f, const void* a, void* S, void* 
                      (cpx<R>*)E);
   dim3 grid((unsigned)((Np + 255) / 256), nchunk);
   prof_begin(p, FIT_K_OTHER, (double)Np * Np * os, "k_m2m");
-  if (io_dtype == FIT_F64)
-    hipLaunchKernelGGL((k_m2m<R, double>), grid, dim3(256), 0, p->stream, g, trunc, p->M, (const cpx<R>*)f, Np,
-                       (const cpx<R>*)A, (const cpx<R>*)E, (const R*)p->ks[0], (const R*)p->ks[1], (const R*)p->ks[2],
-                       pchunk, (double*)Sd, (double*)netpart);
-  else
-    hipLaunchKernelGGL((k_m2m<R, float>), grid, dim3(256), 0, p->stream, g, trunc, p->M, (const cpx<R>*)f, Np,
-                       (const cpx<R>*)A, (const cpx<R>*)E, (const R*)p->ks[0], (const R*)p->ks[1], (const R*)p->ks[2],
-                       pchunk, (float*)Sd, (double*)netpart);
+  auto launch = [&](auto out_tag, auto nd_tag) {
+    using OUT = decltype(out_tag);
+    constexpr int ND = decltype(nd_tag)::value;
+    hipLaunchKernelGGL((k_m2m<R, OUT, ND>), grid, dim3(256), 0, p->stream, g, trunc, p->M, (const cpx<R>*)f, Np,
+                       (const cpx<R>*)A, (const cpx<R>*)E, (const R*)p->ks[0], (const R*)p->ks[1],
+                       (const R*)p->ks[2], pchunk, (OUT*)Sd, (double*)netpart);
+  };
+  auto by_nd = [&](auto out_tag) {
+    if (p->nd == 1) launch(out_tag, std::integral_constant<int, 1>());
+    else if (p->nd == 2) launch(out_tag, std::integral_constant<int, 2>());
+    else launch(out_tag, std::integral_constant<int, 3>());
+  };
+  if (io_dtype == FIT_F64) by_nd(0.0);
+  else by_nd(0.0f);
   prof_end(p);
   launch_reduce(p, (const double*)netpart, nchunk, (int)Np, 1, (double*)netd);
   hipError_t e = hipGetLastError();

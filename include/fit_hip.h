@@ -67,6 +67,12 @@ int fit_plan_destroy(fit_plan* plan);
 /* Device bytes the plan holds. */
 int64_t fit_plan_device_bytes(const fit_plan* plan);
 
+/* Number of plans made by fit_plan_create and not yet destroyed in this process (all threads,
+ * all devices).  No reference counterpart: a check for bindings that must build one plan per
+ * workspace (Workspaces.jl:22 is called once per NonlinearTermWorkspace) and reuse it, and for
+ * plan leaks. */
+int64_t fit_plan_live_count(void);
+
 /* Plan options (no reference counterpart: execution knobs of this backend).
  *   FIT_OPT_PLAIN_PIPELINE = 1: value 1 runs the plain dataflow (unfused, unpruned passes, one
  *     giver at a time, N̂ through HBM) -- the comparison baseline; 0 (default) the fused one.
