@@ -232,16 +232,26 @@ __device__ __forceinline__ void reg_stage(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* 
     constexpr int NB = E / r;
     constexpr bool LAST = S + 1 == rp_nst(L, E);
     static_assert(E % r == 0, "radix must divide the points per thread");
+    // m = j mod NS is the same for every butterfly of the thread when NS divides T: its
+    // twiddles are loaded once per stage.  Index arithmetic is unsigned (t >= 0).
+    constexpr bool MC = NS > 1 && T % NS == 0;
+    cpx<R> wc[r];
+    if constexpr (MC) {
+      const unsigned mc = (unsigned)t % NS;
+#pragma unroll
+      for (int k = 1; k < r; ++k) wc[k] = twiddle<SIGN, R>(tw, (int)(k * mc * (L / (NS * r))));
+    }
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
-      const int j = t + i * T;
-      const int m = j % NS;
+      const unsigned j = (unsigned)t + i * T;
+      const unsigned m = j % NS;
       cpx<R> w[r];
 #pragma unroll
       for (int k = 0; k < r; ++k) w[k] = v[i + k * NB];
       if constexpr (NS > 1) {
 #pragma unroll
-        for (int k = 1; k < r; ++k) w[k] = cmul(w[k], twiddle<SIGN, R>(tw, k * m * (L / (NS * r))));
+        for (int k = 1; k < r; ++k)
+          w[k] = cmul(w[k], MC ? wc[k] : twiddle<SIGN, R>(tw, (int)(k * m * (L / (NS * r)))));
       }
       dft<r, SIGN, R>(w);
       if constexpr (LAST) {
@@ -251,9 +261,10 @@ __device__ __forceinline__ void reg_stage(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* 
       } else {
         cpx<R>* lds = (DB && (S & 1)) ? lds1 : lds0;
         if constexpr (!DB) { if (i == 0) lds_sync<WAVE>(); }  // single buffer: previous readers done
-        const int o = (j - m) * r + m;
+        const int o = (int)((j - m) * r + m);
+        const int io = idx(o);
 #pragma unroll
-        for (int k = 0; k < r; ++k) lds[idx(o + k * NS)] = w[k];
+        for (int k = 0; k < r; ++k) lds[idx.template step<NS, r>(io, o, k)] = w[k];
       }
     }
     if constexpr (!LAST) {
@@ -318,14 +329,26 @@ template <int L> constexpr bool xfence_r() { return L > 192; }
 template <int L> constexpr bool xfence_s() { return L == 384 || L == 512 || L == 3072; }
 
 // line-major LDS index with one pad slot every 16 points (breaks power-of-two strides)
+// step<NS, r>(io, o, k) = idx(o + k NS) given io = idx(o), for the r outputs of one Stockham
+// butterfly: o = (j - m) r + m with m = j mod NS, so the outputs fill one aligned block of NS r
+// slots.  Within a 16-point pad group (NS r dividing 16) the pad term is that of o; for NS a
+// multiple of 16 it grows by k NS / 16; otherwise it is recomputed.
 struct IdxLine {
   int base;
   __device__ __forceinline__ int operator()(int pos) const { return base + pos + (pos >> 4); }
+  template <int NS, int r>
+  __device__ __forceinline__ int step(int io, int o, int k) const {
+    if constexpr (NS % 16 == 0) return io + k * (NS + NS / 16);
+    else if constexpr (16 % (NS * r) == 0) return io + k * NS;
+    else return (*this)(o + k * NS);
+  }
 };
 // column tiles: point-major, TX lines interleaved (x fastest in both HBM and LDS)
 struct IdxCol {
   int x, TX;
   __device__ __forceinline__ int operator()(int pos) const { return pos * TX + x; }
+  template <int NS, int r>
+  __device__ __forceinline__ int step(int io, int o, int k) const { return io + k * NS * TX; }
 };
 
 constexpr int ldsline(int L) { return L + L / 16; }
@@ -355,25 +378,34 @@ __device__ __forceinline__ void wave_stage(cpx<R> (&v)[E], cpx<R>* lds, const ID
     constexpr bool LAST = S + 1 == rp_nst(L, E);
     static_assert(E % r == 0, "radix must divide the points per lane");
     if constexpr (!LAST) wave_lds_sync();  // reads of the previous stage are complete
+    constexpr bool MC = NS > 1 && T % NS == 0;   // same m (and twiddles) for every butterfly
+    cpx<R> wc[r];
+    if constexpr (MC) {
+      const unsigned mc = (unsigned)t % NS;
+#pragma unroll
+      for (int k = 1; k < r; ++k) wc[k] = twiddle<SIGN, R>(tw, (int)(k * mc * (L / (NS * r))));
+    }
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
-      const int j = t + i * T;
-      const int m = j % NS;
+      const unsigned j = (unsigned)t + i * T;
+      const unsigned m = j % NS;
       cpx<R> w[r];
 #pragma unroll
       for (int k = 0; k < r; ++k) w[k] = v[i + k * NB];
       if constexpr (NS > 1) {
 #pragma unroll
-        for (int k = 1; k < r; ++k) w[k] = cmul(w[k], twiddle<SIGN, R>(tw, k * m * (L / (NS * r))));
+        for (int k = 1; k < r; ++k)
+          w[k] = cmul(w[k], MC ? wc[k] : twiddle<SIGN, R>(tw, (int)(k * m * (L / (NS * r)))));
       }
       dft<r, SIGN, R>(w);
       if constexpr (LAST) {
 #pragma unroll
         for (int k = 0; k < r; ++k) v[i + k * NB] = w[k];   // own slots: no exchange
       } else {
-        const int o = (j - m) * r + m;
+        const int o = (int)((j - m) * r + m);
+        const int io = idx(o);
 #pragma unroll
-        for (int k = 0; k < r; ++k) lds[idx(o + k * NS)] = w[k];
+        for (int k = 0; k < r; ++k) lds[idx.template step<NS, r>(io, o, k)] = w[k];
       }
     }
     if constexpr (!LAST) {
@@ -449,8 +481,8 @@ __device__ __forceinline__ void wave_stage_tw(cpx<R> (&v)[E], cpx<R>* lds, const
     if constexpr (!LAST) wave_lds_sync();
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
-      const int j = t + i * T;
-      const int m = j % NS;
+      const unsigned j = (unsigned)t + i * T;
+      const unsigned m = j % NS;
       cpx<R> w[r];
 #pragma unroll
       for (int k = 0; k < r; ++k) w[k] = v[i + k * NB];
@@ -467,9 +499,10 @@ __device__ __forceinline__ void wave_stage_tw(cpx<R> (&v)[E], cpx<R>* lds, const
 #pragma unroll
         for (int k = 0; k < r; ++k) v[i + k * NB] = w[k];
       } else {
-        const int o = (j - m) * r + m;
+        const int o = (int)((j - m) * r + m);
+        const int io = idx(o);
 #pragma unroll
-        for (int k = 0; k < r; ++k) lds[idx(o + k * NS)] = w[k];
+        for (int k = 0; k < r; ++k) lds[idx.template step<NS, r>(io, o, k)] = w[k];
       }
     }
     if constexpr (!LAST) {
