@@ -211,6 +211,54 @@ constexpr int xE(int L) { return rp_pow2(L) ? 8 : 12; }
 // points per thread of the self-advection fused x pass (three velocity lines stay in registers)
 constexpr int sE(int L) { return rp_pow2(L) ? 8 : 12; }
 
+// LDS exchange layout.  LDS serves 8-byte accesses 16 lanes per cycle over 32 four-byte banks,
+// so an exchange is conflict-free when every aligned group of 16 lanes touches 16 distinct
+// 8-byte slots modulo 16.  Point pos of a line sits at pos + C * (pos >> 4): reads of 16
+// consecutive points are conflict-free for any C.  The writes of a stage with stride NS and
+// radix r are: NS = 1 (lanes r points apart) with C = 1; NS < 16 (groups of NS lanes, NS * r
+// points apart) with C = 16 / r, which moves successive groups to distinct slots; NS >= 16
+// (runs of 16 consecutive points) with any C, 0 chosen.  lds_line(L, E) holds the largest.
+constexpr int lds_pad(int NS, int r) { return NS == 1 ? 1 : (NS < 16 ? 16 / r : 0); }
+constexpr int lds_pad_max(int L, int E) {
+  int m = 1, ns = 1;
+  for (int s = 0; s + 1 < rp_nst(L, E); ++s) {
+    const int r = rp_rad(L, E, s);
+    m = lds_pad(ns, r) > m ? lds_pad(ns, r) : m;
+    ns *= r;
+  }
+  return m;
+}
+constexpr int lds_line(int L, int E) { return L + (L / 16) * lds_pad_max(L, E); }
+
+// Stage-major twiddle table: for every stage s >= 1 (stride NS, radix r), entry (k-1)*NS + m at
+// offset tws_off(s) holds W_{NS r}^{k m} = tw[k m L / (NS r)], so the lanes of a stage read
+// consecutive entries (no bank conflicts).  Sum over stages of (r-1)*NS = L - 1 entries.
+constexpr int tws_off(int L, int E, int S) {
+  int off = 0, ns = 1;
+  for (int s = 0; s < S; ++s) {
+    const int r = rp_rad(L, E, s);
+    if (s >= 1) off += (r - 1) * ns;
+    ns *= r;
+  }
+  return off;
+}
+template <int L, int E, int S, int NS, typename R>
+__device__ __forceinline__ void tws_build_stage(cpx<R>* dst, const cpx<R>* __restrict__ tw, int tid, int nth) {
+  if constexpr (S < rp_nst(L, E)) {
+    constexpr int r = rp_rad(L, E, S);
+    if constexpr (S >= 1) {
+      constexpr int off = tws_off(L, E, S);
+      for (int i = tid; i < (r - 1) * NS; i += nth) dst[off + i] = tw[(i / NS + 1) * (i % NS) * (L / (NS * r))];
+    }
+    tws_build_stage<L, E, S + 1, NS * r, R>(dst, tw, tid, nth);
+  }
+}
+// fill dst[0 .. L-1) from the natural table tw[k] = exp(-2 pi i k / L) (caller synchronises)
+template <int L, int E, typename R>
+__device__ __forceinline__ void tws_build(cpx<R>* dst, const cpx<R>* __restrict__ tw, int tid, int nth) {
+  tws_build_stage<L, E, 0, 1, R>(dst, tw, tid, nth);
+}
+
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): this wave's LDS accesses are complete
   __builtin_amdgcn_wave_barrier();      // no code motion across this point
@@ -223,7 +271,8 @@ __device__ __forceinline__ void lds_sync() {
   else __syncthreads();
 }
 
-template <int L, int E, int S, int NS, int SIGN, bool DB, typename R, class IDX, bool WAVE = false>
+// TWS: tw is the stage-major table of tws_build instead of the natural one.
+template <int L, int E, int S, int NS, int SIGN, bool DB, typename R, class IDX, bool WAVE = false, bool TWS = false>
 __device__ __forceinline__ void reg_stage(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* lds1, const IDX& idx, int t,
                                           const cpx<R>* __restrict__ tw) {
   if constexpr (S < rp_nst(L, E)) {
@@ -231,7 +280,13 @@ __device__ __forceinline__ void reg_stage(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* 
     constexpr int T = L / E;
     constexpr int NB = E / r;
     constexpr bool LAST = S + 1 == rp_nst(L, E);
+    constexpr int C = lds_pad(NS, r);
     static_assert(E % r == 0, "radix must divide the points per thread");
+    // twiddle W_{NS r}^{k m} from either table
+    auto twk = [&](int k, unsigned m) {
+      if constexpr (TWS) return twiddle<SIGN, R>(tw, (int)(tws_off(L, E, S) + (k - 1) * NS + m));
+      else return twiddle<SIGN, R>(tw, (int)(k * m * (L / (NS * r))));
+    };
     // m = j mod NS is the same for every butterfly of the thread when NS divides T: its
     // twiddles are loaded once per stage.  Index arithmetic is unsigned (t >= 0).
     constexpr bool MC = NS > 1 && T % NS == 0;
@@ -239,7 +294,7 @@ __device__ __forceinline__ void reg_stage(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* 
     if constexpr (MC) {
       const unsigned mc = (unsigned)t % NS;
 #pragma unroll
-      for (int k = 1; k < r; ++k) wc[k] = twiddle<SIGN, R>(tw, (int)(k * mc * (L / (NS * r))));
+      for (int k = 1; k < r; ++k) wc[k] = twk(k, mc);
     }
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
@@ -250,8 +305,7 @@ __device__ __forceinline__ void reg_stage(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* 
       for (int k = 0; k < r; ++k) w[k] = v[i + k * NB];
       if constexpr (NS > 1) {
 #pragma unroll
-        for (int k = 1; k < r; ++k)
-          w[k] = cmul(w[k], MC ? wc[k] : twiddle<SIGN, R>(tw, (int)(k * m * (L / (NS * r)))));
+        for (int k = 1; k < r; ++k) w[k] = cmul(w[k], MC ? wc[k] : twk(k, m));
       }
       dft<r, SIGN, R>(w);
       if constexpr (LAST) {
@@ -262,17 +316,17 @@ __device__ __forceinline__ void reg_stage(cpx<R> (&v)[E], cpx<R>* lds0, cpx<R>* 
         cpx<R>* lds = (DB && (S & 1)) ? lds1 : lds0;
         if constexpr (!DB) { if (i == 0) lds_sync<WAVE>(); }  // single buffer: previous readers done
         const int o = (int)((j - m) * r + m);
-        const int io = idx(o);
+        const int io = idx.template at<C>(o);
 #pragma unroll
-        for (int k = 0; k < r; ++k) lds[idx.template step<NS, r>(io, o, k)] = w[k];
+        for (int k = 0; k < r; ++k) lds[idx.template step<NS, r, C>(io, o, k)] = w[k];
       }
     }
     if constexpr (!LAST) {
       cpx<R>* lds = (DB && (S & 1)) ? lds1 : lds0;
       lds_sync<WAVE>();
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = lds[idx(t + T * e)];
-      reg_stage<L, E, S + 1, NS * r, SIGN, DB, R, IDX, WAVE>(v, lds0, lds1, idx, t, tw);
+      for (int e = 0; e < E; ++e) v[e] = lds[idx.template at<C>(t + T * e)];
+      reg_stage<L, E, S + 1, NS * r, SIGN, DB, R, IDX, WAVE, TWS>(v, lds0, lds1, idx, t, tw);
     }
   }
 }
@@ -329,29 +383,34 @@ template <int L> constexpr bool xfence_r() { return L > 192; }
 template <int L> constexpr bool xfence_s() { return L == 384 || L == 512 || L == 3072; }
 
 // line-major LDS index with one pad slot every 16 points (breaks power-of-two strides)
-// step<NS, r>(io, o, k) = idx(o + k NS) given io = idx(o), for the r outputs of one Stockham
-// butterfly: o = (j - m) r + m with m = j mod NS, so the outputs fill one aligned block of NS r
-// slots.  Within a 16-point pad group (NS r dividing 16) the pad term is that of o; for NS a
-// multiple of 16 it grows by k NS / 16; otherwise it is recomputed.
+// Line-major LDS index: point pos of the line at base + pos + C * (pos >> 4) (see lds_pad); the
+// call operator is the C = 1 layout used outside the transforms.  step<NS, r, C>(io, o, k) =
+// at<C>(o + k NS) given io = at<C>(o), for the r outputs of one Stockham butterfly: o = (j - m) r
+// + m with m = j mod NS, so the outputs fill one aligned block of NS r slots.  Within a 16-point
+// group (NS r dividing 16) the pad term is that of o; for NS a multiple of 16 it grows by
+// C k NS / 16; otherwise it is recomputed.
 struct IdxLine {
   int base;
-  __device__ __forceinline__ int operator()(int pos) const { return base + pos + (pos >> 4); }
-  template <int NS, int r>
+  template <int C>
+  __device__ __forceinline__ int at(int pos) const { return base + pos + C * (pos >> 4); }
+  __device__ __forceinline__ int operator()(int pos) const { return at<1>(pos); }
+  template <int NS, int r, int C>
   __device__ __forceinline__ int step(int io, int o, int k) const {
-    if constexpr (NS % 16 == 0) return io + k * (NS + NS / 16);
+    if constexpr (NS % 16 == 0) return io + k * (NS + C * (NS / 16));
     else if constexpr (16 % (NS * r) == 0) return io + k * NS;
-    else return (*this)(o + k * NS);
+    else return at<C>(o + k * NS);
   }
 };
 // column tiles: point-major, TX lines interleaved (x fastest in both HBM and LDS)
 struct IdxCol {
   int x, TX;
+  template <int C>
+  __device__ __forceinline__ int at(int pos) const { return pos * TX + x; }
   __device__ __forceinline__ int operator()(int pos) const { return pos * TX + x; }
-  template <int NS, int r>
+  template <int NS, int r, int C>
   __device__ __forceinline__ int step(int io, int o, int k) const { return io + k * NS * TX; }
 };
 
-constexpr int ldsline(int L) { return L + L / 16; }
 
 }  // namespace fit
 
@@ -403,15 +462,15 @@ __device__ __forceinline__ void wave_stage(cpx<R> (&v)[E], cpx<R>* lds, const ID
         for (int k = 0; k < r; ++k) v[i + k * NB] = w[k];   // own slots: no exchange
       } else {
         const int o = (int)((j - m) * r + m);
-        const int io = idx(o);
+        const int io = idx.template at<lds_pad(NS, r)>(o);
 #pragma unroll
-        for (int k = 0; k < r; ++k) lds[idx.template step<NS, r>(io, o, k)] = w[k];
+        for (int k = 0; k < r; ++k) lds[idx.template step<NS, r, lds_pad(NS, r)>(io, o, k)] = w[k];
       }
     }
     if constexpr (!LAST) {
       wave_lds_sync();
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = lds[idx(t + T * e)];
+      for (int e = 0; e < E; ++e) v[e] = lds[idx.template at<lds_pad(NS, r)>(t + T * e)];
       wave_stage<L, E, S + 1, NS * r, SIGN, R, IDX>(v, lds, idx, t, tw);
     }
   }
@@ -500,15 +559,15 @@ __device__ __forceinline__ void wave_stage_tw(cpx<R> (&v)[E], cpx<R>* lds, const
         for (int k = 0; k < r; ++k) v[i + k * NB] = w[k];
       } else {
         const int o = (int)((j - m) * r + m);
-        const int io = idx(o);
+        const int io = idx.template at<lds_pad(NS, r)>(o);
 #pragma unroll
-        for (int k = 0; k < r; ++k) lds[idx.template step<NS, r>(io, o, k)] = w[k];
+        for (int k = 0; k < r; ++k) lds[idx.template step<NS, r, lds_pad(NS, r)>(io, o, k)] = w[k];
       }
     }
     if constexpr (!LAST) {
       wave_lds_sync();
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = lds[idx(t + T * e)];
+      for (int e = 0; e < E; ++e) v[e] = lds[idx.template at<lds_pad(NS, r)>(t + T * e)];
       wave_stage_tw<L, E, S + 1, NS * r, SIGN, R, IDX, NTW>(v, lds, idx, t, twr);
     }
   }

@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(256) k_row_inv(LD ld, ST st, FftDesc d, const 
 template <typename R, int L, class LD, class ST>
 __global__ void __launch_bounds__(256) k_row_inv_r(LD ld, ST st, const cpx<R>* __restrict__ tw, int G2,
                                                    int nrows, int B, int kxa) {
-  constexpr int E = rp_E(L), T = L / E, LL = ldsline(L);
+  constexpr int E = rp_E(L), T = L / E, LL = lds_line(L, E);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* lds0 = reinterpret_cast<cpx<R>*>(smem_raw);
   const int line = threadIdx.x / T, t = threadIdx.x % T;
@@ -669,7 +669,7 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
                                                   int nrows, int B, R gscale, int kxa,
                                                   const int4* __restrict__ fband, int kxb, int nfg,
                                                   const R* __restrict__ kxt, int64_t ugs, int GCM) {
-  constexpr int E = xE(L), T = L / E, LL = ldsline(L);
+  constexpr int E = xE(L), T = L / E, LL = lds_line(L, E);
   constexpr bool WV = wave_lines<L, E>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* lds0 = reinterpret_cast<cpx<R>*>(smem_raw);
@@ -757,7 +757,7 @@ __global__ void __launch_bounds__(256) k_xfused_self(const cpx<R>* __restrict__ 
                                                      int64_t osy, int64_t osz, int64_t osf, int nd, int M, int G2,
                                                      int nrows, int B, R gscale, int kxa, int kxb,
                                                      const R* __restrict__ kxt) {
-  constexpr int T = L / E, LL = ldsline(L);
+  constexpr int T = L / E, LL = lds_line(L, E);
   constexpr bool WV = wave_lines<L, E>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* lds0 = reinterpret_cast<cpx<R>*>(smem_raw);
@@ -866,14 +866,14 @@ __global__ void __launch_bounds__(256) k_xfused_self_stg(const cpx<float>* __res
                                                          int G2, int nrows, float gscale, int kxa, int kxb,
                                                          const float* __restrict__ kxt) {
   using R = float;
-  constexpr int T = L / E, LL = ldsline(L), B = 256 / T;
+  constexpr int T = L / E, LL = lds_line(L, E), B = 256 / T;
   constexpr int SROW = L / 2 + 2;  // staged half row: kx <= L/2 plus one pad point (16-byte units)
   static_assert(T == 64, "one line pair per wave");
   __shared__ __align__(16) cpx<R> xch[B * LL];
   __shared__ __align__(16) cpx<R> stg[B * 2 * SROW];
   __shared__ __align__(16) cpx<R> tws[L];
   __shared__ R kxs[L / 2 + 1];
-  for (int i = threadIdx.x; i < L; i += blockDim.x) tws[i] = tw[i];
+  tws_build<L, E, R>(tws, tw, threadIdx.x, blockDim.x);   // stage-major: conflict-free twiddle reads
   for (int i = threadIdx.x; i <= L / 2; i += blockDim.x) kxs[i] = kxt[i];
   __syncthreads();
   const int line = threadIdx.x / T, t = threadIdx.x % T;
@@ -930,7 +930,7 @@ __global__ void __launch_bounds__(256) k_xfused_self_stg(const cpx<float>* __res
   auto fft = [&](auto sg, cpx<R>(&v)[E]) {
     int tt = t;
     asm volatile("" : "+v"(tt));
-    reg_stage<L, E, 0, 1, decltype(sg)::value, false, R, IdxLine, true>(v, xch, xch, idx, tt, tws);
+    reg_stage<L, E, 0, 1, decltype(sg)::value, false, R, IdxLine, true, true>(v, xch, xch, idx, tt, tws);
   };
   cpx<R> ur[3][E];
   issue(grad);
@@ -996,7 +996,7 @@ __global__ void __launch_bounds__(256) k_xfused_w(const cpx<R>* __restrict__ tw,
                                                   int nrows, R gscale, int kxa, const int4* __restrict__ fband,
                                                   int kxb, int ngiv, int GC, const R* __restrict__ kxt,
                                                   int64_t ugs) {
-  constexpr int E = wE(L), T = L / E, LPW = 64 / T, LL = ldsline(L);
+  constexpr int E = wE(L), T = L / E, LPW = 64 / T, LL = lds_line(L, E);
   constexpr bool UREG = E <= 8;                  // keep u in registers for short lines
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<R>* ldsb = reinterpret_cast<cpx<R>*>(smem_raw);

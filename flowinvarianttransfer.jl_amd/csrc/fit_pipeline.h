@@ -72,7 +72,7 @@ int launch_row_inv(fit_plan* p, LD ld, ST st, int nfields, double extra_bytes_pe
              (!p->plain && reg_supported(L)) ? "k_row_inv_r" : "k_row_inv", L, KTag<ST>::v);
   if (!p->plain && reg_supported(L)) {
     const int T = reg_T(L);
-    const size_t LL = (size_t)L + L / 16;
+    const size_t LL = (size_t)lds_line(L, rp_E(L));
     int B = std::max(1, kThreads / T);
     while (B > 1 && B * LL * csz > g_max_lds) --B;
     dim3 rgrid((npairs + B - 1) / B, 1, nfields);
@@ -356,13 +356,13 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
   if (p->grads_commuted && !reg_supported(L)) return fail(FIT_E_UNSUPPORTED, "shared-transform gradients need a register x pass");
   // wave form for short register footprints (E <= 16); the workgroup form otherwise
   const bool wave_ok = reg_supported(L) && !p->xfused_block && wE(L) <= 16 &&
-                       (size_t)4 * (64 / (L / wE(L))) * ((size_t)L + L / 16) * csz <= g_max_lds;
+                       (size_t)4 * (64 / (L / wE(L))) * (size_t)lds_line(L, wE(L)) * csz <= g_max_lds;
   prof_begin(p, FIT_K_FUSED_X, bytes, wave_ok ? "k_xfused_w" : reg_supported(L) ? "k_xfused_r" : "k_xfused", L);
   if (wave_ok) {
     // wave-per-line kernel: 4 waves per workgroup, 64/T row pairs per wave
     const int T = L / wE(L);
     const int pairs_per_block = 4 * (64 / T);
-    const size_t LL = (size_t)L + L / 16;
+    const size_t LL = (size_t)lds_line(L, wE(L));
     const size_t lds = (size_t)pairs_per_block * LL * csz;
     const int nrg = (npairs + pairs_per_block - 1) / pairs_per_block;
     const int ngiv = nfg / p->M;
@@ -389,7 +389,7 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
   } else if (reg_supported(L)) {
     // workgroup form: B row pairs per workgroup, single LDS exchange buffer
     const int T = xblk_T(L);
-    const size_t LL = (size_t)L + L / 16;
+    const size_t LL = (size_t)lds_line(L, xE(L));
     int B = std::max(1, 256 / T);
     while (B > 1 && B * LL * csz > g_max_lds) --B;
     const int nrg = (npairs + B - 1) / B;
@@ -466,7 +466,7 @@ int v2_xfused_self(fit_plan* p, const int* bi, const int* bo) {
   case LLEN: {                                                                                                  \
     constexpr int T = LLEN / sE(LLEN);                                                                          \
     const int B = std::max(1, kThreads / T);                                                                    \
-    const size_t lds = (size_t)B * ldsline(LLEN) * csz;                                                         \
+    const size_t lds = (size_t)B * lds_line(LLEN, sE(LLEN)) * csz;                                                         \
     const int nrg = (npairs + B - 1) / B;                                                                       \
     hipLaunchKernelGGL((k_xfused_self<R, LLEN, sE(LLEN)>), dim3((unsigned)nrg), dim3(B * T), lds, p->stream,   \
                        (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf, (cpx<R>*)p->nxbuf, osy, \
