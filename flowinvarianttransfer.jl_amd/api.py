@@ -9,6 +9,8 @@ the component axis last, column-major in memory.  Host inputs are numpy arrays; 
 torch tensors on a HIP device (results then stay on the device).
 """
 import ctypes
+import functools
+import operator
 import os
 import threading
 from collections import OrderedDict
@@ -724,10 +726,22 @@ def calculate_helical_partial_fluxes(velocity_hat, ks, **kw):
 # batch axis (the reference's mpi_batch_map semantics on one GPU)
 # ---------------------------------------------------------------------------------------------
 
-def batch_map(f, items):
+def batch_map(f, items, *, reduce="gather"):
     """Apply f to every item and return the results in input order -- the collation contract of
-    the reference's batch axis (MPIExt.jl:14-23, :39-68), executed on one device."""
-    return [f(x) for x in items]
+    the reference's batch axis (MPIExt.jl:14-23, :39-68), executed on one device -- then combine
+    them as `reduce` asks (MPIExt.jl:25-37): "gather" (or ":gather") returns the ordered list,
+    "sum" the left fold with +, "mean" that sum divided by the item count, and a binary callable
+    the left fold with it; anything else raises ValueError (the reference's ArgumentError)."""
+    tag = reduce.lstrip(":") if isinstance(reduce, str) else None
+    if tag not in ("gather", "sum", "mean") and not callable(reduce):
+        raise ValueError(f"reduce must be :gather, :sum, :mean, or a binary function (got {reduce!r}).")
+    flat = [f(x) for x in items]
+    if tag == "gather":
+        return flat
+    if tag in ("sum", "mean"):
+        tot = functools.reduce(operator.add, flat)
+        return tot / len(flat) if tag == "mean" else tot
+    return functools.reduce(reduce, flat)
 
 
 class Ensemble:

@@ -1224,8 +1224,6 @@ int fit_mode_to_mode(fit_plan* p, int inv, const void* f_hat, const void* adv_ha
   int rc = check_plan(p);
   if (rc) return rc;
   if (!S || !net) return fail(FIT_E_ARGUMENT, "NULL output");
-  if (p->dealias == FIT_DEALIAS_PADDED)
-    return fail(FIT_E_UNSUPPORTED, "mode-to-mode supports NoDealiasing and OrszagTwoThirds");
   if ((rc = check_invariant(p, inv, p->M))) return rc;
   const void *f, *a;
   if ((rc = stage_fa(p, f_hat, adv_hat, io_dtype, ptr_kind, &f, &a))) return rc;

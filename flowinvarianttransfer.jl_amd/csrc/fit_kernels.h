@@ -1703,7 +1703,7 @@ __global__ void k_m2m_prep(Grid g, int trunc, int inv, int M, const cpx<R>* __re
     const int64_t om = g.oidx(mx, my, mz);
     const bool kp = !(trunc && g.dealiased(ix, iy, iz));
     const bool km = !(trunc && g.dealiased(mx, my, mz));
-    for (int j = 0; j < g.nd; ++j) {
+    for (int j = 0; Aout && j < g.nd; ++j) {
       const cpx<R> a = kp ? adv[o + j * asf] : mk<R>((R)0, (R)0);
       const cpx<R> b = km ? adv[om + j * asf] : mk<R>((R)0, (R)0);
       Aout[o + j * g.Np] = mk<R>((R)0.5 * (a.x + b.x), (R)0.5 * (a.y - b.y));
@@ -1739,13 +1739,40 @@ __global__ void k_m2m_prep(Grid g, int trunc, int inv, int M, const cpx<R>* __re
   }
 }
 
+// PaddedThreeHalves: A_j on the padded grid G = Hermitian part of the fftshift-embedded â_j
+// (FFTWExt.jl:134-141, 171): A(q) = (emb(q) + conj emb(-q)) / 2, emb(q) = â at the original
+// index of q when q lies in the embedded block, else 0.  No 2/3 mask on the padded path.
+template <typename R>
+__global__ void k_m2m_padA(Grid g, const cpx<R>* __restrict__ adv, int64_t asf, cpx<R>* __restrict__ Aout) {
+  const int64_t Gp = (int64_t)g.G1 * g.G2 * g.G3;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < Gp; q += stride) {
+    const int qx = (int)(q % g.G1), qy = (int)((q / g.G1) % g.G2), qz = (int)(q / ((int64_t)g.G1 * g.G2));
+    const int mx = qx == 0 ? 0 : g.G1 - qx, my = qy == 0 ? 0 : g.G2 - qy, mz = qz == 0 ? 0 : g.G3 - qz;
+    int ox, oy, oz, px, py, pz;
+    const bool in_q = g.pre(qx, qy, qz, ox, oy, oz);
+    const bool in_m = g.pre(mx, my, mz, px, py, pz);
+    for (int j = 0; j < g.nd; ++j) {
+      const cpx<R> a = in_q ? adv[g.oidx(ox, oy, oz) + j * asf] : mk<R>((R)0, (R)0);
+      const cpx<R> b = in_m ? adv[g.oidx(px, py, pz) + j * asf] : mk<R>((R)0, (R)0);
+      Aout[q + j * Gp] = mk<R>((R)0.5 * (a.x + b.x), (R)0.5 * (a.y - b.y));
+    }
+  }
+}
+
+// Transform-grid index of original index i (the inverse of pad_pre): identity without padding,
+// i - n + G for the upper half (the Nyquist bin of an even n lands at -n/2).
+__device__ __forceinline__ int m2m_emb(int i, int n, int G) { return i < (n + 1) / 2 ? i : i - n + G; }
+
 // One thread per receiver k, looping over a chunk of givers p.  S written column-major
 // S[k + Np p] in OUT precision; FP64 partial net sums per chunk.  The giver's grid indices advance
 // incrementally (no integer division in the loop) and the dimension count is a template argument
 // so the wavenumber and carrier arrays stay in registers.
 __device__ __forceinline__ bool m2m_keep1(int i, int n, int h, int t) { return (i <= h ? i : n - i) < t; }
 
-template <typename R, typename OUT, int ND>
+// With PAD the triad sum runs on the padded grid: k and p are placed by the embedding, k ∓ p are
+// taken mod G and A is the padded table (keep = 1, the scale stays 1/(2 Np²)).
+template <typename R, typename OUT, int ND, bool PAD>
 __global__ void __launch_bounds__(256) k_m2m(Grid g, int trunc, int M, const cpx<R>* __restrict__ f, int64_t fsf,
                                              const cpx<R>* __restrict__ A, const cpx<R>* __restrict__ e,
                                              const R* __restrict__ ks0, const R* __restrict__ ks1,
@@ -1758,7 +1785,11 @@ __global__ void __launch_bounds__(256) k_m2m(Grid g, int trunc, int M, const cpx
   const int n1 = g.n1, n2 = ND >= 2 ? g.n2 : 1, n3 = ND >= 3 ? g.n3 : 1;
   const int h1 = n1 / 2, h2 = n2 / 2, h3 = n3 / 2, t1 = n1 / 3, t2 = n2 / 3, t3 = n3 / 3;
   const int np = (int)g.Np;
+  const int G1 = PAD ? g.G1 : n1, G2 = PAD ? (ND >= 2 ? g.G2 : 1) : n2, G3 = PAD ? (ND >= 3 ? g.G3 : 1) : n3;
+  const int ap = G1 * G2 * G3;    // component stride of A
   const int kx = (int)(k % n1), ky = (int)((k / n1) % n2), kz = (int)(k / ((int64_t)n1 * n2));
+  const int ekx = PAD ? m2m_emb(kx, n1, G1) : kx, eky = PAD ? m2m_emb(ky, n2, G2) : ky,
+            ekz = PAD ? m2m_emb(kz, n3, G3) : kz;
   const bool keepk = !trunc || (m2m_keep1(kx, n1, h1, t1) && (ND < 2 || m2m_keep1(ky, n2, h2, t2)) &&
                                 (ND < 3 || m2m_keep1(kz, n3, h3, t3)));
   cpx<R> ek[4];
@@ -1774,18 +1805,21 @@ __global__ void __launch_bounds__(256) k_m2m(Grid g, int trunc, int M, const cpx
     R s = (R)0;
     const bool keepp = !trunc || (keep_yz && m2m_keep1(px, n1, h1, t1));
     if (keepk && keepp) {
-      const int ax = kx - px < 0 ? kx - px + n1 : kx - px;
-      const int bx = kx + px >= n1 ? kx + px - n1 : kx + px;
+      const int epx = PAD ? m2m_emb(px, n1, G1) : px;
+      const int ax = ekx - epx < 0 ? ekx - epx + G1 : ekx - epx;
+      const int bx = ekx + epx >= G1 ? ekx + epx - G1 : ekx + epx;
       int om = ax, op = bx;
       if (ND >= 2) {
-        const int ay = ky - py < 0 ? ky - py + n2 : ky - py;
-        const int by = ky + py >= n2 ? ky + py - n2 : ky + py;
-        om += n1 * ay; op += n1 * by;
+        const int epy = PAD ? m2m_emb(py, n2, G2) : py;
+        const int ay = eky - epy < 0 ? eky - epy + G2 : eky - epy;
+        const int by = eky + epy >= G2 ? eky + epy - G2 : eky + epy;
+        om += G1 * ay; op += G1 * by;
       }
       if (ND >= 3) {
-        const int az = kz - pz < 0 ? kz - pz + n3 : kz - pz;
-        const int bz = kz + pz >= n3 ? kz + pz - n3 : kz + pz;
-        om += n1 * n2 * az; op += n1 * n2 * bz;
+        const int epz = PAD ? m2m_emb(pz, n3, G3) : pz;
+        const int az = ekz - epz < 0 ? ekz - epz + G3 : ekz - epz;
+        const int bz = ekz + epz >= G3 ? ekz + epz - G3 : ekz + epz;
+        om += G1 * G2 * az; op += G1 * G2 * bz;
       }
       R pj[3];
       pj[0] = ks0[px];
@@ -1794,8 +1828,8 @@ __global__ void __launch_bounds__(256) k_m2m(Grid g, int trunc, int M, const cpx
       cpx<R> dm = mk<R>((R)0, (R)0), dp = dm;
 #pragma unroll
       for (int j = 0; j < ND; ++j) {
-        dm = cadd(dm, cscale(A[om + j * np], pj[j]));
-        dp = cadd(dp, cscale(A[op + j * np], pj[j]));
+        dm = cadd(dm, cscale(A[om + j * ap], pj[j]));
+        dp = cadd(dp, cscale(A[op + j * ap], pj[j]));
       }
       const int pi = (int)p;
 #pragma unroll

@@ -1064,7 +1064,9 @@ int m2m_impl(fit_plan* p, int inv, const void* f, const void* a, void* S, void* 
   const int nchunk = (int)((Np + pchunk - 1) / pchunk);
   const size_t os = io_dtype == FIT_F64 ? 8 : 4;
   auto cleanup = [&]() { if (A) (void)hipFree(A); if (E) (void)hipFree(E); if (Sd && Sd != S) (void)hipFree(Sd); if (netpart) (void)hipFree(netpart); if (netd) (void)hipFree(netd); };
-  if (hipMalloc(&A, (size_t)p->nd * Np * cs) != hipSuccess || hipMalloc(&E, (size_t)p->M * Np * cs) != hipSuccess ||
+  const bool pad = p->dealias == FIT_DEALIAS_PADDED;
+  const int64_t Ap = pad ? p->Gp : Np;   // the triad table lives on the transform grid
+  if (hipMalloc(&A, (size_t)p->nd * Ap * cs) != hipSuccess || hipMalloc(&E, (size_t)p->M * Np * cs) != hipSuccess ||
       hipMalloc(&netpart, (size_t)nchunk * Np * sizeof(double)) != hipSuccess ||
       hipMalloc(&netd, (size_t)Np * sizeof(double)) != hipSuccess) {
     cleanup();
@@ -1074,16 +1076,23 @@ int m2m_impl(fit_plan* p, int inv, const void* f, const void* a, void* S, void* 
   else if (hipMalloc(&Sd, (size_t)Np * Np * os) != hipSuccess) { cleanup(); return fail(FIT_E_OUT_OF_MEMORY, "S staging"); }
   const int trunc = p->dealias == FIT_DEALIAS_TWOTHIRDS;
   hipLaunchKernelGGL(k_m2m_prep<R>, dim3(512), dim3(256), 0, p->stream, g, trunc, inv, p->M, (const cpx<R>*)f, Np,
-                     (const cpx<R>*)a, Np, (const R*)p->ks[0], (const R*)p->ks[1], (const R*)p->ks[2], (cpx<R>*)A,
-                     (cpx<R>*)E);
+                     (const cpx<R>*)a, Np, (const R*)p->ks[0], (const R*)p->ks[1], (const R*)p->ks[2],
+                     pad ? (cpx<R>*)nullptr : (cpx<R>*)A, (cpx<R>*)E);
+  if (pad)
+    hipLaunchKernelGGL(k_m2m_padA<R>, dim3(512), dim3(256), 0, p->stream, g, (const cpx<R>*)a, Np, (cpx<R>*)A);
   dim3 grid((unsigned)((Np + 255) / 256), nchunk);
   prof_begin(p, FIT_K_OTHER, (double)Np * Np * os, "k_m2m");
   auto launch = [&](auto out_tag, auto nd_tag) {
     using OUT = decltype(out_tag);
     constexpr int ND = decltype(nd_tag)::value;
-    hipLaunchKernelGGL((k_m2m<R, OUT, ND>), grid, dim3(256), 0, p->stream, g, trunc, p->M, (const cpx<R>*)f, Np,
-                       (const cpx<R>*)A, (const cpx<R>*)E, (const R*)p->ks[0], (const R*)p->ks[1],
-                       (const R*)p->ks[2], pchunk, (OUT*)Sd, (double*)netpart);
+    if (pad)
+      hipLaunchKernelGGL((k_m2m<R, OUT, ND, true>), grid, dim3(256), 0, p->stream, g, 0, p->M, (const cpx<R>*)f, Np,
+                         (const cpx<R>*)A, (const cpx<R>*)E, (const R*)p->ks[0], (const R*)p->ks[1],
+                         (const R*)p->ks[2], pchunk, (OUT*)Sd, (double*)netpart);
+    else
+      hipLaunchKernelGGL((k_m2m<R, OUT, ND, false>), grid, dim3(256), 0, p->stream, g, trunc, p->M, (const cpx<R>*)f,
+                         Np, (const cpx<R>*)A, (const cpx<R>*)E, (const R*)p->ks[0], (const R*)p->ks[1],
+                         (const R*)p->ks[2], pchunk, (OUT*)Sd, (double*)netpart);
   };
   auto by_nd = [&](auto out_tag) {
     if (p->nd == 1) launch(out_tag, std::integral_constant<int, 1>());

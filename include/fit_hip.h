@@ -221,7 +221,10 @@ int fit_shell_to_shell(fit_plan* plan, int invariant, const void* f_hat, const v
  * Mode-resolved triad transfer S(k|p) (ns..., ns...) in io_dtype real (receiver k first) and
  * net(k) = Σ_p S(k|p) (ns...) in io_dtype real.  Replaces calculate_mode_to_mode_transfer's giver
  * loop (ScaleToScaleTransfer.jl:75-88).  S and net must be DEVICE pointers when Np^2 values do
- * not fit host memory; ptr_kind applies to all four pointers.  Dealias NONE or TWOTHIRDS.
+ * not fit host memory; ptr_kind applies to all four pointers.  Every dealiasing: with
+ * PADDED the triad sum runs on the padded grid exactly as the reference's padded nonlinear term
+ * of each one-mode giver field (FFTWExt.jl:160-183).  The plan's transform lengths must be
+ * 2^a 3^b 5^c (no transform runs, but the plan is created like any other).
  */
 int fit_mode_to_mode(fit_plan* plan, int invariant, const void* f_hat, const void* adv_hat,
                      void* S, void* net, int io_dtype, int ptr_kind);

@@ -660,10 +660,15 @@ def mode_to_mode_closed_form(u_hat, ks, invariant="ke", dealiasing="23", advecti
       N̂_p(k) = keep(k)keep(p)/(2Np²) Σ_j p_j [ i f̂(p) A_j(k-p) - i conj(f̂(p)) A_j(k+p) ],
       A_j = Hermitian part of keep·â_j,   S(k|p) = Re Σ_c conj(e_c(k)) N̂_{p,c}(k).
 
-    Evaluated by index arithmetic (no FFT), so it checks the FFT loop independently."""
+    Evaluated by index arithmetic (no FFT), so it checks the FFT loop independently.
+
+    PaddedThreeHalves (FFTWExt.jl:160-183): the same sum on the padded grid G = padded_len(n),
+    with keep = 1, A_j the Hermitian part (on G) of the fftshift-embedded â_j, and k, p placed by
+    the embedding E(i) = i for i < ceil(n/2), i - n + G otherwise (the Nyquist bin of an even n at
+    -n/2); k ∓ p are taken mod G, so a pair whose sum reaches the Nyquist bin of the other side
+    picks up A_j there exactly as the padded transform does.  The scale Mtot/Ntot² of the
+    reference's truncation and the 1/Mtot² of the two padded transforms leave 1/(2 Ntot²)."""
     d = dealias_tag(dealiasing)
-    if d == "pad":
-        raise ValueError("closed form covers NoDealiasing and OrszagTwoThirds")
     if advecting_hat is None:
         advecting_hat = u_hat
     nd = len(ks)
@@ -672,20 +677,26 @@ def mode_to_mode_closed_form(u_hat, ks, invariant="ke", dealiasing="23", advecti
     Np = float(np.prod(ns))
     keep = keep_mask(ns) if d == "23" else np.ones(ns, bool)
     e = effective_carrier(invariant, u_hat, ks)
+    Gs = tuple(padded_len(n) for n in ns) if d == "pad" else tuple(ns)
     A = []
     for j in range(nd):
         g = keep * advecting_hat[..., j]
+        if d == "pad":
+            g = embed_spectrum(g, Gs)
         neg = g[tuple(np.s_[::-1] for _ in range(nd))]
         neg = np.roll(neg, 1, axis=tuple(range(nd)))          # g(-k)
         A.append(0.5 * (g + np.conj(neg)))
+    # transform-grid index of every original index (identity without padding)
+    emb = [np.where(np.arange(n) < (n + 1) // 2, np.arange(n), np.arange(n) - n + G)
+           for n, G in zip(ns, Gs)]
     S = np.zeros(tuple(ns) + tuple(ns))
-    kidx = np.meshgrid(*[np.arange(n) for n in ns], indexing="ij")
+    kidx = np.meshgrid(*[emb[dd] for dd in range(nd)], indexing="ij")
     for p in np.ndindex(*ns):
         if not keep[p]:
             continue
         pk = [ks[j][p[j]] for j in range(nd)]
-        kmp = tuple((kidx[dd] - p[dd]) % ns[dd] for dd in range(nd))
-        kpp = tuple((kidx[dd] + p[dd]) % ns[dd] for dd in range(nd))
+        kmp = tuple((kidx[dd] - emb[dd][p[dd]]) % Gs[dd] for dd in range(nd))
+        kpp = tuple((kidx[dd] + emb[dd][p[dd]]) % Gs[dd] for dd in range(nd))
         dot_m = sum(pk[j] * A[j][kmp] for j in range(nd))
         dot_p = sum(pk[j] * A[j][kpp] for j in range(nd))
         tot = np.zeros(ns)

@@ -183,7 +183,10 @@ def test_scalar_shell_to_shell(fit, oracle):
 
 M2M_CASES = [((8, 8), "ke", "23"), ((6, 6), "ke", "none"), ((8, 8), "enstrophy", "23"),
              ((4, 4, 4), "ke", "none"), ((4, 4, 4), "helicity", "none"), ((4, 4, 4), "enstrophy", "none"),
-             ((8, 8), "scalar", "23")]
+             ((8, 8), "scalar", "23"),
+             # PaddedThreeHalves: (8, 8) pads to exactly 3n/2, (6, 6) to 10, (5, 6) to (9, 10)
+             ((8, 8), "ke", "pad"), ((6, 6), "enstrophy", "pad"), ((5, 6), "ke", "pad"),
+             ((4, 4, 4), "ke", "pad"), ((4, 4, 4), "helicity", "pad"), ((8, 8), "scalar", "pad")]
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
@@ -193,22 +196,22 @@ def test_mode_to_mode_matches_oracle(fit, oracle, ns, inv, dea, prec):
     rng = np.random.default_rng(3)
     nd = len(ns)
     ks = ks_of(oracle, ns)
-    if nd == 2:
+    if nd == 2 and ns[0] == ns[1]:
         u, _ = oracle.streamfunction_velocity_2d(ns[0], 21)
     else:
-        u = oracle.synthetic_field(ns, 3, seed=2)
+        u = oracle.synthetic_field(ns, nd, seed=2)
     dt = np.complex128 if prec == "f64" else np.complex64
     tol = TOL64 if prec == "f64" else TOL32
+    deal = {"none": fit.NoDealiasing(), "23": fit.OrszagTwoThirds(), "pad": fit.PaddedThreeHalves()}[dea]
     if inv == "scalar":
         th = rand_c(rng, ns + (1,))
         net_o, S_o = oracle.calculate_mode_to_mode_transfer(th, ks, "scalar", dea, force=True,
                                                             advecting_hat=u)
         r = fit.calculate_scalar_mode_to_mode_transfer(u.astype(dt), th.astype(dt), ks,
-                                                       dealiasing=fit.OrszagTwoThirds())
+                                                       dealiasing=deal)
     else:
         prim = u + 0.2 * rand_c(rng, u.shape)        # non-Hermitian primary, Hermitian advecting
         I = {"ke": fit.KineticEnergy(), "helicity": fit.Helicity(), "enstrophy": fit.Enstrophy()}[inv]
-        deal = {"none": fit.NoDealiasing(), "23": fit.OrszagTwoThirds()}[dea]
         net_o, S_o = oracle.calculate_mode_to_mode_transfer(prim, ks, inv, dea, force=True,
                                                             advecting_hat=u)
         r = fit.calculate_mode_to_mode_transfer(prim.astype(dt), ks, invariant=I, dealiasing=deal,

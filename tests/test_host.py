@@ -93,3 +93,17 @@ def test_missing_library_fails_loudly(fit, tmp_path):
     from fit_amd import _lib
     with pytest.raises(RuntimeError):
         _lib.load_library(str(tmp_path / "nope.so"))
+
+
+def test_batch_map_reductions(fit):
+    # MPIExt.jl:25-37: :gather keeps input order; :sum / :mean fold with +; a binary function
+    # folds with it; anything else is an ArgumentError (ValueError here).
+    items = [np.arange(3.0) + i for i in range(4)]
+    assert [list(x) for x in fit.batch_map(lambda a: 2 * a, items)] == [list(2 * a) for a in items]
+    assert [list(x) for x in fit.batch_map(lambda a: a, items, reduce=":gather")] == [list(a) for a in items]
+    assert np.array_equal(fit.batch_map(lambda a: a, items, reduce=":sum"), np.array([6.0, 10.0, 14.0]))
+    assert np.array_equal(fit.batch_map(lambda a: a, items, reduce="mean"), np.array([1.5, 2.5, 3.5]))
+    assert np.array_equal(fit.batch_map(lambda a: a, items, reduce=np.maximum), items[-1])
+    assert fit.batch_map(lambda i: i, range(5), reduce=lambda a, b: a * 10 + b) == 1234
+    with pytest.raises(ValueError, match="reduce must be"):
+        fit.batch_map(lambda a: a, items, reduce=":max")

@@ -328,6 +328,28 @@ def test_mode_to_mode_loop_equals_closed_form(oracle, case):
     assert rel(S, Sc) < 1e-12
 
 
+@pytest.mark.parametrize("ns,inv", [((6, 6), "ke"), ((8, 8), "ke"), ((8, 8), "enstrophy"),
+                                    ((5, 7), "ke"), ((4, 4, 4), "ke"), ((4, 4, 4), "helicity"),
+                                    ((6,), "ke")])
+def test_padded_mode_to_mode_loop_equals_closed_form(oracle, ns, inv):
+    # PaddedThreeHalves: the reference's giver loop through the padded nonlinear term (embed,
+    # padded transforms, truncate) against the index-arithmetic sum on the padded grid.  The
+    # primary field is non-Hermitian with Nyquist content; (8, 8) pads to exactly 3n/2 = 12, where
+    # k + p of two Nyquist bins lands on the opposite Nyquist bin of the padded grid.
+    rng = np.random.default_rng(sum(ns) + len(inv))
+    nd = len(ns)
+    ks = oracle.wavenumber_grid(ns, (2 * math.pi,) * nd)
+    prim = rng.standard_normal(ns + (nd,)) + 1j * rng.standard_normal(ns + (nd,))
+    adv = (oracle.synthetic_field(ns, nd, seed=3) if nd > 1
+           else rng.standard_normal(ns + (1,)) + 1j * rng.standard_normal(ns + (1,)))
+    net, S = oracle.calculate_mode_to_mode_transfer(prim, ks, inv, "pad", force=True,
+                                                    advecting_hat=adv)
+    Sc = oracle.mode_to_mode_closed_form(prim, ks, inv, "pad", advecting_hat=adv)
+    assert rel(S, Sc) < 1e-12
+    S23 = oracle.mode_to_mode_closed_form(prim, ks, inv, "none", advecting_hat=adv)
+    assert rel(S, S23) > 1e-3          # padding changes the triads (not the unpadded form)
+
+
 def test_mode_to_mode_reductions(oracle):
     # runtests.jl:1072-1111: antisymmetry, conservation, Σ_p S -> T(k), shell sums -> T(n,m)
     u, ks = oracle.streamfunction_velocity_2d(12, 21)
