@@ -544,6 +544,8 @@ static int create_plan(fit_plan** out, int nd, const int64_t* ns, int M, int D, 
       for (int i = 0; i < nk; ++i) kf[i] = (float)kv[i];
       hipMemcpy(p->tw[d], twf.data(), (size_t)L * cs, hipMemcpyHostToDevice);
       hipMemcpy(p->ks[d], kf.data(), (size_t)nk * rs, hipMemcpyHostToDevice);
+      if ((rc = dalloc(p, &p->tw64[d], (size_t)L * 16))) return bail(rc);
+      hipMemcpy(p->tw64[d], tw.data(), (size_t)L * 16, hipMemcpyHostToDevice);
     }
   }
   if ((rc = setup_commute(p, ks, false))) return bail(rc);
@@ -790,7 +792,7 @@ int fit_plan_destroy(fit_plan* p) {
   p->subs.clear();
   p->sub = nullptr;
   dfree(p->sub_f); dfree(p->sub_a);
-  for (int d = 0; d < 3; ++d) { dfree(p->tw[d]); dfree(p->ks[d]); }
+  for (int d = 0; d < 3; ++d) { dfree(p->tw[d]); dfree(p->tw64[d]); dfree(p->ks[d]); }
   dfree(p->kt[0]); dfree(p->kt[1]);
   void* sh = p->shell; dfree(sh);
   dfree(p->work); dfree(p->uphys); dfree(p->nphys); dfree(p->nhat);
@@ -873,6 +875,10 @@ int fit_plan_set_option(fit_plan* p, int option, int value) {
       for (int d = 0; d < 3; ++d) make_desc(p->G[d], p->fd[d], value);
       p->radix_order = value;
       return FIT_OK;
+    case FIT_OPT_COL_F64:
+      if (value < 0 || value > 3) return fail(FIT_E_ARGUMENT, "FIT_OPT_COL_F64 takes 0 to 3");
+      p->col_f64 = value;
+      return FIT_OK;
     default: return fail(FIT_E_ARGUMENT, "unknown plan option");
   }
 }
@@ -891,6 +897,7 @@ int fit_plan_get_option(const fit_plan* p, int option, int* value) {
     case FIT_OPT_GRAD_SPLIT: *value = p->grad_split; return FIT_OK;
     case FIT_OPT_FUSED_FLUX: *value = p->fused_flux; return FIT_OK;
     case FIT_OPT_RADIX_ORDER: *value = p->radix_order; return FIT_OK;
+    case FIT_OPT_COL_F64: *value = p->col_f64; return FIT_OK;
     default: return fail(FIT_E_ARGUMENT, "unknown plan option");
   }
 }

@@ -138,6 +138,9 @@ struct fit_plan {
   // ordering after a caller's stream (fit_plan_wait_stream)
   hipEvent_t order_ev = nullptr;
   int radix_order = 0;            // FIT_OPT_RADIX_ORDER of the LDS-engine passes
+  int col_f64 = 2;                // FIT_OPT_COL_F64: FP32 plans run the strided register passes in FP64
+                                  // (bit 0: inverse passes, bit 1: forward passes)
+  void* tw64[3] = {nullptr, nullptr, nullptr};  // FP64 twiddles of an FP32 plan (FIT_OPT_COL_F64)
   std::string tune_report;        // the last fit_plan_autotune: workload, times, choices
   // profiling
   bool prof = false;
@@ -207,8 +210,16 @@ int rows_tail(fit_plan* p, int n_inv, double* T, double* Pi);
 
 extern size_t g_max_lds;  // per-workgroup LDS the launches may use
 constexpr int kThreads = 256;
+// LDS the deterministic shell bins may take as private per-wave copies (above it the waves commit
+// in turn): streaming density passes, and the FFT + density passes (which keep their exchange LDS)
+constexpr size_t kDensBinBudget = 64 * 1024, kFftBinBudget = 16 * 1024;
 
 // ---- small shared helpers ---------------------------------------------------------------------
+// twiddle table of axis d in the compute precision C of a pass over fields stored in R
+template <typename R, typename C> inline const cpx<C>* twc(const fit_plan* p, int d) {
+  return (const cpx<C>*)(sizeof(C) != sizeof(R) ? p->tw64[d] : p->tw[d]);
+}
+
 // largest power-of-two tile with 2*L*T*sizeof(cpx) <= 64 KiB (ping-pong LDS), capped at cap
 inline int tile_for(int L, size_t csz, int cap) {
   int t = cap;

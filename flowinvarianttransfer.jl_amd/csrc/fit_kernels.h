@@ -69,6 +69,52 @@ struct Band {
   __host__ __device__ __forceinline__ int at(int t) const { return (full() || t <= a) ? t : G - (2 * a + 1 - t); }
 };
 
+// precision conversion of a complex value (load in storage precision, compute in C)
+template <typename C, typename V> __device__ __forceinline__ cpx<C> cvt(V v) { return mk<C>((C)v.x, (C)v.y); }
+
+// Deterministic FP64 LDS shell bins.  The lanes of one wave instruction that add to the same LDS
+// address are applied in a fixed hardware order, and a wave's instructions in program order; only
+// the interleaving of different waves varies from run to run.  So either every wave owns a private
+// copy of the bins (priv, summed in wave order when the partial row is written), or the waves
+// commit their contributions one after another between workgroup barriers.  With the fixed-order
+// cross-workgroup reduction the shell sums are then bitwise reproducible.
+struct DetBins {
+  double* b;
+  int nb, priv;
+  __device__ __forceinline__ void zero() const {
+    const int n = priv ? nb * (int)(blockDim.x >> 6) : nb;
+    for (int s = threadIdx.x; s < n; s += blockDim.x) b[s] = 0.0;
+  }
+  // f(bins) adds the calling thread's contributions; every thread of the workgroup calls commit
+  // the same number of times (it may hold workgroup barriers)
+  template <class F> __device__ __forceinline__ void commit(F&& f) const {
+    if (priv) {
+      f(b + (threadIdx.x >> 6) * nb);
+      return;
+    }
+    const int nw = (int)(blockDim.x >> 6), w = (int)(threadIdx.x >> 6);
+    for (int i = 0; i < nw; ++i) {
+      if (w == i) f(b);
+      __syncthreads();
+    }
+  }
+  // after a barrier: the workgroup's sum of bin s
+  __device__ __forceinline__ double total(int s) const {
+    if (!priv) return b[s];
+    double v = 0.0;
+    const int nw = (int)(blockDim.x >> 6);
+    for (int w = 0; w < nw; ++w) v += b[w * nb + s];
+    return v;
+  }
+};
+// private copies when they fit the budget; the LDS bytes the bins take
+__host__ __device__ inline int detbins_priv(int nb, int nthreads, size_t budget) {
+  return (size_t)(nthreads / 64) * nb * sizeof(double) <= budget ? 1 : 0;
+}
+__host__ __device__ inline size_t detbins_bytes(int nb, int nthreads, int priv) {
+  return (size_t)(priv ? nthreads / 64 : 1) * (nb > 0 ? nb : 1) * sizeof(double);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Loaders: value at logical (x, y, z) of field f
 // ---------------------------------------------------------------------------------------------
@@ -458,14 +504,20 @@ __global__ void __launch_bounds__(256) k_xfused(FftDesc d, const cpx<R>* __restr
 // Column pass: thread (t, x) with x fastest; TX columns per workgroup; single LDS buffer.
 // FP64 workgroups stay within 256 threads, so a lane may hold all 24 complex doubles of its line
 // and the FFT temporaries (a 1024-thread bound caps a lane at 128 VGPRs and spills ~100).
+// C: the precision the transform computes in (its LDS exchanges and twiddles included); fields are
+// loaded and stored in R.  C = double on an FP32 field rounds each value once per pass instead of
+// at every butterfly stage.
+// FP64 arithmetic holds 12 points per lane at 3·2^a lengths (24 in FP32): 64 lanes per column
+// keep four columns (256 threads) in the 48 KB of LDS that three workgroups per CU share.
 template <typename R> constexpr int col_threads() { return sizeof(R) == 8 ? 256 : 1024; }
-template <typename R, int L, int SIGN, class LD, class ST>
-__global__ void __launch_bounds__(col_threads<R>()) k_col_r(LD ld, ST st, const cpx<R>* __restrict__ tw, int nx, int TX,
+template <typename C> constexpr int colE(int L) { return sizeof(C) == 8 ? gE(L) : rp_E(L); }
+template <typename R, int L, int SIGN, class LD, class ST, typename C = R>
+__global__ void __launch_bounds__(col_threads<C>()) k_col_r(LD ld, ST st, const cpx<C>* __restrict__ tw, int nx, int TX,
                                                 int axis, Band ob, Band ib, Band wb,
                                                 const int4* __restrict__ fband, int fdiv, int Gother) {
-  constexpr int E = rp_E(L), T = L / E;
+  constexpr int E = colE<C>(L), T = L / E;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<C>* lds = reinterpret_cast<cpx<C>*>(smem_raw);
   int bx, by, bz;
   xcd_tile(bx, by, bz);
   const int xl = threadIdx.x % TX, t = threadIdx.x / TX;
@@ -483,21 +535,21 @@ __global__ void __launch_bounds__(col_threads<R>()) k_col_r(LD ld, ST st, const 
     }
   }
   const int x = x0 + xl;
-  cpx<R> v[E];
+  cpx<C> v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = t + T * e;
-    v[e] = mk<R>((R)0, (R)0);
-    if (x < nx && ib.has(i)) v[e] = axis == 1 ? ld(x, i, o, f) : ld(x, o, i, f);
+    v[e] = mk<C>((C)0, (C)0);
+    if (x < nx && ib.has(i)) v[e] = cvt<C>(axis == 1 ? ld(x, i, o, f) : ld(x, o, i, f));
   }
-  reg_fft<L, SIGN, false, R>(v, lds, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_e<L, E, SIGN, false, C>(v, lds, lds, IdxCol{xl, TX}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = t + T * e;
       if (wb.has(i)) {
-        if (axis == 1) st(x, i, o, f, v[e]);
-        else st(x, o, i, f, v[e]);
+        if (axis == 1) st(x, i, o, f, cvt<R>(v[e]));
+        else st(x, o, i, f, cvt<R>(v[e]));
       }
     }
   }
@@ -517,14 +569,14 @@ __global__ void __launch_bounds__(col_threads<R>()) k_col_r(LD ld, ST st, const 
 // instead of nd loads of the source.  Fields are (g, c, j) = (g M + c) nd + j as before.
 // Two lines in flight per thread: at most 512 threads per workgroup keeps them in registers.
 // ---------------------------------------------------------------------------------------------
-template <typename R, int L>
+template <typename R, int L, typename C = R>
 __global__ void __launch_bounds__(512) k_grad1_r(LdSpectral<R> ls, int M, int giver0, StHalf<R> st,
-                                                  const cpx<R>* __restrict__ tw, int nx, int TX, int axis,
+                                                  const cpx<C>* __restrict__ tw, int nx, int TX, int axis,
                                                   Band ob, Band ib, Band wb, const int4* __restrict__ fband,
                                                   int Gother) {
   constexpr int E = gE(L), T = L / E;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<C>* lds = reinterpret_cast<cpx<C>*>(smem_raw);
   int bx, by, bz;
   xcd_tile(bx, by, bz);
   const int xl = threadIdx.x % TX, t = threadIdx.x / TX;
@@ -547,57 +599,58 @@ __global__ void __launch_bounds__(512) k_grad1_r(LdSpectral<R> ls, int M, int gi
   const int nd = s.g.nd;
   const int x = x0 + xl;
   const int mx = x == 0 ? 0 : s.g.G1 - x;
-  cpx<R> v0[E], v1[E];
+  cpx<C> v0[E], v1[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = t + T * e;
-    v0[e] = mk<R>((R)0, (R)0);
+    v0[e] = mk<C>((C)0, (C)0);
     v1[e] = v0[e];
     if (x < nx && ib.has(i)) {
       const int yy = axis == 1 ? i : o, zz = axis == 1 ? o : i;
       const int my = yy == 0 ? 0 : s.g.G2 - yy, mz = zz == 0 ? 0 : s.g.G3 - zz;
-      R ka, kb;
-      const cpx<R> a = s.Graw(x, yy, zz, c + g * s.gstride, axis, ka);
-      const cpx<R> b = s.Graw(mx, my, mz, c + g * s.gstride, axis, kb);
-      v0[e] = mk<R>((R)0.5 * (a.x + b.x), (R)0.5 * (a.y - b.y));
-      v1[e] = mk<R>((R)-0.5 * (ka * a.y + kb * b.y), (R)0.5 * (ka * a.x - kb * b.x));
+      R kar, kbr;
+      const cpx<C> a = cvt<C>(s.Graw(x, yy, zz, c + g * s.gstride, axis, kar));
+      const cpx<C> b = cvt<C>(s.Graw(mx, my, mz, c + g * s.gstride, axis, kbr));
+      const C ka = (C)kar, kb = (C)kbr;
+      v0[e] = mk<C>((C)0.5 * (a.x + b.x), (C)0.5 * (a.y - b.y));
+      v1[e] = mk<C>((C)-0.5 * (ka * a.y + kb * b.y), (C)0.5 * (ka * a.x - kb * b.x));
     }
   }
   const int f0 = gc * nd, fa = gc * nd + nd - 1;
-  reg_fft_m<L, E, 1, true, false, R>(v0, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, false, C>(v0, lds, IdxCol{xl, TX}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = t + T * e;
       if (wb.has(i)) {
-        if (axis == 1) st(x, i, o, f0, v0[e]);
-        else st(x, o, i, f0, v0[e]);
+        if (axis == 1) st(x, i, o, f0, cvt<R>(v0[e]));
+        else st(x, o, i, f0, cvt<R>(v0[e]));
       }
     }
   }
   __syncthreads();
-  reg_fft_m<L, E, 1, true, false, R>(v1, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, false, C>(v1, lds, IdxCol{xl, TX}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = t + T * e;
       if (wb.has(i)) {
-        if (axis == 1) st(x, i, o, fa, v1[e]);
-        else st(x, o, i, fa, v1[e]);
+        if (axis == 1) st(x, i, o, fa, cvt<R>(v1[e]));
+        else st(x, o, i, fa, cvt<R>(v1[e]));
       }
     }
   }
 }
 
 // y pass of the shared-transform gradients (nd = 3), in place on the (g, c, j) fields.
-template <typename R, int L>
+template <typename R, int L, typename C = R>
 __global__ void __launch_bounds__(512) k_grad2_r(cpx<R>* __restrict__ buf, int64_t sy, int64_t sz, int64_t sf,
-                                                  const R* __restrict__ kyt, const cpx<R>* __restrict__ tw,
+                                                  const R* __restrict__ kyt, const cpx<C>* __restrict__ tw,
                                                   int nx, int TX, Band ob, Band ib, Band wb,
                                                   const int4* __restrict__ fband, int M, int slot2) {
   constexpr int E = gE(L), T = L / E;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<C>* lds = reinterpret_cast<cpx<C>*>(smem_raw);
   int bx, by, bz;
   xcd_tile(bx, by, bz);
   const int xl = threadIdx.x % TX, t = threadIdx.x / TX;
@@ -613,46 +666,46 @@ __global__ void __launch_bounds__(512) k_grad2_r(cpx<R>* __restrict__ buf, int64
   cpx<R>* f0 = buf + (int64_t)(gc * 3) * sf + x + o * sz;
   cpx<R>* f1 = f0 + sf;
   cpx<R>* f2 = f1 + sf;
-  cpx<R> v0[E], v1[E];
+  cpx<C> v0[E], v1[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = t + T * e;
-    v0[e] = mk<R>((R)0, (R)0);
-    if (x < nx && ib.has(i)) v0[e] = f0[i * sy];
-    const R k = kyt[i];
-    v1[e] = mk<R>(-k * v0[e].y, k * v0[e].x);
+    v0[e] = mk<C>((C)0, (C)0);
+    if (x < nx && ib.has(i)) v0[e] = cvt<C>(f0[i * sy]);
+    const C k = (C)kyt[i];
+    v1[e] = mk<C>(-k * v0[e].y, k * v0[e].x);
   }
-  reg_fft_m<L, E, 1, true, false, R>(v0, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, false, C>(v0, lds, IdxCol{xl, TX}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = t + T * e;
-      if (wb.has(i)) f0[i * sy] = v0[e];
+      if (wb.has(i)) f0[i * sy] = cvt<R>(v0[e]);
     }
   }
   __syncthreads();
-  reg_fft_m<L, E, 1, true, false, R>(v1, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, false, C>(v1, lds, IdxCol{xl, TX}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = t + T * e;
-      if (wb.has(i)) f1[i * sy] = v1[e];
+      if (wb.has(i)) f1[i * sy] = cvt<R>(v1[e]);
     }
   }
   if (!slot2) return;  // slot 2 runs as a plain column pass
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = t + T * e;
-    v0[e] = mk<R>((R)0, (R)0);
-    if (x < nx && ib.has(i)) v0[e] = f2[i * sy];
+    v0[e] = mk<C>((C)0, (C)0);
+    if (x < nx && ib.has(i)) v0[e] = cvt<C>(f2[i * sy]);
   }
   __syncthreads();
-  reg_fft_m<L, E, 1, true, false, R>(v0, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, false, C>(v0, lds, IdxCol{xl, TX}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = t + T * e;
-      if (wb.has(i)) f2[i * sy] = v0[e];
+      if (wb.has(i)) f2[i * sy] = cvt<R>(v0[e]);
     }
   }
 }
@@ -1136,7 +1189,7 @@ __global__ void __launch_bounds__(256) k_lastdens(FftDesc d, const cpx<R>* __res
                                                   R scale, const cpx<R>* __restrict__ eh, int64_t ehsy,
                                                   int64_t ehsz, int64_t ehsf, const int* __restrict__ shh,
                                                   int nsh, int n1, double* __restrict__ partials,
-                                                  int64_t ehgs) {
+                                                  int64_t ehgs, int dpriv) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int L = d.L;
   cpx<R>* a = reinterpret_cast<cpx<R>*>(smem_raw);
@@ -1149,7 +1202,8 @@ __global__ void __launch_bounds__(256) k_lastdens(FftDesc d, const cpx<R>* __res
   const int g = bz;
   eh += (int64_t)g * ehgs;  // ensembles: each batch index carries its own carrier
   const int tid = threadIdx.x, nth = blockDim.x;
-  for (int s = tid; s < nsh; s += nth) bins[s] = 0.0;
+  const DetBins db{bins, nsh, dpriv};
+  db.zero();
   R tq[QMAX];  // TX*L <= QMAX*256 per launch: one density accumulator per owned line point
 #pragma unroll
   for (int q = 0; q < QMAX; ++q) tq[q] = (R)0;
@@ -1178,24 +1232,26 @@ __global__ void __launch_bounds__(256) k_lastdens(FftDesc d, const cpx<R>* __res
     }
     __syncthreads();  // a / b are refilled by the next component
   }
+  db.commit([&](double* bb) {
 #pragma unroll
-  for (int q = 0; q < QMAX; ++q) {
-    const int e = tid + q * nth;
-    if (e < L * TX) {
-      const int xl = e % TX, i = e / TX, x = x0 + xl;
-      if (x < nx && wb.has(i)) {
-        const int64_t hi = axis == 1 ? (x + i * ehsy + o * ehsz) : (x + o * ehsy + i * ehsz);
-        const int sh = shh[hi];
-        if (sh > 0) {
-          const double w = (x == 0 || 2 * x == n1) ? 1.0 : 2.0;
-          atomicAdd(&bins[sh - 1], w * (double)tq[q]);
+    for (int q = 0; q < QMAX; ++q) {
+      const int e = tid + q * nth;
+      if (e < L * TX) {
+        const int xl = e % TX, i = e / TX, x = x0 + xl;
+        if (x < nx && wb.has(i)) {
+          const int64_t hi = axis == 1 ? (x + i * ehsy + o * ehsz) : (x + o * ehsy + i * ehsz);
+          const int sh = shh[hi];
+          if (sh > 0) {
+            const double w = (x == 0 || 2 * x == n1) ? 1.0 : 2.0;
+            atomicAdd(&bb[sh - 1], w * (double)tq[q]);
+          }
         }
       }
     }
-  }
+  });
   __syncthreads();
   const int64_t blk = (int64_t)g * gridDim.x * gridDim.y + (int64_t)by * gridDim.x + bx;
-  for (int s = tid; s < nsh; s += nth) partials[blk * nsh + s] = bins[s];
+  for (int s = tid; s < nsh; s += nth) partials[blk * nsh + s] = db.total(s);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1264,17 +1320,19 @@ template <typename R, int L, int MODE>
 __global__ void __launch_bounds__(lastdens_threads<R>()) k_lastdens_r(
     const cpx<R>* __restrict__ src, int64_t sy, int64_t sz, int64_t sf, const cpx<R>* __restrict__ tw, int nx,
     int TX, int lax, Band ob, Band wb, int M, R scale, DensArgs<R> da, int nbins, int nsh,
-    double* __restrict__ partials) {
+    double* __restrict__ partials, int dpriv) {
   constexpr int E = gE(L), T = L / E;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* bins = reinterpret_cast<double*>(smem_raw);
-  cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw + (((size_t)nbins * 8 + 15) / 16) * 16);
+  const DetBins db{bins, nbins, dpriv};
+  cpx<R>* lds = reinterpret_cast<cpx<R>*>(
+      smem_raw + ((detbins_bytes(nbins, (int)blockDim.x, dpriv) + 15) / 16) * 16);
   int bx, by, bz;
   xcd_tile(bx, by, bz);
   const int xl = threadIdx.x % TX, t = threadIdx.x / TX;
   const int x = bx * TX + xl;
   const int gi = bz;  // batch index (kDensPair); 0 for kDensFull
-  for (int s = threadIdx.x; s < nbins; s += blockDim.x) bins[s] = 0.0;
+  db.zero();
   const Grid& g = da.g;
   const cpx<R>* ehb = da.eh + (int64_t)gi * da.ehgs;
   const cpx<R>* sb = da.srcs ? da.srcs[gi] : nullptr;
@@ -1366,43 +1424,74 @@ __global__ void __launch_bounds__(lastdens_threads<R>()) k_lastdens_r(
       }
     }
     // shell bins
+    if constexpr (MODE == kDensPair) {
+      if (!dpriv) {
+        // shared bins: stage (shell, weighted density) of every point in the exchange buffer
+        // (free after the last transform; a point takes at most one complex slot), then wave 0
+        // adds them all in point order -- one barrier instead of one per wave
+        int* ssh = reinterpret_cast<int*>(lds);
+        R* sval = reinterpret_cast<R*>(ssh + L * TX);
+        __syncthreads();  // every lane is done reading the exchange buffer
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int i = t + T * e;
-      if (x >= nx || !wb.has(i)) continue;
-      const int yy = lax == 1 ? i : o, zz = lax == 1 ? o : i;
-      if constexpr (MODE == kDensPair) {
-        const int shv = da.shell[x + (int64_t)n1h * (yy + (int64_t)g.n2 * zz)];
-        if (shv > 0) {
-          const double w = (x == 0 || 2 * x == g.n1) ? 1.0 : 2.0;
-          atomicAdd(&bins[shv - 1], w * (double)tq[e]);
-        }
-      } else {
-        int ox, oy, oz;
-        if (g.pre(x, yy, zz, ox, oy, oz) && !(da.trunc && g.dealiased(ox, oy, oz))) {
-          const int shv = da.shell[g.oidx(ox, oy, oz)];
-          if (shv > 0) {
-            atomicAdd(&bins[shv - 1], (double)ta[e][0]);
-            if (da.ninv > 1) atomicAdd(&bins[nsh + shv - 1], (double)ta[e][1]);
+        for (int e = 0; e < E; ++e) {
+          const int i = t + T * e;
+          const int slot = i * TX + xl;
+          int shv = 0;
+          if (x < nx && wb.has(i)) {
+            const int yy = lax == 1 ? i : o, zz = lax == 1 ? o : i;
+            shv = da.shell[x + (int64_t)n1h * (yy + (int64_t)g.n2 * zz)];
           }
+          ssh[slot] = shv;
+          sval[slot] = (x == 0 || 2 * x == g.n1) ? tq[e] : (R)2 * tq[e];
         }
-        const int mx = x == 0 ? 0 : g.G1 - x;
-        if (mx > g.G1 / 2) {
-          const int my = yy == 0 ? 0 : g.G2 - yy, mz = zz == 0 ? 0 : g.G3 - zz;
-          if (g.pre(mx, my, mz, ox, oy, oz) && !(da.trunc && g.dealiased(ox, oy, oz))) {
+        __syncthreads();
+        if (threadIdx.x < 64)
+          for (int q = (int)threadIdx.x; q < L * TX; q += 64) {
+            const int shv = ssh[q];
+            if (shv > 0) atomicAdd(&bins[shv - 1], (double)sval[q]);
+          }
+        continue;   // the next row's first barrier orders its transform after these reads
+      }
+    }
+    db.commit([&](double* bb) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int i = t + T * e;
+        if (x >= nx || !wb.has(i)) continue;
+        const int yy = lax == 1 ? i : o, zz = lax == 1 ? o : i;
+        if constexpr (MODE == kDensPair) {
+          const int shv = da.shell[x + (int64_t)n1h * (yy + (int64_t)g.n2 * zz)];
+          if (shv > 0) {
+            const double w = (x == 0 || 2 * x == g.n1) ? 1.0 : 2.0;
+            atomicAdd(&bb[shv - 1], w * (double)tq[e]);
+          }
+        } else {
+          int ox, oy, oz;
+          if (g.pre(x, yy, zz, ox, oy, oz) && !(da.trunc && g.dealiased(ox, oy, oz))) {
             const int shv = da.shell[g.oidx(ox, oy, oz)];
             if (shv > 0) {
-              atomicAdd(&bins[shv - 1], (double)ta[e][2]);
-              if (da.ninv > 1) atomicAdd(&bins[nsh + shv - 1], (double)ta[e][3]);
+              atomicAdd(&bb[shv - 1], (double)ta[e][0]);
+              if (da.ninv > 1) atomicAdd(&bb[nsh + shv - 1], (double)ta[e][1]);
+            }
+          }
+          const int mx = x == 0 ? 0 : g.G1 - x;
+          if (mx > g.G1 / 2) {
+            const int my = yy == 0 ? 0 : g.G2 - yy, mz = zz == 0 ? 0 : g.G3 - zz;
+            if (g.pre(mx, my, mz, ox, oy, oz) && !(da.trunc && g.dealiased(ox, oy, oz))) {
+              const int shv = da.shell[g.oidx(ox, oy, oz)];
+              if (shv > 0) {
+                atomicAdd(&bb[shv - 1], (double)ta[e][2]);
+                if (da.ninv > 1) atomicAdd(&bb[nsh + shv - 1], (double)ta[e][3]);
+              }
             }
           }
         }
       }
-    }
+    });
   }
   __syncthreads();
   const int64_t blk = ((int64_t)gi * gridDim.y + by) * gridDim.x + bx;
-  for (int s = threadIdx.x; s < nbins; s += blockDim.x) partials[blk * nbins + s] = bins[s];
+  for (int s = threadIdx.x; s < nbins; s += blockDim.x) partials[blk * nbins + s] = db.total(s);
 }
 
 // Hermitian part of the masked effective carrier on the original half grid (x <= n1/2), only on
@@ -1545,13 +1634,14 @@ __global__ void __launch_bounds__(256) k_density(Grid g, int4 invs, int ninv, in
                                                  int64_t csf, const cpx<R>* __restrict__ nh, int64_t nsf,
                                                  const R* __restrict__ ks0, const R* __restrict__ ks1,
                                                  const R* __restrict__ ks2, const int* __restrict__ shell,
-                                                 int nsh, double* __restrict__ partials, R* __restrict__ t_out) {
+                                                 int nsh, double* __restrict__ partials, R* __restrict__ t_out,
+                                                 int dpriv) {
   // ninv invariants (codes invs.x, .y, .z, .w) from one read of carrier and N̂: bins q*nsh + s;
   // t_out (per-mode output) holds the first invariant's density
   extern __shared__ double dbins[];
-  double* bins = dbins;
   const int nb = ninv * nsh;
-  for (int s = threadIdx.x; s < nb; s += blockDim.x) bins[s] = 0.0;
+  const DetBins db{dbins, nb, dpriv};
+  if (shell) db.zero();
   __syncthreads();
   // one x row per iteration: the row's (y, z) and wavenumbers are found once per row instead of
   // by 64-bit division per mode
@@ -1561,21 +1651,30 @@ __global__ void __launch_bounds__(256) k_density(Grid g, int4 invs, int ninv, in
     const R ky = g.nd >= 2 ? ks1[iy] : (R)0;
     const R kz = g.nd >= 3 ? ks2[iz] : (R)0;
     const int64_t base = (int64_t)row * g.n1;
-    for (int ix = threadIdx.x; ix < g.n1; ix += blockDim.x) {
+    for (int x0 = 0; x0 < g.n1; x0 += blockDim.x) {   // uniform trip count (commit may sync)
+      const int ix = x0 + threadIdx.x;
       const int64_t o = base + ix;
-      const R kx = ks0[ix];
-      const int s = shell ? shell[o] : 0;
-      for (int q = 0; q < ninv; ++q) {
-        const int inv = q == 0 ? invs.x : (q == 1 ? invs.y : (q == 2 ? invs.z : invs.w));
-        const R t = density_at<R>(inv, g.nd, M, car, csf, nh, nsf, o, kx, ky, kz);
-        if (q == 0 && t_out) t_out[o] = t;
-        if (s > 0) atomicAdd(&bins[q * nsh + s - 1], (double)t);
+      R tv[4] = {(R)0, (R)0, (R)0, (R)0};
+      int s = 0;
+      if (ix < g.n1) {
+        const R kx = ks0[ix];
+        s = shell ? shell[o] : 0;
+        for (int q = 0; q < ninv; ++q) {
+          const int inv = q == 0 ? invs.x : (q == 1 ? invs.y : (q == 2 ? invs.z : invs.w));
+          tv[q] = density_at<R>(inv, g.nd, M, car, csf, nh, nsf, o, kx, ky, kz);
+        }
+        if (t_out) t_out[o] = tv[0];
       }
+      if (shell)
+        db.commit([&](double* bb) {
+          if (s > 0)
+            for (int q = 0; q < ninv; ++q) atomicAdd(&bb[q * nsh + s - 1], (double)tv[q]);
+        });
     }
   }
   if (shell) {
     __syncthreads();
-    for (int s = threadIdx.x; s < nb; s += blockDim.x) partials[(int64_t)blockIdx.x * nb + s] = bins[s];
+    for (int s = threadIdx.x; s < nb; s += blockDim.x) partials[(int64_t)blockIdx.x * nb + s] = db.total(s);
   }
 }
 
@@ -1617,31 +1716,38 @@ __global__ void __launch_bounds__(256) k_band_density(Grid g, int inv, int M, co
                                                       const R* __restrict__ ks0, const R* __restrict__ ks1,
                                                       const R* __restrict__ ks2, const R* __restrict__ kc,
                                                       const double* __restrict__ cen, int nb, double inv2s2,
-                                                      double* __restrict__ partials) {
+                                                      double* __restrict__ partials, int dpriv) {
   extern __shared__ double dbins[];
-  for (int s = threadIdx.x; s < nb; s += blockDim.x) dbins[s] = 0.0;
+  const DetBins db{dbins, nb, dpriv};
+  db.zero();
   __syncthreads();
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < g.Np; o += stride) {
-    const double kap = (double)kc[o];
-    if (!(kap > 0.0)) continue;
-    const int ix = (int)(o % g.n1);
-    const int iy = (int)((o / g.n1) % g.n2);
-    const int iz = (int)(o / ((int64_t)g.n1 * g.n2));
-    const R kx = ks0[ix];
-    const R ky = g.nd >= 2 ? ks1[iy] : (R)0;
-    const R kz = g.nd >= 3 ? ks2[iz] : (R)0;
-    const double t = (double)density_at<R>(inv, g.nd, M, car, csf, nh, nsf, o, kx, ky, kz);
-    if (t == 0.0) continue;
-    const double sum = band_sum(kap, cen, nb, inv2s2);
-    if (!(sum > 0.0)) continue;
-    for (int n = 0; n < nb; ++n) {
-      const double r = log(kap / cen[n]);
-      atomicAdd(&dbins[n], exp(-r * r * inv2s2) / sum * t);
+  for (int64_t o0 = (int64_t)blockIdx.x * blockDim.x; o0 < g.Np; o0 += stride) {   // uniform trip count
+    const int64_t o = o0 + threadIdx.x;
+    double t = 0.0, kap = 0.0, sum = 0.0;
+    if (o < g.Np) {
+      kap = (double)kc[o];
+      if (kap > 0.0) {
+        const int ix = (int)(o % g.n1);
+        const int iy = (int)((o / g.n1) % g.n2);
+        const int iz = (int)(o / ((int64_t)g.n1 * g.n2));
+        const R kx = ks0[ix];
+        const R ky = g.nd >= 2 ? ks1[iy] : (R)0;
+        const R kz = g.nd >= 3 ? ks2[iz] : (R)0;
+        t = (double)density_at<R>(inv, g.nd, M, car, csf, nh, nsf, o, kx, ky, kz);
+        if (t != 0.0) sum = band_sum(kap, cen, nb, inv2s2);
+      }
     }
+    db.commit([&](double* bb) {
+      if (t != 0.0 && sum > 0.0)
+        for (int n = 0; n < nb; ++n) {
+          const double r = log(kap / cen[n]);
+          atomicAdd(&bb[n], exp(-r * r * inv2s2) / sum * t);
+        }
+    });
   }
   __syncthreads();
-  for (int s = threadIdx.x; s < nb; s += blockDim.x) partials[(int64_t)blockIdx.x * nb + s] = dbins[s];
+  for (int s = threadIdx.x; s < nb; s += blockDim.x) partials[(int64_t)blockIdx.x * nb + s] = db.total(s);
 }
 
 // out[s] = Σ_b partials[b][s], b in increasing order (deterministic FP64 reduction)

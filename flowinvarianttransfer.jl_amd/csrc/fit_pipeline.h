@@ -20,24 +20,30 @@ int launch_col(fit_plan* p, LD ld, ST st, int axis, int nfields, int nx, Band ob
              (!p->plain && reg_supported(L)) ? "k_col_r" : "k_col", L,
              (std::string(SIGN > 0 ? ",inv" : ",fwd") + KTag<LD>::v + KTag<ST>::v).c_str());
   if (!p->plain && reg_supported(L)) {
-    const int T = reg_T(L);
-    // columns per workgroup: 16 (128-byte rows in FP32) within 64 KB of LDS, or the plan's cap
-    // within the 160 KB a workgroup may hold
-    int TX = std::min(p->col_tx > 0 ? p->col_tx : 16, col_threads<R>() / T);
-    const size_t lds_cap = p->col_tx > 0 ? (size_t)160 * 1024 : g_max_lds;
-    while (TX > 1 && (size_t)L * TX * csz > lds_cap) TX /= 2;
-    dim3 grid((nx + TX - 1) / TX, nob, nfields);
-    const size_t lds = (size_t)L * TX * csz;
-    switch (L) {
-#define FIT_CASE(LL)                                                                                       \
-  case LL:                                                                                                 \
-    hipLaunchKernelGGL((k_col_r<R, LL, SIGN, LD, ST>), grid, dim3(reg_T(LL) * TX), lds, p->stream, ld, st, \
-                       (const cpx<R>*)p->tw[axis], nx, TX, axis, ob, ib, wb, fband, fdiv, gother);           \
+    auto run = [&](auto ctag) {
+      using C = decltype(ctag);
+      const size_t ccs = sizeof(cpx<C>);
+      const int T = L / colE<C>(L);
+      // columns per workgroup: 16 (128-byte rows in FP32) within 64 KB of LDS, or the plan's cap
+      // within the 160 KB a workgroup may hold
+      int TX = std::min(p->col_tx > 0 ? p->col_tx : 16, col_threads<C>() / T);
+      const size_t lds_cap = p->col_tx > 0 ? (size_t)160 * 1024 : g_max_lds;
+      while (TX > 1 && (size_t)L * TX * ccs > lds_cap) TX /= 2;
+      dim3 grid((nx + TX - 1) / TX, nob, nfields);
+      const size_t lds = (size_t)L * TX * ccs;
+      switch (L) {
+#define FIT_CASE(LL)                                                                                          \
+  case LL:                                                                                                    \
+    hipLaunchKernelGGL((k_col_r<R, LL, SIGN, LD, ST, C>), grid, dim3(T * TX), lds, p->stream, ld, st,         \
+                       twc<R, C>(p, axis), nx, TX, axis, ob, ib, wb, fband, fdiv, gother);                    \
     break;
-      FIT_REG_LENGTHS(FIT_CASE)
+        FIT_REG_LENGTHS(FIT_CASE)
 #undef FIT_CASE
-      default: break;
-    }
+        default: break;
+      }
+    };
+    if (std::is_same<R, float>::value && (p->col_f64 & (SIGN > 0 ? 1 : 2))) run(0.0);
+    else run((R)0);
   } else {
     const int TX = tile_for(L, csz, 16);
     dim3 grid((nx + TX - 1) / TX, nob, nfields);
@@ -235,51 +241,63 @@ int grads_shared(fit_plan* p, const void* f, int giver0, int ng, const int* bi, 
     ls.cps = p->M;
     ls.gstride = p->M;
   }
+  const bool c64 = std::is_same<R, float>::value && (p->col_f64 & 1);   // FP64 arithmetic (FIT_OPT_COL_F64)
   {
     const int L = p->G[ax];
-    const int TX = grad_tile(p, L, csz);
     const Band ob = nd == 3 ? Band{bi[1], p->G[1]} : full_band(1);
     const Band ib = Band{bi[ax], L};
     const Band wb = full_band(L);
     const int nob = ob.count();
-    dim3 grid((xa + TX - 1) / TX, nob, ngc);
     prof_begin(p, FIT_K_FFT_COLUMN, (double)ngc * xa * nob * (2.0 * ib.count() + 2.0 * wb.count()) * csz,
                "k_grad1_r", L);
-    switch (L) {
-#define FIT_CASE(LL)                                                                                         \
-  case LL:                                                                                                   \
-    hipLaunchKernelGGL((k_grad1_r<R, LL>), grid, dim3(two_T(LL) * TX), (size_t)L * TX * csz, p->stream, ls,  \
-                       p->M, giver0, st, (const cpx<R>*)p->tw[ax], xa, TX, ax, ob, ib, wb, fband, p->G[1]);  \
+    auto run = [&](auto ctag) {
+      using C = decltype(ctag);
+      const int TX = grad_tile(p, L, sizeof(cpx<C>));
+      dim3 grid((xa + TX - 1) / TX, nob, ngc);
+      switch (L) {
+#define FIT_CASE(LL)                                                                                          \
+  case LL:                                                                                                    \
+    hipLaunchKernelGGL((k_grad1_r<R, LL, C>), grid, dim3(two_T(LL) * TX), (size_t)L * TX * sizeof(cpx<C>),    \
+                       p->stream, ls, p->M, giver0, st, twc<R, C>(p, ax), xa, TX, ax, ob, ib, wb, fband,      \
+                       p->G[1]);                                                                              \
     break;
-      FIT_REG_LENGTHS(FIT_CASE)
+        FIT_REG_LENGTHS(FIT_CASE)
 #undef FIT_CASE
-      default: break;
-    }
+        default: break;
+      }
+    };
+    if (c64) run(0.0);
+    else run((R)0);
     prof_end(p);
     FIT_HIP(hipGetLastError());
   }
   if (nd == 3) {
     const int L = p->G[1];
-    const int TX = grad_tile(p, L, csz);
     const Band ob = full_band(p->G[2]);
     const Band ib = Band{bi[1], L};
     const Band wb = full_band(L);
     const bool split = p->grad_split < 0 ? L < 512 : p->grad_split != 0;
-    dim3 grid((xa + TX - 1) / TX, ob.count(), ngc);
     prof_begin(p, FIT_K_FFT_COLUMN,
                (double)ngc * xa * ob.count() * ((split ? 1.0 : 2.0) * ib.count() + (split ? 2.0 : 3.0) * wb.count()) * csz,
                "k_grad2_r", L);
-    switch (L) {
-#define FIT_CASE(LL)                                                                                         \
-  case LL:                                                                                                   \
-    hipLaunchKernelGGL((k_grad2_r<R, LL>), grid, dim3(two_T(LL) * TX), (size_t)L * TX * csz, p->stream,      \
-                       st.p, st.sy, st.sz, st.sf, (const R*)p->kt[1], (const cpx<R>*)p->tw[1], xa, TX, ob,   \
-                       ib, wb, fband, p->M, split ? 0 : 1);                                                  \
+    auto run = [&](auto ctag) {
+      using C = decltype(ctag);
+      const int TX = grad_tile(p, L, sizeof(cpx<C>));
+      dim3 grid((xa + TX - 1) / TX, ob.count(), ngc);
+      switch (L) {
+#define FIT_CASE(LL)                                                                                          \
+  case LL:                                                                                                    \
+    hipLaunchKernelGGL((k_grad2_r<R, LL, C>), grid, dim3(two_T(LL) * TX), (size_t)L * TX * sizeof(cpx<C>),    \
+                       p->stream, st.p, st.sy, st.sz, st.sf, (const R*)p->kt[1], twc<R, C>(p, 1), xa, TX, ob, \
+                       ib, wb, fband, p->M, split ? 0 : 1);                                                   \
     break;
-      FIT_REG_LENGTHS(FIT_CASE)
+        FIT_REG_LENGTHS(FIT_CASE)
 #undef FIT_CASE
-      default: break;
-    }
+        default: break;
+      }
+    };
+    if (c64) run(0.0);
+    else run((R)0);
     prof_end(p);
     FIT_HIP(hipGetLastError());
     if (split) {
@@ -557,14 +575,15 @@ int density_launch(fit_plan* p, int inv, const void* car, const void* nh, void* 
   Grid g = grid_of<R>(p);
   const int nsh = shells ? p->nsh : 0;
   const int nblk = p->nblk_density;
-  const size_t lds = (size_t)std::max(nsh, 1) * sizeof(double);
+  const int dpriv = detbins_priv(nsh, kThreads, kDensBinBudget);
+  const size_t lds = detbins_bytes(nsh, kThreads, dpriv);
   const size_t csz = sizeof(cpx<R>);
   prof_begin(p, FIT_K_DENSITY_SHELL,
              (double)p->Np * (2.0 * p->M * csz + (shells ? 4.0 : 0.0) + (t_out ? sizeof(R) : 0.0)), "k_density");
   hipLaunchKernelGGL(k_density<R>, dim3(nblk), dim3(kThreads), lds, p->stream, g, int4{inv, 0, 0, 0}, 1, p->M,
                      (const cpx<R>*)car, p->Np, (const cpx<R>*)nh, p->Np, (const R*)p->ks[0],
                      (const R*)p->ks[1], (const R*)p->ks[2], shells ? p->shell : nullptr, nsh,
-                     p->partials, (R*)t_out);
+                     p->partials, (R*)t_out, dpriv);
   prof_end(p);
   FIT_HIP(hipGetLastError());
   if (shells) {
@@ -583,10 +602,11 @@ int density_shells_into(fit_plan* p, int inv, const void* car, const void* nh, d
                         double* out) {
   Grid g = grid_of<R>(p);
   const int nsh = p->nsh;
-  hipLaunchKernelGGL(k_density<R>, dim3(nblk), dim3(kThreads), (size_t)nsh * sizeof(double), p->stream, g,
+  const int dpriv = detbins_priv(nsh, kThreads, kDensBinBudget);
+  hipLaunchKernelGGL(k_density<R>, dim3(nblk), dim3(kThreads), detbins_bytes(nsh, kThreads, dpriv), p->stream, g,
                      int4{inv, 0, 0, 0}, 1, p->M, (const cpx<R>*)car, p->Np, (const cpx<R>*)nh, p->Np,
                      (const R*)p->ks[0], (const R*)p->ks[1], (const R*)p->ks[2], p->shell, nsh, partials,
-                     (R*)nullptr);
+                     (R*)nullptr, dpriv);
   FIT_HIP(hipGetLastError());
   launch_reduce(p, partials, nblk, nsh, 1, out);
   FIT_HIP(hipGetLastError());
@@ -606,9 +626,10 @@ int density_multi(fit_plan* p, int ninv, const int* invs, const void* car, const
   if ((rc = grow(p, (void**)&p->dcolG, &p->dcolG_bytes, (size_t)nb * sizeof(double)))) return rc;
   const int4 iv{invs[0], ninv > 1 ? invs[1] : 0, ninv > 2 ? invs[2] : 0, ninv > 3 ? invs[3] : 0};
   prof_begin(p, FIT_K_DENSITY_SHELL, (double)p->Np * (2.0 * p->M * csz + 4.0), "k_density");
-  hipLaunchKernelGGL(k_density<R>, dim3(nblk), dim3(kThreads), (size_t)nb * sizeof(double), p->stream, g, iv, ninv,
-                     p->M, (const cpx<R>*)car, p->Np, (const cpx<R>*)nh, p->Np, (const R*)p->ks[0],
-                     (const R*)p->ks[1], (const R*)p->ks[2], p->shell, nsh, p->partials2, (R*)nullptr);
+  const int dpriv = detbins_priv(nb, kThreads, kDensBinBudget);
+  hipLaunchKernelGGL(k_density<R>, dim3(nblk), dim3(kThreads), detbins_bytes(nb, kThreads, dpriv), p->stream, g, iv,
+                     ninv, p->M, (const cpx<R>*)car, p->Np, (const cpx<R>*)nh, p->Np, (const R*)p->ks[0],
+                     (const R*)p->ks[1], (const R*)p->ks[2], p->shell, nsh, p->partials2, (R*)nullptr, dpriv);
   prof_end(p);
   FIT_HIP(hipGetLastError());
   prof_begin(p, FIT_K_REDUCE, (double)nblk * nb * 8.0 + nb * 8.0, "k_reduce_rows");
@@ -748,7 +769,8 @@ int lastdens_launch(fit_plan* p, int gb, const int* bo, int mode, const DensArgs
   int rc;
   if ((rc = grow(p, (void**)&p->partials2, &p->partials2_bytes, (size_t)gb * nblk * nbins * sizeof(double))))
     return rc;
-  const size_t lds = ((size_t)nbins * 8 + 15) / 16 * 16 + (size_t)L * TX * csz;
+  const int dpriv = detbins_priv(nbins, T * TX, kFftBinBudget);
+  const size_t lds = (detbins_bytes(nbins, T * TX, dpriv) + 15) / 16 * 16 + (size_t)L * TX * csz;
   const double bytes = (double)gb * p->M * xb * nrow * L * csz +
                        (mode == kDensFull ? (double)p->M * p->Np * csz + 4.0 * p->Np
                                           : (double)gb * p->M * xb * nrow * wb.count() * csz * (da.srcs ? 2.0 : 1.0));
@@ -761,11 +783,11 @@ int lastdens_launch(fit_plan* p, int gb, const int* bo, int mode, const DensArgs
     if (mode == kDensFull)                                                                                       \
       hipLaunchKernelGGL((k_lastdens_r<R, LL, kDensFull>), grid, block, lds, p->stream, (const cpx<R>*)p->nxbuf, sy, \
                          sz, sf, (const cpx<R>*)p->tw[lax], xb, TX, lax, ob, wb, p->M, scale, da, nbins, p->nsh,  \
-                         p->partials2);                                                                          \
+                         p->partials2, dpriv);                                                                   \
     else                                                                                                         \
       hipLaunchKernelGGL((k_lastdens_r<R, LL, kDensPair>), grid, block, lds, p->stream, (const cpx<R>*)p->nxbuf, sy, \
                          sz, sf, (const cpx<R>*)p->tw[lax], xb, TX, lax, ob, wb, p->M, scale, da, nbins, p->nsh,  \
-                         p->partials2);                                                                          \
+                         p->partials2, dpriv);                                                                   \
     break;
     FIT_REG_LENGTHS(FIT_CASE)
 #undef FIT_CASE
@@ -822,11 +844,13 @@ int batched_lastdens_lds(fit_plan* p, int gb, const void* eh, int64_t ehgs, doub
              (double)gb * p->M * xb * gy * L * csz + (double)gb * p->M * xb * gy * wb.count() * csz, "k_lastdens", L);
   // per-thread accumulators: the smallest register array that covers TX*L points per workgroup
   const int qneed = (L * TX + kThreads - 1) / kThreads;
+  const int dpriv = detbins_priv(nsh, kThreads, kFftBinBudget);
+  const size_t lds = (size_t)2 * L * TX * csz + detbins_bytes(nsh, kThreads, dpriv);
 #define FIT_LASTDENS(QQ)                                                                                         \
-  hipLaunchKernelGGL((k_lastdens<R, QQ>), dim3(gx, gy, gb), dim3(kThreads), (size_t)2 * L * TX * csz + (size_t)nsh * 8, \
+  hipLaunchKernelGGL((k_lastdens<R, QQ>), dim3(gx, gy, gb), dim3(kThreads), lds,                                  \
                      p->stream, p->fd[lax], (const cpx<R>*)p->tw[lax], (const cpx<R>*)p->nxbuf, sy, sz, sf, xb, TX, \
                      lax, ob, wb, p->M, (R)scale, (const cpx<R>*)eh, (int64_t)n1h, (int64_t)n1h * p->n[1], Hn,     \
-                     p->shh, nsh, p->n[0], p->partials2, ehgs)
+                     p->shh, nsh, p->n[0], p->partials2, ehgs, dpriv)
   if (qneed <= 4) FIT_LASTDENS(4);
   else if (qneed <= 8) FIT_LASTDENS(8);
   else if (qneed <= 16) FIT_LASTDENS(16);
@@ -1036,9 +1060,10 @@ int b2b_impl(fit_plan* p, int inv, const void* f, const void* a, const R* kc, co
     rc = (p->plain || p->nd == 1) ? nonlinear_from_u<R>(p, p->in_c, 0) : v2_nonlinear<R>(p, p->in_c);
     if (rc) return rc;
     prof_begin(p, FIT_K_DENSITY_SHELL, (double)p->Np * (2.0 * p->M * cs + sizeof(R)), "k_band_density");
-    hipLaunchKernelGGL(k_band_density<R>, dim3(nblk), dim3(kThreads), (size_t)nb * sizeof(double), p->stream, g, inv,
-                       p->M, (const cpx<R>*)f, p->Np, (const cpx<R>*)p->nhat, p->Np, (const R*)p->ks[0],
-                       (const R*)p->ks[1], (const R*)p->ks[2], kc, cen, nb, inv2s2, p->partials2);
+    const int dpriv = detbins_priv(nb, kThreads, kDensBinBudget);
+    hipLaunchKernelGGL(k_band_density<R>, dim3(nblk), dim3(kThreads), detbins_bytes(nb, kThreads, dpriv), p->stream,
+                       g, inv, p->M, (const cpx<R>*)f, p->Np, (const cpx<R>*)p->nhat, p->Np, (const R*)p->ks[0],
+                       (const R*)p->ks[1], (const R*)p->ks[2], kc, cen, nb, inv2s2, p->partials2, dpriv);
     prof_end(p);
     FIT_HIP(hipGetLastError());
     prof_begin(p, FIT_K_REDUCE, (double)nblk * nb * 8.0, "k_reduce_rows");

@@ -99,10 +99,20 @@ int64_t fit_plan_live_count(void);
  *     pass (measured faster on the 512³ flux with 3/2 padding).
  *   FIT_OPT_RADIX_ORDER = 11: radix order of the LDS-engine FFT passes (lengths without a
  *     register FFT): 0 (default) radix 8 first, then 4, 2, 3, 5; 1 radix 4 first; 2 the odd
- *     radices first. */
+ *     radices first.
+ *   FIT_OPT_COL_F64 = 12: FP32 plans only; which strided-axis register-FFT passes (y and z
+ *     transforms) run their arithmetic in FP64, with FP64 twiddles and LDS exchanges, loading and
+ *     storing the fields in FP32, so that each value is rounded to FP32 once per pass instead of at
+ *     every butterfly stage: bit 0 the inverse passes (gradient passes included), bit 1 the
+ *     forward passes.  2 (default): the forward passes, which halves the FP32 error of Π(K) at
+ *     512³ with 3/2 padding (rms 6.3e-7 -> 3.1e-7 over eight seeded fields, worst 1.3e-6 ->
+ *     3.9e-7) for 2.8 ms more per flux; 3 also the inverse passes (rms 2.5e-7, 6.6 ms more);
+ *     0 all in FP32.
+ */
 enum { FIT_OPT_PLAIN_PIPELINE = 1, FIT_OPT_GIVER_BATCH = 2, FIT_OPT_XPASS_GIVERS = 3,
        FIT_OPT_XPASS_BLOCK = 4, FIT_OPT_SUBGRID = 5, FIT_OPT_COL_TILE = 6, FIT_OPT_XPASS_SELF = 7,
-       FIT_OPT_XPASS_STAGE = 8, FIT_OPT_GRAD_SPLIT = 9, FIT_OPT_FUSED_FLUX = 10, FIT_OPT_RADIX_ORDER = 11 };
+       FIT_OPT_XPASS_STAGE = 8, FIT_OPT_GRAD_SPLIT = 9, FIT_OPT_FUSED_FLUX = 10, FIT_OPT_RADIX_ORDER = 11,
+       FIT_OPT_COL_F64 = 12 };
 int fit_plan_set_option(fit_plan* plan, int option, int value);
 int fit_plan_get_option(const fit_plan* plan, int option, int* value);
 

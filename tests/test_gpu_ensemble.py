@@ -70,11 +70,12 @@ def test_ensemble_2d_uneven_matches_oracle(fit, oracle, prec):
     us2, ths2, _ = make_ensemble(oracle, SHAPES_2D, 5000)
     res2 = ens.fluxes([to_dev(u, dt) for u in us2], [to_dev(t, dt) for t in ths2])
     check(fit, oracle, res2, us2, ths2, kss, tol)
-    # and once more on the first arrays: a replay reproduces the eager run (up to the order of
-    # the FP64 shell-bin additions inside a workgroup)
+    # and once more on the first arrays: a replay reproduces the eager run bit for bit (the shell
+    # bins add in a fixed order inside and across workgroups)
     res3 = ens.fluxes(ud, td)
     for (a, b), (c, d) in zip(res, res3):
-        assert nerr(c.flux, a.flux) < 1e-12 and nerr(d.flux, b.flux) < 1e-12
+        assert np.array_equal(c.flux, a.flux) and np.array_equal(d.flux, b.flux)
+        assert np.array_equal(c.transfer_spectrum, a.transfer_spectrum)
     ens.close()
 
 

@@ -146,3 +146,31 @@ def test_gradient_pass_split(fit, oracle, ns, dea, split, prec):
     e = rel(got, ref)
     say("  err", e)
     assert e < (1e-11 if prec == "f64" else 2e-5)
+
+
+COLF64_SHAPES = [((6, 768), "23"), ((4, 4, 768), "none"), ((4, 512, 4), "23"), ((4, 192, 128), "none"),
+                 ((8, 6, 8), "pad"), ((4, 6, 256), "pad"), ((2, 3072), "none")]
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("ns,dea", COLF64_SHAPES)
+def test_col_f64_arithmetic(fit, oracle, ns, dea, mode):
+    """FP32 plans with the strided register passes computing in FP64 (FIT_OPT_COL_F64: inverse,
+    forward, both, neither) against the oracle, at FP32 tolerance, and bitwise equal on a rerun."""
+    from fit_amd import _lib
+    say("case col_f64", ns, dea, mode)
+    rng = np.random.default_rng(sum(ns) + mode)
+    nd = len(ns)
+    ks = oracle.wavenumber_grid(ns, (2 * math.pi,) * nd)
+    u = rng.standard_normal(ns + (nd,)) + 1j * rng.standard_normal(ns + (nd,))
+    ref = oracle.compute_nonlinear_term(u, ks, dea)
+    deal = {"none": fit.NoDealiasing(), "23": fit.OrszagTwoThirds(), "pad": fit.PaddedThreeHalves()}[dea]
+    with fit.Plan(ns, nd, nd, ks, deal, np.float32) as plan:
+        plan.set_option(_lib.OPT_COL_F64, mode)
+        assert plan.get_option(_lib.OPT_COL_F64) == mode
+        got = plan.nonlinear_term(u)
+        again = plan.nonlinear_term(u)
+    e = rel(got, ref)
+    say("  err", e)
+    assert e < 2e-5
+    assert np.array_equal(got, again)
