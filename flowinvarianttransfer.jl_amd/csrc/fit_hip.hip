@@ -503,7 +503,9 @@ static int create_plan(fit_plan** out, int nd, const int64_t* ns, int M, int D, 
       delete p;
       return fail(FIT_E_UNSUPPORTED, "transform length " + std::to_string(bad) + " is not of the form 2^a 3^b 5^c");
     }
-    if ((size_t)2 * p->G[d] * (compute_dtype == FIT_F64 ? 16 : 8) > 65536) {
+    // the LDS-engine passes hold two lines per workgroup (up to the 160 KB of LDS a gfx950
+    // workgroup may take: FP64 lines up to 5120 points, FP32 up to 10240)
+    if ((size_t)2 * p->G[d] * (compute_dtype == FIT_F64 ? 16 : 8) > (size_t)160 * 1024) {
       int bad = p->G[d];
       delete p;
       return fail(FIT_E_UNSUPPORTED, "transform length " + std::to_string(bad) + " exceeds the LDS line limit");
@@ -879,6 +881,10 @@ int fit_plan_set_option(fit_plan* p, int option, int value) {
       if (value < 0 || value > 3) return fail(FIT_E_ARGUMENT, "FIT_OPT_COL_F64 takes 0 to 3");
       p->col_f64 = value;
       return FIT_OK;
+    case FIT_OPT_COL_PIPE:
+      if (value < 0 || value > 64) return fail(FIT_E_ARGUMENT, "FIT_OPT_COL_PIPE takes 0 to 64");
+      p->col_pipe = value;
+      return FIT_OK;
     default: return fail(FIT_E_ARGUMENT, "unknown plan option");
   }
 }
@@ -898,6 +904,7 @@ int fit_plan_get_option(const fit_plan* p, int option, int* value) {
     case FIT_OPT_FUSED_FLUX: *value = p->fused_flux; return FIT_OK;
     case FIT_OPT_RADIX_ORDER: *value = p->radix_order; return FIT_OK;
     case FIT_OPT_COL_F64: *value = p->col_f64; return FIT_OK;
+    case FIT_OPT_COL_PIPE: *value = p->col_pipe; return FIT_OK;
     default: return fail(FIT_E_ARGUMENT, "unknown plan option");
   }
 }

@@ -555,6 +555,78 @@ __global__ void __launch_bounds__(col_threads<C>()) k_col_r(LD ld, ST st, const 
   }
 }
 
+// Software-pipelined form of k_col_r: each workgroup walks the tiles ty = blockIdx.y,
+// + gridDim.y, ... (< nob) of its columns and issues the loads of the next tile before it
+// transforms the current one, so they are in flight during the transform's LDS exchanges (the
+// exchange barriers wait on LDS traffic only).  The prefetched line is held in the storage
+// precision R; 12 points per lane at 3·2^a lengths keep both lines in registers.
+template <typename R, int L, int SIGN, class LD, class ST, typename C = R>
+__global__ void __launch_bounds__(512) k_col_p(LD ld, ST st, const cpx<C>* __restrict__ tw, int nx, int TX,
+                                               int axis, Band ob, Band ib, Band wb,
+                                               const int4* __restrict__ fband, int fdiv, int Gother, int nob) {
+  constexpr int E = gE(L), T = L / E;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<C>* lds = reinterpret_cast<cpx<C>*>(smem_raw);
+  int bx, by, bz;
+  xcd_tile(bx, by, bz);
+  const int xl = threadIdx.x % TX, t = threadIdx.x / TX;
+  const int x0 = bx * TX;
+  const int f = bz;
+  bool ochk = false;         // axis 2 with per-field bounds: tiles outside the field's y band skip
+  Band obf = ob;
+  if (fband) {
+    const int4 fb = fband[f / fdiv];
+    if (x0 > fb.x) return;
+    if (axis == 2) {
+      ochk = true;
+      obf = Band{fb.y, Gother};
+      ib = Band{fb.z, L};
+    } else {
+      ib = Band{fb.y, L};
+    }
+  }
+  const int x = x0 + xl;
+  const int ys = (int)gridDim.y;
+  auto next = [&](int ty) {
+    while (ty < nob && ochk && !obf.has(ob.at(ty))) ty += ys;
+    return ty;
+  };
+  cpx<R> vn[E];
+  auto load = [&](int ty) {
+    const int o = ob.at(ty);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = t + T * e;
+      vn[e] = mk<R>((R)0, (R)0);
+      if (x < nx && ib.has(i)) vn[e] = axis == 1 ? ld(x, i, o, f) : ld(x, o, i, f);
+    }
+  };
+  int ty = next(by);
+  if (ty >= nob) return;
+  load(ty);
+  while (true) {
+    cpx<C> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = cvt<C>(vn[e]);
+    const int tn = next(ty + ys);
+    if (tn < nob) load(tn);
+    reg_fft_e<L, E, SIGN, false, C>(v, lds, lds, IdxCol{xl, TX}, t, tw);
+    const int o = ob.at(ty);
+    if (x < nx) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int i = t + T * e;
+        if (wb.has(i)) {
+          if (axis == 1) st(x, i, o, f, cvt<R>(v[e]));
+          else st(x, o, i, f, cvt<R>(v[e]));
+        }
+      }
+    }
+    if (tn >= nob) break;
+    ty = tn;
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Gradient inverse passes with shared transforms.  The Hermitian part of i k_d·f̂ equals
 // i k_t(d)·h, h the Hermitian part of f̂, whenever the wavenumbers are odd under k -> -k on every
@@ -707,6 +779,80 @@ __global__ void __launch_bounds__(512) k_grad2_r(cpx<R>* __restrict__ buf, int64
       const int i = t + T * e;
       if (wb.has(i)) f2[i * sy] = cvt<R>(v0[e]);
     }
+  }
+}
+
+// Software-pipelined form of k_grad2_r (see k_col_p): per tile, the slot-2 line and the next
+// tile's slot-0 line are loaded before the transforms that precede their use.
+template <typename R, int L, typename C = R>
+__global__ void __launch_bounds__(512) k_grad2_p(cpx<R>* __restrict__ buf, int64_t sy, int64_t sz, int64_t sf,
+                                                  const R* __restrict__ kyt, const cpx<C>* __restrict__ tw,
+                                                  int nx, int TX, Band ob, Band ib, Band wb,
+                                                  const int4* __restrict__ fband, int M, int slot2, int nob) {
+  constexpr int E = gE(L), T = L / E;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<C>* lds = reinterpret_cast<cpx<C>*>(smem_raw);
+  int bx, by, bz;
+  xcd_tile(bx, by, bz);
+  const int xl = threadIdx.x % TX, t = threadIdx.x / TX;
+  const int x0 = bx * TX;
+  const int gc = bz;
+  if (fband) {
+    const int4 fb = fband[gc / M];
+    if (x0 > fb.x) return;
+    ib = Band{fb.y, L};
+  }
+  const int x = x0 + xl;
+  const int ys = (int)gridDim.y;
+  cpx<R>* fb0 = buf + (int64_t)(gc * 3) * sf + x;
+  cpx<R> s0[E], s2[E];
+  auto load = [&](cpx<R> (&d)[E], int slot, int ty) {
+    const cpx<R>* src = fb0 + slot * sf + ob.at(ty) * sz;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = t + T * e;
+      d[e] = mk<R>((R)0, (R)0);
+      if (x < nx && ib.has(i)) d[e] = src[i * sy];
+    }
+  };
+  auto store = [&](const cpx<C> (&v)[E], int slot, int ty) {
+    cpx<R>* dst = fb0 + slot * sf + ob.at(ty) * sz;
+    if (x < nx) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int i = t + T * e;
+        if (wb.has(i)) dst[i * sy] = cvt<R>(v[e]);
+      }
+    }
+  };
+  int ty = by;
+  if (ty >= nob) return;
+  load(s0, 0, ty);
+  if (slot2) load(s2, 2, ty);
+  while (true) {
+    const int tn = ty + ys;
+    cpx<C> v0[E], v1[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = t + T * e;
+      v0[e] = cvt<C>(s0[e]);
+      const C k = (C)kyt[i];
+      v1[e] = mk<C>(-k * v0[e].y, k * v0[e].x);
+    }
+    if (tn < nob) load(s0, 0, tn);
+    reg_fft_m<L, E, 1, true, false, C>(v0, lds, IdxCol{xl, TX}, t, tw);
+    store(v0, 0, ty);
+    reg_fft_m<L, E, 1, true, false, C>(v1, lds, IdxCol{xl, TX}, t, tw);
+    store(v1, 1, ty);
+    if (slot2) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v0[e] = cvt<C>(s2[e]);
+      if (tn < nob) load(s2, 2, tn);
+      reg_fft_m<L, E, 1, true, false, C>(v0, lds, IdxCol{xl, TX}, t, tw);
+      store(v0, 2, ty);
+    }
+    if (tn >= nob) break;
+    ty = tn;
   }
 }
 

@@ -19,7 +19,32 @@ int launch_col(fit_plan* p, LD ld, ST st, int axis, int nfields, int nx, Band ob
   prof_begin(p, FIT_K_FFT_COLUMN, (double)nx * nob * (rows_in + rows_out) * csz * nfields,
              (!p->plain && reg_supported(L)) ? "k_col_r" : "k_col", L,
              (std::string(SIGN > 0 ? ",inv" : ",fwd") + KTag<LD>::v + KTag<ST>::v).c_str());
-  if (!p->plain && reg_supported(L)) {
+  if (!p->plain && reg_supported(L) && p->col_pipe > 0) {
+    // software-pipelined form: col_pipe tiles of the other index per workgroup
+    auto run = [&](auto ctag) {
+      using C = decltype(ctag);
+      const size_t ccs = sizeof(cpx<C>);
+      const int T = L / gE(L);
+      int TX = std::min(p->col_tx > 0 ? p->col_tx : 16, 512 / T);
+      const size_t lds_cap = p->col_tx > 0 ? (size_t)160 * 1024 : g_max_lds;
+      while (TX > 1 && (size_t)L * TX * ccs > lds_cap) TX /= 2;
+      const int gy = std::max(1, (nob + p->col_pipe - 1) / p->col_pipe);
+      dim3 grid((nx + TX - 1) / TX, gy, nfields);
+      const size_t lds = (size_t)L * TX * ccs;
+      switch (L) {
+#define FIT_CASE(LL)                                                                                          \
+  case LL:                                                                                                    \
+    hipLaunchKernelGGL((k_col_p<R, LL, SIGN, LD, ST, C>), grid, dim3(T * TX), lds, p->stream, ld, st,         \
+                       twc<R, C>(p, axis), nx, TX, axis, ob, ib, wb, fband, fdiv, gother, nob);               \
+    break;
+        FIT_REG_LENGTHS(FIT_CASE)
+#undef FIT_CASE
+        default: break;
+      }
+    };
+    if (std::is_same<R, float>::value && (p->col_f64 & (SIGN > 0 ? 1 : 2))) run(0.0);
+    else run((R)0);
+  } else if (!p->plain && reg_supported(L)) {
     auto run = [&](auto ctag) {
       using C = decltype(ctag);
       const size_t ccs = sizeof(cpx<C>);
@@ -283,13 +308,20 @@ int grads_shared(fit_plan* p, const void* f, int giver0, int ng, const int* bi, 
     auto run = [&](auto ctag) {
       using C = decltype(ctag);
       const int TX = grad_tile(p, L, sizeof(cpx<C>));
-      dim3 grid((xa + TX - 1) / TX, ob.count(), ngc);
+      const int nob = ob.count();
+      const int gy = p->col_pipe > 0 ? std::max(1, (nob + p->col_pipe - 1) / p->col_pipe) : nob;
+      dim3 grid((xa + TX - 1) / TX, gy, ngc);
       switch (L) {
 #define FIT_CASE(LL)                                                                                          \
   case LL:                                                                                                    \
-    hipLaunchKernelGGL((k_grad2_r<R, LL, C>), grid, dim3(two_T(LL) * TX), (size_t)L * TX * sizeof(cpx<C>),    \
-                       p->stream, st.p, st.sy, st.sz, st.sf, (const R*)p->kt[1], twc<R, C>(p, 1), xa, TX, ob, \
-                       ib, wb, fband, p->M, split ? 0 : 1);                                                   \
+    if (p->col_pipe > 0)                                                                                      \
+      hipLaunchKernelGGL((k_grad2_p<R, LL, C>), grid, dim3(two_T(LL) * TX), (size_t)L * TX * sizeof(cpx<C>),  \
+                         p->stream, st.p, st.sy, st.sz, st.sf, (const R*)p->kt[1], twc<R, C>(p, 1), xa, TX,   \
+                         ob, ib, wb, fband, p->M, split ? 0 : 1, nob);                                        \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_grad2_r<R, LL, C>), grid, dim3(two_T(LL) * TX), (size_t)L * TX * sizeof(cpx<C>),  \
+                         p->stream, st.p, st.sy, st.sz, st.sf, (const R*)p->kt[1], twc<R, C>(p, 1), xa, TX,   \
+                         ob, ib, wb, fband, p->M, split ? 0 : 1);                                             \
     break;
         FIT_REG_LENGTHS(FIT_CASE)
 #undef FIT_CASE

@@ -108,11 +108,14 @@ int64_t fit_plan_live_count(void);
  *     512³ with 3/2 padding (rms 6.3e-7 -> 3.1e-7 over eight seeded fields, worst 1.3e-6 ->
  *     3.9e-7) for 2.8 ms more per flux; 3 also the inverse passes (rms 2.5e-7, 6.6 ms more);
  *     0 all in FP32.
+ *   FIT_OPT_COL_PIPE = 13: 0 (default) one tile of the other index per workgroup in the strided
+ *     register-FFT passes; n > 0 the software-pipelined form, each workgroup walking n tiles and
+ *     loading the next tile's lines while it transforms the current one.
  */
 enum { FIT_OPT_PLAIN_PIPELINE = 1, FIT_OPT_GIVER_BATCH = 2, FIT_OPT_XPASS_GIVERS = 3,
        FIT_OPT_XPASS_BLOCK = 4, FIT_OPT_SUBGRID = 5, FIT_OPT_COL_TILE = 6, FIT_OPT_XPASS_SELF = 7,
        FIT_OPT_XPASS_STAGE = 8, FIT_OPT_GRAD_SPLIT = 9, FIT_OPT_FUSED_FLUX = 10, FIT_OPT_RADIX_ORDER = 11,
-       FIT_OPT_COL_F64 = 12 };
+       FIT_OPT_COL_F64 = 12, FIT_OPT_COL_PIPE = 13 };
 int fit_plan_set_option(fit_plan* plan, int option, int value);
 int fit_plan_get_option(const fit_plan* plan, int option, int* value);
 

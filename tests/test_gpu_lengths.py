@@ -57,9 +57,6 @@ SELF_CASES = [((128, 192), "23"), ((192, 128), "23"), ((256, 384), "23"), ((384,
 def test_self_advection_register_lengths(fit, oracle, ns, dea, xself, prec):
     from fit_amd import _lib
     say("case self-advection lengths", ns, dea, xself, prec)
-    G = [n * 3 // 2 if dea == "pad" else n for n in ns]
-    if prec == "f64" and max(G) > 2048:
-        pytest.skip("FP64 transforms longer than 2048 exceed the plan's 64 KB LDS line limit")
     rng = np.random.default_rng(sum(ns) + 7)
     nd = len(ns)
     ks = oracle.wavenumber_grid(ns, (2 * math.pi,) * nd)
