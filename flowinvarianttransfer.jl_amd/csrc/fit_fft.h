@@ -204,7 +204,10 @@ constexpr int rp_rad(int L, int E, int s) {
 constexpr bool rp_pow2(int L) { return L > 0 && (L & (L - 1)) == 0; }
 constexpr int rp_E(int L) { return rp_pow2(L) ? 8 : 24; }
 // points per thread for kernels that keep two lines in registers: radix-4 stages for 3·2^a
-constexpr int gE(int L) { return rp_pow2(L) ? 8 : 12; }
+// (16 at power-of-two lengths from 1024: a column then spans L/16 lanes, so a 512-thread
+// workgroup holds four 2048-point or eight 1024-point columns and reads 32- or 64-byte row
+// segments instead of 16 or 32)
+constexpr int gE(int L) { return rp_pow2(L) ? (L >= 1024 ? 16 : 8) : 12; }
 // points per thread of the workgroup-form fused x pass (radix-4 stages for 3·2^a: at 768 they
 // halve the registers of the radix-8 form and run 10% faster)
 constexpr int xE(int L) { return rp_pow2(L) ? 8 : 12; }

@@ -878,7 +878,7 @@ int fit_plan_set_option(fit_plan* p, int option, int value) {
       p->radix_order = value;
       return FIT_OK;
     case FIT_OPT_COL_F64:
-      if (value < 0 || value > 3) return fail(FIT_E_ARGUMENT, "FIT_OPT_COL_F64 takes 0 to 3");
+      if (value < 0 || value > 7) return fail(FIT_E_ARGUMENT, "FIT_OPT_COL_F64 takes 0 to 7");
       p->col_f64 = value;
       return FIT_OK;
     case FIT_OPT_COL_PIPE:

@@ -139,8 +139,8 @@ struct fit_plan {
   hipEvent_t order_ev = nullptr;
   int radix_order = 0;            // FIT_OPT_RADIX_ORDER of the LDS-engine passes
   int col_pipe = 0;               // FIT_OPT_COL_PIPE: tiles per workgroup of the pipelined column passes (0: off)
-  int col_f64 = 2;                // FIT_OPT_COL_F64: FP32 plans run the strided register passes in FP64
-                                  // (bit 0: inverse passes, bit 1: forward passes)
+  int col_f64 = 6;                // FIT_OPT_COL_F64: FP32 plans run the strided register passes in FP64
+                                  // (bit 0: inverse passes, bit 1: forward y, bit 2: forward z)
   void* tw64[3] = {nullptr, nullptr, nullptr};  // FP64 twiddles of an FP32 plan (FIT_OPT_COL_F64)
   std::string tune_report;        // the last fit_plan_autotune: workload, times, choices
   // profiling

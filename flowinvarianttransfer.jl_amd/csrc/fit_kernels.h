@@ -510,7 +510,7 @@ __global__ void __launch_bounds__(256) k_xfused(FftDesc d, const cpx<R>* __restr
 // FP64 arithmetic holds 12 points per lane at 3·2^a lengths (24 in FP32): 64 lanes per column
 // keep four columns (256 threads) in the 48 KB of LDS that three workgroups per CU share.
 template <typename R> constexpr int col_threads() { return sizeof(R) == 8 ? 256 : 1024; }
-template <typename C> constexpr int colE(int L) { return sizeof(C) == 8 ? gE(L) : rp_E(L); }
+template <typename C> constexpr int colE(int L) { return (sizeof(C) == 8 || L >= 1024) ? gE(L) : rp_E(L); }
 template <typename R, int L, int SIGN, class LD, class ST, typename C = R>
 __global__ void __launch_bounds__(col_threads<C>()) k_col_r(LD ld, ST st, const cpx<C>* __restrict__ tw, int nx, int TX,
                                                 int axis, Band ob, Band ib, Band wb,

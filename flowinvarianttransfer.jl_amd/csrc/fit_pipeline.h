@@ -42,7 +42,7 @@ int launch_col(fit_plan* p, LD ld, ST st, int axis, int nfields, int nx, Band ob
         default: break;
       }
     };
-    if (std::is_same<R, float>::value && (p->col_f64 & (SIGN > 0 ? 1 : 2))) run(0.0);
+    if (std::is_same<R, float>::value && (p->col_f64 & (SIGN > 0 ? 1 : (axis == 1 ? 2 : 4)))) run(0.0);
     else run((R)0);
   } else if (!p->plain && reg_supported(L)) {
     auto run = [&](auto ctag) {
@@ -67,7 +67,7 @@ int launch_col(fit_plan* p, LD ld, ST st, int axis, int nfields, int nx, Band ob
         default: break;
       }
     };
-    if (std::is_same<R, float>::value && (p->col_f64 & (SIGN > 0 ? 1 : 2))) run(0.0);
+    if (std::is_same<R, float>::value && (p->col_f64 & (SIGN > 0 ? 1 : (axis == 1 ? 2 : 4)))) run(0.0);
     else run((R)0);
   } else {
     const int TX = tile_for(L, csz, 16);

@@ -103,11 +103,11 @@ int64_t fit_plan_live_count(void);
  *   FIT_OPT_COL_F64 = 12: FP32 plans only; which strided-axis register-FFT passes (y and z
  *     transforms) run their arithmetic in FP64, with FP64 twiddles and LDS exchanges, loading and
  *     storing the fields in FP32, so that each value is rounded to FP32 once per pass instead of at
- *     every butterfly stage: bit 0 the inverse passes (gradient passes included), bit 1 the
- *     forward passes.  2 (default): the forward passes, which halves the FP32 error of Π(K) at
- *     512³ with 3/2 padding (rms 6.3e-7 -> 3.1e-7 over eight seeded fields, worst 1.3e-6 ->
- *     3.9e-7) for 2.8 ms more per flux; 3 also the inverse passes (rms 2.5e-7, 6.6 ms more);
- *     0 all in FP32.
+ *     every butterfly stage: bit 0 the inverse passes (gradient passes included), bit 1 the forward
+ *     y pass, bit 2 the forward z pass.  6 (default): both forward passes, which halves the FP32
+ *     error of Π(K) at 512³ with 3/2 padding (eight seeded fields: rms 6.6e-7 -> 3.4e-7, worst
+ *     1.28e-6 -> 5.3e-7) for 2.5 ms more per flux; 4 or 2 alone: rms 4.2e-7, worst 7.2e-7 /
+ *     7.3e-7 for 1.0 / 1.5 ms; 7 also the inverse passes (rms 2.5e-7, 6.6 ms more); 0 all FP32.
  *   FIT_OPT_COL_PIPE = 13: 0 (default) one tile of the other index per workgroup in the strided
  *     register-FFT passes; n > 0 the software-pipelined form, each workgroup walking n tiles and
  *     loading the next tile's lines while it transforms the current one.

@@ -149,11 +149,12 @@ COLF64_SHAPES = [((6, 768), "23"), ((4, 4, 768), "none"), ((4, 512, 4), "23"), (
                  ((8, 6, 8), "pad"), ((4, 6, 256), "pad"), ((2, 3072), "none")]
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 4, 6, 7])
 @pytest.mark.parametrize("ns,dea", COLF64_SHAPES)
 def test_col_f64_arithmetic(fit, oracle, ns, dea, mode):
     """FP32 plans with the strided register passes computing in FP64 (FIT_OPT_COL_F64: inverse,
-    forward, both, neither) against the oracle, at FP32 tolerance, and bitwise equal on a rerun."""
+    forward y, forward z and their combinations, or none) against the oracle, at FP32 tolerance,
+    and bitwise equal on a rerun."""
     from fit_amd import _lib
     say("case col_f64", ns, dea, mode)
     rng = np.random.default_rng(sum(ns) + mode)
