@@ -208,6 +208,9 @@ constexpr int rp_E(int L) { return rp_pow2(L) ? 8 : 24; }
 // workgroup holds four 2048-point or eight 1024-point columns and reads 32- or 64-byte row
 // segments instead of 16 or 32)
 constexpr int gE(int L) { return rp_pow2(L) ? (L >= 1024 ? 16 : 8) : 12; }
+// points per thread of the fused last pass + density, which also holds E density accumulators:
+// at 16 points per lane it spills under its 512-thread bound, so it keeps 8
+constexpr int ldE(int L) { return rp_pow2(L) ? 8 : 12; }
 // points per thread of the workgroup-form fused x pass (radix-4 stages for 3·2^a: at 768 they
 // halve the registers of the radix-8 form and run 10% faster)
 constexpr int xE(int L) { return rp_pow2(L) ? 8 : 12; }

@@ -1467,7 +1467,7 @@ __global__ void __launch_bounds__(lastdens_threads<R>()) k_lastdens_r(
     const cpx<R>* __restrict__ src, int64_t sy, int64_t sz, int64_t sf, const cpx<R>* __restrict__ tw, int nx,
     int TX, int lax, Band ob, Band wb, int M, R scale, DensArgs<R> da, int nbins, int nsh,
     double* __restrict__ partials, int dpriv) {
-  constexpr int E = gE(L), T = L / E;
+  constexpr int E = ldE(L), T = L / E;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* bins = reinterpret_cast<double*>(smem_raw);
   const DetBins db{bins, nbins, dpriv};

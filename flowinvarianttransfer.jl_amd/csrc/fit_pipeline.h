@@ -788,7 +788,7 @@ int lastdens_launch(fit_plan* p, int gb, const int* bo, int mode, const DensArgs
   const int64_t sy = pb, sz = (int64_t)pb * p->G[1], sf = sz * p->G[2];
   const Band ob = p->nd == 3 ? Band{bo[1], p->G[1]} : full_band(1);
   const Band wb = p->nd == 3 ? Band{bo[2], p->G[2]} : Band{bo[1], p->G[1]};
-  const int T = L / gE(L);
+  const int T = L / ldE(L);
   const int nthr = std::is_same<R, double>::value ? lastdens_threads<double>() : lastdens_threads<float>();
   // columns per workgroup: 16 (128-byte rows in FP32) within the thread bound and 64 KB of LDS
   int TX = std::max(1, std::min(16, nthr / T));
