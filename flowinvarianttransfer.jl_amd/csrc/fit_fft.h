@@ -407,6 +407,17 @@ struct IdxLine {
     else return at<C>(o + k * NS);
   }
 };
+// column tiles with a compile-time tile width: every LDS offset of an exchange is the lane's
+// base plus an immediate
+template <int TX>
+struct IdxColC {
+  int x;
+  template <int C>
+  __device__ __forceinline__ int at(int pos) const { return pos * TX + x; }
+  __device__ __forceinline__ int operator()(int pos) const { return pos * TX + x; }
+  template <int NS, int r, int C>
+  __device__ __forceinline__ int step(int io, int o, int k) const { return io + k * NS * TX; }
+};
 // column tiles: point-major, TX lines interleaved (x fastest in both HBM and LDS)
 struct IdxCol {
   int x, TX;

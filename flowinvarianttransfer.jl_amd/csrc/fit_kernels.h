@@ -641,12 +641,22 @@ __global__ void __launch_bounds__(512) k_col_p(LD ld, ST st, const cpx<C>* __res
 // instead of nd loads of the source.  Fields are (g, c, j) = (g M + c) nd + j as before.
 // Two lines in flight per thread: at most 512 threads per workgroup keeps them in registers.
 // ---------------------------------------------------------------------------------------------
+// Columns per workgroup of the two-line gradient passes, a compile-time function of the length and
+// the arithmetic precision (16 within 512 threads and 64 KB of LDS), so that every LDS offset of
+// their exchanges is an immediate.
+template <int L, typename C> constexpr int grad_tx() {
+  int tx = 512 / (L / gE(L));
+  tx = tx > 16 ? 16 : tx;
+  while (tx > 1 && (long)L * tx * 2 * (long)sizeof(C) > 65536) tx /= 2;
+  return tx;
+}
+
 template <typename R, int L, typename C = R>
 __global__ void __launch_bounds__(512) k_grad1_r(LdSpectral<R> ls, int M, int giver0, StHalf<R> st,
-                                                  const cpx<C>* __restrict__ tw, int nx, int TX, int axis,
+                                                  const cpx<C>* __restrict__ tw, int nx, int TXr, int axis,
                                                   Band ob, Band ib, Band wb, const int4* __restrict__ fband,
                                                   int Gother) {
-  constexpr int E = gE(L), T = L / E;
+  constexpr int E = gE(L), T = L / E, TX = grad_tx<L, C>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<C>* lds = reinterpret_cast<cpx<C>*>(smem_raw);
   int bx, by, bz;
@@ -689,7 +699,7 @@ __global__ void __launch_bounds__(512) k_grad1_r(LdSpectral<R> ls, int M, int gi
     }
   }
   const int f0 = gc * nd, fa = gc * nd + nd - 1;
-  reg_fft_m<L, E, 1, true, false, C>(v0, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, false, C>(v0, lds, IdxColC<TX>{xl}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -701,7 +711,7 @@ __global__ void __launch_bounds__(512) k_grad1_r(LdSpectral<R> ls, int M, int gi
     }
   }
   __syncthreads();
-  reg_fft_m<L, E, 1, true, false, C>(v1, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, false, C>(v1, lds, IdxColC<TX>{xl}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -718,9 +728,9 @@ __global__ void __launch_bounds__(512) k_grad1_r(LdSpectral<R> ls, int M, int gi
 template <typename R, int L, typename C = R>
 __global__ void __launch_bounds__(512) k_grad2_r(cpx<R>* __restrict__ buf, int64_t sy, int64_t sz, int64_t sf,
                                                   const R* __restrict__ kyt, const cpx<C>* __restrict__ tw,
-                                                  int nx, int TX, Band ob, Band ib, Band wb,
+                                                  int nx, int TXr, Band ob, Band ib, Band wb,
                                                   const int4* __restrict__ fband, int M, int slot2) {
-  constexpr int E = gE(L), T = L / E;
+  constexpr int E = gE(L), T = L / E, TX = grad_tx<L, C>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<C>* lds = reinterpret_cast<cpx<C>*>(smem_raw);
   int bx, by, bz;
@@ -735,33 +745,35 @@ __global__ void __launch_bounds__(512) k_grad2_r(cpx<R>* __restrict__ buf, int64
     ib = Band{fb.y, L};
   }
   const int x = x0 + xl;
-  cpx<R>* f0 = buf + (int64_t)(gc * 3) * sf + x + o * sz;
+  // the lane's row t of slot 0; element e is T*e rows further (a uniform offset)
+  cpx<R>* f0 = buf + (int64_t)(gc * 3) * sf + x + o * sz + (int64_t)t * sy;
   cpx<R>* f1 = f0 + sf;
   cpx<R>* f2 = f1 + sf;
+  const int64_t ry = (int64_t)T * sy;
   cpx<C> v0[E], v1[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = t + T * e;
     v0[e] = mk<C>((C)0, (C)0);
-    if (x < nx && ib.has(i)) v0[e] = cvt<C>(f0[i * sy]);
+    if (x < nx && ib.has(i)) v0[e] = cvt<C>(f0[e * ry]);
     const C k = (C)kyt[i];
     v1[e] = mk<C>(-k * v0[e].y, k * v0[e].x);
   }
-  reg_fft_m<L, E, 1, true, false, C>(v0, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, false, C>(v0, lds, IdxColC<TX>{xl}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = t + T * e;
-      if (wb.has(i)) f0[i * sy] = cvt<R>(v0[e]);
+      if (wb.has(i)) f0[e * ry] = cvt<R>(v0[e]);
     }
   }
   __syncthreads();
-  reg_fft_m<L, E, 1, true, false, C>(v1, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, false, C>(v1, lds, IdxColC<TX>{xl}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = t + T * e;
-      if (wb.has(i)) f1[i * sy] = cvt<R>(v1[e]);
+      if (wb.has(i)) f1[e * ry] = cvt<R>(v1[e]);
     }
   }
   if (!slot2) return;  // slot 2 runs as a plain column pass
@@ -769,15 +781,15 @@ __global__ void __launch_bounds__(512) k_grad2_r(cpx<R>* __restrict__ buf, int64
   for (int e = 0; e < E; ++e) {
     const int i = t + T * e;
     v0[e] = mk<C>((C)0, (C)0);
-    if (x < nx && ib.has(i)) v0[e] = cvt<C>(f2[i * sy]);
+    if (x < nx && ib.has(i)) v0[e] = cvt<C>(f2[e * ry]);
   }
   __syncthreads();
-  reg_fft_m<L, E, 1, true, false, C>(v0, lds, IdxCol{xl, TX}, t, tw);
+  reg_fft_m<L, E, 1, true, false, C>(v0, lds, IdxColC<TX>{xl}, t, tw);
   if (x < nx) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = t + T * e;
-      if (wb.has(i)) f2[i * sy] = cvt<R>(v0[e]);
+      if (wb.has(i)) f2[e * ry] = cvt<R>(v0[e]);
     }
   }
 }

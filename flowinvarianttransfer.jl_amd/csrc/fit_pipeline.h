@@ -277,15 +277,15 @@ int grads_shared(fit_plan* p, const void* f, int giver0, int ng, const int* bi, 
                "k_grad1_r", L);
     auto run = [&](auto ctag) {
       using C = decltype(ctag);
-      const int TX = grad_tile(p, L, sizeof(cpx<C>));
-      dim3 grid((xa + TX - 1) / TX, nob, ngc);
       switch (L) {
 #define FIT_CASE(LL)                                                                                          \
-  case LL:                                                                                                    \
+  case LL: {                                                                                                  \
+    constexpr int TX = grad_tx<LL, C>();                                                                      \
+    dim3 grid((xa + TX - 1) / TX, nob, ngc);                                                                  \
     hipLaunchKernelGGL((k_grad1_r<R, LL, C>), grid, dim3(two_T(LL) * TX), (size_t)L * TX * sizeof(cpx<C>),    \
                        p->stream, ls, p->M, giver0, st, twc<R, C>(p, ax), xa, TX, ax, ob, ib, wb, fband,      \
                        p->G[1]);                                                                              \
-    break;
+  } break;
         FIT_REG_LENGTHS(FIT_CASE)
 #undef FIT_CASE
         default: break;
@@ -307,22 +307,25 @@ int grads_shared(fit_plan* p, const void* f, int giver0, int ng, const int* bi, 
                "k_grad2_r", L);
     auto run = [&](auto ctag) {
       using C = decltype(ctag);
-      const int TX = grad_tile(p, L, sizeof(cpx<C>));
       const int nob = ob.count();
       const int gy = p->col_pipe > 0 ? std::max(1, (nob + p->col_pipe - 1) / p->col_pipe) : nob;
-      dim3 grid((xa + TX - 1) / TX, gy, ngc);
       switch (L) {
 #define FIT_CASE(LL)                                                                                          \
-  case LL:                                                                                                    \
-    if (p->col_pipe > 0)                                                                                      \
+  case LL: {                                                                                                  \
+    if (p->col_pipe > 0) {                                                                                    \
+      const int TX = grad_tile(p, L, sizeof(cpx<C>));                                                         \
+      dim3 grid((xa + TX - 1) / TX, gy, ngc);                                                                 \
       hipLaunchKernelGGL((k_grad2_p<R, LL, C>), grid, dim3(two_T(LL) * TX), (size_t)L * TX * sizeof(cpx<C>),  \
                          p->stream, st.p, st.sy, st.sz, st.sf, (const R*)p->kt[1], twc<R, C>(p, 1), xa, TX,   \
                          ob, ib, wb, fband, p->M, split ? 0 : 1, nob);                                        \
-    else                                                                                                      \
+    } else {                                                                                                  \
+      constexpr int TX = grad_tx<LL, C>();                                                                    \
+      dim3 grid((xa + TX - 1) / TX, gy, ngc);                                                                 \
       hipLaunchKernelGGL((k_grad2_r<R, LL, C>), grid, dim3(two_T(LL) * TX), (size_t)L * TX * sizeof(cpx<C>),  \
                          p->stream, st.p, st.sy, st.sz, st.sf, (const R*)p->kt[1], twc<R, C>(p, 1), xa, TX,   \
                          ob, ib, wb, fband, p->M, split ? 0 : 1);                                             \
-    break;
+    }                                                                                                         \
+  } break;
         FIT_REG_LENGTHS(FIT_CASE)
 #undef FIT_CASE
         default: break;
