@@ -67,6 +67,9 @@ struct Band {
   __host__ __device__ __forceinline__ int count() const { return full() ? G : (a < 0 ? 0 : 2 * a + 1); }
   __host__ __device__ __forceinline__ bool has(int i) const { return full() || i <= a || i >= G - a; }
   __host__ __device__ __forceinline__ int at(int t) const { return (full() || t <= a) ? t : G - (2 * a + 1 - t); }
+  // an in-band index for every i (i itself when in band): a load from it stays in the lines the
+  // band's own loads touch, so a line can be loaded unconditionally and the result selected
+  __host__ __device__ __forceinline__ int clamp(int i) const { return has(i) ? i : (a < 0 ? 0 : a); }
 };
 
 // precision conversion of a complex value (load in storage precision, compute in C)
@@ -536,11 +539,25 @@ __global__ void __launch_bounds__(col_threads<C>()) k_col_r(LD ld, ST st, const 
   }
   const int x = x0 + xl;
   cpx<C> v[E];
+  if constexpr (SIGN < 0) {
+    // forward passes read dense bands: unconditional loads from a clamped column and in-band row,
+    // results selected (a load under a divergent branch is waited for at the join, serialising
+    // the line's loads: 2.8 -> 2.1 ms at 768 with FP64 arithmetic).  The inverse passes of the
+    // giver loop read narrow bands, where skipping the loads wins.
+    const int xs = x < nx ? x : nx - 1;
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int i = t + T * e;
-    v[e] = mk<C>((C)0, (C)0);
-    if (x < nx && ib.has(i)) v[e] = cvt<C>(axis == 1 ? ld(x, i, o, f) : ld(x, o, i, f));
+    for (int e = 0; e < E; ++e) {
+      const int i = t + T * e, ic = ib.clamp(i);
+      const cpx<R> a = axis == 1 ? ld(xs, ic, o, f) : ld(xs, o, ic, f);
+      v[e] = (x < nx && ib.has(i)) ? cvt<C>(a) : mk<C>((C)0, (C)0);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = t + T * e;
+      v[e] = mk<C>((C)0, (C)0);
+      if (x < nx && ib.has(i)) v[e] = cvt<C>(axis == 1 ? ld(x, i, o, f) : ld(x, o, i, f));
+    }
   }
   reg_fft_e<L, E, SIGN, false, C>(v, lds, lds, IdxCol{xl, TX}, t, tw);
   if (x < nx) {
@@ -592,13 +609,14 @@ __global__ void __launch_bounds__(512) k_col_p(LD ld, ST st, const cpx<C>* __res
     return ty;
   };
   cpx<R> vn[E];
+  const int xs = x < nx ? x : nx - 1;
   auto load = [&](int ty) {
     const int o = ob.at(ty);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const int i = t + T * e;
-      vn[e] = mk<R>((R)0, (R)0);
-      if (x < nx && ib.has(i)) vn[e] = axis == 1 ? ld(x, i, o, f) : ld(x, o, i, f);
+      const int i = t + T * e, ic = ib.clamp(i);
+      const cpx<R> a = axis == 1 ? ld(xs, ic, o, f) : ld(xs, o, ic, f);
+      vn[e] = (x < nx && ib.has(i)) ? a : mk<R>((R)0, (R)0);
     }
   };
   int ty = next(by);
@@ -818,13 +836,14 @@ __global__ void __launch_bounds__(512) k_grad2_p(cpx<R>* __restrict__ buf, int64
   const int ys = (int)gridDim.y;
   cpx<R>* fb0 = buf + (int64_t)(gc * 3) * sf + x;
   cpx<R> s0[E], s2[E];
+  const int xs = x < nx ? x : nx - 1;
   auto load = [&](cpx<R> (&d)[E], int slot, int ty) {
-    const cpx<R>* src = fb0 + slot * sf + ob.at(ty) * sz;
+    const cpx<R>* src = fb0 + (xs - x) + slot * sf + ob.at(ty) * sz;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = t + T * e;
-      d[e] = mk<R>((R)0, (R)0);
-      if (x < nx && ib.has(i)) d[e] = src[i * sy];
+      const cpx<R> a = src[(int64_t)ib.clamp(i) * sy];
+      d[e] = (x < nx && ib.has(i)) ? a : mk<R>((R)0, (R)0);
     }
   };
   auto store = [&](const cpx<C> (&v)[E], int slot, int ty) {
@@ -1560,24 +1579,39 @@ __global__ void __launch_bounds__(lastdens_threads<R>()) k_lastdens_r(
       for (int c = 0; c < M; ++c) {
         cpx<R> v[E];
         column(c, v);
+        if (sb) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-          const int i = t + T * e;
-          if (x < nx && wb.has(i)) {
-            const int yy = lax == 1 ? i : o, zz = lax == 1 ? o : i;
-            cpx<R> ev;
-            if (sb) {
+          for (int e = 0; e < E; ++e) {
+            const int i = t + T * e;
+            if (x < nx && wb.has(i)) {
               // eh from the snapshot array: Hermitian part of the masked field at (x, yy, zz)
+              const int yy = lax == 1 ? i : o, zz = lax == 1 ? o : i;
               const int my = yy == 0 ? 0 : g.n2 - yy, mz = zz == 0 ? 0 : g.n3 - zz, mx = x == 0 ? 0 : g.n1 - x;
               const bool keep = !(da.trunc && g.dealiased(x, yy, zz));
               const cpx<R> a = sb[g.oidx(x, yy, zz) + c * da.ssf], b = sb[g.oidx(mx, my, mz) + c * da.ssf];
-              ev = keep ? mk<R>((R)0.5 * (a.x + b.x), (R)0.5 * (a.y - b.y)) : mk<R>((R)0, (R)0);
-            } else {
-              ev = ehb[x + yy * da.ehsy + zz * da.ehsz + c * da.ehsf];
+              const cpx<R> ev = keep ? mk<R>((R)0.5 * (a.x + b.x), (R)0.5 * (a.y - b.y)) : mk<R>((R)0, (R)0);
+              const cpx<R> w = cscale(v[e], scale);
+              tq[e] += ev.x * w.x + ev.y * w.y;
             }
-            const cpx<R> w = cscale(v[e], scale);
-            tq[e] += ev.x * w.x + ev.y * w.y;
           }
+        } else {
+          // the precomputed carrier: every point's load issued unconditionally from a clamped
+          // column and in-band row, then selected (a load under a divergent branch is waited for
+          // at the join; 0.45 -> 0.36 ms for the 48-giver last pass at 192)
+          const int xs = x < nx ? x : nx - 1;
+          cpx<R> ev[E];
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            const int ic = wb.clamp(t + T * e);
+            const int yy = lax == 1 ? ic : o, zz = lax == 1 ? o : ic;
+            ev[e] = ehb[xs + yy * da.ehsy + zz * da.ehsz + c * da.ehsf];
+          }
+#pragma unroll
+          for (int e = 0; e < E; ++e)
+            if (x < nx && wb.has(t + T * e)) {
+              const cpx<R> w = cscale(v[e], scale);
+              tq[e] += ev[e].x * w.x + ev[e].y * w.y;
+            }
         }
       }
     }
