@@ -747,8 +747,9 @@ def batch_map(f, items, *, reduce="gather"):
 class Ensemble:
     """Device-resident ensemble of snapshots (fit_ensemble): the reference's batch axis
     (mpi_batch_map, MPIExt.jl:39-68) with the kinetic-energy flux and, optionally, the scalar
-    flux of every snapshot.  Snapshots on the same grid run batched in the same launches and the
-    whole ensemble replays as one captured HIP graph.
+    flux of every snapshot.  Snapshots on the same grid run batched in the same launches (on a
+    3/2-padded grid: one after another through the padded pipeline) and the whole ensemble
+    replays as one captured HIP graph.
 
     shapes: per-snapshot grid sizes; ks_list: per-snapshot wavenumber vectors."""
 
@@ -904,9 +905,8 @@ def calculate_ensemble_fluxes(velocities, ks_list, scalars=None, *, binning=None
     velocities[j]: (ns_j..., D) array; ks_list[j]: its wavenumber vectors; scalars[j] (optional):
     a scalar advected by velocities[j]; physical=True: the arrays are physical fields, transformed
     to û = fft(u)/Np on the device first.  Returns a list of (flux of `invariant`, scalar-variance
-    flux or None).  Kinetic-energy ensembles of 2D / 3D snapshots (D = nd, no padding) run as one
-    batched, graph-captured fit_ensemble; anything else runs snapshot by snapshot on cached
-    plans."""
+    flux or None).  Kinetic-energy ensembles of 2D / 3D snapshots (D = nd) run as one
+    graph-captured fit_ensemble; anything else runs snapshot by snapshot on cached plans."""
     _check_spectral(spectral)
     if physical:
         # physical snapshots: û = fft(u)/Np on the device first (PencilFFTsExt.jl:83-88)
@@ -921,7 +921,6 @@ def calculate_ensemble_fluxes(velocities, ks_list, scalars=None, *, binning=None
         return []
     nd = len(ks_list[0])
     batched = (isinstance(invariant, KineticEnergy) and nd in (2, 3)
-               and not isinstance(dealiasing, PaddedThreeHalves)
                and all(len(k) == nd and int(v.shape[nd]) == nd and len(v.shape) == nd + 1
                        for v, k in zip(velocities, ks_list))
                and len({_real_dtype(v) for v in velocities}) == 1

@@ -1403,8 +1403,6 @@ int fit_ensemble_create(fit_ensemble** out, int n_snap, int nd, const int64_t* n
   if (n_snap < 1) return fail(FIT_E_ARGUMENT, "an ensemble needs at least one snapshot");
   if (nd != 2 && nd != 3) return fail(FIT_E_UNSUPPORTED, "ensembles run 2D or 3D snapshots");
   if (!ns || !ks || !shell_idx || !n_shells) return fail(FIT_E_ARGUMENT, "ns / ks / shell_idx / n_shells is NULL");
-  if (dealias == FIT_DEALIAS_PADDED)
-    return fail(FIT_E_UNSUPPORTED, "ensembles run NoDealiasing or OrszagTwoThirds (use a plan per snapshot for padding)");
   fit_ensemble* e = new fit_ensemble();
   e->n = n_snap; e->nd = nd; e->dtype = compute_dtype; e->dealias = dealias; e->device = device;
   e->with_scalar = with_scalar != 0;
@@ -1565,7 +1563,12 @@ int fit_ensemble_flux(fit_ensemble* e, const void* const* u_hat, const void* con
     FIT_HIP(hipMemcpyAsync(G.d_tab, G.h_tab.data(), G.h_tab.size() * sizeof(void*), hipMemcpyHostToDevice, e->stream));
   }
   const int io_key = ptr_kind * 2 + io_dtype;
-  if (e->exec && e->graph_io == io_key && !e->prof) {
+  // groups on padded grids pass the snapshot pointers themselves as kernel arguments (the others
+  // read them from the device table): a graph replays only for the pointers it was captured with
+  std::vector<void*> ptrs;
+  for (EnsGroup& G : e->groups)
+    if (G.p->dealias == FIT_DEALIAS_PADDED) ptrs.insert(ptrs.end(), G.h_tab.begin(), G.h_tab.end());
+  if (e->exec && e->graph_io == io_key && e->graph_ptrs == ptrs && !e->prof) {
     FIT_HIP(hipGraphLaunch(e->exec, e->stream));
   } else {
     int rc = ens_run_any(e);  // allocates every workspace the graph will reuse
@@ -1581,6 +1584,7 @@ int fit_ensemble_flux(fit_ensemble* e, const void* const* u_hat, const void* con
       e->graph = g;
       FIT_HIP(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
       e->graph_io = io_key;
+      e->graph_ptrs = ptrs;
     }
   }
   for (EnsGroup& G : e->groups)

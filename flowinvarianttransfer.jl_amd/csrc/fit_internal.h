@@ -183,6 +183,7 @@ struct fit_ensemble {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_io = -1;              // (ptr_kind, io_dtype) the graph was captured for
+  std::vector<void*> graph_ptrs;  // padded groups' input pointers the graph's launches hold
   hipEvent_t order_ev = nullptr;  // ordering after a caller's stream (fit_ensemble_wait_stream)
 };
 

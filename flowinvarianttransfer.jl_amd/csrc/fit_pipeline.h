@@ -1269,9 +1269,41 @@ int ens_batch_size(fit_ensemble* e, EnsGroup& G) {
   return FIT_OK;
 }
 
+// A group on a 3/2-padded grid: each snapshot through the plan's own padded flux pipeline
+// (velocities and products on the G-point grid, densities on the original modes), its rows copied
+// from dcolG into the group's row table.  Kinetic energy: f = a = û; scalar: f = θ̂, a = û.
+template <typename R>
+int ens_padded(fit_ensemble* e, EnsGroup& G) {
+  fit_plan* p = G.p;
+  const int B = (int)G.snaps.size(), nsh = p->nsh;
+  int rc;
+  for (int b = 0; b < B; ++b) {
+    const void* u = G.h_tab[(size_t)b];
+    {
+      const int inv = FIT_KINETIC_ENERGY;
+      if ((rc = flux_multi<R>(p, 1, &inv, u, u))) return rc;
+      FIT_HIP(hipMemcpyAsync(G.rows + (size_t)b * nsh, p->dcolG, (size_t)nsh * sizeof(double),
+                             hipMemcpyDeviceToDevice, p->stream));
+    }
+    if (e->with_scalar) {
+      MGuard mg(p, 1);
+      const int inv = FIT_PASSIVE_SCALAR;
+      if ((rc = flux_multi<R>(p, 1, &inv, G.h_tab[(size_t)B + b], u))) return rc;
+      FIT_HIP(hipMemcpyAsync(G.rows + (size_t)(B + b) * nsh, p->dcolG, (size_t)nsh * sizeof(double),
+                             hipMemcpyDeviceToDevice, p->stream));
+    }
+  }
+  return FIT_OK;
+}
+
 template <typename R>
 int ens_run(fit_ensemble* e) {
   for (EnsGroup& G : e->groups) {
+    if (G.p->dealias == FIT_DEALIAS_PADDED) {
+      int rc = ens_padded<R>(e, G);
+      if (rc) return rc;
+      continue;
+    }
     int rc = ens_batch_size<R>(e, G);
     if (rc) return rc;
     const int B = (int)G.snaps.size();

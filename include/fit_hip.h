@@ -300,7 +300,9 @@ typedef struct fit_ensemble fit_ensemble;
  * one HIP graph on the first call and replayed afterwards.
  *   nd in {2, 3}; ns: n_snap*nd sizes (snapshot j: ns[j*nd .. j*nd+nd-1]); velocities have D = nd
  *   components; ks: n_snap*nd arrays (snapshot j, axis d at ks[j*nd + d]); shell_idx[j]: Np_j
- *   Int64 shell numbers (as fit_plan_create); n_shells[j] >= 1; dealias NONE or TWOTHIRDS.
+ *   Int64 shell numbers (as fit_plan_create); n_shells[j] >= 1; any dealiasing.  With PADDED
+ *   each snapshot runs the plan's padded flux pipeline in turn (the batched launches serve
+ *   NONE and TWOTHIRDS), and the graph is captured again whenever the snapshot pointers change.
  */
 int fit_ensemble_create(fit_ensemble** ens, int n_snap, int nd, const int64_t* ns, int compute_dtype,
                         int dealias, const double* const* ks, const int64_t* const* shell_idx,
