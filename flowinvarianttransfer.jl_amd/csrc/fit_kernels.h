@@ -1819,6 +1819,36 @@ __device__ __forceinline__ R density_at(int inv, int nd, int M, const cpx<R>* ca
   return (wx.x * vx.x + wx.y * vx.y) + (wy.x * vy.x + wy.y * vy.y) + (wz.x * vz.x + wz.y * vz.y);
 }
 
+// density_at from values already in registers (M <= 3 components of carrier u and N̂ n)
+template <typename R>
+__device__ __forceinline__ R density_vals(int inv, int nd, int M, const cpx<R> (&u)[3], const cpx<R> (&n)[3], R kx,
+                                          R ky, R kz) {
+  if (inv == 0 || inv == 3) {
+    R t = u[0].x * n[0].x + u[0].y * n[0].y;
+    if (M > 1) t += u[1].x * n[1].x + u[1].y * n[1].y;
+    if (M > 2) t += u[2].x * n[2].x + u[2].y * n[2].y;
+    return t;
+  }
+  if (inv == 1) {
+    const cpx<R> wx = csub(cscale(u[2], ky), cscale(u[1], kz));
+    const cpx<R> wy = csub(cscale(u[0], kz), cscale(u[2], kx));
+    const cpx<R> wz = csub(cscale(u[1], kx), cscale(u[0], ky));
+    return (wx.x * n[0].y - wx.y * n[0].x) + (wy.x * n[1].y - wy.y * n[1].x) + (wz.x * n[2].y - wz.y * n[2].x);
+  }
+  if (nd == 2) {
+    const cpx<R> w = csub(cscale(u[1], kx), cscale(u[0], ky));
+    const cpx<R> v = csub(cscale(n[1], kx), cscale(n[0], ky));
+    return w.x * v.x + w.y * v.y;
+  }
+  const cpx<R> wx = csub(cscale(u[2], ky), cscale(u[1], kz));
+  const cpx<R> wy = csub(cscale(u[0], kz), cscale(u[2], kx));
+  const cpx<R> wz = csub(cscale(u[1], kx), cscale(u[0], ky));
+  const cpx<R> vx = csub(cscale(n[2], ky), cscale(n[1], kz));
+  const cpx<R> vy = csub(cscale(n[0], kz), cscale(n[2], kx));
+  const cpx<R> vz = csub(cscale(n[1], kx), cscale(n[0], ky));
+  return (wx.x * vx.x + wx.y * vx.y) + (wy.x * vy.x + wy.y * vy.y) + (wz.x * vz.x + wz.y * vz.y);
+}
+
 // density over the full original grid; optional per-mode output t_out; optional shell partial
 // sums (FP64 LDS bins, one row of nsh partials per workgroup, reduced later in fixed order).
 template <typename R>
@@ -1835,32 +1865,68 @@ __global__ void __launch_bounds__(256) k_density(Grid g, int4 invs, int ninv, in
   const DetBins db{dbins, nb, dpriv};
   if (shell) db.zero();
   __syncthreads();
-  // one x row per iteration: the row's (y, z) and wavenumbers are found once per row instead of
-  // by 64-bit division per mode
+  // x rows: the row's (y, z) and wavenumbers are found once per row instead of by 64-bit division
+  // per mode.  With M <= 3 two rows run per iteration and every load of both (carrier, N̂, shell
+  // map) is issued before any arithmetic: twice the bytes in flight per wave of this
+  // latency-bound stream (the bins cap a CU at a few workgroups).
   const int nrow = g.n2 * g.n3;
-  for (int row = blockIdx.x; row < nrow; row += gridDim.x) {
-    const int iy = row % g.n2, iz = row / g.n2;
-    const R ky = g.nd >= 2 ? ks1[iy] : (R)0;
-    const R kz = g.nd >= 3 ? ks2[iz] : (R)0;
-    const int64_t base = (int64_t)row * g.n1;
+  const int RP = M <= 3 ? 2 : 1;
+  for (int row = blockIdx.x; row < nrow; row += RP * gridDim.x) {
+    int rows[2] = {row, row + (int)gridDim.x};
+    const bool two = RP == 2 && rows[1] < nrow;   // uniform
+    if (!two) rows[1] = row;
+    R ky[2], kz[2];
+    int64_t base[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int iy = rows[r] % g.n2, iz = rows[r] / g.n2;
+      ky[r] = g.nd >= 2 ? ks1[iy] : (R)0;
+      kz[r] = g.nd >= 3 ? ks2[iz] : (R)0;
+      base[r] = (int64_t)rows[r] * g.n1;
+    }
     for (int x0 = 0; x0 < g.n1; x0 += blockDim.x) {   // uniform trip count (commit may sync)
       const int ix = x0 + threadIdx.x;
-      const int64_t o = base + ix;
-      R tv[4] = {(R)0, (R)0, (R)0, (R)0};
-      int s = 0;
+      R tv[2][4] = {{(R)0, (R)0, (R)0, (R)0}, {(R)0, (R)0, (R)0, (R)0}};
+      int s[2] = {0, 0};
       if (ix < g.n1) {
         const R kx = ks0[ix];
-        s = shell ? shell[o] : 0;
-        for (int q = 0; q < ninv; ++q) {
-          const int inv = q == 0 ? invs.x : (q == 1 ? invs.y : (q == 2 ? invs.z : invs.w));
-          tv[q] = density_at<R>(inv, g.nd, M, car, csf, nh, nsf, o, kx, ky, kz);
+        if (RP == 2) {
+          cpx<R> u[2][3], n[2][3];
+#pragma unroll
+          for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const int cc = c < M ? c : 0;
+              u[r][c] = car[base[r] + ix + cc * csf];
+              n[r][c] = nh[base[r] + ix + cc * nsf];
+            }
+#pragma unroll
+          for (int r = 0; r < 2; ++r) s[r] = shell ? shell[base[r] + ix] : 0;
+#pragma unroll
+          for (int r = 0; r < 2; ++r)
+            for (int q = 0; q < ninv; ++q) {
+              const int inv = q == 0 ? invs.x : (q == 1 ? invs.y : (q == 2 ? invs.z : invs.w));
+              tv[r][q] = density_vals<R>(inv, g.nd, M, u[r], n[r], kx, ky[r], kz[r]);
+            }
+        } else {
+          const int64_t o = base[0] + ix;
+          s[0] = shell ? shell[o] : 0;
+          for (int q = 0; q < ninv; ++q) {
+            const int inv = q == 0 ? invs.x : (q == 1 ? invs.y : (q == 2 ? invs.z : invs.w));
+            tv[0][q] = density_at<R>(inv, g.nd, M, car, csf, nh, nsf, o, kx, ky[0], kz[0]);
+          }
         }
-        if (t_out) t_out[o] = tv[0];
+        if (t_out) {
+          t_out[base[0] + ix] = tv[0][0];
+          if (two) t_out[base[1] + ix] = tv[1][0];
+        }
       }
       if (shell)
         db.commit([&](double* bb) {
-          if (s > 0)
-            for (int q = 0; q < ninv; ++q) atomicAdd(&bb[q * nsh + s - 1], (double)tv[q]);
+          if (s[0] > 0)
+            for (int q = 0; q < ninv; ++q) atomicAdd(&bb[q * nsh + s[0] - 1], (double)tv[0][q]);
+          if (two && s[1] > 0)
+            for (int q = 0; q < ninv; ++q) atomicAdd(&bb[q * nsh + s[1] - 1], (double)tv[1][q]);
         });
     }
   }
