@@ -1188,9 +1188,10 @@ __global__ void __launch_bounds__(256) k_xfused_self_stg(const cpx<float>* __res
         const int q = c * nd + j + 1;
         if (q < M * nd) issue(grad + (int64_t)q * gsf);
         fft(std::integral_constant<int, 1>{}, v);
+        // the gradient's 1/Gp is applied once to the transformed sum (hs below)
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-          const cpx<R> tt = mk<R>(ur[j][e].x * (v[e].x * gscale), ur[j][e].y * (v[e].y * gscale));
+          const cpx<R> tt = mk<R>(ur[j][e].x * v[e].x, ur[j][e].y * v[e].y);
           acc[e] = j == 0 ? tt : cadd(acc[e], tt);
         }
       }
@@ -1202,11 +1203,12 @@ __global__ void __launch_bounds__(256) k_xfused_self_stg(const cpx<float>* __res
     for (int e = 0; e < E; ++e) xch[idx(t + T * e)] = acc[e];
     wave_lds_sync();
     cpx<R>* of = out + (int64_t)c * osf;
+    const R hs = (R)0.5 * gscale;
     for (int k = t; k <= kxb; k += T) {
       const cpx<R> Zk = xch[idx(k)];
       const cpx<R> Zm = xch[idx((L - k) % L)];
-      const cpx<R> A = mk<R>((R)0.5 * (Zk.x + Zm.x), (R)0.5 * (Zk.y - Zm.y));
-      const cpx<R> Bv = mk<R>((R)0.5 * (Zk.y + Zm.y), (R)-0.5 * (Zk.x - Zm.x));
+      const cpx<R> A = mk<R>(hs * (Zk.x + Zm.x), hs * (Zk.y - Zm.y));
+      const cpx<R> Bv = mk<R>(hs * (Zk.y + Zm.y), -hs * (Zk.x - Zm.x));
       if (ok0) of[k + oo0] = A;
       if (ok1) of[k + oo1] = Bv;
     }
@@ -1866,19 +1868,22 @@ __global__ void __launch_bounds__(256) k_density(Grid g, int4 invs, int ninv, in
   if (shell) db.zero();
   __syncthreads();
   // x rows: the row's (y, z) and wavenumbers are found once per row instead of by 64-bit division
-  // per mode.  With M <= 3 two rows run per iteration and every load of both (carrier, N̂, shell
-  // map) is issued before any arithmetic: twice the bytes in flight per wave of this
+  // per mode.  With M <= 3, RP rows run per iteration and every load of all of them (carrier, N̂,
+  // shell map) is issued before any arithmetic: RP times the bytes in flight per wave of this
   // latency-bound stream (the bins cap a CU at a few workgroups).
+  constexpr int RP = 4;
   const int nrow = g.n2 * g.n3;
-  const int RP = M <= 3 ? 2 : 1;
-  for (int row = blockIdx.x; row < nrow; row += RP * gridDim.x) {
-    int rows[2] = {row, row + (int)gridDim.x};
-    const bool two = RP == 2 && rows[1] < nrow;   // uniform
-    if (!two) rows[1] = row;
-    R ky[2], kz[2];
-    int64_t base[2];
+  const int rp = M <= 3 ? RP : 1;
+  for (int row = blockIdx.x; row < nrow; row += rp * gridDim.x) {
+    int rows[RP];
+    bool live[RP];
+    R ky[RP], kz[RP];
+    int64_t base[RP];
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
+    for (int r = 0; r < RP; ++r) {
+      const int rr = row + r * (int)gridDim.x;
+      live[r] = r < rp && rr < nrow;   // uniform
+      rows[r] = live[r] ? rr : row;
       const int iy = rows[r] % g.n2, iz = rows[r] / g.n2;
       ky[r] = g.nd >= 2 ? ks1[iy] : (R)0;
       kz[r] = g.nd >= 3 ? ks2[iz] : (R)0;
@@ -1886,14 +1891,19 @@ __global__ void __launch_bounds__(256) k_density(Grid g, int4 invs, int ninv, in
     }
     for (int x0 = 0; x0 < g.n1; x0 += blockDim.x) {   // uniform trip count (commit may sync)
       const int ix = x0 + threadIdx.x;
-      R tv[2][4] = {{(R)0, (R)0, (R)0, (R)0}, {(R)0, (R)0, (R)0, (R)0}};
-      int s[2] = {0, 0};
+      R tv[RP][4];
+      int s[RP];
+#pragma unroll
+      for (int r = 0; r < RP; ++r) {
+        s[r] = 0;
+        tv[r][0] = tv[r][1] = tv[r][2] = tv[r][3] = (R)0;
+      }
       if (ix < g.n1) {
         const R kx = ks0[ix];
-        if (RP == 2) {
-          cpx<R> u[2][3], n[2][3];
+        if (rp == RP) {
+          cpx<R> u[RP][3], n[RP][3];
 #pragma unroll
-          for (int r = 0; r < 2; ++r)
+          for (int r = 0; r < RP; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
               const int cc = c < M ? c : 0;
@@ -1901,9 +1911,9 @@ __global__ void __launch_bounds__(256) k_density(Grid g, int4 invs, int ninv, in
               n[r][c] = nh[base[r] + ix + cc * nsf];
             }
 #pragma unroll
-          for (int r = 0; r < 2; ++r) s[r] = shell ? shell[base[r] + ix] : 0;
+          for (int r = 0; r < RP; ++r) s[r] = shell ? shell[base[r] + ix] : 0;
 #pragma unroll
-          for (int r = 0; r < 2; ++r)
+          for (int r = 0; r < RP; ++r)
             for (int q = 0; q < ninv; ++q) {
               const int inv = q == 0 ? invs.x : (q == 1 ? invs.y : (q == 2 ? invs.z : invs.w));
               tv[r][q] = density_vals<R>(inv, g.nd, M, u[r], n[r], kx, ky[r], kz[r]);
@@ -1916,17 +1926,17 @@ __global__ void __launch_bounds__(256) k_density(Grid g, int4 invs, int ninv, in
             tv[0][q] = density_at<R>(inv, g.nd, M, car, csf, nh, nsf, o, kx, ky[0], kz[0]);
           }
         }
-        if (t_out) {
-          t_out[base[0] + ix] = tv[0][0];
-          if (two) t_out[base[1] + ix] = tv[1][0];
-        }
+        if (t_out)
+#pragma unroll
+          for (int r = 0; r < RP; ++r)
+            if (live[r]) t_out[base[r] + ix] = tv[r][0];
       }
       if (shell)
         db.commit([&](double* bb) {
-          if (s[0] > 0)
-            for (int q = 0; q < ninv; ++q) atomicAdd(&bb[q * nsh + s[0] - 1], (double)tv[0][q]);
-          if (two && s[1] > 0)
-            for (int q = 0; q < ninv; ++q) atomicAdd(&bb[q * nsh + s[1] - 1], (double)tv[1][q]);
+#pragma unroll
+          for (int r = 0; r < RP; ++r)
+            if (live[r] && s[r] > 0)
+              for (int q = 0; q < ninv; ++q) atomicAdd(&bb[q * nsh + s[r] - 1], (double)tv[r][q]);
         });
     }
   }
