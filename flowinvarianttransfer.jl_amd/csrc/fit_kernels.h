@@ -1582,16 +1582,26 @@ __global__ void __launch_bounds__(lastdens_threads<R>()) k_lastdens_r(
         cpx<R> v[E];
         column(c, v);
         if (sb) {
+          // eh from the snapshot array: Hermitian part of the masked field at (x, yy, zz); both
+          // loads of every point issued unconditionally from a clamped column and in-band row
+          const int xs = x < nx ? x : nx - 1, mx = xs == 0 ? 0 : g.n1 - xs;
+          cpx<R> a[E], b[E];
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            const int ic = wb.clamp(t + T * e);
+            const int yy = lax == 1 ? ic : o, zz = lax == 1 ? o : ic;
+            const int my = yy == 0 ? 0 : g.n2 - yy, mz = zz == 0 ? 0 : g.n3 - zz;
+            a[e] = sb[g.oidx(xs, yy, zz) + c * da.ssf];
+            b[e] = sb[g.oidx(mx, my, mz) + c * da.ssf];
+          }
 #pragma unroll
           for (int e = 0; e < E; ++e) {
             const int i = t + T * e;
             if (x < nx && wb.has(i)) {
-              // eh from the snapshot array: Hermitian part of the masked field at (x, yy, zz)
               const int yy = lax == 1 ? i : o, zz = lax == 1 ? o : i;
-              const int my = yy == 0 ? 0 : g.n2 - yy, mz = zz == 0 ? 0 : g.n3 - zz, mx = x == 0 ? 0 : g.n1 - x;
               const bool keep = !(da.trunc && g.dealiased(x, yy, zz));
-              const cpx<R> a = sb[g.oidx(x, yy, zz) + c * da.ssf], b = sb[g.oidx(mx, my, mz) + c * da.ssf];
-              const cpx<R> ev = keep ? mk<R>((R)0.5 * (a.x + b.x), (R)0.5 * (a.y - b.y)) : mk<R>((R)0, (R)0);
+              const cpx<R> ev = keep ? mk<R>((R)0.5 * (a[e].x + b[e].x), (R)0.5 * (a[e].y - b[e].y))
+                                     : mk<R>((R)0, (R)0);
               const cpx<R> w = cscale(v[e], scale);
               tq[e] += ev.x * w.x + ev.y * w.y;
             }
