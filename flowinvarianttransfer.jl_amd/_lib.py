@@ -21,11 +21,12 @@ OPT_FUSED_FLUX = 10
 OPT_RADIX_ORDER = 11
 OPT_COL_F64 = 12
 OPT_COL_PIPE = 13
+OPT_LAST_BINS = 14
 TUNE_FLUX, TUNE_SHELL_TO_SHELL = 1, 2
 OPT_NAMES = {1: "plain_pipeline", 2: "giver_batch", 3: "xpass_givers", 4: "xpass_block", 5: "subgrid",
              6: "col_tile", 7: "xpass_self", 8: "xpass_stage", 9: "grad_split", 10: "fused_flux",
-             11: "radix_order", 12: "col_f64", 13: "col_pipe"}
-ENS_OPT_GRAPH, ENS_OPT_BATCH = 1, 2
+             11: "radix_order", 12: "col_f64", 13: "col_pipe", 14: "last_bins"}
+ENS_OPT_GRAPH, ENS_OPT_BATCH, ENS_OPT_PLAN = 1, 2, 1000
 DECOMP_HELICAL, DECOMP_TOROIDAL_POLOIDAL = 1, 2
 K_CLASSES = ("fft_column", "fft_row_inverse", "fft_row_forward", "density_shell", "reduce", "other",
              "fused_x")

@@ -819,6 +819,10 @@ class Ensemble:
     def set_batch(self, n):
         self.set_option(_lib.ENS_OPT_BATCH, int(n))
 
+    def set_plan_option(self, option, value):
+        """Plan option `option` (the plans' OPT_* codes) on every same-grid group's plan."""
+        self.set_option(_lib.ENS_OPT_PLAN + int(option), int(value))
+
     def stream(self):
         return self.lib.fit_ensemble_stream(self.handle)
 

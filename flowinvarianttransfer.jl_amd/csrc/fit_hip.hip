@@ -885,6 +885,10 @@ int fit_plan_set_option(fit_plan* p, int option, int value) {
       if (value < 0 || value > 64) return fail(FIT_E_ARGUMENT, "FIT_OPT_COL_PIPE takes 0 to 64");
       p->col_pipe = value;
       return FIT_OK;
+    case FIT_OPT_LAST_BINS:
+      if (value < 1 || value > 128) return fail(FIT_E_ARGUMENT, "FIT_OPT_LAST_BINS takes 1 to 128 (KiB)");
+      p->last_bins_kb = value;
+      return FIT_OK;
     default: return fail(FIT_E_ARGUMENT, "unknown plan option");
   }
 }
@@ -905,6 +909,7 @@ int fit_plan_get_option(const fit_plan* p, int option, int* value) {
     case FIT_OPT_RADIX_ORDER: *value = p->radix_order; return FIT_OK;
     case FIT_OPT_COL_F64: *value = p->col_f64; return FIT_OK;
     case FIT_OPT_COL_PIPE: *value = p->col_pipe; return FIT_OK;
+    case FIT_OPT_LAST_BINS: *value = p->last_bins_kb; return FIT_OK;
     default: return fail(FIT_E_ARGUMENT, "unknown plan option");
   }
 }
@@ -1499,6 +1504,16 @@ int fit_ensemble_set_option(fit_ensemble* e, int option, int value) {
       ens_drop_graph(e);
       return FIT_OK;
     default:
+      if (option > FIT_ENS_OPT_PLAN) {
+        // a plan option (FIT_OPT_*) on every group's plan
+        for (EnsGroup& G : e->groups) {
+          const int rc = fit_plan_set_option(G.p, option - FIT_ENS_OPT_PLAN, value);
+          if (rc) return rc;
+          G.ng = 0;
+        }
+        ens_drop_graph(e);
+        return FIT_OK;
+      }
       return fail(FIT_E_ARGUMENT, "unknown ensemble option");
   }
 }

@@ -139,6 +139,7 @@ struct fit_plan {
   hipEvent_t order_ev = nullptr;
   int radix_order = 0;            // FIT_OPT_RADIX_ORDER of the LDS-engine passes
   int col_pipe = 0;               // FIT_OPT_COL_PIPE: tiles per workgroup of the pipelined column passes (0: off)
+  int last_bins_kb = 64;          // FIT_OPT_LAST_BINS: LDS budget of private shell bins in the fused last pass
   int col_f64 = 6;                // FIT_OPT_COL_F64: FP32 plans run the strided register passes in FP64
                                   // (bit 0: inverse passes, bit 1: forward y, bit 2: forward z)
   void* tw64[3] = {nullptr, nullptr, nullptr};  // FP64 twiddles of an FP32 plan (FIT_OPT_COL_F64)

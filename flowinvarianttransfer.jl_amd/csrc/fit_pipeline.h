@@ -407,8 +407,10 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
   const double bytes = (double)nfg * nrows * (p->nd * (xa * csz) + xb * csz) + (double)p->nd * nrows * L * sizeof(R);
   const R* kxt = p->grads_commuted ? (const R*)p->kt[0] : nullptr;
   if (p->grads_commuted && !reg_supported(L)) return fail(FIT_E_UNSUPPORTED, "shared-transform gradients need a register x pass");
-  // wave form for short register footprints (E <= 16); the workgroup form otherwise
-  const bool wave_ok = reg_supported(L) && !p->xfused_block && wE(L) <= 16 &&
+  // wave form when the velocity lines stay in registers (E <= 8); the workgroup form otherwise
+  // (at 1024 points the wave form holds 16 points per lane in 164 VGPRs: 1.01 ms against
+  // 0.34 ms for the workgroup form in the 2D ensemble)
+  const bool wave_ok = reg_supported(L) && !p->xfused_block && wE(L) <= 8 &&
                        (size_t)4 * (64 / (L / wE(L))) * (size_t)lds_line(L, wE(L)) * csz <= g_max_lds;
   prof_begin(p, FIT_K_FUSED_X, bytes, wave_ok ? "k_xfused_w" : reg_supported(L) ? "k_xfused_r" : "k_xfused", L);
   if (wave_ok) {
@@ -421,18 +423,18 @@ int v2_xfused(fit_plan* p, int ng, const int* bi, const int4* fband, const int* 
     const int ngiv = nfg / p->M;
     const int GC = ugs ? 1 : std::max(1, std::min(ngiv, p->xfused_gc > 0 ? p->xfused_gc : 4));
     const int nchunk = (ngiv + GC - 1) / GC;
-    // only lengths with at most 16 points per lane are instantiated (the others never qualify)
+    // only lengths with at most 8 points per lane are instantiated (the others never qualify)
     switch (L) {
 #define FIT_CASE(LLEN)                                                                                            \
   case LLEN:                                                                                                      \
-    if constexpr (wE(LLEN) <= 16) {                                                                               \
+    if constexpr (wE(LLEN) <= 8) {                                                                                \
       hipLaunchKernelGGL((k_xfused_w<R, LLEN>), dim3((unsigned)((int64_t)nrg * nchunk)), dim3(256), lds,          \
                          p->stream, (const cpx<R>*)p->tw[0], (const cpx<R>*)p->work2, gsy, gsz, gsf, (const R*)u, \
                          usy, usz, p->Gp, (cpx<R>*)p->nxbuf, osy, osz, osf, p->nd, p->M, p->G[1], nrows,          \
                          (R)(1.0 / (double)p->Gp), xa - 1, fband, xb - 1, ngiv, GC, kxt, ugs);                     \
     } else {                                                                                                      \
       prof_end(p);                                                                                                \
-      return fail(FIT_E_UNSUPPORTED, "wave-form x pass needs at most 16 points per lane");                        \
+      return fail(FIT_E_UNSUPPORTED, "wave-form x pass needs at most 8 points per lane");                         \
     }                                                                                                             \
     break;
       FIT_REG_LENGTHS(FIT_CASE)
@@ -796,6 +798,18 @@ int lastdens_launch(fit_plan* p, int gb, const int* bo, int mode, const DensArgs
   // columns per workgroup: 16 (128-byte rows in FP32) within the thread bound and 64 KB of LDS
   int TX = std::max(1, std::min(16, nthr / T));
   while (TX > 1 && (size_t)L * TX * csz > 65536) TX /= 2;
+  // private bins per wave within the budget (and the workgroup's LDS), with half the columns per
+  // workgroup if only that makes them fit
+  const size_t bin_budget = (size_t)p->last_bins_kb * 1024;
+  auto priv_fits = [&](int tx) {
+    return detbins_priv(nbins, T * tx, bin_budget) &&
+           (detbins_bytes(nbins, T * tx, 1) + 15) / 16 * 16 + (size_t)L * tx * csz <= g_max_lds;
+  };
+  int dpriv = priv_fits(TX) ? 1 : 0;
+  if (!dpriv && TX > 1 && priv_fits(TX / 2)) {
+    TX /= 2;
+    dpriv = 1;
+  }
   const int gx = (xb + TX - 1) / TX;
   const int nrow = ob.count();
   // rows per workgroup: enough workgroups to fill the chip, few partial rows to reduce
@@ -804,7 +818,6 @@ int lastdens_launch(fit_plan* p, int gb, const int* bo, int mode, const DensArgs
   int rc;
   if ((rc = grow(p, (void**)&p->partials2, &p->partials2_bytes, (size_t)gb * nblk * nbins * sizeof(double))))
     return rc;
-  const int dpriv = detbins_priv(nbins, T * TX, kFftBinBudget);
   const size_t lds = (detbins_bytes(nbins, T * TX, dpriv) + 15) / 16 * 16 + (size_t)L * TX * csz;
   const double bytes = (double)gb * p->M * xb * nrow * L * csz +
                        (mode == kDensFull ? (double)p->M * p->Np * csz + 4.0 * p->Np

@@ -111,11 +111,16 @@ int64_t fit_plan_live_count(void);
  *   FIT_OPT_COL_PIPE = 13: 0 (default) one tile of the other index per workgroup in the strided
  *     register-FFT passes; n > 0 the software-pipelined form, each workgroup walking n tiles and
  *     loading the next tile's lines while it transforms the current one.
+ *   FIT_OPT_LAST_BINS = 14: LDS budget in KiB (1 to 128, default 64) for per-wave private FP64
+ *     shell bins in the fused last pass; above it the waves share one copy and add their points
+ *     in order through one wave.  When the private copies fit only with half the columns per
+ *     workgroup, the pass runs half the columns (2D ensemble, 2048² and 1024² last passes: 1.52
+ *     -> 1.39 and 0.32 -> 0.26 ms against a 16 KiB budget).
  */
 enum { FIT_OPT_PLAIN_PIPELINE = 1, FIT_OPT_GIVER_BATCH = 2, FIT_OPT_XPASS_GIVERS = 3,
        FIT_OPT_XPASS_BLOCK = 4, FIT_OPT_SUBGRID = 5, FIT_OPT_COL_TILE = 6, FIT_OPT_XPASS_SELF = 7,
        FIT_OPT_XPASS_STAGE = 8, FIT_OPT_GRAD_SPLIT = 9, FIT_OPT_FUSED_FLUX = 10, FIT_OPT_RADIX_ORDER = 11,
-       FIT_OPT_COL_F64 = 12, FIT_OPT_COL_PIPE = 13 };
+       FIT_OPT_COL_F64 = 12, FIT_OPT_COL_PIPE = 13, FIT_OPT_LAST_BINS = 14 };
 int fit_plan_set_option(fit_plan* plan, int option, int value);
 int fit_plan_get_option(const fit_plan* plan, int option, int* value);
 
@@ -315,8 +320,9 @@ int fit_ensemble_groups(const fit_ensemble* ens);
 
 /* Ensemble options (no reference counterpart):
  *   FIT_ENS_OPT_GRAPH = 1: 1 (default) capture the ensemble into a HIP graph, 0 run eagerly.
- *   FIT_ENS_OPT_BATCH = 2: cap on snapshots per batched launch (0 = device memory budget). */
-enum { FIT_ENS_OPT_GRAPH = 1, FIT_ENS_OPT_BATCH = 2 };
+ *   FIT_ENS_OPT_BATCH = 2: cap on snapshots per batched launch (0 = device memory budget).
+ *   FIT_ENS_OPT_PLAN + o: plan option o (FIT_OPT_*) on every group's plan. */
+enum { FIT_ENS_OPT_GRAPH = 1, FIT_ENS_OPT_BATCH = 2, FIT_ENS_OPT_PLAN = 1000 };
 int fit_ensemble_set_option(fit_ensemble* ens, int option, int value);
 
 /*

@@ -79,9 +79,10 @@ def test_shell_to_shell_rerun_bitwise(fit, oracle, ns, nsh, prec):
 
 
 def test_ensemble_ordered_commit_rerun_bitwise(fit, oracle):
-    # a 1024² snapshot: 725 shells in the fused last pass of eight waves exceed the private-copy
-    # budget, so the waves commit in turn; replays of the captured graph return the same bits
+    # a 1024² snapshot: with a 1 KiB bin budget its 725 shells exceed the private copies, so the
+    # waves commit in turn; replays of the captured graph return the same bits
     import torch
+    from fit_amd import _lib
     shapes = [(1024, 1024), (64, 64), (1024, 1024)]
     us, kss = [], []
     for j, s in enumerate(shapes):
@@ -90,6 +91,7 @@ def test_ensemble_ordered_commit_rerun_bitwise(fit, oracle):
         kss.append(ks)
     ud = [to_dev(u, torch.complex64) for u in us]
     ens = fit.Ensemble(shapes, kss, binning=fit.LinearBinning(1.0), precision=np.float32)
+    ens.set_plan_option(_lib.OPT_LAST_BINS, 1)
     a = ens.fluxes(ud)
     b = ens.fluxes(ud)
     c = ens.fluxes(ud)
@@ -99,6 +101,14 @@ def test_ensemble_ordered_commit_rerun_bitwise(fit, oracle):
     e = rel(a[1][0].flux, Po)
     say("  64² Pi err", e)
     assert e < 2e-5
+    # the default 64 KiB budget gives the 1024-point last pass private per-wave bins at half the
+    # columns per workgroup: the same fluxes to rounding, again bitwise on replay
+    ens.set_plan_option(_lib.OPT_LAST_BINS, 64)
+    d = ens.fluxes(ud)
+    f = ens.fluxes(ud)
+    for x, y, z in zip(a, d, f):
+        assert np.array_equal(y[0].flux, z[0].flux)
+        assert rel(y[0].flux, x[0].flux) < 1e-5
     ens.close()
 
 
