@@ -594,8 +594,8 @@ static int create_plan(fit_plan** out, int nd, const int64_t* ns, int M, int D, 
     for (int d = 0; d < 3; ++d) p->band_rx[d] = std::max(p->band_rx[d], 0);
     if ((rc = dalloc(p, (void**)&p->d_shell_band, (size_t)n_shells * sizeof(int4)))) return bail(rc);
     hipMemcpy(p->d_shell_band, p->shell_band.data(), (size_t)n_shells * sizeof(int4), hipMemcpyHostToDevice);
-    // shell map on the original half grid (x <= n1/2) for the fused density pass
-    if (dealias != FIT_DEALIAS_PADDED) {
+    // shell map on the original half grid (x <= n1/2) for the pair density passes
+    {
       const int n1h = p->n[0] / 2 + 1;
       const int64_t Hn = (int64_t)n1h * p->n[1] * p->n[2];
       std::vector<int> shh((size_t)Hn);

@@ -81,6 +81,8 @@ struct fit_plan {
   void* nphys = nullptr;   // one physical product
   void* nhat = nullptr;    // M full original-grid N̂
   void* nhat_dst = nullptr;  // caller's device N̂ array when written in place of nhat
+  bool nhat_half_req = false;   // flux: the forward pass may store N̂ on the original half grid
+  bool nhat_half_done = false;  // ... and did (nhat then holds M half-grid components)
   void* in_f = nullptr;    // staged primary field (M comps)
   void* in_a = nullptr;    // staged advecting field (D comps)
   void* in_c = nullptr;    // staged carrier / extra input (M comps)
@@ -291,6 +293,7 @@ template <typename R> struct KTag<StHalf<R>> { static constexpr const char* v = 
 template <typename R> struct KTag<StReal<R>> { static constexpr const char* v = ",StReal"; };
 template <typename R> struct KTag<StProduct<R>> { static constexpr const char* v = ",StProduct"; };
 template <typename R> struct KTag<StNhat<R>> { static constexpr const char* v = ",StNhat"; };
+template <typename R> struct KTag<StNhatHalf<R>> { static constexpr const char* v = ",StNhatHalf"; };
 
 inline size_t csize(int dtype) { return dtype == FIT_F64 ? 16 : 8; }
 

@@ -273,6 +273,39 @@ template <typename R> struct StNhat {
   }
 };
 
+// N̂ on the original half grid x <= n1/2 (index x + (n1/2+1)(y + n2 z)): the transform-grid bins
+// whose preimage lies there, plus, with padding, the Nyquist plane x = n1/2 reached through the
+// mirror of the transform half grid.  With padding an even-length y (z) axis keeps its Nyquist
+// index n/2 at padded frequency -n/2 for both k and -k, so there N̂(-k) is not conj N̂(k): the
+// bins x > n1/2 of those planes are kept as well (ext, per component w1 = n1 - (n1/2+1) columns:
+// y = n2/2 plane w1 × n3, then z = n3/2 plane w1 × n2; ext == nullptr without padding).
+template <typename R> struct StNhatHalf {
+  Grid g; cpx<R>* out; int64_t sf; int trunc; R scale; cpx<R>* ext; int64_t esf;
+  __device__ __forceinline__ void put(int ox, int oy, int oz, int f, cpx<R> v) const {
+    if (trunc && g.dealiased(ox, oy, oz)) v = mk<R>((R)0, (R)0);
+    out[ox + (int64_t)(g.n1 / 2 + 1) * (oy + (int64_t)g.n2 * oz) + f * sf] = cscale(v, scale);
+  }
+  __device__ __forceinline__ void operator()(int x, int y, int z, int f, cpx<R> v) const {
+    int ox, oy, oz;
+    if (g.pre(x, y, z, ox, oy, oz) && 2 * ox <= g.n1) put(ox, oy, oz, f, v);
+    const int mx = x == 0 ? 0 : g.G1 - x;
+    if (mx > g.G1 / 2) {
+      const int my = y == 0 ? 0 : g.G2 - y;
+      const int mz = z == 0 ? 0 : g.G3 - z;
+      if (g.pre(mx, my, mz, ox, oy, oz)) {
+        if (2 * ox <= g.n1) {
+          put(ox, oy, oz, f, cconj(v));
+        } else if (ext) {
+          const int n1h = g.n1 / 2 + 1, w1 = g.n1 - n1h;
+          const cpx<R> w = cscale(cconj(v), scale);
+          if (g.n2 % 2 == 0 && 2 * oy == g.n2) ext[(ox - n1h) + (int64_t)w1 * oz + f * esf] = w;
+          if (g.n3 % 2 == 0 && 2 * oz == g.n3) ext[(int64_t)w1 * g.n3 + (ox - n1h) + (int64_t)w1 * oy + f * esf] = w;
+        }
+      }
+    }
+  }
+};
+
 // ---------------------------------------------------------------------------------------------
 // FFT pass kernels
 // ---------------------------------------------------------------------------------------------
@@ -1954,6 +1987,142 @@ __global__ void __launch_bounds__(256) k_density(Grid g, int4 invs, int ninv, in
     __syncthreads();
     for (int s = threadIdx.x; s < nb; s += blockDim.x) partials[(int64_t)blockIdx.x * nb + s] = db.total(s);
   }
+}
+
+// Component c of the effective carrier e (t = Re Σ_c conj(e_c) N̂_c) from the carrier's components
+// u (M <= 3) at one bin with wavenumbers (kx, ky, kz): carrier_comp on register values.
+template <typename R>
+__device__ __forceinline__ cpx<R> carrier_vals(int inv, int nd, int c, const cpx<R> (&u)[3], R kx, R ky, R kz) {
+  if (inv == 0 || inv == 3) return u[c];
+  if (inv == 2 && nd == 2) {
+    if (c > 1) return mk<R>((R)0, (R)0);
+    const cpx<R> w = cmuli<1>(csub(cscale(u[1], kx), cscale(u[0], ky)));
+    return c == 0 ? cmuli<1>(cscale(w, ky)) : cmuli<-1>(cscale(w, kx));
+  }
+  if (inv == 1) {
+    if (c == 0) return cmuli<1>(csub(cscale(u[2], ky), cscale(u[1], kz)));
+    if (c == 1) return cmuli<1>(csub(cscale(u[0], kz), cscale(u[2], kx)));
+    return cmuli<1>(csub(cscale(u[1], kx), cscale(u[0], ky)));
+  }
+  const cpx<R> wx = cmuli<1>(csub(cscale(u[2], ky), cscale(u[1], kz)));
+  const cpx<R> wy = cmuli<1>(csub(cscale(u[0], kz), cscale(u[2], kx)));
+  const cpx<R> wz = cmuli<1>(csub(cscale(u[1], kx), cscale(u[0], ky)));
+  if (c == 0) return cmuli<1>(csub(cscale(wz, ky), cscale(wy, kz)));
+  if (c == 1) return cmuli<1>(csub(cscale(wx, kz), cscale(wz, kx)));
+  return cmuli<1>(csub(cscale(wy, kx), cscale(wx, ky)));
+}
+
+// Pair density over the original half grid (N̂ stored there, M <= 3), into the shell of k (which
+// −k shares: symmetric shell maps).  On the planes x = 0 and x = n1/2, where −k is a half-grid bin
+// of its own, t(k) alone.  Elsewhere t(k) + t(−k) with e(−k) from the carrier at the mirror bin
+// and its own wavenumbers and N̂(−k) = conj N̂(k), i.e. Re Σ_c [conj e_c(k) + e_c(−k)] N̂_c(k);
+// on the Nyquist rows of a padded grid (ext != nullptr) N̂(−k) comes from ext (StNhatHalf).
+// FP64 LDS bins as in k_density; two rows of loads in flight per lane (measured: 1.50 ms for the
+// 512³ KE + helicity pair density, against 1.73 ms at four rows and 1.65 ms with one thread per
+// half-row bin in 320-thread workgroups).
+template <typename R>
+__global__ void __launch_bounds__(256) k_density_pair(Grid g, int4 invs, int ninv, int M, const cpx<R>* __restrict__ car,
+                                                      int64_t csf, const cpx<R>* __restrict__ nh, int64_t nsf,
+                                                      const R* __restrict__ ks0, const R* __restrict__ ks1,
+                                                      const R* __restrict__ ks2, const int* __restrict__ shh,
+                                                      int nsh, double* __restrict__ partials, int dpriv,
+                                                      const cpx<R>* __restrict__ ext, int64_t esf) {
+  extern __shared__ double dbins[];
+  const int nb = ninv * nsh;
+  const DetBins db{dbins, nb, dpriv};
+  db.zero();
+  __syncthreads();
+  constexpr int RP = 2;
+  const int n1h = g.n1 / 2 + 1, w1 = g.n1 - n1h;
+  const int nrow = g.n2 * g.n3;
+  for (int row = blockIdx.x; row < nrow; row += RP * gridDim.x) {
+    bool live[RP];
+    R ky[RP], kz[RP], kym[RP], kzm[RP];
+    int64_t base[RP], mbase[RP], hbase[RP], ebase[RP];
+    bool eown[RP];
+#pragma unroll
+    for (int r = 0; r < RP; ++r) {
+      const int rr = row + r * (int)gridDim.x;
+      live[r] = rr < nrow;   // uniform
+      const int rw = live[r] ? rr : row;
+      const int iy = rw % g.n2, iz = rw / g.n2;
+      const int my = iy == 0 ? 0 : g.n2 - iy, mz = iz == 0 ? 0 : g.n3 - iz;
+      ky[r] = g.nd >= 2 ? ks1[iy] : (R)0;
+      kz[r] = g.nd >= 3 ? ks2[iz] : (R)0;
+      kym[r] = g.nd >= 2 ? ks1[my] : (R)0;
+      kzm[r] = g.nd >= 3 ? ks2[mz] : (R)0;
+      base[r] = (int64_t)rw * g.n1;
+      mbase[r] = ((int64_t)my + (int64_t)g.n2 * mz) * g.n1;
+      hbase[r] = (int64_t)rw * n1h;
+      // Nyquist row of a padded grid: N̂(−k) at ext[ebase + n1 − x]
+      eown[r] = false;
+      ebase[r] = 0;
+      if (ext && g.n2 % 2 == 0 && 2 * iy == g.n2) {
+        eown[r] = true;
+        ebase[r] = (int64_t)w1 * mz - n1h;
+      } else if (ext && g.n3 % 2 == 0 && 2 * iz == g.n3) {
+        eown[r] = true;
+        ebase[r] = (int64_t)w1 * g.n3 + (int64_t)w1 * my - n1h;
+      }
+    }
+    for (int x0 = 0; x0 < n1h; x0 += blockDim.x) {   // uniform trip count (commit may sync)
+      const int ix = x0 + threadIdx.x;
+      R tv[RP][4];
+      int s[RP];
+#pragma unroll
+      for (int r = 0; r < RP; ++r) {
+        s[r] = 0;
+        tv[r][0] = tv[r][1] = tv[r][2] = tv[r][3] = (R)0;
+      }
+      if (ix < n1h) {
+        const int mx = ix == 0 ? 0 : g.n1 - ix;
+        const R kx = ks0[ix], kxm = ks0[mx];
+        const bool selfc = ix == 0 || 2 * ix == g.n1;
+        cpx<R> up[RP][3], um[RP][3], n[RP][3];
+#pragma unroll
+        for (int r = 0; r < RP; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const int cc = c < M ? c : 0;
+            up[r][c] = car[base[r] + ix + cc * csf];
+            um[r][c] = car[mbase[r] + mx + cc * csf];
+            n[r][c] = nh[hbase[r] + ix + cc * nsf];
+          }
+#pragma unroll
+        for (int r = 0; r < RP; ++r) s[r] = shh[hbase[r] + ix];
+#pragma unroll
+        for (int r = 0; r < RP; ++r) {
+          const bool own = !selfc && eown[r];   // uniform in the row but for the two planes
+          cpx<R> nm[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) nm[c] = own ? ext[ebase[r] + mx + (c < M ? c : 0) * esf] : mk<R>((R)0, (R)0);
+          for (int q = 0; q < ninv; ++q) {
+            const int inv = q == 0 ? invs.x : (q == 1 ? invs.y : (q == 2 ? invs.z : invs.w));
+            R t = (R)0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              if (c >= M) break;
+              const cpx<R> e = carrier_vals<R>(inv, g.nd, c, up[r], kx, ky[r], kz[r]);
+              const cpx<R> em = carrier_vals<R>(inv, g.nd, c, um[r], kxm, kym[r], kzm[r]);
+              const cpx<R> nc = n[r][c];
+              if (selfc) t += e.x * nc.x + e.y * nc.y;                                   // t(k)
+              else if (own) t += (e.x * nc.x + e.y * nc.y) + (em.x * nm[c].x + em.y * nm[c].y);
+              else t += (e.x + em.x) * nc.x + (e.y - em.y) * nc.y;                   // t(k) + t(−k)
+            }
+            tv[r][q] = t;
+          }
+        }
+      }
+      db.commit([&](double* bb) {
+#pragma unroll
+        for (int r = 0; r < RP; ++r)
+          if (live[r] && s[r] > 0)
+            for (int q = 0; q < ninv; ++q) atomicAdd(&bb[q * nsh + s[r] - 1], (double)tv[r][q]);
+      });
+    }
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < nb; s += blockDim.x) partials[(int64_t)blockIdx.x * nb + s] = db.total(s);
 }
 
 // ---------------------------------------------------------------------------------------------

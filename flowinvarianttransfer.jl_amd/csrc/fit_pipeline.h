@@ -539,11 +539,25 @@ int v2_xfused_self(fit_plan* p, const int* bi, const int* bo) {
   return FIT_OK;
 }
 
-// Remaining forward passes of nxbuf (M fields, one field group per component) into full N̂.
 template <typename R> int lastdens_launch(fit_plan* p, int gb, const int* bo, int mode, const DensArgs<R>& da,
                                           int nbins, double* out);
 template <typename R> int carrier_half(fit_plan* p, int inv, const void* f, const void* const* srcs, int nb, void* eh);
 
+// Padded plans: the Nyquist-plane extras of a half-grid N̂ (StNhatHalf), after its M components
+// in the same buffer; nullptr without padding.
+template <typename R>
+cpx<R>* nhat_ext(fit_plan* p, void* nh, int64_t* esf) {
+  *esf = 0;
+  if (p->dealias != FIT_DEALIAS_PADDED) return nullptr;
+  const int n1h = p->n[0] / 2 + 1, w1 = p->n[0] - n1h;
+  const int64_t Hn = (int64_t)n1h * p->n[1] * p->n[2];
+  *esf = (int64_t)w1 * (p->n[1] + p->n[2]);
+  return (cpx<R>*)nh + (int64_t)p->M * Hn;
+}
+
+// Remaining forward passes of nxbuf (M fields, one field group per component) into N̂: the full
+// original grid, or for the flux (nhat_half_req) the original half grid plus the padded Nyquist
+// extras.
 template <typename R>
 int v2_forward_nhat(fit_plan* p, const int* bo) {
   const int xb = xext(p, bo[0]), pb = xpitch(xb);
@@ -583,17 +597,29 @@ int v2_forward_nhat(fit_plan* p, const int* bo) {
   const bool trunc = p->dealias == FIT_DEALIAS_TWOTHIRDS;
   const double scale = (double)p->Gp / ((double)p->Np * (double)p->Np);
   void* dst = p->nhat_dst ? p->nhat_dst : p->nhat;
-  StNhat<R> sn{g, (cpx<R>*)dst, p->Np, trunc, (R)scale};
+  // the flux reads N̂ only through the pair density: then the original half grid suffices
+  const bool half = p->nhat_half_req && !p->nhat_dst;
+  const int64_t nsf = half ? (int64_t)(p->n[0] / 2 + 1) * p->n[1] * p->n[2] : p->Np;
   int rc;
-  if (trunc) FIT_HIP(hipMemsetAsync(dst, 0, (size_t)p->M * p->Np * sizeof(cpx<R>), p->stream));
-  if (p->nd == 3) {
-    if ((rc = launch_col<R, -1>(p, ld, st, 1, p->M, xb, full_band(p->G[2]), full_band(p->G[1]),
-                                Band{bo[1], p->G[1]})))
-      return rc;
-    return launch_col<R, -1>(p, ld, sn, 2, p->M, xb, Band{bo[1], p->G[1]}, full_band(p->G[2]),
-                             Band{bo[2], p->G[2]});
+  if (trunc) FIT_HIP(hipMemsetAsync(dst, 0, (size_t)p->M * nsf * sizeof(cpx<R>), p->stream));
+  auto last = [&](auto sn) {
+    if (p->nd == 3) {
+      int rc2 = launch_col<R, -1>(p, ld, st, 1, p->M, xb, full_band(p->G[2]), full_band(p->G[1]),
+                                  Band{bo[1], p->G[1]});
+      if (rc2) return rc2;
+      return launch_col<R, -1>(p, ld, sn, 2, p->M, xb, Band{bo[1], p->G[1]}, full_band(p->G[2]),
+                               Band{bo[2], p->G[2]});
+    }
+    return launch_col<R, -1>(p, ld, sn, 1, p->M, xb, full_band(1), full_band(p->G[1]), Band{bo[1], p->G[1]});
+  };
+  if (half) {
+    int64_t esf = 0;
+    cpx<R>* ext = nhat_ext<R>(p, dst, &esf);
+    if ((rc = last(StNhatHalf<R>{g, (cpx<R>*)dst, nsf, trunc, (R)scale, ext, esf}))) return rc;
+    p->nhat_half_done = true;
+    return FIT_OK;
   }
-  return launch_col<R, -1>(p, ld, sn, 1, p->M, xb, full_band(1), full_band(p->G[1]), Band{bo[1], p->G[1]});
+  return last(StNhat<R>{g, (cpx<R>*)dst, p->Np, trunc, (R)scale});
 }
 
 // N̂ of f advected by u (already in uphys) through the fused pipeline.
@@ -676,6 +702,38 @@ int density_multi(fit_plan* p, int ninv, const int* invs, const void* car, const
   return FIT_OK;
 }
 
+// density_multi with N̂ on the original half grid (nh, M components of Hn bins): the pair density
+// of every half-grid bin and its mirror into the shared shell (k_density_pair), into dcolG.
+template <typename R>
+int density_multi_pair(fit_plan* p, int ninv, const int* invs, const void* car, const void* nh) {
+  Grid g = grid_of<R>(p);
+  const int nsh = p->nsh, nb = ninv * nsh;
+  const int nblk = p->nblk_density;
+  const size_t csz = sizeof(cpx<R>);
+  const int64_t Hn = (int64_t)(p->n[0] / 2 + 1) * p->n[1] * p->n[2];
+  int rc;
+  if ((rc = grow(p, (void**)&p->partials2, &p->partials2_bytes, (size_t)nblk * nb * sizeof(double)))) return rc;
+  if ((rc = grow(p, (void**)&p->dcolG, &p->dcolG_bytes, (size_t)nb * sizeof(double)))) return rc;
+  const int4 iv{invs[0], ninv > 1 ? invs[1] : 0, ninv > 2 ? invs[2] : 0, ninv > 3 ? invs[3] : 0};
+  // carrier at k and -k (every full-grid bin about once), N̂ and the shell map on the half grid
+  const int ncar = p->M;
+  prof_begin(p, FIT_K_DENSITY_SHELL, (double)p->Np * ncar * csz + (double)Hn * (p->M * csz + 4.0), "k_density_pair");
+  const int thr = kThreads;
+  const int dpriv = detbins_priv(nb, thr, kDensBinBudget);
+  int64_t esf = 0;
+  const cpx<R>* ext = nhat_ext<R>(p, const_cast<void*>(nh), &esf);
+  hipLaunchKernelGGL(k_density_pair<R>, dim3(nblk), dim3(thr), detbins_bytes(nb, thr, dpriv), p->stream, g,
+                     iv, ninv, p->M, (const cpx<R>*)car, p->Np, (const cpx<R>*)nh, Hn, (const R*)p->ks[0],
+                     (const R*)p->ks[1], (const R*)p->ks[2], p->shh, nsh, p->partials2, dpriv, ext, esf);
+  prof_end(p);
+  FIT_HIP(hipGetLastError());
+  prof_begin(p, FIT_K_REDUCE, (double)nblk * nb * 8.0 + nb * 8.0, "k_reduce_rows");
+  launch_reduce(p, p->partials2, nblk, nb, 1, p->dcolG);
+  prof_end(p);
+  FIT_HIP(hipGetLastError());
+  return FIT_OK;
+}
+
 
 template <typename R>
 int nonlinear_term_impl(fit_plan* p, const void* f, const void* a) {
@@ -729,7 +787,16 @@ int flux_multi(fit_plan* p, int ninv, const int* invs, const void* f, const void
     // the pipeline took an unfused route and stored N̂
     return density_multi<R>(p, ninv, invs, f, p->nhat);
   }
-  if ((rc = nonlinear_term_impl<R>(p, f, a))) return rc;
+  // N̂ on the half grid and the pair density when both modes of a pair share a shell
+  p->nhat_half_req = p->shh && p->shell_symmetric && p->M <= 3 && p->nd >= 2 && !p->plain;
+  p->nhat_half_done = false;
+  rc = nonlinear_term_impl<R>(p, f, a);
+  p->nhat_half_req = false;
+  if (rc) return rc;
+  if (p->nhat_half_done) {
+    p->nhat_half_done = false;
+    return density_multi_pair<R>(p, ninv, invs, f, p->nhat);
+  }
   return density_multi<R>(p, ninv, invs, f, p->nhat);
 }
 

@@ -93,6 +93,31 @@ FLUX_CASES = [
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("ns,dea", [((8, 8, 8), "pad"), ((6, 8, 10), "pad"), ((5, 9, 8), "pad"), ((10, 8, 6), "23"),
+                                    ((9, 6, 5), "none"), ((12, 12), "pad"), ((9, 10), "pad")])
+def test_flux_full_band_field_nyquist_rows(fit, oracle, ns, dea, prec):
+    """Kinetic-energy and helicity fluxes of a real random field with content at every bin: the
+    flux stores N̂ on the half grid and sums pair densities, which on a padded grid needs the
+    Nyquist rows' own N̂(−k) (there N̂(−k) != conj N̂(k))."""
+    say("case full-band flux", ns, dea, prec)
+    nd = len(ns)
+    ks = ks_of(oracle, ns)
+    rng = np.random.default_rng(17)
+    u = np.fft.fftn(rng.standard_normal(ns + (nd,)), axes=tuple(range(nd))) / np.prod(ns)
+    invs = ("ke", "helicity") if nd == 3 else ("ke",)
+    I = {"ke": fit.KineticEnergy(), "helicity": fit.Helicity()}
+    deal = {"none": fit.NoDealiasing(), "23": fit.OrszagTwoThirds(), "pad": fit.PaddedThreeHalves()}[dea]
+    dt = np.complex128 if prec == "f64" else np.complex64
+    for name in invs:
+        r = fit.calculate_spectral_flux(u.astype(dt), ks, binning=fit.LinearBinning(1.0), dealiasing=deal,
+                                        invariant=I[name])
+        _, T, Pi = oracle.calculate_spectral_flux(u, ks, oracle.LinearBinning(1.0), dea, name)
+        e = max(rel(r.transfer_spectrum, T), rel(r.flux, Pi))
+        say("  ", name, "err", e)
+        assert e < (TOL64 if prec == "f64" else 2e-5)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("ns,dea,inv,binn,geom", FLUX_CASES)
 def test_spectral_flux_matches_oracle(fit, oracle, ns, dea, inv, binn, geom, prec):
     say("case flux", ns, dea, inv, binn, geom, prec)

@@ -294,6 +294,66 @@ def test_padded_shell_to_shell_reduces(oracle):
     assert nrm > 0 and np.max(np.abs(net - Tk)) < 1e-10 * nrm and asym < 1e-10 * nrm
 
 
+def _pair_density_half_grid(N, e, sh, pad):
+    """Shell sums of t = Re Σ_c conj(e_c) N̂_c from the original half grid x <= n1/2 only, as the
+    device pair density forms them: t(k) alone on the planes x = 0 and x = n1/2 (−k is a half-grid
+    bin there), t(k) + t(−k) elsewhere with N̂(−k) = conj N̂(k), except on the Nyquist rows of an
+    even y / z axis of a padded grid, where N̂(−k) is its own stored bin."""
+    n1, n2, n3 = N.shape[:3]
+    out = np.zeros(sh.max() + 1)
+    for ix in range(n1 // 2 + 1):
+        mx = (n1 - ix) % n1
+        for iy in range(n2):
+            my = (n2 - iy) % n2
+            for iz in range(n3):
+                mz = (n3 - iz) % n3
+                ek, em, nk = e[ix, iy, iz], e[mx, my, mz], N[ix, iy, iz]
+                own = pad and ((n2 % 2 == 0 and 2 * iy == n2) or (n3 % 2 == 0 and 2 * iz == n3))
+                if ix == 0 or 2 * ix == n1:
+                    t = np.sum(np.real(np.conj(ek) * nk))
+                elif own:
+                    t = np.sum(np.real(np.conj(ek) * nk)) + np.sum(np.real(np.conj(em) * N[mx, my, mz]))
+                else:
+                    t = np.sum((ek.real + em.real) * nk.real + (ek.imag - em.imag) * nk.imag)
+                out[sh[ix, iy, iz]] += t
+    return out
+
+
+@pytest.mark.parametrize("ns,dea,inv", [((12, 12), "pad", "ke"), ((10, 10), "23", "ke"), ((8, 8, 8), "pad", "ke"),
+                                        ((8, 8, 8), "pad", "helicity"), ((6, 8, 10), "pad", "ke"),
+                                        ((7, 9, 8), "pad", "helicity"), ((9, 8, 7), "none", "ke")])
+def test_half_grid_pair_density_equals_full_grid(oracle, ns, dea, inv):
+    # the flux's device density reads N̂ on the half grid only; its shell sums equal the full-grid
+    # density's for symmetric shells, including the padded Nyquist rows where N̂(−k) is not
+    # conj N̂(k) (the padded grid holds both at frequency −n/2); the plain pair formula there fails
+    nd = len(ns)
+    ks = oracle.wavenumber_grid(ns, (2 * math.pi,) * nd)
+    if nd == 3:
+        # a real random field: content at every bin, the Nyquist rows included
+        rng = np.random.default_rng(3)
+        u = np.fft.fftn(rng.standard_normal(ns + (3,)), axes=(0, 1, 2)) / np.prod(ns)
+    else:
+        u, _ = oracle.streamfunction_velocity_2d(ns[0], 3)
+    N = oracle.compute_nonlinear_term(u, ks, dea)
+    K = np.meshgrid(*ks, indexing="ij")
+    kz = K[2] if nd == 3 else np.zeros_like(K[0])
+    if inv == "ke":
+        e = u
+    else:
+        ux, uy, uz = u[..., 0], u[..., 1], u[..., 2]
+        e = 1j * np.stack([uz * K[1] - uy * kz, ux * kz - uz * K[0], uy * K[0] - ux * K[1]], -1)
+    sh = np.floor(np.sqrt(K[0] ** 2 + K[1] ** 2 + kz ** 2) + 0.5).astype(int)
+    full = np.bincount(sh.ravel(), weights=np.real(np.sum(np.conj(e) * N, -1)).ravel(), minlength=sh.max() + 1)
+    if nd == 2:
+        N, e, sh = N[:, :, None, :], e[:, :, None, :], sh[:, :, None]
+    half = _pair_density_half_grid(N, e, sh, dea == "pad")
+    assert rel(half, full) < 1e-12
+    if dea == "pad" and (ns[1] % 2 == 0 or (nd == 3 and ns[2] % 2 == 0)):
+        # without the Nyquist-row rule the half-grid sums are wrong
+        plain = _pair_density_half_grid(N, e, sh, False)
+        assert rel(plain, full) > 1e-6
+
+
 # ----------------------------------------------------------------------------- mode-to-mode
 
 @pytest.mark.parametrize("case", ["2d_ke_23", "2d_ke_none", "2d_ens", "3d_ke", "3d_hel", "3d_ens",
