@@ -48,14 +48,24 @@ def _real_dtype(a):
     return np.float64 if dt in (np.complex128, np.float64) else np.float32
 
 
+def _fortran_order(t):
+    """True when tensor t is column-major contiguous (read from its strides; no view is made)."""
+    acc = 1
+    for n, st in zip(t.shape, t.stride()):
+        if n != 1 and st != acc:
+            return False
+        acc *= n
+    return True
+
+
 def _marshal(a, io):
     """-> (pointer, keep-alive object, ptr_kind) for a complex array in io precision (F32/F64),
     column-major."""
     if _is_device(a):
         want = torch.complex64 if io == _lib.F32 else torch.complex128
         t = a if a.dtype == want else a.to(want)
-        rev = tuple(reversed(range(t.dim())))
-        if not t.permute(rev).is_contiguous():
+        if not _fortran_order(t):
+            rev = tuple(reversed(range(t.dim())))
             t = t.permute(rev).contiguous().permute(rev)
         return t.data_ptr(), t, _lib.PTR_DEVICE
     want = np.complex64 if io == _lib.F32 else np.complex128
@@ -73,8 +83,8 @@ def _marshal_real(a, io):
     array's own precision (F32 / F64)."""
     if _is_device(a):
         t = a if a.dtype in (torch.float32, torch.float64) else a.to(torch.float64)
-        rev = tuple(reversed(range(t.dim())))
-        if not t.permute(rev).is_contiguous():
+        if not _fortran_order(t):
+            rev = tuple(reversed(range(t.dim())))
             t = t.permute(rev).contiguous().permute(rev)
         return t.data_ptr(), t, _lib.PTR_DEVICE, (_lib.F64 if t.dtype == torch.float64 else _lib.F32)
     arr = np.asarray(a)
