@@ -1150,17 +1150,23 @@ __global__ void __launch_bounds__(256) k_xfused_self_stg(const cpx<float>* __res
   const IdxLine idx{line * LL};
   cpx<R>* sl = stg + line * 2 * SROW;
   const int nun = (kxa + 2) / 2;  // 16-byte units (two points) covering kx <= kxa
+  // the staged points kx > kxa read as zero: those past the last unit are cleared once here (no
+  // transfer writes them), the one inside it (kx = kxa + 1 for even kxa) after every transfer
+  for (int i = 2 * nun + t; i < SROW; i += T) {
+    sl[i] = mk<R>((R)0, (R)0);
+    sl[SROW + i] = mk<R>((R)0, (R)0);
+  }
   // stage rows r0, r1 of field gf (kx <= kxa; rows start 128-byte aligned, the pitch covers
-  // the odd point of the last unit)
+  // the odd point of the last unit).  A row past the grid stages row 0 (offset 0): finite values
+  // whose transform is never stored, so the transforms need no row test.
   auto issue = [&](const cpx<R>* gf) {
 #pragma unroll
     for (int rr = 0; rr < 2; ++rr) {
-      const bool ok = rr ? ok1 : ok0;
       const cpx<R>* src = gf + (rr ? ro1 : ro0);
 #pragma unroll
       for (int i = 0; i < (SROW / 2 + 63) / 64; ++i) {
         const int ui = i * 64 + t;
-        if (ok && ui < nun)
+        if (ui < nun)
           lds_dma16(src + 2 * ui,
                     __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)(lds_void*)(sl + rr * SROW + i * 128)));
       }
@@ -1170,15 +1176,16 @@ __global__ void __launch_bounds__(256) k_xfused_self_stg(const cpx<float>* __res
   auto consume = [&](bool dx, cpx<R> (&v)[E]) {
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the staged rows have landed
     __builtin_amdgcn_wave_barrier();
+    if (t == 0 && kxa + 1 <= L / 2) {
+      sl[kxa + 1] = mk<R>((R)0, (R)0);
+      sl[SROW + kxa + 1] = mk<R>((R)0, (R)0);
+    }
+    wave_lds_sync();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = t + T * e;
       const int kk = k <= L / 2 ? k : L - k;
-      cpx<R> A = mk<R>((R)0, (R)0), Bv = A;
-      if (kk <= kxa) {
-        if (ok0) A = sl[kk];
-        if (ok1) Bv = sl[SROW + kk];
-      }
+      cpx<R> A = sl[kk], Bv = sl[SROW + kk];
       if (dx) {
         const R kx = kxs[kk];
         A = mk<R>(-kx * A.y, kx * A.x);
