@@ -977,16 +977,37 @@ __global__ void __launch_bounds__(256) k_xfused_r(const cpx<R>* __restrict__ tw,
         if (k > L / 2) { A.y = -A.y; Bv.y = -Bv.y; }
         v[e] = mk<R>(A.x - Bv.y, A.y + Bv.x);
       }
-      lds_sync<WV>();  // the previous transform / output loop is done with the LDS buffer
-      reg_fft_m<L, E, 1, xfence_r<L>(), WV, R>(v, lds0, idx, t, tw);
       const R* uj = ug + (int64_t)j * usf;
+      if constexpr (WV) {
+        // wave-synchronised exchanges: the velocity rows are requested before the transform,
+        // which they then overlap (row offsets are 0 past the grid: in-bounds loads, selected
+        // to zero).  192-point T(n,m) pass: 2.92 -> 2.78 ms; the workgroup-barrier form waits for
+        // them at its first barrier and only pays the registers (2048 points: 1.41 -> 1.51 ms).
+        R u0[E], u1[E];
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int xx = t + T * e;
-        const R u0 = ok0 ? uj[xx + uo0] : (R)0;
-        const R u1 = ok1 ? uj[xx + uo1] : (R)0;
-        const cpx<R> tt = mk<R>(u0 * (v[e].x * gscale), u1 * (v[e].y * gscale));
-        acc[e] = j == 0 ? tt : cadd(acc[e], tt);
+        for (int e = 0; e < E; ++e) {
+          const int xx = t + T * e;
+          u0[e] = uj[xx + uo0];
+          u1[e] = uj[xx + uo1];
+        }
+        lds_sync<WV>();  // the previous transform / output loop is done with the LDS buffer
+        reg_fft_m<L, E, 1, xfence_r<L>(), WV, R>(v, lds0, idx, t, tw);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const cpx<R> tt = mk<R>((ok0 ? u0[e] : (R)0) * (v[e].x * gscale), (ok1 ? u1[e] : (R)0) * (v[e].y * gscale));
+          acc[e] = j == 0 ? tt : cadd(acc[e], tt);
+        }
+      } else {
+        lds_sync<WV>();  // the previous transform / output loop is done with the LDS buffer
+        reg_fft_m<L, E, 1, xfence_r<L>(), WV, R>(v, lds0, idx, t, tw);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int xx = t + T * e;
+          const R u0 = ok0 ? uj[xx + uo0] : (R)0;
+          const R u1 = ok1 ? uj[xx + uo1] : (R)0;
+          const cpx<R> tt = mk<R>(u0 * (v[e].x * gscale), u1 * (v[e].y * gscale));
+          acc[e] = j == 0 ? tt : cadd(acc[e], tt);
+        }
       }
     }
     lds_sync<WV>();
