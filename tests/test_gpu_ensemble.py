@@ -128,6 +128,27 @@ def test_ensemble_api_uses_batched_path(fit, oracle):
     check(fit, oracle, res, us, ths, kss, 1e-5)
 
 
+def test_ensemble_plan_options_reach_every_group(fit, oracle):
+    """FIT_ENS_OPT_PLAN + o sets plan option o on every group's plan: the workgroup-form x pass and
+    a 1 KiB bin budget (ordered commits) give the same fluxes; bad values are rejected."""
+    import torch
+    from fit_amd import _lib
+    us, ths, kss = make_ensemble(oracle, SHAPES_2D[:4], 4600)
+    ud = [to_dev(u, torch.complex64) for u in us]
+    td = [to_dev(t, torch.complex64) for t in ths]
+    ens = fit.Ensemble(SHAPES_2D[:4], kss, with_scalar=True, binning=fit.LinearBinning(1.0))
+    base = ens.raw_fluxes(ud, td)
+    ens.set_plan_option(_lib.OPT_XPASS_BLOCK, 1)
+    ens.set_plan_option(_lib.OPT_LAST_BINS, 1)
+    for a, b in zip(ens.raw_fluxes(ud, td), base):
+        assert nerr(a, b) < 1e-5
+    with pytest.raises(ValueError):
+        ens.set_plan_option(_lib.OPT_LAST_BINS, 0)
+    with pytest.raises(ValueError):
+        ens.set_plan_option(999, 1)
+    ens.close()
+
+
 PADDED_SHAPES = {2: [(16, 16), (32, 32), (16, 16), (24, 24), (16, 24)],
                  3: [(8, 8, 8), (12, 12, 12), (8, 8, 8)]}
 
