@@ -104,10 +104,10 @@ int64_t fit_plan_live_count(void);
  *     transforms) run their arithmetic in FP64, with FP64 twiddles and LDS exchanges, loading and
  *     storing the fields in FP32, so that each value is rounded to FP32 once per pass instead of at
  *     every butterfly stage: bit 0 the inverse passes (gradient passes included), bit 1 the forward
- *     y pass, bit 2 the forward z pass.  6 (default): both forward passes, which halves the FP32
- *     error of Π(K) at 512³ with 3/2 padding (eight seeded fields: rms 6.6e-7 -> 3.4e-7, worst
- *     1.28e-6 -> 5.3e-7) for 2.5 ms more per flux; 4 or 2 alone: rms 4.2e-7, worst 7.2e-7 /
- *     7.3e-7 for 1.0 / 1.5 ms; 7 also the inverse passes (rms 2.5e-7, 6.6 ms more); 0 all FP32.
+ *     y pass, bit 2 the forward z pass.  6 (default): both forward passes.  FP32 error of the
+ *     kinetic-energy Π(K) at 512³ with 3/2 padding against FP64 (eight seeded fields, rms /
+ *     worst): 6: 3.0e-7 / 4.9e-7 for 1.2 ms more per flux than 0; 4: 3.5e-7 / 6.8e-7 (+0.6 ms);
+ *     2: 4.2e-7 / 6.0e-7 (+0.7 ms); 0 (all FP32): 4.5e-7 / 6.7e-7; 7 also the inverse passes.
  *   FIT_OPT_COL_PIPE = 13: 0 (default) one tile of the other index per workgroup in the strided
  *     register-FFT passes; n > 0 the software-pipelined form, each workgroup walking n tiles and
  *     loading the next tile's lines while it transforms the current one.
